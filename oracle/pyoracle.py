@@ -1,0 +1,160 @@
+"""ORACLE (test infrastructure only): ctypes access to liboracle.so (the C
+restatement of the reference's BSIM4 / Newton / transient path) and to
+_ref/libklu_ref.so (the reference's own KLU, compiled from its sources).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_klu = None
+
+N_ROLES = ['d', 'g', 's', 'b', 'dp', 'gp', 'gm', 'sp', 'db', 'bp', 'sb', 'q']
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, 'liboracle.so')
+        if not os.path.exists(path):
+            raise RuntimeError('oracle not built: run make -C oracle')
+        L = ctypes.CDLL(path)
+        vp, d, i, cp = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_char_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_model_new.restype = vp
+        L.or_model_free.argtypes = [vp]
+        L.or_model_param.argtypes = [vp, cp, d]
+        L.or_model_param.restype = i
+        L.or_model_finish.argtypes = [vp, d, d]
+        L.or_model_finish.restype = i
+        L.or_model_get.argtypes = [vp, cp]
+        L.or_model_get.restype = d
+        L.or_inst_new.argtypes = [vp]
+        L.or_inst_new.restype = vp
+        L.or_inst_free.argtypes = [vp]
+        L.or_inst_param.argtypes = [vp, cp, d]
+        L.or_inst_param.restype = i
+        L.or_inst_defaults.argtypes = [vp]
+        L.or_inst_defaults.restype = i
+        L.or_inst_finish.argtypes = [vp, ip, i, d]
+        L.or_inst_finish.restype = i
+        L.or_qa_dc.argtypes = [vp, d, d, d, d, d, d, dp, dp]
+        L.or_qa_dc.restype = i
+        L.or_inst_load.argtypes = [vp, i, d, d, d, d, d, dp, i, i, d, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_inst_load.restype = i
+        L.or_inst_get.argtypes = [vp, cp]
+        L.or_inst_get.restype = d
+        L.or_inst_node.argtypes = [vp, i]
+        L.or_inst_node.restype = i
+        L.or_counts.argtypes = [ip, ip, ip]
+        L.or_slot_rc.argtypes = [i, ip, ip]
+        L.or_slot_rc.restype = i
+        _lib = L
+    return _lib
+
+
+def klu_ref():
+    """The reference's KLU, or None when oracle/_ref was not built."""
+    global _klu
+    if _klu is None:
+        path = os.path.join(HERE, '_ref', 'libklu_ref.so')
+        if not os.path.exists(path):
+            return None
+        _klu = ctypes.CDLL(path)
+    return _klu
+
+
+def counts():
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().or_counts(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def slot_roles():
+    n, _, _ = counts()
+    out = []
+    for s in range(n):
+        r, c = ctypes.c_int(), ctypes.c_int()
+        lib().or_slot_rc(s, ctypes.byref(r), ctypes.byref(c))
+        out.append((r.value, c.value))
+    return out
+
+
+class Model:
+    def __init__(self, params, nom_temp=300.15, temp=300.15):
+        L = lib()
+        self.h = L.or_model_new()
+        for k, v in params.items():
+            if L.or_model_param(self.h, k.encode(), float(v)) != 0:
+                raise KeyError('unknown BSIM4 model parameter %s' % k)
+        rc = L.or_model_finish(self.h, nom_temp, temp)
+        if rc != 0:
+            raise ValueError('model setup failed (%d)' % rc)
+
+    def get(self, what):
+        return lib().or_model_get(self.h, what.encode())
+
+    def __del__(self):
+        try:
+            lib().or_model_free(self.h)
+        except Exception:
+            pass
+
+
+class Instance:
+    def __init__(self, model, params):
+        L = lib()
+        self.model = model
+        self.h = L.or_inst_new(model.h)
+        for k, v in params.items():
+            if L.or_inst_param(self.h, k.encode(), float(v)) != 0:
+                raise KeyError('unknown BSIM4 instance parameter %s' % k)
+        self.bits = L.or_inst_defaults(self.h)
+        if self.bits < 0:
+            raise ValueError('instance setup failed (%d)' % self.bits)
+
+    def finish(self, nodes, state, temp):
+        arr = (ctypes.c_int * 12)(*nodes)
+        rc = lib().or_inst_finish(self.h, arr, state, temp)
+        if rc != 0:
+            raise ValueError('instance temperature setup failed (%d)' % rc)
+
+    def qa_dc(self, temp, vd, vg, vs, vb, gmin=1e-12, want_nodes=False):
+        cur = (ctypes.c_double * 4)()
+        vout = (ctypes.c_double * 16)()
+        rc = lib().or_qa_dc(self.h, temp, vd, vg, vs, vb, gmin, cur, vout)
+        if rc < 0:
+            raise RuntimeError('QA DC solve failed (%d)' % rc)
+        if want_nodes:
+            return list(cur), list(vout)
+        return list(cur)
+
+    def node(self, role):
+        return lib().or_inst_node(self.h, role)
+
+    def load(self, mode, temp, rhs_old, states, ag=(0.0,) * 7, order=1, method=1,
+             delta=0.0, delta_old=(0.0,) * 7, gmin=1e-12, reltol=1e-3, abstol=1e-12,
+             vntol=1e-6):
+        """One load; states is a list of 4 ctypes double arrays (state0..3)."""
+        nslot, nrole, _ = counts()
+        D = ctypes.c_double
+        st = (D * nslot)()
+        rh = (D * nrole)()
+        agv = (D * 7)(*ag)
+        dov = (D * 7)(*delta_old)
+        nc = lib().or_inst_load(self.h, mode, temp, gmin, reltol, abstol, vntol, agv, order,
+                                method, delta, dov, rhs_old, states[0], states[1], states[2],
+                                states[3], st, rh)
+        return list(st), list(rh), nc
+
+    def get(self, what):
+        return lib().or_inst_get(self.h, what.encode())
+
+    def __del__(self):
+        try:
+            lib().or_inst_free(self.h)
+        except Exception:
+            pass
