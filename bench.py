@@ -1,0 +1,141 @@
+"""Benchmark driver (one JSON line on stdout).
+
+Workloads
+  eval : 2**20 isolated BSIM4 instances of the 45 nm base card, one
+         backward-Euler transient load each plus deterministic assembly of
+         their 16 Jacobian entries into a random 2**21-row CSC matrix.  A step
+         is one evaluation pass + one assembly pass.
+
+Inputs are resident in HBM before the timed region.  Kernel durations come
+from HIP events recorded on the stream the kernels are launched on.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, 'ngspice-sf-mirror_amd'))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bench_eval(args):
+    import torch
+    import ngmi
+    import workloads
+
+    n = args.n
+    t0 = time.time()
+    wl = workloads.device_eval_workload(n=n, seed=1)
+    models = [ngmi.Model(workloads.BASE_NMOS), ngmi.Model(workloads.BASE_PMOS)]
+    s = ngmi.InstanceSet()
+    s.add(models, wl['model_idx'], wl['params'])
+    s.finalize()
+    icols, mcols, tbytes = s.layout()
+    Ap, Ai, e2z = ngmi.csc_from_elements(wl['nrows'], wl['rows_e'], wl['cols_e'])
+    nnz = len(Ai)
+    gat = ngmi.Gather(nnz, e2z)
+    log('setup %.1fs: n=%d inst cols %d model cols %d nnz %d' % (time.time() - t0, n, icols,
+                                                                  mcols, nnz))
+    st = [torch.from_numpy(x).cuda() for x in workloads.eval_states(wl)]
+    st0_init = st[0].clone()
+    vt = torch.from_numpy(wl['vterm']).cuda()
+    cache = torch.zeros(38 * n, dtype=torch.float64, device='cuda')
+    stamp = torch.zeros(16 * n, dtype=torch.float64, device='cuda')
+    rhs = torch.zeros(4 * n, dtype=torch.float64, device='cuda')
+    nc = torch.zeros(n, dtype=torch.int32, device='cuda')
+    ax = torch.zeros(nnz, dtype=torch.float64, device='cuda')
+    stream = torch.cuda.current_stream().cuda_stream
+    h = workloads.H_STEP
+    ag = (1 / h, -1 / h, 0, 0, 0, 0, 0)
+    mode = ngmi.MODETRAN | ngmi.MODEINITFLOAT
+    ptrs = [x.data_ptr() for x in st]
+
+    def step(t_eval=None, t_gat=None):
+        if t_eval:
+            t_eval.start(stream)
+        s.eval(mode, vt.data_ptr(), ptrs, cache.data_ptr(), stamp.data_ptr(), rhs.data_ptr(),
+               nc.data_ptr(), ag=ag, order=1, stream=stream)
+        if t_eval:
+            t_eval.stop(stream)
+        if t_gat:
+            t_gat.start(stream)
+        gat.run(stamp.data_ptr(), ax.data_ptr(), stream)
+        if t_gat:
+            t_gat.stop(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    te, tg = ngmi.Timer(), ngmi.Timer()
+    ev_ms, ga_ms = [], []
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(te, tg)
+        ev_ms.append(te.ms())
+        ga_ms.append(tg.ms())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ev = float(np.mean(ev_ms))
+    ga = float(np.mean(ga_ms))
+    ms_step = wall / args.steps * 1e3
+    # algorithmic bytes of one evaluation launch (per instance): per-instance
+    # table columns, 4 terminal voltages, state reads (4 voltages + 3 charges of
+    # level 1 + 2 junction charges of level 0), state writes (12 voltage/charge
+    # rows + 3 cqX + 4 charges), cache 38 read + 38 written, 16 + 4 outputs,
+    # 1 int vote, 1 int model index
+    per_inst = 8 * (icols + 4 + 9 + 19 + 76 + 20) + 4 + 4
+    achieved = per_inst * n / (ev * 1e-3) / 1e9
+    return {
+        'metric': 'BSIM4 device evaluations/s (1M instances, with CSC assembly)',
+        'value': n / (ms_step * 1e-3),
+        'unit': 'evaluations/s',
+        'ms_per_step': ms_step,
+        'dtype': 'f64',
+        'config': {'workload': 'device-eval: 2^20 BSIM4 instances, 45nm base card, '
+                               'random 2^21-row CSC (~8 nnz/row)', 'nnz': nnz},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'kernel': 'k_bsim4_eval', 'kernel_ms': ev, 'bytes_per_instance': per_inst},
+        'extra': {'gather_ms': ga, 'eval_ms': ev, 'inst_cols': icols, 'model_cols': mcols},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='eval', choices=['eval'])
+    ap.add_argument('--n', type=int, default=1 << 20)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only (the engine has no multi-GPU path); '
+              'got --gpus %d' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py needs a GPU', file=sys.stderr)
+        sys.exit(2)
+    res = bench_eval(args)
+    out = {'metric': res['metric'], 'value': res['value'], 'unit': res['unit'], 'n_gpus': 1,
+           'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': res['ms_per_step'],
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': res['dtype'], 'data': 'synthetic (seeded Philox)', 'config': res['config'],
+           'roofline': res['roofline'], 'cpu_baseline': None, 'extra': res['extra']}
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

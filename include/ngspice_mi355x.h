@@ -1,0 +1,161 @@
+/*
+ * ngspice_mi355x.h -- C ABI of the MI355X transient engine.
+ *
+ * Plain C types only: host pointers, device pointers (as void* / double*
+ * obtained from the engine or from hipMalloc), sizes and mode words.  Every
+ * entry point returns OK (0) or one of the reference's error numbers
+ * (src/include/ngspice/iferrmsg.h:17-38, sperror.h:15-21) unless stated.
+ *
+ * The entry points replace, one for one, the reference interfaces cited on
+ * each declaration; the batched variants take N circuits in one call.
+ */
+#ifndef NGSPICE_MI355X_H
+#define NGSPICE_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error numbers: iferrmsg.h:17-38, sperror.h:15-21 */
+#define NGMI_OK          0
+#define NGMI_E_PANIC     1
+#define NGMI_E_NOMOD     4
+#define NGMI_E_NOTERM    6
+#define NGMI_E_BADPARM   7
+#define NGMI_E_NOMEM     8
+#define NGMI_E_UNSUPP    10
+#define NGMI_E_PARMVAL   11
+#define NGMI_E_NOTFOUND  14
+#define NGMI_E_SINGULAR  102
+#define NGMI_E_ITERLIM   103
+#define NGMI_E_TIMESTEP  106
+#define NGMI_E_DEVICE    120   /* HIP runtime failure (no GPU, launch error) */
+
+/* CKTmode bits, cktdefs.h:171-199 */
+#define NGMI_MODETRAN        0x1
+#define NGMI_MODEAC          0x2
+#define NGMI_MODEDCOP        0x10
+#define NGMI_MODETRANOP      0x20
+#define NGMI_MODEDCTRANCURVE 0x40
+#define NGMI_MODEINITFLOAT   0x100
+#define NGMI_MODEINITJCT     0x200
+#define NGMI_MODEINITFIX     0x400
+#define NGMI_MODEINITSMSIG   0x800
+#define NGMI_MODEINITTRAN    0x1000
+#define NGMI_MODEINITPRED    0x2000
+#define NGMI_MODEUIC         0x10000
+
+/* last error text of the calling thread (never NULL) */
+const char *ngmi_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* BSIM4 model cards                                                   */
+/* ------------------------------------------------------------------ */
+typedef struct ngmi_bsim4_model ngmi_bsim4_model;
+
+/* replaces creating a BSIM4 GENmodel (DEVmodSize, devdefs.h:84) */
+ngmi_bsim4_model *ngmi_bsim4_model_new(void);
+void ngmi_bsim4_model_free(ngmi_bsim4_model *m);
+/* BSIM4mParam (b4mpar.c:20): netlist name (lower case) -> value; "nmos"/"pmos" set the type.
+ * Returns NGMI_E_BADPARM for an unknown name. */
+int ngmi_bsim4_model_param(ngmi_bsim4_model *m, const char *name, double value);
+/* BSIM4setup model defaults (b4set.c:79-2300) + BSIM4temp model part (b4temp.c:90-417).
+ * Temperatures in kelvin.  NGMI_E_UNSUPP for selectors the engine does not build
+ * (capMod != 2, rdsMod, rgateMod, rbodyMod, NQS, mtrlMod). */
+int ngmi_bsim4_model_setup(ngmi_bsim4_model *m, double tnom_kelvin, double temp_kelvin);
+/* a derived model quantity by name ("type", "coxe", "vtm", "vcrit") */
+double ngmi_bsim4_model_get(const ngmi_bsim4_model *m, const char *what);
+
+/* ------------------------------------------------------------------ */
+/* BSIM4 instance sets: host resolution + device-resident tables       */
+/* ------------------------------------------------------------------ */
+typedef struct ngmi_bsim4_set ngmi_bsim4_set;
+
+ngmi_bsim4_set *ngmi_bsim4_set_new(void);
+void ngmi_bsim4_set_free(ngmi_bsim4_set *s);
+/* BSIM4param (b4par.c:22) for n instances at once: instance k uses
+ * models[model_idx[k]]; params[j*n + k] is the value of names[j] for
+ * instance k.  Appends; returns the index of the first new instance or a
+ * negative error number. */
+int ngmi_bsim4_set_add(ngmi_bsim4_set *s, ngmi_bsim4_model *const *models, int nmodels,
+                       const int *model_idx, int n, const char *const *names, int nparams,
+                       const double *params);
+/* BSIM4temp instance part (b4temp.c:418-2400) for every instance and upload
+ * of the evaluation tables to the current HIP device. */
+int ngmi_bsim4_set_finalize(ngmi_bsim4_set *s);
+int ngmi_bsim4_set_size(const ngmi_bsim4_set *s);
+/* copy of one resolved table field of instance k (field names of the engine's
+ * evaluation table, e.g. "tvth0", "leff", "weff", "coxe") */
+double ngmi_bsim4_set_field(const ngmi_bsim4_set *s, int k, const char *field);
+/* number of per-instance / per-model table columns and table bytes on the device */
+int ngmi_bsim4_set_layout(const ngmi_bsim4_set *s, int *inst_cols, int *model_cols,
+                          size_t *device_bytes);
+
+/* Per-instance evaluation, BSIM4load (b4ld.c:69-5399) on isolated
+ * instances: terminal voltages come from vterm (4 x n, rows d g s b, device
+ * memory, circuit polarity), the circuit scalars from the arguments.
+ * Device arrays (all n-strided, row r of instance k at [r*n + k]):
+ *   state[0..2]   29 rows each (bsim4def.h:534-567 order), read/written;
+ *   cache         38 rows, values kept between calls (b4ld.c:444-507);
+ *   stamp         16 rows out (Jacobian, row-major over d g s b);
+ *   rhs           4 rows out (d g s b);
+ *   noncon        n ints out (1 where the device convergence test fails).
+ * Asynchronous on stream (a hipStream_t, NULL for the default stream). */
+typedef struct {
+    int mode, order, method;   /* CKTmode; integration order; 1 trap, 2 Gear */
+    double ag[7];              /* NIcomCof coefficients */
+    double xfact;              /* CKTdelta / CKTdeltaOld[1] (INITPRED) */
+    double gmin, reltol, abstol;
+} ngmi_ckt_scalars;
+
+int ngmi_bsim4_eval(const ngmi_bsim4_set *s, const ngmi_ckt_scalars *ck, const double *vterm,
+                    double *const state[3], double *cache, double *stamp, double *rhs,
+                    int *noncon, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Sparse pattern: COO element list -> CSC + per-element slot map      */
+/* ------------------------------------------------------------------ */
+/* SMPmakeElt (klusmp.c:417-440) + SMPconvertCOOtoCSC (klusmp.c:137-323):
+ * elements (row[e], col[e]) 0-based, n x n.  Writes Ap[n+1], and returns in
+ * *nnz the number of distinct entries; Ai must hold ne entries, elt2nz[e]
+ * receives the CSC position of element e.  Within a column, rows ascend. */
+int ngmi_csc_from_elements(int n, int64_t ne, const int *row, const int *col, int64_t *Ap,
+                           int *Ai, int64_t *elt2nz, int64_t *nnz);
+
+/* Deterministic assembly ("CKTload" accumulation): Ax[z] = sum over the
+ * elements e mapped to z, in ascending e order, of src[e].  The map is
+ * inverted once; the plan lives on the device. */
+typedef struct ngmi_gather ngmi_gather;
+ngmi_gather *ngmi_gather_new(int64_t nnz, int64_t ne, const int64_t *elt2nz);
+void ngmi_gather_free(ngmi_gather *g);
+int ngmi_gather_run(const ngmi_gather *g, const double *src, double *Ax, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Device memory helpers for callers without a HIP binding             */
+/* ------------------------------------------------------------------ */
+void *ngmi_dev_alloc(size_t bytes);
+int ngmi_dev_free(void *p);
+int ngmi_copy_h2d(void *dst, const void *src, size_t bytes);
+int ngmi_copy_d2h(void *dst, const void *src, size_t bytes);
+int ngmi_dev_memset(void *dst, int byte, size_t bytes);
+int ngmi_sync(void);
+/* device name, compute units, and clock (kHz) of the current device */
+int ngmi_device_info(char *name, int name_len, int *cus, int *clock_khz);
+
+/* HIP events around work on a stream, for callers that time kernels on the
+ * stream they launch on */
+typedef struct ngmi_timer ngmi_timer;
+ngmi_timer *ngmi_timer_new(void);
+void ngmi_timer_free(ngmi_timer *t);
+int ngmi_timer_start(ngmi_timer *t, void *stream);
+int ngmi_timer_stop(ngmi_timer *t, void *stream);
+/* milliseconds between start and stop (synchronises on the stop event) */
+double ngmi_timer_ms(ngmi_timer *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

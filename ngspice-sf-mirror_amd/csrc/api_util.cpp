@@ -1,0 +1,128 @@
+/*
+ * Error text, device memory, timing and device queries of the C ABI.
+ */
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include "ngmi_internal.h"
+
+namespace ngmi {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+}  // namespace ngmi
+
+using namespace ngmi;
+
+extern "C" {
+
+const char *ngmi_last_error(void) { return g_err; }
+
+void *ngmi_dev_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return nullptr;
+    }
+    return p;
+}
+
+int ngmi_dev_free(void *p)
+{
+    if (p) NGMI_HIP(hipFree(p));
+    return NGMI_OK;
+}
+
+int ngmi_copy_h2d(void *dst, const void *src, size_t bytes)
+{
+    NGMI_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return NGMI_OK;
+}
+
+int ngmi_copy_d2h(void *dst, const void *src, size_t bytes)
+{
+    NGMI_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return NGMI_OK;
+}
+
+int ngmi_dev_memset(void *dst, int byte, size_t bytes)
+{
+    NGMI_HIP(hipMemset(dst, byte, bytes));
+    return NGMI_OK;
+}
+
+int ngmi_sync(void)
+{
+    NGMI_HIP(hipDeviceSynchronize());
+    return NGMI_OK;
+}
+
+int ngmi_device_info(char *name, int name_len, int *cus, int *clock_khz)
+{
+    int dev = 0;
+    NGMI_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    NGMI_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cus) *cus = p.multiProcessorCount;
+    if (clock_khz) *clock_khz = p.clockRate;
+    return NGMI_OK;
+}
+
+struct ngmi_timer {
+    hipEvent_t a, b;
+};
+
+ngmi_timer *ngmi_timer_new(void)
+{
+    ngmi_timer *t = new ngmi_timer;
+    if (hipEventCreate(&t->a) != hipSuccess || hipEventCreate(&t->b) != hipSuccess) {
+        set_error("hipEventCreate failed");
+        delete t;
+        return nullptr;
+    }
+    return t;
+}
+
+void ngmi_timer_free(ngmi_timer *t)
+{
+    if (!t) return;
+    (void)hipEventDestroy(t->a);
+    (void)hipEventDestroy(t->b);
+    delete t;
+}
+
+int ngmi_timer_start(ngmi_timer *t, void *stream)
+{
+    NGMI_HIP(hipEventRecord(t->a, (hipStream_t)stream));
+    return NGMI_OK;
+}
+
+int ngmi_timer_stop(ngmi_timer *t, void *stream)
+{
+    NGMI_HIP(hipEventRecord(t->b, (hipStream_t)stream));
+    return NGMI_OK;
+}
+
+double ngmi_timer_ms(ngmi_timer *t)
+{
+    float ms = -1.0f;
+    if (hipEventSynchronize(t->b) != hipSuccess) return -1.0;
+    if (hipEventElapsedTime(&ms, t->a, t->b) != hipSuccess) return -1.0;
+    return (double)ms;
+}
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+/*
+ * Internal helpers shared by the engine's translation units: error text,
+ * HIP status checks, and the device-side view of a BSIM4 instance table.
+ */
+#ifndef NGMI_INTERNAL_H
+#define NGMI_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "../../include/ngspice_mi355x.h"
+#include "bsim4_fields.h"
+
+namespace ngmi {
+
+/* thread-local last error */
+void set_error(const char *fmt, ...);
+
+#define NGMI_HIP(call)                                                          \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::ngmi::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                              __FILE__, __LINE__);                              \
+            return NGMI_E_DEVICE;                                               \
+        }                                                                       \
+    } while (0)
+
+/*
+ * Device view of a BSIM4 evaluation table.  Each field is either a
+ * per-instance column (itab[col * n + i]) or, when every instance of a
+ * model card has the same value, a per-model column (mtab[col * nm + m]).
+ * code[f] = col for a per-instance field and ~col for a per-model one; the
+ * code array lives in device memory and is read with a scalar load next to
+ * each use (a kernel-argument copy would pin ~190 SGPRs for the whole
+ * kernel).
+ */
+struct B4Tab {
+    const double *itab;
+    const double *mtab;
+    const int *model_of;
+    int64_t n;       /* instances (row stride of itab) */
+    int nm;          /* models (row stride of mtab) */
+    const int *code;
+};
+
+}  // namespace ngmi
+
+#endif

@@ -1,0 +1,259 @@
+"""Python host mirror of the engine's C ABI (include/ngspice_mi355x.h).
+
+A thin ctypes layer: every call goes to libngspice_mi355x.so, which runs the
+HIP kernels.  Device buffers are passed as integer addresses (for example
+``torch.Tensor.data_ptr()`` of a tensor on ``cuda:0``).  There is no CPU
+fallback: loading fails with an error when the library is missing.
+
+Names mirror the reference's device API: ``Model.param`` is BSIM4mParam
+(b4mpar.c:20), ``InstanceSet.add`` is BSIM4param (b4par.c:22) for many
+instances, ``InstanceSet.finalize`` is the instance part of BSIM4temp
+(b4temp.c:418-2400), ``InstanceSet.eval`` is BSIM4load (b4ld.c:69-5399).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libngspice_mi355x.so')
+
+OK = 0
+E_BADPARM = 7
+E_UNSUPP = 10
+E_SINGULAR = 102
+E_ITERLIM = 103
+E_TIMESTEP = 106
+E_DEVICE = 120
+
+MODETRAN = 0x1
+MODEAC = 0x2
+MODEDCOP = 0x10
+MODETRANOP = 0x20
+MODEDCTRANCURVE = 0x40
+MODEINITFLOAT = 0x100
+MODEINITJCT = 0x200
+MODEINITFIX = 0x400
+MODEINITSMSIG = 0x800
+MODEINITTRAN = 0x1000
+MODEINITPRED = 0x2000
+MODEUIC = 0x10000
+
+N_STATES = 29
+N_CACHE = 38
+N_STAMP = 16
+N_RHS = 4
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__('%s: error %d: %s' % (what, code, last_error()))
+        self.code = code
+
+
+class _Scalars(ctypes.Structure):
+    _fields_ = [('mode', ctypes.c_int), ('order', ctypes.c_int), ('method', ctypes.c_int),
+                ('ag', ctypes.c_double * 7), ('xfact', ctypes.c_double),
+                ('gmin', ctypes.c_double), ('reltol', ctypes.c_double),
+                ('abstol', ctypes.c_double)]
+
+
+def lib():
+    """The engine library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError('engine library %s not built (run make -C %s)' % (LIB_PATH, HERE))
+        L = ctypes.CDLL(LIB_PATH)
+        vp, d, i, cp = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_char_p
+        i64, sz = ctypes.c_int64, ctypes.c_size_t
+        L.ngmi_last_error.restype = cp
+        L.ngmi_bsim4_model_new.restype = vp
+        L.ngmi_bsim4_model_free.argtypes = [vp]
+        L.ngmi_bsim4_model_param.argtypes = [vp, cp, d]
+        L.ngmi_bsim4_model_setup.argtypes = [vp, d, d]
+        L.ngmi_bsim4_model_get.argtypes = [vp, cp]
+        L.ngmi_bsim4_model_get.restype = d
+        L.ngmi_bsim4_set_new.restype = vp
+        L.ngmi_bsim4_set_free.argtypes = [vp]
+        L.ngmi_bsim4_set_add.argtypes = [vp, ctypes.POINTER(vp), i, ctypes.POINTER(i), i,
+                                         ctypes.POINTER(cp), i, ctypes.POINTER(d)]
+        L.ngmi_bsim4_set_finalize.argtypes = [vp]
+        L.ngmi_bsim4_set_size.argtypes = [vp]
+        L.ngmi_bsim4_set_field.argtypes = [vp, i, cp]
+        L.ngmi_bsim4_set_field.restype = d
+        L.ngmi_bsim4_set_layout.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i),
+                                            ctypes.POINTER(sz)]
+        L.ngmi_bsim4_eval.argtypes = [vp, ctypes.POINTER(_Scalars), vp, ctypes.POINTER(vp),
+                                      vp, vp, vp, vp, vp]
+        L.ngmi_csc_from_elements.argtypes = [i, i64, vp, vp, vp, vp, vp, ctypes.POINTER(i64)]
+        L.ngmi_gather_new.argtypes = [i64, i64, vp]
+        L.ngmi_gather_new.restype = vp
+        L.ngmi_gather_free.argtypes = [vp]
+        L.ngmi_gather_run.argtypes = [vp, vp, vp, vp]
+        L.ngmi_device_info.argtypes = [cp, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.ngmi_timer_new.restype = vp
+        L.ngmi_timer_free.argtypes = [vp]
+        L.ngmi_timer_start.argtypes = [vp, vp]
+        L.ngmi_timer_stop.argtypes = [vp, vp]
+        L.ngmi_timer_ms.argtypes = [vp]
+        L.ngmi_timer_ms.restype = d
+        L.ngmi_sync.argtypes = []
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return (lib().ngmi_last_error() or b'').decode()
+
+
+def check(rc, what):
+    if rc != OK:
+        raise EngineError(rc, what)
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    cu, clk = ctypes.c_int(), ctypes.c_int()
+    check(lib().ngmi_device_info(buf, 256, ctypes.byref(cu), ctypes.byref(clk)), 'device_info')
+    return buf.value.decode(), cu.value, clk.value
+
+
+def sync():
+    check(lib().ngmi_sync(), 'sync')
+
+
+class Model:
+    """One BSIM4 .model card at one temperature."""
+
+    def __init__(self, params, tnom=300.15, temp=300.15):
+        L = lib()
+        self.h = L.ngmi_bsim4_model_new()
+        for k, v in params.items():
+            check(L.ngmi_bsim4_model_param(self.h, k.encode(), float(v)), 'model param %s' % k)
+        check(L.ngmi_bsim4_model_setup(self.h, tnom, temp), 'model setup')
+
+    def get(self, what):
+        return lib().ngmi_bsim4_model_get(self.h, what.encode())
+
+    def __del__(self):
+        try:
+            lib().ngmi_bsim4_model_free(self.h)
+        except Exception:
+            pass
+
+
+class InstanceSet:
+    """Device-resident BSIM4 instance tables."""
+
+    def __init__(self):
+        self.h = lib().ngmi_bsim4_set_new()
+        self.models = []
+
+    def add(self, models, model_idx, params):
+        """params: dict name -> array of n values (instance parameters)."""
+        L = lib()
+        model_idx = np.ascontiguousarray(model_idx, dtype=np.int32)
+        n = len(model_idx)
+        names = list(params.keys())
+        vals = np.ascontiguousarray(
+            np.stack([np.broadcast_to(np.asarray(params[k], dtype=np.float64), (n,))
+                      for k in names]) if names else np.zeros((0, n)))
+        self.models.extend(models)
+        mh = (ctypes.c_void_p * len(models))(*[m.h for m in models])
+        nh = (ctypes.c_char_p * max(len(names), 1))(*[k.encode() for k in names])
+        rc = L.ngmi_bsim4_set_add(self.h, mh, len(models),
+                                  model_idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n, nh,
+                                  len(names), vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc < 0:
+            raise EngineError(-rc, 'set add')
+        return rc
+
+    def finalize(self):
+        check(lib().ngmi_bsim4_set_finalize(self.h), 'set finalize')
+
+    def __len__(self):
+        return lib().ngmi_bsim4_set_size(self.h)
+
+    def field(self, k, name):
+        return lib().ngmi_bsim4_set_field(self.h, k, name.encode())
+
+    def layout(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        check(lib().ngmi_bsim4_set_layout(self.h, ctypes.byref(a), ctypes.byref(b),
+                                          ctypes.byref(c)), 'layout')
+        return a.value, b.value, c.value
+
+    def eval(self, mode, vterm, states, cache, stamp, rhs, noncon, ag=(0.0,) * 7, order=1,
+             method=1, xfact=0.0, gmin=1e-12, reltol=1e-3, abstol=1e-12, stream=0):
+        """BSIM4load over the set; all buffers are device addresses."""
+        sc = _Scalars(mode, order, method, (ctypes.c_double * 7)(*ag), xfact, gmin, reltol,
+                      abstol)
+        st = (ctypes.c_void_p * 3)(*states)
+        check(lib().ngmi_bsim4_eval(self.h, ctypes.byref(sc), vterm, st, cache, stamp, rhs,
+                                    noncon, stream), 'bsim4 eval')
+
+    def __del__(self):
+        try:
+            lib().ngmi_bsim4_set_free(self.h)
+        except Exception:
+            pass
+
+
+def csc_from_elements(n, rows, cols):
+    """SMPmakeElt + SMPconvertCOOtoCSC: returns (Ap, Ai, elt2nz)."""
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    ne = len(rows)
+    Ap = np.zeros(n + 1, dtype=np.int64)
+    Ai = np.zeros(max(ne, 1), dtype=np.int32)
+    e2z = np.zeros(max(ne, 1), dtype=np.int64)
+    nnz = ctypes.c_int64()
+    check(lib().ngmi_csc_from_elements(n, ne, rows.ctypes.data, cols.ctypes.data, Ap.ctypes.data,
+                                       Ai.ctypes.data, e2z.ctypes.data, ctypes.byref(nnz)),
+          'csc_from_elements')
+    return Ap, Ai[:nnz.value].copy(), e2z[:ne]
+
+
+class Gather:
+    """Deterministic assembly plan: Ax[z] = sum of src[e] over elements e -> z."""
+
+    def __init__(self, nnz, elt2nz):
+        e2z = np.ascontiguousarray(elt2nz, dtype=np.int64)
+        self.nnz = nnz
+        self.h = lib().ngmi_gather_new(nnz, len(e2z), e2z.ctypes.data)
+        if not self.h:
+            raise EngineError(1, 'gather plan')
+
+    def run(self, src, Ax, stream=0):
+        check(lib().ngmi_gather_run(self.h, src, Ax, stream), 'gather')
+
+    def __del__(self):
+        try:
+            lib().ngmi_gather_free(self.h)
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP events recorded on the stream the timed kernels run on."""
+
+    def __init__(self):
+        self.h = lib().ngmi_timer_new()
+
+    def start(self, stream=0):
+        check(lib().ngmi_timer_start(self.h, stream), 'timer start')
+
+    def stop(self, stream=0):
+        check(lib().ngmi_timer_stop(self.h, stream), 'timer stop')
+
+    def ms(self):
+        return lib().ngmi_timer_ms(self.h)
+
+    def __del__(self):
+        try:
+            lib().ngmi_timer_free(self.h)
+        except Exception:
+            pass

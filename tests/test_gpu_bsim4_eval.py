@@ -1,0 +1,161 @@
+"""GPU parity of the BSIM4 evaluation kernel against the oracle.
+
+Bar: per instance, every Jacobian entry, RHS entry, charge state and
+terminal-voltage state agrees with the oracle within 1e-12 of that
+instance's largest entry of the same kind (currents and conductances are
+sums of terms of both signs; the scale is the instance's own magnitude),
+and every entry larger than 1e-6 of that scale agrees within 1e-10
+relative.  Device convergence votes must be identical.
+"""
+import numpy as np
+import pytest
+
+from engine_path import ngmi, workloads
+from oracle_eval import OracleBatch
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+TOL_SCALED = 1e-12
+TOL_REL = 1e-10
+CHARGE_STATES = [11, 13, 15, 28, 19, 21, 12, 14, 16]   # qb qg qd qs qbs qbd cqb cqg cqd
+VOLT_STATES = [0, 1, 2, 3]
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _compare(name, gpu, ref):
+    """gpu, ref: [rows, n].  Returns (scaled error, relative error)."""
+    scale = np.abs(ref).max(axis=0)
+    scale[scale == 0] = 1e-300
+    d = np.abs(gpu - ref)
+    scaled = (d / scale).max()
+    big = np.abs(ref) > 1e-6 * scale
+    rel = (d[big] / np.abs(ref[big])).max() if big.any() else 0.0
+    print('%-8s scaled %.2e rel %.2e' % (name, scaled, rel), flush=True)
+    assert scaled <= TOL_SCALED, (name, scaled)
+    assert rel <= TOL_REL, (name, rel)
+    return scaled, rel
+
+
+class GpuEval:
+    def __init__(self, models, model_idx, params):
+        self.m = [ngmi.Model(mp) for mp in models]
+        self.s = ngmi.InstanceSet()
+        self.s.add(self.m, model_idx, params)
+        self.s.finalize()
+        n = len(model_idx)
+        self.n = n
+        self.cache = torch.zeros(38 * n, dtype=torch.float64, device='cuda')
+        self.stamp = torch.zeros(16 * n, dtype=torch.float64, device='cuda')
+        self.rhs = torch.zeros(4 * n, dtype=torch.float64, device='cuda')
+        self.nc = torch.zeros(n, dtype=torch.int32, device='cuda')
+
+    def load(self, mode, vterm, states, **kw):
+        st = [_dev(s) for s in states]
+        vt = _dev(vterm)
+        self.s.eval(mode, vt.data_ptr(), [x.data_ptr() for x in st], self.cache.data_ptr(),
+                    self.stamp.data_ptr(), self.rhs.data_ptr(), self.nc.data_ptr(), **kw)
+        torch.cuda.synchronize()
+        for lv in range(3):
+            states[lv][:] = st[lv].cpu().numpy().reshape(29, self.n)
+        return (self.stamp.cpu().numpy().reshape(16, self.n),
+                self.rhs.cpu().numpy().reshape(4, self.n), self.nc.cpu().numpy())
+
+
+def _check_all(gs, gr, gnc, gst, os_, or_, onc, ost):
+    _compare('stamp', gs, os_)
+    _compare('rhs', gr, or_)
+    for lv in range(3):
+        _compare('q%d' % lv, gst[lv][CHARGE_STATES], ost[lv][CHARGE_STATES])
+        _compare('v%d' % lv, gst[lv][VOLT_STATES], ost[lv][VOLT_STATES])
+    assert np.array_equal(gnc, onc)
+
+
+def test_device_eval_workload_slice():
+    """The device-evaluation workload's distribution (first 1024 NMOS and
+    1024 PMOS draws of seed 1), backward-Euler transient load."""
+    wl = workloads.device_eval_workload(n=2048, seed=1, with_matrix=False)
+    models = [workloads.BASE_NMOS, workloads.BASE_PMOS]
+    print('case: device_eval_workload n=2048 seed=1', flush=True)
+    g = GpuEval(models, wl['model_idx'], wl['params'])
+    o = OracleBatch(models, wl['model_idx'], wl['params'])
+    h = workloads.H_STEP
+    kw = dict(ag=(1 / h, -1 / h, 0, 0, 0, 0, 0), order=1)
+    gst = workloads.eval_states(wl)
+    ost = [s.copy() for s in gst]
+    mode = ngmi.MODETRAN | ngmi.MODEINITFLOAT
+    gs, gr, gnc = g.load(mode, wl['vterm'], gst, **kw)
+    os_, or_, onc = o.load(mode, wl['vterm'], ost, **kw)
+    _check_all(gs, gr, gnc, gst, os_, or_, onc, ost)
+    assert np.abs(gs).max() > 0 and np.abs(gst[0][CHARGE_STATES]).max() > 0
+
+
+VARIANTS = [{}, {'igcmod': 1, 'igbmod': 1}, {'igcmod': 2, 'gidlmod': 1}, {'mobmod': 2},
+            {'mobmod': 3, 'diomod': 2}, {'mobmod': 6, 'tempmod': 3}, {'cvchargemod': 1},
+            {'diomod': 0, 'tempmod': 1, 'xpart': 0.0}, {'lambda': 0.1, 'vtl': 2e5}]
+
+
+@pytest.mark.parametrize('var', range(len(VARIANTS)))
+def test_newton_mode_sequence(var):
+    """INITJCT -> DC INITFLOAT -> INITTRAN -> INITPRED -> transient INITFLOAT
+    iterations (trapezoidal order 2), each compared after the call, with the
+    cache of the previous call feeding limiting and the convergence test."""
+    rng = np.random.default_rng(100 + var)
+    n = 256
+    nm = dict(workloads.BASE_NMOS, **VARIANTS[var])
+    pm = dict(workloads.BASE_PMOS, **VARIANTS[var])
+    model_idx = (np.arange(n) >= n // 2).astype(np.int32)
+    params = {'l': np.exp(rng.uniform(np.log(45e-9), np.log(1e-6), n)),
+              'w': np.exp(rng.uniform(np.log(100e-9), np.log(10e-6), n)),
+              'delvto': rng.normal(0, 0.03, n),
+              'ad': rng.uniform(0, 1e-12, n), 'as': rng.uniform(0, 1e-12, n),
+              'pd': rng.uniform(0, 2e-6, n), 'ps': rng.uniform(0, 2e-6, n),
+              'nf': rng.integers(1, 4, n).astype(float)}
+    print('case: mode sequence variant %r' % (VARIANTS[var],), flush=True)
+    g = GpuEval([nm, pm], model_idx, params)
+    o = OracleBatch([nm, pm], model_idx, params)
+    typ = np.where(model_idx == 0, 1.0, -1.0)
+    gst = [np.zeros((29, n)) for _ in range(3)]
+    ost = [np.zeros((29, n)) for _ in range(3)]
+    h = 1e-11
+    steps = [
+        (ngmi.MODEDCOP | ngmi.MODEINITJCT, dict(ag=(0,) * 7, order=1)),
+        (ngmi.MODEDCOP | ngmi.MODEINITFLOAT, dict(ag=(0,) * 7, order=1)),
+        (ngmi.MODETRAN | ngmi.MODEINITTRAN, dict(ag=(1 / h, -1 / h, 0, 0, 0, 0, 0), order=1)),
+        (ngmi.MODETRAN | ngmi.MODEINITPRED, dict(ag=(2 / h, 1.0, 0, 0, 0, 0, 0), order=2)),
+        (ngmi.MODETRAN | ngmi.MODEINITFLOAT, dict(ag=(2 / h, 1.0, 0, 0, 0, 0, 0), order=2)),
+        (ngmi.MODETRAN | ngmi.MODEINITFLOAT, dict(ag=(2 / h, 1.0, 0, 0, 0, 0, 0), order=2)),
+    ]
+    for k, (mode, kw) in enumerate(steps):
+        v = np.stack([typ * rng.uniform(-0.2, 1.1, n), typ * rng.uniform(-0.2, 1.2, n),
+                      typ * rng.uniform(-0.05, 0.05, n), typ * rng.uniform(-0.4, 0.1, n)])
+        okw = dict(kw)
+        gkw = dict(kw)
+        if mode & ngmi.MODEINITPRED:
+            okw.update(delta=h, delta_old=(h, 0.8 * h, 0.7 * h, 0, 0, 0, 0))
+            gkw.update(xfact=h / (0.8 * h))
+        print('  step %d mode %#x' % (k, mode), flush=True)
+        gs, gr, gnc = g.load(mode, v, gst, **gkw)
+        os_, or_, onc = o.load(mode, v, ost, **okw)
+        _check_all(gs, gr, gnc, gst, os_, or_, onc, ost)
+
+
+def test_gather_assembly_matches_ordered_sum():
+    """Deterministic assembly into a random 2**14-row CSC equals numpy's
+    ordered accumulation (np.add.at visits elements in ascending order)."""
+    wl = workloads.device_eval_workload(n=4096, seed=3, rows=1 << 14)
+    n = wl['n']
+    Ap, Ai, e2z = ngmi.csc_from_elements(wl['nrows'], wl['rows_e'], wl['cols_e'])
+    nnz = len(Ai)
+    gat = ngmi.Gather(nnz, e2z)
+    src = np.random.default_rng(0).normal(size=16 * n)
+    d_src = _dev(src)
+    d_ax = torch.zeros(nnz, dtype=torch.float64, device='cuda')
+    gat.run(d_src.data_ptr(), d_ax.data_ptr())
+    torch.cuda.synchronize()
+    ref = np.zeros(nnz)
+    np.add.at(ref, e2z, src)
+    assert np.array_equal(d_ax.cpu().numpy(), ref)
