@@ -134,6 +134,58 @@ void ngmi_gather_free(ngmi_gather *g);
 int ngmi_gather_run(const ngmi_gather *g, const double *src, double *Ax, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Circuits and batched transient analysis                             */
+/* ------------------------------------------------------------------ */
+typedef struct ngmi_circuit ngmi_circuit;
+typedef struct ngmi_batch ngmi_batch;
+
+/* A netlist with nodes 1..nnodes (0 = ground): the engine's counterpart of
+ * building a CKTcircuit (CKTinit + the device setup calls of CKTsetup,
+ * cktsetup.c:33-210).  Element values are per circuit; a batch needs the
+ * same topology in every circuit. */
+ngmi_circuit *ngmi_circuit_new(int nnodes);
+void ngmi_circuit_free(ngmi_circuit *c);
+/* one BSIM4 instance (BSIM4param, b4par.c:22, for its instance parameters) */
+int ngmi_circuit_add_bsim4(ngmi_circuit *c, ngmi_bsim4_model *m, int d, int g, int s, int b,
+                           int nparams, const char *const *names, const double *values);
+/* capacitor (CAPload, capload.c:19-86), resistor (RESload, resload.c:16-38) */
+int ngmi_circuit_add_cap(ngmi_circuit *c, int p, int n, double farad);
+int ngmi_circuit_add_res(ngmi_circuit *c, int p, int n, double ohm);
+/* independent voltage source (VSRCload, vsrcload.c:40-170): DC when pulse is
+ * NULL, else PULSE(v1 v2 td tr tf pw per) with npulse of the 7 values given */
+int ngmi_circuit_add_vsrc(ngmi_circuit *c, int p, int n, double dc, const double *pulse,
+                          int npulse);
+/* .ic v(node)=v for the transient operating point (cktload.c:146-173) */
+int ngmi_circuit_set_ic(ngmi_circuit *c, int node, double v);
+/* .options reltol abstol vntol chgtol trtol gmin method(1 trap, 2 gear) maxord */
+int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, double vntol,
+                             double chgtol, double trtol, double gmin, int method, int maxord);
+
+/* A batch of C circuits on the current device: SMPpreOrder/SMPreorder
+ * (klusmp.c:752-806, 1255-1285) once for the shared topology, device tables,
+ * state and waveform buffers (probe: unknowns recorded at every accepted
+ * point, NULL = all; max_points per circuit). */
+ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, int nprobe,
+                           int64_t max_points);
+void ngmi_batch_free(ngmi_batch *b);
+/* unknowns, L+U nonzeros, factor terms, and levels [factor, forward, backward] */
+int ngmi_batch_info(const ngmi_batch *b, int *n_unknowns, int *nslots, int *nterms, int *levels);
+/* .tran tstep tstop [tmax] for every circuit: CKTop then DCtran
+ * (dctran.c:66-971) with NIiter (niiter.c:28-367) per circuit, all on the
+ * device; returns when every circuit has finished or failed, or after
+ * max_passes Newton passes (NGMI_E_ITERLIM).  Asynchronous work on stream,
+ * synchronised before return. */
+int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int64_t max_passes,
+                    void *stream, int64_t *passes);
+/* per circuit: status (0 finished, -1 running, else the error number),
+ * recorded points, accepted / rejected timepoints, Newton iterations */
+int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int64_t *accepted,
+                     int64_t *rejected, int64_t *iterations);
+/* waveform of circuit c (CKTdump rows, cktdump.c:33-44): t[n], x[n][nprobe] */
+int ngmi_batch_waveform(const ngmi_batch *b, int c, double *t, double *x, int64_t maxpts,
+                        int64_t *npts);
+
+/* ------------------------------------------------------------------ */
 /* Device memory helpers for callers without a HIP binding             */
 /* ------------------------------------------------------------------ */
 void *ngmi_dev_alloc(size_t bytes);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstring>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 #include "ngmi_internal.h"
 #include "bsim4_setup.h"
@@ -45,6 +46,7 @@ struct ngmi_bsim4_model {
 
 struct ngmi_bsim4_set {
     std::vector<ngmi_bsim4_model *> models;   /* distinct cards, in first-use order */
+    std::unordered_map<const ngmi_bsim4_model *, int> model_pos;
     std::vector<int> model_of;
     std::vector<B4Inst> inst;
     std::vector<double> rows;                 /* n x B4F__COUNT, after finalize */
@@ -144,12 +146,13 @@ int ngmi_bsim4_set_add(ngmi_bsim4_set *s, ngmi_bsim4_model *const *models, int n
             return NGMI_E_NOMOD;
         }
         if (local[mk] < 0) {
-            auto it = std::find(s->models.begin(), s->models.end(), models[mk]);
-            if (it == s->models.end()) {
+            auto it = s->model_pos.find(models[mk]);
+            if (it == s->model_pos.end()) {
                 s->models.push_back(models[mk]);
                 local[mk] = (int)s->models.size() - 1;
+                s->model_pos[models[mk]] = local[mk];
             } else {
-                local[mk] = (int)(it - s->models.begin());
+                local[mk] = it->second;
             }
         }
         B4Inst in;

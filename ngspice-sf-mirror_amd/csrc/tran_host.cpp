@@ -1,0 +1,766 @@
+/*
+ * Host side of the batched transient engine: circuit construction (the
+ * engine's counterpart of the netlist -> CKTcircuit build), the static
+ * symbolic analysis shared by every circuit of a batch, and the driver loop
+ * that launches one Newton pass at a time until every circuit has reached
+ * its final time.
+ *
+ * Symbolic analysis (the role of SMPpreOrder / klu_analyze, klusmp.c:1255,
+ * and of the pivot sequence klu_factor fixes for klu_refactor,
+ * klu_refactor.c:18-482):
+ *   1. unknowns 1..NN are node voltages, NN+1.. the voltage-source branch
+ *      currents in source order (the order CKTmkCur gives them);
+ *   2. a static row permutation puts each source's branch equation on the
+ *      row of its positive node (and that node's KCL on the branch row), so
+ *      both diagonals are the exact +-1 entries of the source stamp;
+ *   3. a minimum-degree symmetric ordering of the permuted pattern limits
+ *      fill; the remaining pivots are the (permuted) diagonal;
+ *   4. the filled L+U pattern gives an entry-wise Doolittle program grouped
+ *      into dependency levels, executed by one workgroup per circuit.
+ * The pivot sequence is fixed for the whole run and for every circuit of the
+ * batch; a numerically zero pivot is reported as E_SINGULAR for that
+ * circuit's Newton iteration.
+ */
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "ngmi_internal.h"
+#include "tran_types.h"
+
+namespace ngmi {
+int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s);
+int set_tran_lds_limit(size_t bytes);
+const B4Tab *set_table(const ngmi_bsim4_set *s);
+}
+
+using namespace ngmi;
+
+struct ngmi_circuit {
+    int nnodes = 0;
+    struct Mos {
+        ngmi_bsim4_model *model;
+        int node[4];
+        std::vector<std::string> names;
+        std::vector<double> values;
+    };
+    std::vector<Mos> mos;
+    std::vector<std::array<double, 3>> caps;   /* p n C */
+    std::vector<std::array<double, 3>> res;    /* p n R */
+    struct Vs {
+        int p, n;
+        double dc;
+        int npulse;
+        double pulse[7];
+    };
+    std::vector<Vs> vs;
+    std::vector<double> ic;
+    std::vector<char> ic_given;
+    double reltol = 1e-3, abstol = 1e-12, vntol = 1e-6, chgtol = 1e-14, trtol = 7.0, gmin = 1e-12;
+    int method = 1, maxord = 2;
+};
+
+struct ngmi_batch {
+    int C = 0, N = 0, NN = 0, NM = 0, NCAP = 0, NRES = 0, NVS = 0;
+    ngmi_bsim4_set *set = nullptr;
+    TranDev D{};
+    size_t lds = 0;
+    std::vector<void *> allocs;
+    std::vector<int> probe;
+    int64_t cap_pts = 0;
+    /* host copies of the program statistics */
+    int nslots = 0, nterms = 0, nlev_f = 0, nlev_fw = 0, nlev_bw = 0;
+    std::vector<double> ic_host;
+    std::vector<char> ic_nodes_host;
+    int method = 1, maxord = 2;
+    ~ngmi_batch()
+    {
+        for (void *p : allocs) (void)hipFree(p);
+        if (set) ngmi_bsim4_set_free(set);
+    }
+};
+
+namespace {
+
+template <class T>
+int upload(ngmi_batch *b, const std::vector<T> &h, T **d)
+{
+    const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(T);
+    void *p = nullptr;
+    NGMI_HIP(hipMalloc(&p, bytes));
+    b->allocs.push_back(p);
+    if (!h.empty()) NGMI_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    *d = reinterpret_cast<T *>(p);
+    return NGMI_OK;
+}
+
+template <class T>
+int alloc(ngmi_batch *b, size_t count, T **d)
+{
+    void *p = nullptr;
+    NGMI_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    NGMI_HIP(hipMemset(p, 0, std::max<size_t>(count, 1) * sizeof(T)));
+    b->allocs.push_back(p);
+    *d = reinterpret_cast<T *>(p);
+    return NGMI_OK;
+}
+
+bool same_topology(const ngmi_circuit &a, const ngmi_circuit &b)
+{
+    if (a.nnodes != b.nnodes || a.mos.size() != b.mos.size() || a.caps.size() != b.caps.size() ||
+        a.res.size() != b.res.size() || a.vs.size() != b.vs.size())
+        return false;
+    for (size_t k = 0; k < a.mos.size(); k++)
+        for (int r = 0; r < 4; r++)
+            if (a.mos[k].node[r] != b.mos[k].node[r]) return false;
+    for (size_t k = 0; k < a.caps.size(); k++)
+        if (a.caps[k][0] != b.caps[k][0] || a.caps[k][1] != b.caps[k][1]) return false;
+    for (size_t k = 0; k < a.res.size(); k++)
+        if (a.res[k][0] != b.res[k][0] || a.res[k][1] != b.res[k][1]) return false;
+    for (size_t k = 0; k < a.vs.size(); k++)
+        if (a.vs[k].p != b.vs[k].p || a.vs[k].n != b.vs[k].n) return false;
+    for (int u = 0; u <= a.nnodes; u++)
+        if (a.ic_given[u] != b.ic_given[u]) return false;
+    return true;
+}
+
+/* one matrix contribution: original (row, col) 1-based, code */
+struct Elt {
+    int r, c;
+    uint32_t code;
+};
+
+uint32_t mk(uint32_t type, uint32_t idx, bool neg) { return (type << CT_TYPE_SHIFT) | (neg ? CT_NEG : 0u) | idx; }
+
+/* greedy minimum degree on an undirected graph (adjacency bitsets) */
+std::vector<int> min_degree(int n, std::vector<std::vector<uint64_t>> adj)
+{
+    const int W = (n + 63) / 64;
+    std::vector<char> gone(n, 0);
+    std::vector<int> order;
+    order.reserve(n);
+    for (int step = 0; step < n; step++) {
+        int best = -1, bestdeg = 1 << 30;
+        for (int v = 0; v < n; v++) {
+            if (gone[v]) continue;
+            int d = 0;
+            for (int w = 0; w < W; w++) d += __builtin_popcountll(adj[v][w]);
+            if (d < bestdeg) { bestdeg = d; best = v; }
+        }
+        const int v = best;
+        gone[v] = 1;
+        order.push_back(v);
+        std::vector<int> nb;
+        for (int w = 0; w < W; w++) {
+            uint64_t m = adj[v][w];
+            while (m) {
+                const int bit = __builtin_ctzll(m);
+                m &= m - 1;
+                nb.push_back(w * 64 + bit);
+            }
+        }
+        for (int a : nb) {
+            adj[a][v >> 6] &= ~(1ull << (v & 63));
+            for (int b2 : nb)
+                if (b2 != a) adj[a][b2 >> 6] |= 1ull << (b2 & 63);
+        }
+    }
+    return order;
+}
+
+}  // namespace
+
+extern "C" {
+
+ngmi_circuit *ngmi_circuit_new(int nnodes)
+{
+    if (nnodes < 1) {
+        set_error("a circuit needs at least one node");
+        return nullptr;
+    }
+    ngmi_circuit *c = new ngmi_circuit;
+    c->nnodes = nnodes;
+    c->ic.assign(nnodes + 1, 0.0);
+    c->ic_given.assign(nnodes + 1, 0);
+    return c;
+}
+
+void ngmi_circuit_free(ngmi_circuit *c) { delete c; }
+
+static bool node_ok(const ngmi_circuit *c, int n) { return n >= 0 && n <= c->nnodes; }
+
+int ngmi_circuit_add_bsim4(ngmi_circuit *c, ngmi_bsim4_model *m, int d, int g, int s, int b,
+                           int nparams, const char *const *names, const double *values)
+{
+    if (!c || !m) return NGMI_E_PANIC;
+    if (!node_ok(c, d) || !node_ok(c, g) || !node_ok(c, s) || !node_ok(c, b)) {
+        set_error("MOSFET terminal outside 0..%d", c->nnodes);
+        return NGMI_E_NOTERM;
+    }
+    ngmi_circuit::Mos mo;
+    mo.model = m;
+    mo.node[0] = d; mo.node[1] = g; mo.node[2] = s; mo.node[3] = b;
+    for (int k = 0; k < nparams; k++) {
+        mo.names.emplace_back(names[k]);
+        mo.values.push_back(values[k]);
+    }
+    c->mos.push_back(std::move(mo));
+    return NGMI_OK;
+}
+
+int ngmi_circuit_add_cap(ngmi_circuit *c, int p, int n, double cap)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (!node_ok(c, p) || !node_ok(c, n)) return NGMI_E_NOTERM;
+    c->caps.push_back({(double)p, (double)n, cap});
+    return NGMI_OK;
+}
+
+int ngmi_circuit_add_res(ngmi_circuit *c, int p, int n, double r)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (!node_ok(c, p) || !node_ok(c, n)) return NGMI_E_NOTERM;
+    if (r == 0.0) {
+        set_error("zero resistance");
+        return NGMI_E_PARMVAL;
+    }
+    c->res.push_back({(double)p, (double)n, r});
+    return NGMI_OK;
+}
+
+int ngmi_circuit_add_vsrc(ngmi_circuit *c, int p, int n, double dc, const double *pulse, int npulse)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (!node_ok(c, p) || !node_ok(c, n) || p == n) return NGMI_E_NOTERM;
+    if (npulse < 0 || npulse > 7 || (npulse > 0 && npulse < 2)) return NGMI_E_PARMVAL;
+    ngmi_circuit::Vs v{};
+    v.p = p; v.n = n; v.dc = dc; v.npulse = pulse ? npulse : 0;
+    for (int k = 0; k < v.npulse; k++) v.pulse[k] = pulse[k];
+    c->vs.push_back(v);
+    return NGMI_OK;
+}
+
+int ngmi_circuit_set_ic(ngmi_circuit *c, int node, double v)
+{
+    if (!c || node < 1 || node > c->nnodes) return NGMI_E_NOTERM;
+    c->ic[node] = v;
+    c->ic_given[node] = 1;
+    return NGMI_OK;
+}
+
+int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, double vntol,
+                             double chgtol, double trtol, double gmin, int method, int maxord)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (method != 1 && method != 2) return NGMI_E_PARMVAL;
+    if (maxord < 1 || maxord > (method == 1 ? 2 : 2)) {
+        set_error("maximum order %d not supported (state ring keeps order <= 2)", maxord);
+        return NGMI_E_UNSUPP;
+    }
+    c->reltol = reltol; c->abstol = abstol; c->vntol = vntol; c->chgtol = chgtol;
+    c->trtol = trtol; c->gmin = gmin; c->method = method; c->maxord = maxord;
+    return NGMI_OK;
+}
+
+ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, int nprobe,
+                           int64_t max_points)
+{
+    if (!ckts || C < 1) {
+        set_error("empty batch");
+        return nullptr;
+    }
+    const ngmi_circuit &c0 = *ckts[0];
+    for (int c = 1; c < C; c++) {
+        if (!same_topology(c0, *ckts[c])) {
+            set_error("circuit %d has a different topology than circuit 0", c);
+            return nullptr;
+        }
+    }
+    ngmi_batch *b = new ngmi_batch;
+    auto bail = [&](const char *what) -> ngmi_batch * {
+        if (what) set_error("%s", what);
+        delete b;
+        return nullptr;
+    };
+    b->C = C;
+    b->NN = c0.nnodes;
+    b->NVS = (int)c0.vs.size();
+    b->N = b->NN + b->NVS;
+    b->NM = (int)c0.mos.size();
+    b->NCAP = (int)c0.caps.size();
+    b->NRES = (int)c0.res.size();
+    const int N = b->N, NN = b->NN, NM = b->NM, NCAP = b->NCAP, NRES = b->NRES, NVS = b->NVS;
+
+    /* ---- matrix and RHS contributions (original rows/cols, 1-based) */
+    std::vector<Elt> elts;
+    std::vector<std::vector<uint32_t>> rhs_codes(N + 1);
+    static const int rr[16] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3};
+    static const int cc[16] = {0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3};
+    for (int j = 0; j < NM; j++) {
+        const int *nd = c0.mos[j].node;
+        for (int s = 0; s < 16; s++) {
+            const int r = nd[rr[s]], col = nd[cc[s]];
+            if (r && col) elts.push_back({r, col, mk(CT_MOS, (uint32_t)(s * NM + j), false)});
+        }
+        for (int r = 0; r < 4; r++)
+            if (nd[r]) rhs_codes[nd[r]].push_back(mk(CT_MOS, (uint32_t)(r * NM + j), false));
+    }
+    for (int k = 0; k < NCAP; k++) {
+        const int p = (int)c0.caps[k][0], q = (int)c0.caps[k][1];
+        if (p) elts.push_back({p, p, mk(CT_CAP, k, false)});
+        if (q) elts.push_back({q, q, mk(CT_CAP, k, false)});
+        if (p && q) {
+            elts.push_back({p, q, mk(CT_CAP, k, true)});
+            elts.push_back({q, p, mk(CT_CAP, k, true)});
+        }
+        if (p) rhs_codes[p].push_back(mk(CT_CAP, k, true));
+        if (q) rhs_codes[q].push_back(mk(CT_CAP, k, false));
+    }
+    for (int k = 0; k < NRES; k++) {
+        const int p = (int)c0.res[k][0], q = (int)c0.res[k][1];
+        if (p) elts.push_back({p, p, mk(CT_RES, k, false)});
+        if (q) elts.push_back({q, q, mk(CT_RES, k, false)});
+        if (p && q) {
+            elts.push_back({p, q, mk(CT_RES, k, true)});
+            elts.push_back({q, p, mk(CT_RES, k, true)});
+        }
+    }
+    for (int k = 0; k < NVS; k++) {
+        const int p = c0.vs[k].p, q = c0.vs[k].n, br = NN + 1 + k;
+        if (p) { elts.push_back({p, br, mk(CT_ONE, 0, false)}); elts.push_back({br, p, mk(CT_ONE, 0, false)}); }
+        if (q) { elts.push_back({q, br, mk(CT_ONE, 0, true)}); elts.push_back({br, q, mk(CT_ONE, 0, true)}); }
+        rhs_codes[br].push_back(mk(CT_VSRC, k, false));
+    }
+    /* .ic rows: voltage columns are zeroed in the transient operating point
+     * (cktload.c:146-173); the row must hold no branch-current column */
+    b->ic_nodes_host.assign(NN + 1, 0);
+    for (int u = 1; u <= NN; u++) b->ic_nodes_host[u] = c0.ic_given[u];
+    for (Elt &e : elts) {
+        if (e.r <= NN && b->ic_nodes_host[e.r]) {
+            if (e.c > NN) return bail(".ic on a node with a voltage source is not supported");
+            e.code |= CT_ICZ;
+        }
+    }
+
+    /* ---- static row permutation: branch row <-> its positive node row */
+    std::vector<int> prow(N + 1), prow_inv(N + 1);
+    for (int u = 0; u <= N; u++) prow[u] = u;
+    for (int k = 0; k < NVS; k++) {
+        const int br = NN + 1 + k;
+        const int node = c0.vs[k].p ? c0.vs[k].p : c0.vs[k].n;
+        if (prow[node] != node) return bail("two voltage sources share a node row");
+        prow[node] = br;
+        prow[br] = node;
+    }
+    for (int u = 0; u <= N; u++) prow_inv[prow[u]] = u;
+
+    /* ---- pattern of B = P A (0-based), minimum-degree ordering of B + B^T */
+    std::vector<std::vector<char>> Bp(N, std::vector<char>(N, 0));
+    for (const Elt &e : elts) Bp[prow_inv[e.r] - 1][e.c - 1] = 1;
+    for (int a = 0; a < N; a++) Bp[a][a] = 1;
+    const int W = (N + 63) / 64;
+    std::vector<std::vector<uint64_t>> adj(N, std::vector<uint64_t>(W, 0));
+    for (int a = 0; a < N; a++)
+        for (int c = 0; c < N; c++)
+            if (a != c && (Bp[a][c] || Bp[c][a])) adj[a][c >> 6] |= 1ull << (c & 63);
+    const std::vector<int> order = min_degree(N, adj);   /* order[q] = B index eliminated q-th */
+    std::vector<int> oinv(N);
+    for (int q = 0; q < N; q++) oinv[order[q]] = q;
+
+    /* ---- filled pattern of M = Q^T B Q */
+    std::vector<std::vector<char>> S(N, std::vector<char>(N, 0));
+    for (int a = 0; a < N; a++)
+        for (int c = 0; c < N; c++)
+            if (Bp[a][c]) S[oinv[a]][oinv[c]] = 1;
+    for (int k = 0; k < N; k++)
+        for (int i = k + 1; i < N; i++)
+            if (S[i][k])
+                for (int j = k + 1; j < N; j++)
+                    if (S[k][j]) S[i][j] = 1;
+    std::vector<int> slot((size_t)N * N, -1);
+    int nslots = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++)
+            if (S[i][j]) slot[(size_t)i * N + j] = nslots++;
+    b->nslots = nslots;
+
+    /* ---- factor program: entries ordered by m = min(i, j), diagonal first */
+    std::vector<int> lev(nslots, 0);
+    struct Ent { int t, piv, lev; std::vector<int2> terms; };
+    std::vector<Ent> ents;
+    ents.reserve(nslots);
+    for (int m = 0; m < N; m++) {
+        std::vector<std::pair<int, int>> list;
+        list.push_back({m, m});
+        for (int j = m + 1; j < N; j++) if (S[m][j]) list.push_back({m, j});
+        for (int i = m + 1; i < N; i++) if (S[i][m]) list.push_back({i, m});
+        for (auto [i, j] : list) {
+            Ent e;
+            e.t = slot[(size_t)i * N + j];
+            e.piv = (i > j) ? slot[(size_t)j * N + j] : -1;
+            int L = 0;
+            for (int k = 0; k < m; k++) {
+                if (S[i][k] && S[k][j]) {
+                    const int a = slot[(size_t)i * N + k], c2 = slot[(size_t)k * N + j];
+                    e.terms.push_back(make_int2(a, c2));
+                    L = std::max(L, std::max(lev[a], lev[c2]) + 1);
+                }
+            }
+            if (e.piv >= 0) L = std::max(L, lev[e.piv] + 1);
+            lev[e.t] = L;
+            e.lev = L;
+            ents.push_back(std::move(e));
+        }
+    }
+    int nlev = 0;
+    for (const Ent &e : ents) nlev = std::max(nlev, e.lev + 1);
+    std::vector<int> f_lev(nlev + 1, 0), f_ent;
+    std::vector<int2> f_term;
+    {
+        std::vector<std::vector<int>> by(nlev);
+        for (int k = 0; k < (int)ents.size(); k++) by[ents[k].lev].push_back(k);
+        for (int L = 0; L < nlev; L++) {
+            f_lev[L] = (int)(f_ent.size() / 4);
+            for (int k : by[L]) {
+                const Ent &e = ents[k];
+                f_ent.push_back(e.t);
+                f_ent.push_back(e.piv);
+                f_ent.push_back((int)f_term.size());
+                f_term.insert(f_term.end(), e.terms.begin(), e.terms.end());
+                f_ent.push_back((int)f_term.size());
+            }
+        }
+        f_lev[nlev] = (int)(f_ent.size() / 4);
+    }
+    b->nlev_f = nlev;
+    b->nterms = (int)f_term.size();
+
+    /* ---- forward (unit L) and backward programs */
+    std::vector<int> fw_lev, fw_ent, bw_lev, bw_ent;
+    std::vector<int2> fw_term, bw_term;
+    {
+        std::vector<int> lv(N, 0);
+        int nl = 0;
+        for (int a = 0; a < N; a++) {
+            int L = 0;
+            for (int k = 0; k < a; k++) if (S[a][k]) L = std::max(L, lv[k] + 1);
+            lv[a] = L;
+            nl = std::max(nl, L + 1);
+        }
+        std::vector<std::vector<int>> by(nl);
+        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
+        for (int L = 0; L < nl; L++) {
+            fw_lev.push_back((int)(fw_ent.size() / 3));
+            for (int a : by[L]) {
+                fw_ent.push_back(a);
+                fw_ent.push_back((int)fw_term.size());
+                for (int k = 0; k < a; k++)
+                    if (S[a][k]) fw_term.push_back(make_int2(slot[(size_t)a * N + k], k));
+                fw_ent.push_back((int)fw_term.size());
+            }
+        }
+        fw_lev.push_back((int)(fw_ent.size() / 3));
+        b->nlev_fw = nl;
+    }
+    {
+        std::vector<int> lv(N, 0);
+        int nl = 0;
+        for (int a = N - 1; a >= 0; a--) {
+            int L = 0;
+            for (int j = a + 1; j < N; j++) if (S[a][j]) L = std::max(L, lv[j] + 1);
+            lv[a] = L;
+            nl = std::max(nl, L + 1);
+        }
+        std::vector<std::vector<int>> by(nl);
+        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
+        for (int L = 0; L < nl; L++) {
+            bw_lev.push_back((int)(bw_ent.size() / 4));
+            for (int a : by[L]) {
+                bw_ent.push_back(a);
+                bw_ent.push_back(slot[(size_t)a * N + a]);
+                bw_ent.push_back((int)bw_term.size());
+                for (int j = a + 1; j < N; j++)
+                    if (S[a][j]) bw_term.push_back(make_int2(slot[(size_t)a * N + j], j));
+                bw_ent.push_back((int)bw_term.size());
+            }
+        }
+        bw_lev.push_back((int)(bw_ent.size() / 4));
+        b->nlev_bw = nl;
+    }
+
+    /* ---- assembly plans in permuted coordinates */
+    std::vector<std::vector<uint32_t>> per_slot(nslots);
+    std::vector<int> slot_flags(nslots, 0);
+    for (const Elt &e : elts) {
+        const int i = oinv[prow_inv[e.r] - 1], j = oinv[e.c - 1];
+        per_slot[slot[(size_t)i * N + j]].push_back(e.code);
+    }
+    for (int u = 1; u <= NN; u++) {
+        if (!b->ic_nodes_host[u]) continue;
+        const int i = oinv[prow_inv[u] - 1], j = oinv[u - 1];
+        slot_flags[slot[(size_t)i * N + j]] |= 1;
+    }
+    std::vector<int> a_ptr(nslots + 1, 0);
+    std::vector<uint32_t> a_code;
+    for (int s = 0; s < nslots; s++) {
+        a_ptr[s] = (int)a_code.size();
+        a_code.insert(a_code.end(), per_slot[s].begin(), per_slot[s].end());
+    }
+    a_ptr[nslots] = (int)a_code.size();
+    std::vector<int> perm(2 * N), r_ptr(N + 1, 0);
+    std::vector<uint32_t> r_code;
+    for (int a = 0; a < N; a++) {
+        const int bidx = order[a];            /* B index in permuted position a */
+        perm[a] = bidx + 1;                   /* unknown solved in column a */
+        perm[N + a] = prow[bidx + 1];         /* original row assembled in row a */
+        r_ptr[a] = (int)r_code.size();
+        const auto &rc = rhs_codes[prow[bidx + 1]];
+        r_code.insert(r_code.end(), rc.begin(), rc.end());
+    }
+    r_ptr[N] = (int)r_code.size();
+
+    /* ---- BSIM4 instances, circuit-major */
+    b->set = ngmi_bsim4_set_new();
+    for (int c = 0; c < C; c++) {
+        for (int j = 0; j < NM; j++) {
+            const ngmi_circuit::Mos &mo = ckts[c]->mos[j];
+            std::vector<const char *> nm;
+            for (const auto &s : mo.names) nm.push_back(s.c_str());
+            ngmi_bsim4_model *mp = mo.model;
+            const int zero = 0;
+            const int rc = ngmi_bsim4_set_add(b->set, &mp, 1, &zero, 1, nm.data(), (int)nm.size(),
+                                              mo.values.data());
+            if (rc < 0) return bail(nullptr);
+        }
+    }
+    if (ngmi_bsim4_set_finalize(b->set) != NGMI_OK) return bail(nullptr);
+
+    /* ---- per-circuit element values */
+    std::vector<double> cap_c((size_t)C * NCAP), res_g((size_t)C * NRES), vs_par((size_t)C * NVS * VSP);
+    std::vector<double> ic_val((size_t)C * (NN + 1), 0.0);
+    for (int c = 0; c < C; c++) {
+        const ngmi_circuit &ck = *ckts[c];
+        for (int k = 0; k < NCAP; k++) cap_c[(size_t)c * NCAP + k] = ck.caps[k][2];
+        for (int k = 0; k < NRES; k++) res_g[(size_t)c * NRES + k] = 1.0 / ck.res[k][2];
+        for (int k = 0; k < NVS; k++) {
+            double *v = &vs_par[((size_t)c * NVS + k) * VSP];
+            v[0] = ck.vs[k].dc;
+            v[1] = ck.vs[k].npulse > 0 ? 1.0 : 0.0;
+            v[2] = ck.vs[k].npulse;
+            for (int q = 0; q < 7; q++) v[3 + q] = q < ck.vs[k].npulse ? ck.vs[k].pulse[q] : 0.0;
+        }
+        for (int u = 1; u <= NN; u++) ic_val[(size_t)c * (NN + 1) + u] = ck.ic[u];
+    }
+    std::vector<int> mos_nodes(4 * (size_t)NM), cap_nodes(2 * (size_t)NCAP), res_nodes(2 * (size_t)NRES),
+        vs_nodes(3 * (size_t)NVS), ic_nodes(NN + 1);
+    for (int j = 0; j < NM; j++)
+        for (int r = 0; r < 4; r++) mos_nodes[(size_t)r * NM + j] = c0.mos[j].node[r];
+    for (int k = 0; k < NCAP; k++) {
+        cap_nodes[k] = (int)c0.caps[k][0];
+        cap_nodes[NCAP + k] = (int)c0.caps[k][1];
+    }
+    for (int k = 0; k < NRES; k++) {
+        res_nodes[k] = (int)c0.res[k][0];
+        res_nodes[NRES + k] = (int)c0.res[k][1];
+    }
+    for (int k = 0; k < NVS; k++) {
+        vs_nodes[k] = c0.vs[k].p;
+        vs_nodes[NVS + k] = c0.vs[k].n;
+        vs_nodes[2 * NVS + k] = NN + 1 + k;
+    }
+    for (int u = 0; u <= NN; u++) ic_nodes[u] = b->ic_nodes_host[u];
+    if (probe && nprobe > 0) b->probe.assign(probe, probe + nprobe);
+    else for (int u = 1; u <= N; u++) b->probe.push_back(u);
+    for (int u : b->probe)
+        if (u < 0 || u > N) return bail("probe outside the unknowns");
+    b->cap_pts = max_points > 0 ? max_points : 1;
+
+    TranDev &D = b->D;
+    D.C = C; D.N = N; D.NN = NN; D.NM = NM; D.NCAP = NCAP; D.NRES = NRES; D.NVS = NVS;
+    D.nslots = nslots; D.nlev_f = b->nlev_f; D.nlev_fw = b->nlev_fw; D.nlev_bw = b->nlev_bw;
+    D.nprobe = (int)b->probe.size();
+    D.cap_pts = b->cap_pts;
+    D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
+    D.trtol = c0.trtol; D.gmin = c0.gmin;
+    int *ip;
+    double *dp;
+    uint32_t *up;
+    int2 *i2;
+#define UP(vec, field, T, tmp) do { if (upload<T>(b, vec, &tmp) != NGMI_OK) return bail(nullptr); D.field = tmp; } while (0)
+    UP(mos_nodes, mos_nodes, int, ip);
+    UP(cap_nodes, cap_nodes, int, ip);
+    UP(res_nodes, res_nodes, int, ip);
+    UP(vs_nodes, vs_nodes, int, ip);
+    UP(ic_nodes, ic_nodes, int, ip);
+    UP(cap_c, cap_c, double, dp);
+    UP(res_g, res_g, double, dp);
+    UP(vs_par, vs_par, double, dp);
+    UP(ic_val, ic_val, double, dp);
+    UP(a_ptr, a_ptr, int, ip);
+    UP(a_code, a_code, uint32_t, up);
+    UP(slot_flags, slot_flags, int, ip);
+    UP(r_ptr, r_ptr, int, ip);
+    UP(r_code, r_code, uint32_t, up);
+    UP(f_lev, f_lev, int, ip);
+    UP(f_ent, f_ent, int, ip);
+    UP(f_term, f_term, int2, i2);
+    UP(fw_lev, fw_lev, int, ip);
+    UP(fw_ent, fw_ent, int, ip);
+    UP(fw_term, fw_term, int2, i2);
+    UP(bw_lev, bw_lev, int, ip);
+    UP(bw_ent, bw_ent, int, ip);
+    UP(bw_term, bw_term, int2, i2);
+    UP(perm, perm_unknown, int, ip);
+    UP(b->probe, probe, int, ip);
+#undef UP
+    const int64_t cnm = (int64_t)C * NM, cnc = (int64_t)C * NCAP;
+    if (alloc(b, (size_t)C * NVS, &dp) != NGMI_OK) return bail(nullptr);
+    D.vs_brk = dp;
+    CktCtl *ctl;
+    if (alloc(b, (size_t)C, &ctl) != NGMI_OK) return bail(nullptr);
+    D.ctl = ctl;
+    if (alloc(b, (size_t)2 * C * (N + 1), &dp) != NGMI_OK) return bail(nullptr);
+    D.x = dp;
+    if (alloc(b, (size_t)NLEV * B4X__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
+    D.mstate = dp;
+    if (alloc(b, (size_t)NLEV * 2 * cnc, &dp) != NGMI_OK) return bail(nullptr);
+    D.cstate = dp;
+    if (alloc(b, (size_t)B4C__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
+    D.cache = dp;
+    if (alloc(b, (size_t)B4S__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
+    D.stamp = dp;
+    if (alloc(b, (size_t)B4R__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
+    D.mrhs = dp;
+    if (alloc(b, (size_t)cnm, &ip) != NGMI_OK) return bail(nullptr);
+    D.noncon = ip;
+    if (alloc(b, (size_t)C * b->cap_pts, &dp) != NGMI_OK) return bail(nullptr);
+    D.out_t = dp;
+    if (alloc(b, (size_t)C * b->cap_pts * D.nprobe, &dp) != NGMI_OK) return bail(nullptr);
+    D.out_x = dp;
+    if (alloc(b, 1, &ip) != NGMI_OK) return bail(nullptr);
+    D.done_count = ip;
+    b->lds = sizeof(double) * ((size_t)nslots + N + 2 * (size_t)NCAP + NVS);
+    if (b->lds > 150 * 1024) return bail("circuit too large for the per-circuit LDS factorisation");
+    if (b->lds > 64 * 1024 && set_tran_lds_limit(b->lds) != NGMI_OK) return bail(nullptr);
+    b->method = c0.method;
+    b->maxord = c0.maxord;
+    return b;
+}
+
+void ngmi_batch_free(ngmi_batch *b) { delete b; }
+
+int ngmi_batch_info(const ngmi_batch *b, int *n_unknowns, int *nslots, int *nterms, int *levels)
+{
+    if (!b) return NGMI_E_PANIC;
+    if (n_unknowns) *n_unknowns = b->N;
+    if (nslots) *nslots = b->nslots;
+    if (nterms) *nterms = b->nterms;
+    if (levels) {
+        levels[0] = b->nlev_f;
+        levels[1] = b->nlev_fw;
+        levels[2] = b->nlev_bw;
+    }
+    return NGMI_OK;
+}
+
+/* TRANinit (traninit.c:17-40) + DCtran start (dctran.c:117-200) per circuit,
+ * then passes until every circuit is done or max_passes is reached. */
+int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int64_t max_passes,
+                    void *stream, int64_t *passes_out)
+{
+    if (!b) return NGMI_E_PANIC;
+    if (!(tstep > 0) || !(tstop > 0)) return NGMI_E_PARMVAL;
+    TranDev &D = b->D;
+    const hipStream_t s = (hipStream_t)stream;
+    const int C = b->C;
+    std::vector<CktCtl> ctl(C);
+    double maxStep = tmax;
+    if (maxStep == 0) maxStep = (tstep < tstop / 50.0) ? tstep : tstop / 50.0;
+    for (int c = 0; c < C; c++) {
+        CktCtl &k = ctl[c];
+        std::memset(&k, 0, sizeof(k));
+        k.finalTime = tstop;
+        k.step = tstep;
+        k.maxStep = maxStep;
+        k.delmin = 1e-11 * maxStep;
+        k.minBreak = 10.0 * k.delmin;
+        k.delta = std::min(tstop / 100, tstep) / 10;
+        k.breaks[0] = 0;
+        k.breaks[1] = tstop;
+        k.nbreaks = 2;
+        k.time = 0;
+        k.firsttime = 1;
+        k.mode = NGMI_MODETRANOP | NGMI_MODEINITJCT;
+        k.order = 1;
+        k.method = b->method;
+        k.maxOrder = b->maxord;
+        k.phase = PH_OP;
+        for (int i = 0; i < 7; i++) k.deltaOld[i] = maxStep;
+    }
+    NGMI_HIP(hipMemcpyAsync(D.ctl, ctl.data(), sizeof(CktCtl) * C, hipMemcpyHostToDevice, s));
+    const int64_t cnm = (int64_t)C * b->NM, cnc = (int64_t)C * b->NCAP;
+    NGMI_HIP(hipMemsetAsync(D.x, 0, sizeof(double) * 2 * C * (size_t)(b->N + 1), s));
+    NGMI_HIP(hipMemsetAsync(D.mstate, 0, sizeof(double) * NLEV * B4X__COUNT * (size_t)cnm, s));
+    NGMI_HIP(hipMemsetAsync(D.cstate, 0, sizeof(double) * NLEV * 2 * (size_t)cnc, s));
+    NGMI_HIP(hipMemsetAsync(D.cache, 0, sizeof(double) * B4C__COUNT * (size_t)cnm, s));
+    NGMI_HIP(hipMemsetAsync(D.done_count, 0, sizeof(int), s));
+    {
+        std::vector<double> brk((size_t)C * b->NVS, -1.0);
+        NGMI_HIP(hipMemcpyAsync(D.vs_brk, brk.data(), brk.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+        NGMI_HIP(hipStreamSynchronize(s));
+    }
+    const B4Tab *tab = set_table(b->set);
+    int64_t pass = 0;
+    int done = 0;
+    const int check_every = 32;
+    while (pass < max_passes) {
+        for (int k = 0; k < check_every && pass < max_passes; k++, pass++) {
+            const int rc = launch_tran_pass(*tab, D, b->lds, s);
+            if (rc != NGMI_OK) return rc;
+        }
+        NGMI_HIP(hipMemcpyAsync(&done, D.done_count, sizeof(int), hipMemcpyDeviceToHost, s));
+        NGMI_HIP(hipStreamSynchronize(s));
+        if (done >= C) break;
+    }
+    if (passes_out) *passes_out = pass;
+    return done >= C ? NGMI_OK : NGMI_E_ITERLIM;
+}
+
+/* per-circuit results: status (0 ok, else error number), counts */
+int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int64_t *accepted,
+                     int64_t *rejected, int64_t *iterations)
+{
+    if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
+    CktCtl k;
+    NGMI_HIP(hipMemcpy(&k, b->D.ctl + c, sizeof(k), hipMemcpyDeviceToHost));
+    if (status) *status = k.phase == PH_DONE ? 0 : (k.phase == PH_FAIL ? k.status : -1);
+    if (npts) *npts = k.npts;
+    if (accepted) *accepted = k.accepted;
+    if (rejected) *rejected = k.rejected;
+    if (iterations) *iterations = k.iters;
+    return NGMI_OK;
+}
+
+/* waveform of circuit c: t[npts], x[npts][nprobe] (host buffers, up to maxpts) */
+int ngmi_batch_waveform(const ngmi_batch *b, int c, double *t, double *x, int64_t maxpts,
+                        int64_t *npts)
+{
+    if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
+    int64_t n = 0;
+    CktCtl k;
+    NGMI_HIP(hipMemcpy(&k, b->D.ctl + c, sizeof(k), hipMemcpyDeviceToHost));
+    n = std::min<int64_t>(std::min<int64_t>(k.npts, b->cap_pts), maxpts);
+    if (n > 0) {
+        NGMI_HIP(hipMemcpy(t, b->D.out_t + (size_t)c * b->cap_pts, n * sizeof(double),
+                           hipMemcpyDeviceToHost));
+        NGMI_HIP(hipMemcpy(x, b->D.out_x + (size_t)c * b->cap_pts * b->D.nprobe,
+                           n * b->D.nprobe * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (npts) *npts = n;
+    return NGMI_OK;
+}
+
+}  // extern "C"

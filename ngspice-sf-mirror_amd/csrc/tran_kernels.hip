@@ -1,0 +1,743 @@
+/*
+ * Batched transient engine kernels.  One pass = one Newton iteration
+ * (NIiter body) for every circuit still running:
+ *
+ *   k_tran_mos   one thread per (circuit, MOSFET): BSIM4load for the
+ *                circuit's current mode, writing 16 Jacobian and 4 RHS
+ *                entries, the charge states and the convergence vote;
+ *   k_tran_step  one workgroup per circuit: capacitor / resistor / source
+ *                loads, assembly into the static L+U slots in LDS, LU
+ *                factorisation and forward/back substitution by the
+ *                level-scheduled program, NIconvTest, the NIiter INITF state
+ *                machine and, when NIiter returns, the DCtran step control
+ *                (LTE, order, breakpoints, acceptance, output).
+ *
+ * Restated control flow: dctran.c:66-971 (XSPICE build without event
+ * instances), niiter.c:28-367, nicomcof.c:17-130, niconv.c:21-90,
+ * cktterr.c:13-78, ckttrunc.c:20-56, capload.c:19-86, vsrcload.c:40-170,
+ * vsrcacct.c:20-160, cktsetbk.c, cktclrbk.c, equality.c:22-60,
+ * cktload.c:146-173 (.ic rows in the transient operating point).
+ */
+#include "ngmi_internal.h"
+#include "bsim4_eval.h"
+#include "tran_types.h"
+
+namespace ngmi {
+
+#define MODEDC_MASK (M_DCOP | M_TRANOP | M_DCTRANCURVE)
+#define INITF_MASK 0x3f00
+
+__device__ __forceinline__ int lvl(int rb, int l) { return (rb + l) & (NLEV - 1); }
+
+/* ---------------- MOSFET evaluation ---------------- */
+
+struct TranPrm {
+    const B4Tab &t;
+    int64_t i;
+    int mi;
+    __device__ __forceinline__ double operator()(int f) const
+    {
+        const int c = t.code[f];
+        return c >= 0 ? t.itab[(int64_t)c * t.n + i] : t.mtab[(int64_t)(~c) * t.nm + mi];
+    }
+};
+
+struct TranState {
+    double *base;
+    int64_t cnm, i;
+    int rb;
+    __device__ __forceinline__ double *p(int l, int k) const
+    {
+        return base + ((int64_t)lvl(rb, l) * B4X__COUNT + k) * cnm + i;
+    }
+    __device__ __forceinline__ double s(int l, int k) const { return *p(l, k); }
+    __device__ __forceinline__ void set(int l, int k, double v) { *p(l, k) = v; }
+};
+
+struct TranCache {
+    double *c_;
+    int64_t cnm, i;
+    __device__ __forceinline__ double c(int k) const { return c_[k * cnm + i]; }
+    __device__ __forceinline__ void setc(int k, double v) { c_[k * cnm + i] = v; }
+};
+
+__global__ void __launch_bounds__(256)
+k_tran_mos(B4Tab tab, TranDev D)
+{
+    const int64_t cnm = (int64_t)D.C * D.NM;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnm) return;
+    const int c = (int)(i / D.NM);
+    const int j = (int)(i - (int64_t)c * D.NM);
+    const CktCtl *K = D.ctl + c;
+    if (K->phase >= PH_DONE) return;
+    B4Ckt ck;
+    ck.mode = K->mode;
+    ck.order = K->order;
+    ck.method = K->method;
+#pragma unroll
+    for (int k = 0; k < 7; k++) ck.ag[k] = K->ag[k];
+    ck.xfact = K->delta / K->deltaOld[1];
+    ck.gmin = D.gmin;
+    ck.reltol = D.reltol;
+    ck.abstol = D.abstol;
+    const double *xo = D.x + ((int64_t)K->xsel * D.C + c) * (D.N + 1);
+    double v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = xo[D.mos_nodes[r * D.NM + j]];
+    TranPrm p{tab, i, tab.model_of[i]};
+    TranState st{D.mstate, cnm, i, K->rb};
+    TranCache ca{D.cache, cnm, i};
+    double J[B4S__COUNT], R[B4R__COUNT];
+    const int nc = b4_load(p, st, ca, ck, v, J, R);
+#pragma unroll
+    for (int k = 0; k < B4S__COUNT; k++) D.stamp[k * cnm + i] = J[k];
+#pragma unroll
+    for (int k = 0; k < B4R__COUNT; k++) D.mrhs[k * cnm + i] = R[k];
+    D.noncon[i] = nc;
+}
+
+/* ---------------- scalar helpers (thread 0 of a circuit) ---------------- */
+
+__device__ int almost_equal_ulps(double A, double B, int maxUlps)
+{   /* equality.c:22-60 */
+    if (A == B) return 1;
+    int64_t a = __double_as_longlong(A), b = __double_as_longlong(B);
+    if (a < 0) a = (int64_t)0x8000000000000000LL - a;
+    if (b < 0) b = (int64_t)0x8000000000000000LL - b;
+    int64_t d = a - b;
+    if (d < 0) d = -d;
+    return d <= maxUlps;
+}
+
+__device__ int set_break(CktCtl *K, double time)
+{   /* cktsetbk.c:19-100 */
+    if (almost_equal_ulps(time, K->time, 3)) return 0;
+    if (K->time > time) return -1;
+    for (int i = 0; i < K->nbreaks; i++) {
+        if (K->breaks[i] > time) {
+            if ((K->breaks[i] - time) <= K->minBreak) {
+                K->breaks[i] = time;
+                return 0;
+            }
+            if (i > 0 && time - K->breaks[i - 1] <= K->minBreak) return 0;
+            if (K->nbreaks >= MAXBRK) return -2;
+            for (int j = K->nbreaks; j > i; j--) K->breaks[j] = K->breaks[j - 1];
+            K->breaks[i] = time;
+            K->nbreaks++;
+            return 0;
+        }
+    }
+    if (K->nbreaks && time - K->breaks[K->nbreaks - 1] <= K->minBreak) return 0;
+    if (K->nbreaks >= MAXBRK) return -2;
+    K->breaks[K->nbreaks++] = time;
+    return 0;
+}
+
+__device__ void clr_break(CktCtl *K)
+{   /* cktclrbk.c:15-40 */
+    if (K->nbreaks > 2) {
+        for (int j = 1; j < K->nbreaks; j++) K->breaks[j - 1] = K->breaks[j];
+        K->nbreaks--;
+    } else {
+        K->breaks[0] = K->breaks[1];
+        K->breaks[1] = K->finalTime;
+    }
+}
+
+struct Pulse {
+    double V1, V2, TD, TR, TF, PW, PER;
+};
+
+__device__ Pulse pulse_params(const double *vp, double step)
+{   /* vsrcload.c:96-125 */
+    Pulse P;
+    const int order = (int)vp[2];
+    const double *c = vp + 3;
+    P.V1 = c[0];
+    P.V2 = c[1];
+    P.TD = order > 2 ? c[2] : 0.0;
+    P.TR = order > 3 && c[3] > 0.0 ? c[3] : step;
+    P.TF = order > 4 && c[4] > 0.0 ? c[4] : step;
+    P.PW = order > 5 && c[5] >= 0.0 ? c[5] : 0.0;
+    P.PER = order > 6 && c[6] > 0.0 ? c[6] : P.TR + P.TF + P.PW;
+    return P;
+}
+
+__device__ double vsrc_value(const double *vp, int mode, double tnow, double step)
+{   /* vsrcload.c:78-170 (DC, PULSE) */
+    if (vp[1] == 0.0) return vp[0];
+    const Pulse P = pulse_params(vp, step);
+    double time = (mode & MODEDC_MASK) ? 0.0 : tnow;
+    time -= P.TD;
+    if (time > P.PER) {
+        const double basetime = P.PER * floor(time / P.PER);
+        time -= basetime;
+    }
+    if (time <= 0 || time >= P.TR + P.PW + P.TF) return P.V1;
+    if (time >= P.TR && time <= P.TR + P.PW) return P.V2;
+    if (time > 0 && time < P.TR) return P.V1 + (P.V2 - P.V1) * (time) / P.TR;
+    return P.V2 + (P.V1 - P.V2) * (time - (P.TR + P.PW)) / P.TF;
+}
+
+__device__ int vsrc_accept(const TranDev &D, int c, CktCtl *K)
+{   /* vsrcacct.c:20-160 (PULSE corners) */
+    if (!(K->mode & (M_TRAN | M_TRANOP))) return 0;
+    for (int k = 0; k < D.NVS; k++) {
+        const double *vp = D.vs_par + ((int64_t)c * D.NVS + k) * VSP;
+        if (vp[1] == 0.0) continue;
+        double *brk = D.vs_brk + (int64_t)c * D.NVS + k;
+        const Pulse P = pulse_params(vp, K->step);
+        double time = K->time - P.TD;
+        if (K->time >= *brk) {
+            double wait;
+            if (time >= P.PER) {
+                const double basetime = P.PER * floor(time / P.PER);
+                time -= basetime;
+            }
+            const double atime = time + K->minBreak;
+            if (atime < 0.0) wait = -time;
+            else if (atime < P.TR) wait = P.TR - time;
+            else if (atime < P.TR + P.PW) wait = P.TR + P.PW - time;
+            else if (atime < P.TR + P.PW + P.TF) wait = P.TR + P.PW + P.TF - time;
+            else wait = P.PER - time;
+            *brk = K->time + wait;
+            if (set_break(K, *brk)) return -1;
+            *brk -= K->minBreak;
+        }
+    }
+    return 0;
+}
+
+__device__ int com_cof(CktCtl *K)
+{   /* nicomcof.c:17-130 */
+    if (K->method == 1) {
+        if (K->order == 1) {
+            K->ag[0] = 1 / K->delta;
+            K->ag[1] = -1 / K->delta;
+        } else if (K->order == 2) {
+            const double xmu = 0.5;
+            K->ag[0] = 1.0 / K->delta / (1.0 - xmu);
+            K->ag[1] = xmu / (1.0 - xmu);
+        } else return 104;
+        return 0;
+    }
+    const int ord = K->order;
+    if (ord < 1 || ord > 6) return 104;
+    double mat[7][7];
+    for (int k = 0; k < 7; k++) K->ag[k] = 0.0;
+    K->ag[1] = -1 / K->delta;
+    double arg = 0, arg1;
+    for (int i = 0; i <= ord; i++) mat[0][i] = 1;
+    for (int i = 1; i <= ord; i++) mat[i][0] = 0;
+    for (int i = 1; i <= ord; i++) {
+        arg += K->deltaOld[i - 1];
+        arg1 = 1;
+        for (int j = 1; j <= ord; j++) {
+            arg1 *= arg / K->delta;
+            mat[j][i] = arg1;
+        }
+    }
+    for (int i = 1; i <= ord; i++)
+        for (int j = i + 1; j <= ord; j++) {
+            mat[j][i] /= mat[i][i];
+            for (int k = i + 1; k <= ord; k++) mat[j][k] -= mat[j][i] * mat[i][k];
+        }
+    for (int i = 1; i <= ord; i++)
+        for (int j = i + 1; j <= ord; j++) K->ag[j] = K->ag[j] - mat[j][i] * K->ag[i];
+    K->ag[ord] /= mat[ord][ord];
+    for (int i = ord - 1; i >= 0; i--) {
+        for (int j = i + 1; j <= ord; j++) K->ag[i] = K->ag[i] - mat[i][j] * K->ag[j];
+        K->ag[i] /= mat[i][i];
+    }
+    return 0;
+}
+
+/* CKTterr (cktterr.c:13-78) on one charge state; q[l], cq[l] = level l */
+__device__ double terr_one(const double *q, const double *cq, const CktCtl *K, const TranDev &D)
+{
+    const double trapCoeff[2] = {.5, .08333333333};
+    const double gearCoeff[6] = {.5, .2222222222, .1363636364, .096, .07299270073, .05830903790};
+    const int order = K->order;
+    const double volttol = D.abstol + D.reltol * fmax(fabs(cq[0]), fabs(cq[1]));
+    double chargetol = fmax(fabs(q[0]), fabs(q[1]));
+    chargetol = D.reltol * fmax(chargetol, D.chgtol) / K->delta;
+    const double tol = fmax(volttol, chargetol);
+    double diff[8], deltmp[8];
+    for (int i = order + 1; i >= 0; i--) diff[i] = q[i];
+    for (int i = 0; i <= order; i++) deltmp[i] = K->deltaOld[i];
+    int j = order;
+    for (;;) {
+        for (int i = 0; i <= j; i++) diff[i] = (diff[i] - diff[i + 1]) / deltmp[i];
+        if (--j < 0) break;
+        for (int i = 0; i <= j; i++) deltmp[i] = deltmp[i + 1] + K->deltaOld[i];
+    }
+    const double factor = (K->method == 2) ? gearCoeff[order - 1] : trapCoeff[order - 1];
+    double del = D.trtol * tol / fmax(D.abstol, factor * fabs(diff[0]));
+    if (order == 2) del = sqrt(del);
+    else if (order == 3) del = cbrt(del);
+    else if (order > 3) del = exp(log(del) / order);
+    return del;
+}
+
+/* ---------------- per-circuit step kernel ---------------- */
+
+enum : int { ACT_NONE = 0, ACT_ACCEPT = 1, ACT_LTE1 = 2, ACT_RETRY = 3, ACT_STOP = 4 };
+
+/* block-wide minimum of one value per thread */
+__device__ double block_min(double v, double *red)
+{
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmin(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+/* LTE over every charge state of the circuit (CAPtrunc + BSIM4trunc) */
+__device__ double circuit_terr(const TranDev &D, const CktCtl *K, int c, double *red)
+{
+    const int rb = K->rb, order = K->order;
+    const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
+    double m = 1e308;
+    for (int k = threadIdx.x; k < D.NCAP; k += blockDim.x) {
+        const int64_t ci = (int64_t)c * D.NCAP + k;
+        double q[4], cq[2];
+        for (int l = 0; l <= order + 1; l++) q[l] = D.cstate[((int64_t)lvl(rb, l) * 2) * cnc + ci];
+        for (int l = 0; l < 2; l++) cq[l] = D.cstate[((int64_t)lvl(rb, l) * 2 + 1) * cnc + ci];
+        m = fmin(m, terr_one(q, cq, K, D));
+    }
+    const int qs[3] = {B4X_qb, B4X_qg, B4X_qd};
+    for (int t = threadIdx.x; t < 3 * D.NM; t += blockDim.x) {
+        const int j = t / 3, which = qs[t - 3 * j];
+        const int64_t mi = (int64_t)c * D.NM + j;
+        double q[4], cq[2];
+        for (int l = 0; l <= order + 1; l++)
+            q[l] = D.mstate[((int64_t)lvl(rb, l) * B4X__COUNT + which) * cnm + mi];
+        for (int l = 0; l < 2; l++)
+            cq[l] = D.mstate[((int64_t)lvl(rb, l) * B4X__COUNT + which + 1) * cnm + mi];
+        m = fmin(m, terr_one(q, cq, K, D));
+    }
+    return block_min(m, red);
+}
+
+/* copy state level `from` to level `to` for every state of the circuit */
+__device__ void copy_level(const TranDev &D, int c, int rb, int from, int to)
+{
+    const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
+    const int pf = lvl(rb, from), pt = lvl(rb, to);
+    for (int t = threadIdx.x; t < B4X__COUNT * D.NM; t += blockDim.x) {
+        const int k = t / D.NM, j = t - k * D.NM;
+        const int64_t mi = (int64_t)c * D.NM + j;
+        D.mstate[((int64_t)pt * B4X__COUNT + k) * cnm + mi] =
+            D.mstate[((int64_t)pf * B4X__COUNT + k) * cnm + mi];
+    }
+    for (int t = threadIdx.x; t < 2 * D.NCAP; t += blockDim.x) {
+        const int k = t / D.NCAP, j = t - k * D.NCAP;
+        const int64_t ci = (int64_t)c * D.NCAP + j;
+        D.cstate[((int64_t)pt * 2 + k) * cnc + ci] = D.cstate[((int64_t)pf * 2 + k) * cnc + ci];
+    }
+}
+
+__device__ void fail(const TranDev &D, CktCtl *K, int code)
+{
+    K->phase = PH_FAIL;
+    K->status = code;
+    atomicAdd(D.done_count, 1);
+}
+
+/* DCtran "resume:" block (dctran.c:459-600, XSPICE branch) and state rotation */
+__device__ void resume_block(CktCtl *K)
+{
+    K->delta = fmin(K->delta, K->maxStep);
+    if (almost_equal_ulps(K->time, K->breaks[0], 100) || K->breaks[0] - K->time <= K->delmin) {
+        K->order = 1;
+        K->delta = fmin(K->delta, .1 * fmin(K->saveDelta, K->breaks[1] - K->breaks[0]));
+        if (K->firsttime) K->delta /= 10;
+    }
+    while ((K->breaks[0] <= K->time + K->minBreak || almost_equal_ulps(K->breaks[0], K->time, 100)) &&
+           K->breaks[0] < K->finalTime)
+        clr_break(K);
+    if (K->time + K->delta > K->breaks[0]) {
+        K->saveDelta = K->delta;
+        K->delta = K->breaks[0] - K->time;
+    } else if (K->time + 1.9 * K->delta > K->breaks[0]) {
+        K->saveDelta = K->delta;
+        K->delta = (K->breaks[0] - K->time) / 2.;
+    }
+    for (int i = 5; i >= 0; i--) K->deltaOld[i + 1] = K->deltaOld[i];
+    K->deltaOld[0] = K->delta;
+    K->rb = (K->rb + NLEV - 1) & (NLEV - 1);   /* states[i+1] = states[i], states[0] = old last */
+}
+
+/* top of the DCtran inner loop: advance time, integration coefficients */
+__device__ int loop_top(CktCtl *K)
+{
+    K->olddelta = K->delta;
+    K->time += K->delta;
+    K->deltaOld[0] = K->delta;
+    K->iterno = 0;
+    return com_cof(K);
+}
+
+__global__ void __launch_bounds__(256)
+k_tran_step(TranDev D)
+{
+    extern __shared__ double sm[];
+    __shared__ double red[256];
+    __shared__ int s_flag[4];
+    __shared__ int s_act;
+
+    const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    CktCtl *K = D.ctl + c;
+    if (K->phase >= PH_DONE) return;
+    const int N = D.N;
+    double *lu = sm;
+    double *z = lu + D.nslots;
+    double *capg = z + N;
+    double *capq = capg + D.NCAP;
+    double *vsv = capq + D.NCAP;
+
+    const int mode = K->mode, rb = K->rb, xs = K->xsel;
+    const double tnow = K->time;
+    const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
+    double *xo = D.x + ((int64_t)xs * D.C + c) * (N + 1);
+    double *xn = D.x + ((int64_t)(xs ^ 1) * D.C + c) * (N + 1);
+    if (tid < 4) s_flag[tid] = 0;
+
+    /* capload.c:19-86 */
+    for (int k = tid; k < D.NCAP; k += nt) {
+        const int p = D.cap_nodes[k], q = D.cap_nodes[D.NCAP + k];
+        const double Cv = D.cap_c[(int64_t)c * D.NCAP + k];
+        const int64_t ci = (int64_t)c * D.NCAP + k;
+        double geq = 0.0, ceq = 0.0;
+        if (mode & (M_TRAN | M_AC | M_TRANOP)) {
+            const int cond1 = ((mode & MODEDC_MASK) && (mode & M_INITJCT)) ||
+                              ((mode & M_UIC) && (mode & M_INITTRAN));
+            const double vcap = cond1 ? 0.0 : xo[p] - xo[q];
+            double *q0 = D.cstate + ((int64_t)lvl(rb, 0) * 2) * cnc + ci;
+            double *q1 = D.cstate + ((int64_t)lvl(rb, 1) * 2) * cnc + ci;
+            double *cc0 = D.cstate + ((int64_t)lvl(rb, 0) * 2 + 1) * cnc + ci;
+            double *cc1 = D.cstate + ((int64_t)lvl(rb, 1) * 2 + 1) * cnc + ci;
+            if (mode & (M_TRAN | M_AC)) {
+                if (mode & M_INITPRED) *q0 = *q1;
+                else {
+                    *q0 = Cv * vcap;
+                    if (mode & M_INITTRAN) *q1 = *q0;
+                }
+                /* NIintegrate, niinteg.c:18-80 */
+                double cc;
+                if (K->method == 1) {
+                    if (K->order == 1) cc = K->ag[0] * *q0 + K->ag[1] * *q1;
+                    else cc = -*cc1 * K->ag[1] + K->ag[0] * (*q0 - *q1);
+                } else {
+                    cc = 0;
+                    for (int l = K->order; l >= 2; l--)
+                        cc += K->ag[l] * D.cstate[((int64_t)lvl(rb, l) * 2) * cnc + ci];
+                    cc += K->ag[1] * *q1;
+                    cc += K->ag[0] * *q0;
+                }
+                *cc0 = cc;
+                ceq = cc - K->ag[0] * *q0;
+                geq = K->ag[0] * Cv;
+                if (mode & M_INITTRAN) *cc1 = *cc0;
+            } else {
+                *q0 = Cv * vcap;
+            }
+        }
+        capg[k] = geq;
+        capq[k] = ceq;
+    }
+    for (int k = tid; k < D.NVS; k += nt)
+        vsv[k] = vsrc_value(D.vs_par + ((int64_t)c * D.NVS + k) * VSP, mode, tnow, K->step);
+    __syncthreads();
+
+    /* assembly into the permuted L+U slots (CKTload order is not kept: the
+     * sum per entry runs in element order of the plan) */
+    const bool icz = (mode & M_TRANOP) && !(mode & M_UIC);
+    for (int s = tid; s < D.nslots; s += nt) {
+        double acc = 0.0;
+        for (int q = D.a_ptr[s]; q < D.a_ptr[s + 1]; q++) {
+            const uint32_t code = D.a_code[q];
+            if ((code & CT_ICZ) && icz) continue;
+            const uint32_t idx = code & CT_INDEX_MASK;
+            double v;
+            switch (code >> CT_TYPE_SHIFT) {
+            case CT_MOS: {
+                const int sl = idx / D.NM, j = idx - sl * D.NM;
+                v = D.stamp[sl * cnm + (int64_t)c * D.NM + j];
+                break;
+            }
+            case CT_CAP: v = capg[idx]; break;
+            case CT_RES: v = D.res_g[(int64_t)c * D.NRES + idx]; break;
+            default: v = 1.0; break;
+            }
+            acc += (code & CT_NEG) ? -v : v;
+        }
+        if ((D.slot_flags[s] & 1) && icz) acc = 1.0;
+        lu[s] = acc;
+    }
+    for (int a = tid; a < N; a += nt) {
+        double acc = 0.0;
+        const int u = D.perm_unknown[N + a];   /* original row of permuted row a */
+        if (icz && u <= D.NN && D.ic_nodes[u]) {
+            acc = D.ic_val[(int64_t)c * (D.NN + 1) + u];
+        } else {
+            for (int q = D.r_ptr[a]; q < D.r_ptr[a + 1]; q++) {
+                const uint32_t code = D.r_code[q];
+                const uint32_t idx = code & CT_INDEX_MASK;
+                double v;
+                switch (code >> CT_TYPE_SHIFT) {
+                case CT_MOS: {
+                    const int r = idx / D.NM, j = idx - r * D.NM;
+                    v = D.mrhs[r * cnm + (int64_t)c * D.NM + j];
+                    break;
+                }
+                case CT_CAP: v = capq[idx]; break;
+                case CT_VSRC: v = vsv[idx]; break;
+                default: v = 0.0; break;
+                }
+                acc += (code & CT_NEG) ? -v : v;
+            }
+        }
+        z[a] = acc;
+    }
+    __syncthreads();
+
+    /* LU factorisation, entry-wise Doolittle by dependency levels */
+    for (int L = 0; L < D.nlev_f; L++) {
+        for (int e = D.f_lev[L] + tid; e < D.f_lev[L + 1]; e += nt) {
+            const int4 en = reinterpret_cast<const int4 *>(D.f_ent)[e];
+            double acc = lu[en.x];
+            for (int q = en.z; q < en.w; q++) {
+                const int2 t = D.f_term[q];
+                acc -= lu[t.x] * lu[t.y];
+            }
+            if (en.y >= 0) acc /= lu[en.y];
+            lu[en.x] = acc;
+        }
+        __syncthreads();
+    }
+    /* forward substitution (unit L) */
+    for (int L = 0; L < D.nlev_fw; L++) {
+        for (int e = D.fw_lev[L] + tid; e < D.fw_lev[L + 1]; e += nt) {
+            const int row = D.fw_ent[3 * e], tb = D.fw_ent[3 * e + 1], te = D.fw_ent[3 * e + 2];
+            double acc = z[row];
+            for (int q = tb; q < te; q++) {
+                const int2 t = D.fw_term[q];
+                acc -= lu[t.x] * z[t.y];
+            }
+            z[row] = acc;
+        }
+        __syncthreads();
+    }
+    /* back substitution */
+    for (int L = 0; L < D.nlev_bw; L++) {
+        for (int e = D.bw_lev[L] + tid; e < D.bw_lev[L + 1]; e += nt) {
+            const int4 en = reinterpret_cast<const int4 *>(D.bw_ent)[e];
+            double acc = z[en.x];
+            for (int q = en.z; q < en.w; q++) {
+                const int2 t = D.bw_term[q];
+                acc -= lu[t.x] * z[t.y];
+            }
+            const double piv = lu[en.y];
+            if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
+            z[en.x] = acc / piv;
+        }
+        __syncthreads();
+    }
+    /* new iterate; NIconvTest (niconv.c:21-90) and the device votes */
+    for (int a = tid; a < N; a += nt) xn[D.perm_unknown[a]] = z[a];
+    if (tid == 0) xn[0] = 0.0;
+    int vote = 0;
+    for (int j = tid; j < D.NM; j += nt) vote |= D.noncon[(int64_t)c * D.NM + j];
+    int conv = 0;
+    for (int a = tid; a < N; a += nt) {
+        const int u = D.perm_unknown[a];
+        const double nw = z[a], old = xo[u];
+        if (isnan(nw)) { conv = 1; continue; }
+        const double tol = D.reltol * fmax(fabs(old), fabs(nw)) + (u <= D.NN ? D.vntol : D.abstol);
+        if (fabs(nw - old) > tol) conv = 1;
+    }
+    if (vote) s_flag[1] = 1;
+    if (conv) s_flag[2] = 1;
+    __syncthreads();
+
+    /* NIiter bookkeeping (niiter.c:95-365); thread 0 owns the scalars */
+    if (tid == 0) {
+        int m = K->mode;
+        int rc = -1;   /* -1: keep iterating */
+        const int iterno = K->iterno + 1;
+        K->iterno = iterno;
+        K->iters++;
+        if (s_flag[0]) rc = 102;
+        else if (iterno > 100) rc = 103;
+        else {
+            const int noncon = (s_flag[1] == 0 && iterno != 1) ? s_flag[2] : 1;
+            if (m & M_INITFLOAT) {
+                if (noncon == 0) rc = 0;
+            } else if (m & M_INITJCT) {
+                m = (m & ~INITF_MASK) | M_INITFIX;
+            } else if (m & M_INITFIX) {
+                if (noncon == 0) m = (m & ~INITF_MASK) | M_INITFLOAT;
+            } else {
+                m = (m & ~INITF_MASK) | M_INITFLOAT;   /* INITSMSIG, INITTRAN, INITPRED */
+            }
+        }
+        K->mode = m;
+        if (rc == -1) K->xsel ^= 1;   /* SWAP(rhs, rhsOld) */
+        s_act = ACT_NONE;
+        if (rc >= 0) {
+            if (K->phase == PH_OP) {
+                if (rc != 0) {
+                    fail(D, K, rc);
+                    s_act = ACT_STOP;
+                } else {
+                    /* dctran.c:268-330: into the transient */
+                    K->phase = PH_TRAN;
+                    K->order = 1;
+                    for (int i = 0; i < 7; i++) K->deltaOld[i] = K->maxStep;
+                    K->saveDelta = K->finalTime / 50;
+                    K->mode = M_TRAN | M_INITTRAN;
+                    K->ag[0] = K->ag[1] = 0;
+                    s_act = 10;   /* copy state0 -> state1, then accept */
+                }
+            } else {
+                K->mode = M_TRAN | M_INITPRED;
+                s_act = K->firsttime ? 20 : 21;   /* 20: copy state1 -> state2, state3 first */
+                s_flag[3] = rc;
+            }
+        }
+    }
+    __syncthreads();
+    int act = s_act;
+    if (act == ACT_NONE || act == ACT_STOP) return;
+    if (act == 10) {
+        copy_level(D, c, K->rb, 0, 1);
+        __syncthreads();
+        act = ACT_ACCEPT;
+    } else {
+        if (act == 20) {
+            copy_level(D, c, K->rb, 1, 2);
+            copy_level(D, c, K->rb, 1, 3);
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const int rc = s_flag[3];
+            if (rc != 0) {
+                K->time = K->time - K->delta;
+                K->rejected++;
+                K->delta = K->delta / 8;
+                if (K->firsttime) K->mode = M_TRAN | M_INITTRAN;
+                K->order = 1;
+                s_act = ACT_RETRY;
+            } else if (K->firsttime) {
+                K->firsttime = 0;
+                s_act = ACT_ACCEPT;
+            } else {
+                s_act = ACT_LTE1;
+            }
+        }
+        __syncthreads();
+        act = s_act;
+        if (act == ACT_LTE1) {
+            const double t1 = circuit_terr(D, K, c, red);
+            if (tid == 0) {
+                const double nd = fmin(2 * K->delta, t1);
+
+                if (nd > .9 * K->delta) {
+                    s_act = (K->order == 1 && K->maxOrder > 1) ? 30 : ACT_ACCEPT;
+                    if (s_act == 30) K->order = 2;
+                    else K->delta = nd;
+                } else {
+                    K->time = K->time - K->delta;
+                    K->rejected++;
+                    K->delta = nd;
+                    s_act = ACT_RETRY;
+                }
+            }
+            __syncthreads();
+            act = s_act;
+            if (act == 30) {
+                const double t2 = circuit_terr(D, K, c, red);
+                if (tid == 0) {
+                    const double nd = fmin(2 * K->delta, t2);
+                    if (nd <= 1.05 * K->delta) K->order = 1;
+                    K->delta = nd;
+                }
+                act = ACT_ACCEPT;
+            }
+        }
+    }
+    if (act == ACT_ACCEPT) {
+        /* nextTime: (dctran.c:352-455) */
+        if (tid == 0) {
+            s_act = ACT_ACCEPT;
+            if (vsrc_accept(D, c, K)) {
+                fail(D, K, 1);
+                s_act = ACT_STOP;
+            } else {
+                if (K->time > K->breaks[0]) clr_break(K);
+                K->accepted++;
+            }
+        }
+        __syncthreads();
+        if (s_act == ACT_STOP) return;
+        /* CKTdump: rhsOld of the accepted point */
+        const double *xold = D.x + ((int64_t)K->xsel * D.C + c) * (N + 1);
+        const int64_t np = K->npts;
+        if (np < D.cap_pts) {
+            double *row = D.out_x + ((int64_t)c * D.cap_pts + np) * D.nprobe;
+            for (int k = tid; k < D.nprobe; k += nt) row[k] = xold[D.probe[k]];
+            if (tid == 0) D.out_t[(int64_t)c * D.cap_pts + np] = K->time;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            K->npts = np + 1;
+            if (almost_equal_ulps(K->time, K->finalTime, 100)) {
+                K->phase = PH_DONE;
+                atomicAdd(D.done_count, 1);
+            } else {
+                resume_block(K);
+                if (loop_top(K)) fail(D, K, 104);
+            }
+        }
+        return;
+    }
+    if (act == ACT_RETRY && tid == 0) {
+        if (K->delta <= K->delmin) {
+            if (K->olddelta > K->delmin) K->delta = K->delmin;
+            else {
+                fail(D, K, 106);
+                return;
+            }
+        }
+        if (loop_top(K)) fail(D, K, 104);
+    }
+}
+
+int set_tran_lds_limit(size_t bytes)
+{
+    NGMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tran_step),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return NGMI_OK;
+}
+
+int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s)
+{
+    const int64_t cnm = (int64_t)D.C * D.NM;
+    if (cnm > 0) {
+        hipLaunchKernelGGL(k_tran_mos, dim3((unsigned)((cnm + 255) / 256)), dim3(256), 0, s, tab, D);
+        NGMI_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_tran_step, dim3((unsigned)D.C), dim3(256), lds_bytes, s, D);
+    NGMI_HIP(hipGetLastError());
+    return NGMI_OK;
+}
+
+}  // namespace ngmi

@@ -1,0 +1,103 @@
+/*
+ * Data shared between the host side of the batched transient engine and its
+ * kernels: per-circuit control blocks, the static LU program and the
+ * assembly plans.  All device arrays are circuit-major: quantity q of
+ * element j of circuit c lives at [q][c * count + j].
+ */
+#ifndef NGMI_TRAN_TYPES_H
+#define NGMI_TRAN_TYPES_H
+
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace ngmi {
+
+constexpr int MAXBRK = 24;       /* breakpoint table per circuit */
+constexpr int NLEV = 4;          /* state levels kept: max order 2 + 2 */
+constexpr int VSP = 10;          /* doubles per voltage source */
+
+/* circuit phases */
+enum : int {
+    PH_OP = 0,        /* operating point Newton (CKTop) */
+    PH_TRAN = 1,      /* transient Newton (NIiter under DCtran) */
+    PH_DONE = 2,      /* reached the final time */
+    PH_FAIL = 3       /* error; see status */
+};
+
+/* per-circuit control block (scalars of CKTcircuit used by DCtran/NIiter) */
+struct CktCtl {
+    double time, delta, saveDelta, olddelta;
+    double deltaOld[7];
+    double ag[7];
+    double xfact;
+    double breaks[MAXBRK];
+    double finalTime, step, maxStep, delmin, minBreak;
+    int mode, order, method, maxOrder;
+    int phase, status, firsttime, iterno;
+    int nbreaks, rb, xsel, pad;
+    int64_t npts, accepted, rejected, iters;
+};
+
+/* contribution codes of the assembly plans */
+enum : uint32_t {
+    CT_MOS = 0u, CT_CAP = 1u, CT_RES = 2u, CT_ONE = 3u, CT_VSRC = 4u,
+    CT_TYPE_SHIFT = 28u, CT_NEG = 1u << 27, CT_ICZ = 1u << 26,
+    CT_INDEX_MASK = (1u << 26) - 1u
+};
+
+/* one batch (all circuits share the topology) */
+struct TranDev {
+    int C;            /* circuits */
+    int N;            /* unknowns (1..N, 0 = ground) */
+    int NN;           /* node unknowns (1..NN); N - NN branch unknowns */
+    int NM, NCAP, NRES, NVS;
+    int nslots;       /* nonzeros of L + U */
+    int nlev_f, nlev_fw, nlev_bw;
+    int nprobe;
+    int64_t cap_pts;  /* waveform capacity per circuit */
+    double reltol, abstol, vntol, chgtol, trtol, gmin;
+    /* topology (shared) */
+    const int *mos_nodes;     /* [4][NM] d g s b */
+    const int *cap_nodes;     /* [2][NCAP] */
+    const int *res_nodes;     /* [2][NRES] */
+    const int *vs_nodes;      /* [3][NVS] p n branch */
+    const int *ic_nodes;      /* [NN+1] 1 where the node has an .ic */
+    /* per-circuit element values */
+    const double *cap_c;      /* [C][NCAP] */
+    const double *res_g;      /* [C][NRES] */
+    const double *vs_par;     /* [C][NVS][VSP]: dc, pulse flag, coefficient count, 7 coefficients */
+    double *vs_brk;           /* [C][NVS] break_time */
+    const double *ic_val;     /* [C][NN+1] */
+    /* LU program */
+    const int *a_ptr;         /* [nslots+1] */
+    const uint32_t *a_code;   /* contributions */
+    const int *slot_flags;    /* [nslots] bit0: .ic diagonal */
+    const int *r_ptr;         /* [N+1] RHS gather per permuted row */
+    const uint32_t *r_code;
+    const int *f_lev;         /* [nlev_f+1] -> entries */
+    const int *f_ent;         /* [ne][4]: target, pivot (-1 for U), term begin, term end */
+    const int2 *f_term;       /* (a, b) slot pairs */
+    const int *fw_lev, *fw_ent;   /* forward solve: [ne][3] row, term begin, end */
+    const int2 *fw_term;          /* (L slot, z index) */
+    const int *bw_lev, *bw_ent;   /* backward solve: [ne][4] row, diag slot, term begin, end */
+    const int2 *bw_term;          /* (U slot, z index) */
+    const int *perm_unknown;  /* [2N]: [a] unknown (1-based) solved in permuted column a;
+                                 [N + a] original row assembled into permuted row a */
+    const int *probe;         /* [nprobe] unknowns written at each accepted point */
+    /* state */
+    CktCtl *ctl;              /* [C] */
+    double *x;                /* [2][C][N+1] iterate buffers */
+    double *mstate;           /* [NLEV][29][C*NM] */
+    double *cstate;           /* [NLEV][2][C*NCAP] */
+    double *cache;            /* [38][C*NM] */
+    double *stamp;            /* [16][C*NM] */
+    double *mrhs;             /* [4][C*NM] */
+    int *noncon;              /* [C*NM] */
+    double *out_t;            /* [C][cap_pts] */
+    double *out_x;            /* [C][cap_pts][nprobe] */
+    int *done_count;          /* circuits finished or failed */
+};
+
+}  // namespace ngmi
+
+#endif

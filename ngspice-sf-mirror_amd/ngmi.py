@@ -257,3 +257,130 @@ class Timer:
             lib().ngmi_timer_free(self.h)
         except Exception:
             pass
+
+
+def _tran_protos(L):
+    if getattr(L, '_tran_ready', False):
+        return L
+    vp, d, i, cp = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_char_p
+    i64 = ctypes.c_int64
+    L.ngmi_circuit_new.argtypes = [i]
+    L.ngmi_circuit_new.restype = vp
+    L.ngmi_circuit_free.argtypes = [vp]
+    L.ngmi_circuit_add_bsim4.argtypes = [vp, vp, i, i, i, i, i, ctypes.POINTER(cp),
+                                         ctypes.POINTER(d)]
+    L.ngmi_circuit_add_cap.argtypes = [vp, i, i, d]
+    L.ngmi_circuit_add_res.argtypes = [vp, i, i, d]
+    L.ngmi_circuit_add_vsrc.argtypes = [vp, i, i, d, ctypes.POINTER(d), i]
+    L.ngmi_circuit_set_ic.argtypes = [vp, i, d]
+    L.ngmi_circuit_set_options.argtypes = [vp, d, d, d, d, d, d, i, i]
+    L.ngmi_batch_new.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(i), i, i64]
+    L.ngmi_batch_new.restype = vp
+    L.ngmi_batch_free.argtypes = [vp]
+    L.ngmi_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
+                                  ctypes.POINTER(i)]
+    L.ngmi_batch_tran.argtypes = [vp, d, d, d, i64, vp, ctypes.POINTER(i64)]
+    L.ngmi_batch_stats.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i64),
+                                   ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.ngmi_batch_waveform.argtypes = [vp, i, ctypes.POINTER(d), ctypes.POINTER(d), i64,
+                                      ctypes.POINTER(i64)]
+    L._tran_ready = True
+    return L
+
+
+class Circuit:
+    """A netlist for the engine from a circuit description dict (nnodes,
+    models, mos, caps, res, vsrc, ic, options); see workloads.py."""
+
+    def __init__(self, desc, temp=300.15):
+        L = _tran_protos(lib())
+        self.models = {k: Model(v, temp=temp) for k, v in desc['models'].items()}
+        self.h = L.ngmi_circuit_new(desc['nnodes'])
+        if not self.h:
+            raise EngineError(1, 'circuit')
+        o = dict(reltol=1e-3, abstol=1e-12, vntol=1e-6, chgtol=1e-14, trtol=7.0, gmin=1e-12,
+                 method=1, maxord=2)
+        o.update(desc.get('options', {}))
+        check(L.ngmi_circuit_set_options(self.h, o['reltol'], o['abstol'], o['vntol'],
+                                         o['chgtol'], o['trtol'], o['gmin'], o['method'],
+                                         o['maxord']), 'options')
+        for key, dn, gn, sn, bn, ip in desc['mos']:
+            names = (ctypes.c_char_p * max(len(ip), 1))(*[k.encode() for k in ip])
+            vals = (ctypes.c_double * max(len(ip), 1))(*[float(v) for v in ip.values()])
+            check(L.ngmi_circuit_add_bsim4(self.h, self.models[key].h, dn, gn, sn, bn, len(ip),
+                                           names, vals), 'add bsim4')
+        for p, n, cv in desc['caps']:
+            check(L.ngmi_circuit_add_cap(self.h, p, n, cv), 'add cap')
+        for p, n, r in desc.get('res', []):
+            check(L.ngmi_circuit_add_res(self.h, p, n, r), 'add res')
+        for p, n, dc, pulse in desc['vsrc']:
+            if pulse is None:
+                check(L.ngmi_circuit_add_vsrc(self.h, p, n, dc, None, 0), 'add vsrc')
+            else:
+                arr = (ctypes.c_double * len(pulse))(*pulse)
+                check(L.ngmi_circuit_add_vsrc(self.h, p, n, dc, arr, len(pulse)), 'add vsrc')
+        for node, v in desc.get('ic', {}).items():
+            check(L.ngmi_circuit_set_ic(self.h, node, v), 'set ic')
+
+    def __del__(self):
+        try:
+            lib().ngmi_circuit_free(self.h)
+        except Exception:
+            pass
+
+
+class Batch:
+    """C circuits of one topology advanced together on the GPU."""
+
+    def __init__(self, circuits, probe=None, max_points=20000):
+        L = _tran_protos(lib())
+        self.circuits = circuits
+        arr = (ctypes.c_void_p * len(circuits))(*[c.h for c in circuits])
+        if probe is None:
+            pp, npb = None, 0
+        else:
+            pp = (ctypes.c_int * len(probe))(*probe)
+            npb = len(probe)
+        self.h = L.ngmi_batch_new(arr, len(circuits), pp, npb, max_points)
+        if not self.h:
+            raise EngineError(1, 'batch')
+        self.C = len(circuits)
+
+    def info(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lv = (ctypes.c_int * 3)()
+        check(lib().ngmi_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                    lv), 'info')
+        return dict(unknowns=a.value, lu_nonzeros=b.value, factor_terms=c.value,
+                    levels=tuple(lv))
+
+    def tran(self, tstep, tstop, tmax=0.0, max_passes=10 ** 7, stream=0):
+        passes = ctypes.c_int64()
+        rc = lib().ngmi_batch_tran(self.h, tstep, tstop, tmax, max_passes, stream,
+                                   ctypes.byref(passes))
+        return rc, passes.value
+
+    def stats(self, c):
+        st = ctypes.c_int()
+        a = [ctypes.c_int64() for _ in range(4)]
+        check(lib().ngmi_batch_stats(self.h, c, ctypes.byref(st), *[ctypes.byref(x) for x in a]),
+              'stats')
+        return dict(status=st.value, npts=a[0].value, accepted=a[1].value, rejected=a[2].value,
+                    iterations=a[3].value)
+
+    def waveform(self, c, nprobe):
+        s = self.stats(c)
+        n = max(s['npts'], 1)
+        t = np.zeros(n)
+        x = np.zeros((n, nprobe))
+        got = ctypes.c_int64()
+        check(lib().ngmi_batch_waveform(self.h, c, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                        ctypes.byref(got)), 'waveform')
+        return t[:got.value], x[:got.value]
+
+    def __del__(self):
+        try:
+            lib().ngmi_batch_free(self.h)
+        except Exception:
+            pass

@@ -78,3 +78,114 @@ def eval_states(wl):
     s0[1] = wl['vbs']          # vbs
     s0[0] = wl['vbs'] - wl['vds']   # vbd
     return [s0, np.zeros((29, n)), np.zeros((29, n))]
+
+
+# ---------------------------------------------------------------------------
+# Transient circuits.  A circuit is a plain dict (the same description feeds
+# the engine and the oracle):
+#   nnodes, models {key: card}, mos [(key, d, g, s, b, {inst params})],
+#   caps [(p, n, C)], res [(p, n, R)], vsrc [(p, n, dc, pulse or None)],
+#   ic {node: v}, tstep, tstop.
+# Node 0 is ground; nodes are numbered 1..nnodes.
+# ---------------------------------------------------------------------------
+
+RING_STAGES = 101
+RING_WN, RING_WP, RING_L = 200e-9, 400e-9, 45e-9
+VDD = 1.0
+
+
+def ring_oscillator(stages=RING_STAGES, cload=1e-15, ic_node1=0.51, tstep=10e-12, tstop=200e-9):
+    """CMOS inverter ring: stage k drives node k+1 from node k (node
+    stages+1 wraps to node 1), 1 fF from every ring node to ground, VDD on
+    node stages+1; node 1 starts from .ic v(1) = 0.51 V."""
+    vdd = stages + 1
+    mos, caps = [], []
+    for k in range(1, stages + 1):
+        a = k
+        y = k + 1 if k < stages else 1
+        mos.append(('n', y, a, 0, 0, {'w': RING_WN, 'l': RING_L}))
+        mos.append(('p', y, a, vdd, vdd, {'w': RING_WP, 'l': RING_L}))
+        caps.append((k, 0, cload))
+    return {'nnodes': vdd, 'models': {'n': dict(BASE_NMOS), 'p': dict(BASE_PMOS)}, 'mos': mos,
+            'caps': caps, 'res': [], 'vsrc': [(vdd, 0, VDD, None)], 'ic': {1: ic_node1},
+            'tstep': tstep, 'tstop': tstop}
+
+
+def mirror_adder_topology(bits=16):
+    """Node map and transistor list of a ripple-carry adder of 28-transistor
+    mirror full adders (carry: 10 T, sum: 14 T, two output inverters).
+    Returns (nnodes, mos list with model keys 'n'/'p' and no params, named
+    nodes dict)."""
+    nodes = {'vdd': 1}
+    nxt = [2]
+
+    def new(name):
+        nodes[name] = nxt[0]
+        nxt[0] += 1
+        return nodes[name]
+
+    A = [new('a%d' % i) for i in range(bits)]
+    B = [new('b%d' % i) for i in range(bits)]
+    cin = new('cin')
+    vdd = 1
+    mos = []
+    ci = cin
+    for i in range(bits):
+        a, b = A[i], B[i]
+        cob, sb = new('cob%d' % i), new('sb%d' % i)
+        co, s = new('co%d' % i), new('s%d' % i)
+        na, nx, pa, px = new('na%d' % i), new('nx%d' % i), new('pa%d' % i), new('px%d' % i)
+        n1, n2, ny = new('n1_%d' % i), new('n2_%d' % i), new('ny%d' % i)
+        p1, p2, py = new('p1_%d' % i), new('p2_%d' % i), new('py%d' % i)
+        # carry: Co_b = not(a b + ci (a + b))
+        mos += [('n', cob, a, na, 0), ('n', na, b, 0, 0),
+                ('n', cob, ci, nx, 0), ('n', nx, a, 0, 0), ('n', nx, b, 0, 0),
+                ('p', cob, a, pa, vdd), ('p', pa, b, vdd, vdd),
+                ('p', cob, ci, px, vdd), ('p', px, a, vdd, vdd), ('p', px, b, vdd, vdd)]
+        # sum: S_b = not(a b ci + Co_b (a + b + ci))
+        mos += [('n', sb, a, n1, 0), ('n', n1, b, n2, 0), ('n', n2, ci, 0, 0),
+                ('n', sb, cob, ny, 0), ('n', ny, a, 0, 0), ('n', ny, b, 0, 0), ('n', ny, ci, 0, 0),
+                ('p', sb, a, p1, vdd), ('p', p1, b, p2, vdd), ('p', p2, ci, vdd, vdd),
+                ('p', sb, cob, py, vdd), ('p', py, a, vdd, vdd), ('p', py, b, vdd, vdd),
+                ('p', py, ci, vdd, vdd)]
+        # output inverters
+        mos += [('n', co, cob, 0, 0), ('p', co, cob, vdd, vdd),
+                ('n', s, sb, 0, 0), ('p', s, sb, vdd, vdd)]
+        ci = co
+    return nxt[0] - 1, mos, nodes
+
+
+ADDER_BITS = 16
+ADDER_TSTEP, ADDER_TSTOP = 5e-12, 20e-9
+ADDER_SWITCH = 1e-9
+ADDER_EDGE = 20e-12
+
+
+def mc_adder(seed, bits=ADDER_BITS, tstep=ADDER_TSTEP, tstop=ADDER_TSTOP):
+    """One Monte Carlo instance of the ripple-carry adder: its own model card
+    pair (toxe ~ N(1.8 nm, 2 %)), per transistor delvto ~ N(0, 30 mV) and
+    L ~ N(45 nm, 3 %), W = 200 nm (NMOS) / 400 nm (PMOS), and its own input
+    vector: every input moves from a random bit to a random bit at 1 ns
+    (PULSE v1 v2 1n 20p 20p 1 2)."""
+    nn, topo, names = mirror_adder_topology(bits)
+    g = rng(2, seed)
+    toxe = 1.8e-9 * (1.0 + 0.02 * g.standard_normal())
+    models = {'n': dict(BASE_NMOS, toxe=toxe), 'p': dict(BASE_PMOS, toxe=toxe)}
+    nm = len(topo)
+    dv = g.normal(0.0, 0.030, nm)
+    ll = 45e-9 * (1.0 + 0.03 * g.standard_normal(nm))
+    mos = []
+    for k, (key, d, gg, s, b) in enumerate(topo):
+        w = RING_WN if key == 'n' else RING_WP
+        mos.append((key, d, gg, s, b, {'w': w, 'l': float(ll[k]), 'delvto': float(dv[k])}))
+    bits0 = g.integers(0, 2, 2 * bits + 1)
+    bits1 = g.integers(0, 2, 2 * bits + 1)
+    inputs = [names['a%d' % i] for i in range(bits)] + [names['b%d' % i] for i in range(bits)] \
+        + [names['cin']]
+    vsrc = [(1, 0, VDD, None)]
+    for k, node in enumerate(inputs):
+        v1, v2 = VDD * bits0[k], VDD * bits1[k]
+        vsrc.append((node, 0, v1, (v1, v2, ADDER_SWITCH, ADDER_EDGE, ADDER_EDGE, 1.0, 2.0)))
+    return {'nnodes': nn, 'models': models, 'mos': mos, 'caps': [], 'res': [], 'vsrc': vsrc,
+            'ic': {}, 'tstep': tstep, 'tstop': tstop, 'names': names,
+            'bits0': bits0, 'bits1': bits1}

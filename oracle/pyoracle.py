@@ -158,3 +158,100 @@ class Instance:
             lib().or_inst_free(self.h)
         except Exception:
             pass
+
+
+def _tran_protos():
+    L = lib()
+    if getattr(L, '_tran_ready', False):
+        return L
+    vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.or_tran_new.argtypes = [i, i, i, i, i]
+    L.or_tran_new.restype = vp
+    L.or_tran_set_options.argtypes = [vp, d, d, d, d, d, d, i, i, i]
+    L.or_tran_add_mos.argtypes = [vp, vp, i, i, i, i]
+    L.or_tran_add_cap.argtypes = [vp, i, i, d]
+    L.or_tran_add_res.argtypes = [vp, i, i, d]
+    L.or_tran_add_vsrc.argtypes = [vp, i, i, d, dp, i]
+    L.or_tran_set_ic.argtypes = [vp, i, d]
+    L.or_tran_setup.argtypes = [vp]
+    L.or_tran_run.argtypes = [vp, d, d, d]
+    L.or_tran_npts.argtypes = [vp]
+    L.or_tran_npts.restype = ctypes.c_int64
+    L.or_tran_n.argtypes = [vp]
+    L.or_tran_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int64)]
+    L.or_tran_output.argtypes = [vp, dp, dp]
+    L.or_tran_free.argtypes = [vp]
+    L._tran_ready = True
+    return L
+
+
+class Transient:
+    """DCtran restatement on one circuit (see tran_ref.c).
+
+    circuit: dict with nnodes, models {key: params}, mos [(key, d, g, s, b,
+    {inst params})], caps [(p, n, C)], res [(p, n, R)], vsrc [(p, n, dc,
+    pulse tuple or None)], ic {node: v}, temp (K), options {reltol, abstol,
+    vntol, chgtol, trtol, gmin, method, maxord, uic}."""
+
+    def __init__(self, circuit):
+        import numpy as np
+        L = _tran_protos()
+        c = circuit
+        self.np = np
+        temp = c.get('temp', 300.15)
+        self.models = {k: Model(v, nom_temp=300.15, temp=temp) for k, v in c['models'].items()}
+        self.h = L.or_tran_new(c['nnodes'], len(c['mos']), len(c['caps']), len(c.get('res', [])),
+                               len(c['vsrc']))
+        o = dict(reltol=1e-3, abstol=1e-12, vntol=1e-6, chgtol=1e-14, trtol=7.0, gmin=1e-12,
+                 method=1, maxord=2, uic=0)
+        o.update(c.get('options', {}))
+        L.or_tran_set_options(self.h, o['reltol'], o['abstol'], o['vntol'], o['chgtol'],
+                              o['trtol'], o['gmin'], o['method'], o['maxord'], o['uic'])
+        self.inst = []
+        for key, dn, gn, sn, bn, ip in c['mos']:
+            inst = Instance(self.models[key], ip)
+            if L.or_tran_add_mos(self.h, inst.h, dn, gn, sn, bn) != 0:
+                raise ValueError('MOSFET needs internal nodes')
+            self.inst.append(inst)
+        for p, n, cv in c['caps']:
+            L.or_tran_add_cap(self.h, p, n, cv)
+        for p, n, r in c.get('res', []):
+            L.or_tran_add_res(self.h, p, n, r)
+        for p, n, dc, pulse in c['vsrc']:
+            if pulse is None:
+                L.or_tran_add_vsrc(self.h, p, n, dc, None, 0)
+            else:
+                arr = (ctypes.c_double * len(pulse))(*pulse)
+                L.or_tran_add_vsrc(self.h, p, n, dc, arr, len(pulse))
+        for node, v in c.get('ic', {}).items():
+            L.or_tran_set_ic(self.h, node, v)
+        if L.or_tran_setup(self.h) != 0:
+            raise RuntimeError('transient setup failed')
+
+    def run(self, tstep, tstop, tmax=0.0):
+        """Returns (rc, t[npts], x[npts, n+1]) with x[:, 0] = ground."""
+        L = _tran_protos()
+        np = self.np
+        rc = L.or_tran_run(self.h, tstep, tstop, tmax)
+        npts = L.or_tran_npts(self.h)
+        n = L.or_tran_n(self.h)
+        t = np.zeros(max(npts, 1))
+        x = np.zeros((max(npts, 1), n + 1))
+        if npts:
+            L.or_tran_output(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return rc, t[:npts], x[:npts]
+
+    def stats(self):
+        L = _tran_protos()
+        a, r, it = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        L.or_tran_stats(self.h, ctypes.byref(a), ctypes.byref(r), ctypes.byref(it))
+        return dict(accepted=a.value, rejected=r.value, iterations=it.value)
+
+    def __del__(self):
+        try:
+            _tran_protos().or_tran_free(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,722 @@
+/*
+ * ORACLE (test infrastructure only).  Scalar C restatement of ngspice 46's
+ * transient path for circuits of BSIM4 MOSFETs, capacitors, resistors and
+ * independent voltage sources (DC or PULSE), used by tests/ and by the CPU
+ * baseline of bench.py to check the engine's waveforms.
+ *
+ * Follows:
+ *   src/spicelib/analysis/traninit.c:17-40   TMAX / delmin
+ *   src/spicelib/analysis/dctran.c:66-971    DCtran (the XSPICE build, no
+ *                                             event-driven instances)
+ *   src/spicelib/analysis/cktop.c:26-116     CKTop (direct Newton only; the
+ *                                             gmin/source-stepping fallbacks
+ *                                             are not restated and report
+ *                                             failure instead)
+ *   src/maths/ni/niiter.c:28-367             NIiter (INITF state machine,
+ *                                             >= 2 iterations, maxIter >= 100)
+ *   src/maths/ni/nicomcof.c:17-130           NIcomCof (trapezoidal, Gear)
+ *   src/maths/ni/niconv.c:21-90              NIconvTest
+ *   src/spicelib/analysis/cktload.c:31-180   CKTload incl. .ic rows in TRANOP
+ *   src/spicelib/analysis/ckttrunc.c:20-56   CKTtrunc (min(2 dt, LTE))
+ *   src/spicelib/analysis/cktsetbk.c, cktclrbk.c   breakpoint table
+ *   src/spicelib/devices/cap/capload.c:19-86, captrunc.c:17-30
+ *   src/spicelib/devices/res/resload.c:16-38
+ *   src/spicelib/devices/vsrc/vsrcload.c:40-170 (PULSE), vsrcacct.c:20-160
+ *   src/maths/misc/equality.c:22-60          AlmostEqualUlps
+ *
+ * The linear solve is a dense LU with partial pivoting on the MNA matrix
+ * (the reference uses KLU; results agree to rounding).  Device loads sum into
+ * the matrix in device-type order CAP, BSIM4, RES, VSRC as CKTload does.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "bsim4_ref.h"
+
+int or_slot_rc(int slot, int *row_role, int *col_role);
+
+#define MAXBRK 64
+
+typedef struct {
+    int p, n, br;
+    double dc;
+    int pulse;          /* 1: PULSE(v1 v2 td tr tf pw per) */
+    int order;          /* number of PULSE coefficients given */
+    double c[7];
+    double break_time;
+} ORVsrc;
+
+typedef struct {
+    /* topology */
+    int nnodes, nbr, n;       /* unknowns 1..n, 0 = ground */
+    int nmos, ncap, nres, nvs;
+    ORInst **mos;
+    int *cap_p, *cap_n, *cap_q;
+    double *cap_c;
+    int *res_p, *res_n;
+    double *res_g;
+    ORVsrc *vs;
+    int *ic_given;
+    double *ic;
+    /* options */
+    double reltol, abstol, vntol, chgtol, trtol, gmin, temp;
+    int maxOrder, method, uic;
+    /* state */
+    int nstates;
+    double *stbuf;            /* 4 x nstates */
+    double *states[4];
+    double *A;                /* (n+1) x (n+1), row 0 / col 0 are trash */
+    double *rhs, *rhsOld;
+    double *lu;
+    int *piv;
+    ORCkt ck;
+    double time, delta, saveDelta, finalTime, step, maxStep, delmin, minBreak;
+    double breaks[MAXBRK];
+    int nbreaks;
+    /* output */
+    int64_t npts, cap_pts;
+    double *out_t, *out_x;
+    int64_t niter, accepted, rejected;
+    int noncon_dc_fail;
+} ORTran;
+
+/* equality.c:22-60 */
+static int almost_equal_ulps(double A, double B, int maxUlps)
+{
+    int64_t aInt, bInt, intDiff;
+    union { double d; int64_t i; } uA, uB;
+    if (A == B) return 1;
+    uA.d = A;
+    aInt = uA.i;
+    if (aInt < 0) aInt = INT64_MIN - aInt;
+    uB.d = B;
+    bInt = uB.i;
+    if (bInt < 0) bInt = INT64_MIN - bInt;
+    intDiff = llabs(aInt - bInt);
+    return intDiff <= maxUlps;
+}
+
+/* cktsetbk.c:19-100 */
+static int set_break(ORTran *T, double time)
+{
+    int i, j;
+    if (almost_equal_ulps(time, T->time, 3)) return 0;
+    if (T->time > time) return -1;
+    for (i = 0; i < T->nbreaks; i++) {
+        if (T->breaks[i] > time) {
+            if ((T->breaks[i] - time) <= T->minBreak) {
+                T->breaks[i] = time;
+                return 0;
+            }
+            if (i > 0 && time - T->breaks[i - 1] <= T->minBreak) return 0;
+            if (T->nbreaks >= MAXBRK) return -2;
+            for (j = T->nbreaks; j > i; j--) T->breaks[j] = T->breaks[j - 1];
+            T->breaks[i] = time;
+            T->nbreaks++;
+            return 0;
+        }
+    }
+    if (T->nbreaks && time - T->breaks[T->nbreaks - 1] <= T->minBreak) return 0;
+    if (T->nbreaks >= MAXBRK) return -2;
+    T->breaks[T->nbreaks++] = time;
+    return 0;
+}
+
+/* cktclrbk.c:15-40 */
+static void clr_break(ORTran *T)
+{
+    int j;
+    if (T->nbreaks > 2) {
+        for (j = 1; j < T->nbreaks; j++) T->breaks[j - 1] = T->breaks[j];
+        T->nbreaks--;
+    } else {
+        T->breaks[0] = T->breaks[1];
+        T->breaks[1] = T->finalTime;
+    }
+}
+
+/* PULSE parameter defaults, vsrcload.c:96-125 */
+static void pulse_params(const ORTran *T, const ORVsrc *v, double *V1, double *V2, double *TD,
+                         double *TR, double *TF, double *PW, double *PER)
+{
+    *V1 = v->c[0];
+    *V2 = v->c[1];
+    *TD = v->order > 2 ? v->c[2] : 0.0;
+    *TR = v->order > 3 && v->c[3] > 0.0 ? v->c[3] : T->step;
+    *TF = v->order > 4 && v->c[4] > 0.0 ? v->c[4] : T->step;
+    *PW = v->order > 5 && v->c[5] >= 0.0 ? v->c[5] : 0.0;
+    *PER = v->order > 6 && v->c[6] > 0.0 ? v->c[6] : *TR + *TF + *PW;
+}
+
+/* vsrcload.c:78-170 for DC and PULSE */
+static double vsrc_value(const ORTran *T, const ORVsrc *v, int mode)
+{
+    double time, V1, V2, TD, TR, TF, PW, PER, basetime;
+    if ((mode & (OR_MODEDCOP | OR_MODEDCTRANCURVE)) && !v->pulse) return v->dc;
+    if (!v->pulse) return v->dc;
+    time = (mode & (OR_MODEDCOP | OR_MODETRANOP | OR_MODEDCTRANCURVE)) ? 0.0 : T->time;
+    pulse_params(T, v, &V1, &V2, &TD, &TR, &TF, &PW, &PER);
+    time -= TD;
+    if (time > PER) {
+        basetime = PER * floor(time / PER);
+        time -= basetime;
+    }
+    if (time <= 0 || time >= TR + PW + TF) return V1;
+    if (time >= TR && time <= TR + PW) return V2;
+    if (time > 0 && time < TR) return V1 + (V2 - V1) * (time) / TR;
+    return V2 + (V1 - V2) * (time - (TR + PW)) / TF;
+}
+
+/* vsrcacct.c:20-160 for PULSE: next corner as a breakpoint */
+static int vsrc_accept(ORTran *T, int mode)
+{
+    int k;
+    if (!(mode & (OR_MODETRAN | OR_MODETRANOP))) return 0;
+    for (k = 0; k < T->nvs; k++) {
+        ORVsrc *v = &T->vs[k];
+        double V1, V2, TD, TR, TF, PW, PER, time, basetime, wait, atime;
+        if (!v->pulse) continue;
+        pulse_params(T, v, &V1, &V2, &TD, &TR, &TF, &PW, &PER);
+        time = T->time - TD;
+        if (T->time >= v->break_time) {
+            if (time >= PER) {
+                basetime = PER * floor(time / PER);
+                time -= basetime;
+            }
+            atime = time + T->minBreak;
+            if (atime < 0.0) wait = -time;
+            else if (atime < TR) wait = TR - time;
+            else if (atime < TR + PW) wait = TR + PW - time;
+            else if (atime < TR + PW + TF) wait = TR + PW + TF - time;
+            else wait = PER - time;
+            v->break_time = T->time + wait;
+            if (set_break(T, v->break_time)) return -1;
+            v->break_time -= T->minBreak;
+        }
+    }
+    return 0;
+}
+
+#define AX(r, c) T->A[(size_t)(r) * (T->n + 1) + (c)]
+
+/* CKTload (cktload.c:31-180): CAP, BSIM4, RES, VSRC, then .ic rows */
+static int ckt_load(ORTran *T)
+{
+    ORCkt *ck = &T->ck;
+    const int mode = ck->mode, n = T->n;
+    int k, s;
+    memset(T->A, 0, sizeof(double) * (size_t)(n + 1) * (n + 1));
+    memset(T->rhs, 0, sizeof(double) * (size_t)(n + 1));
+    ck->rhsOld = T->rhsOld;
+    /* capload.c:19-86 */
+    if (mode & (OR_MODETRAN | OR_MODEAC | OR_MODETRANOP)) {
+        const int cond1 = ((mode & (OR_MODEDCOP | OR_MODETRANOP | OR_MODEDCTRANCURVE)) &&
+                           (mode & OR_MODEINITJCT)) ||
+                          ((mode & OR_MODEUIC) && (mode & OR_MODEINITTRAN));
+        for (k = 0; k < T->ncap; k++) {
+            const int p = T->cap_p[k], q = T->cap_n[k], qc = T->cap_q[k];
+            double vcap, geq, ceq;
+            vcap = cond1 ? 0.0 : T->rhsOld[p] - T->rhsOld[q];
+            if (mode & (OR_MODETRAN | OR_MODEAC)) {
+                if (mode & OR_MODEINITPRED) {
+                    ck->states[0][qc] = ck->states[1][qc];
+                } else {
+                    ck->states[0][qc] = T->cap_c[k] * vcap;
+                    if (mode & OR_MODEINITTRAN) ck->states[1][qc] = ck->states[0][qc];
+                }
+                if (or_integrate(ck, &geq, &ceq, T->cap_c[k], qc)) return -1;
+                if (mode & OR_MODEINITTRAN) ck->states[1][qc + 1] = ck->states[0][qc + 1];
+                AX(p, p) += geq;
+                AX(q, q) += geq;
+                AX(p, q) -= geq;
+                AX(q, p) -= geq;
+                T->rhs[p] -= ceq;
+                T->rhs[q] += ceq;
+            } else {
+                ck->states[0][qc] = T->cap_c[k] * vcap;
+            }
+        }
+    }
+    /* b4ld.c:69-5433 */
+    for (k = 0; k < T->nmos; k++) {
+        ORInst *h = T->mos[k];
+        if (or_bsim4_load(h, ck)) return -1;
+        for (s = 0; s < P__COUNT; s++) {
+            int rr, cc, r, c;
+            or_slot_rc(s, &rr, &cc);
+            r = h->node[rr];
+            c = h->node[cc];
+            if (r && c) AX(r, c) += h->stamp[s];
+        }
+        for (s = 0; s < N__COUNT; s++) {
+            int r = h->node[s];
+            if (r) T->rhs[r] += h->rhs[s];
+        }
+    }
+    /* resload.c:16-38 */
+    for (k = 0; k < T->nres; k++) {
+        const int p = T->res_p[k], q = T->res_n[k];
+        const double g = T->res_g[k];
+        AX(p, p) += g;
+        AX(q, q) += g;
+        AX(p, q) -= g;
+        AX(q, p) -= g;
+    }
+    /* vsrcload.c:40-60 + value */
+    for (k = 0; k < T->nvs; k++) {
+        const ORVsrc *v = &T->vs[k];
+        AX(v->p, v->br) += 1.0;
+        AX(v->n, v->br) -= 1.0;
+        AX(v->br, v->p) += 1.0;
+        AX(v->br, v->n) -= 1.0;
+        T->rhs[v->br] += vsrc_value(T, v, mode);
+    }
+    /* .ic rows in the transient operating point (cktload.c:146-173) */
+    if ((mode & (OR_MODEDCOP | OR_MODETRANOP | OR_MODEDCTRANCURVE)) &&
+        (mode & OR_MODETRANOP) && !(mode & OR_MODEUIC)) {
+        int node;
+        for (node = 1; node <= T->nnodes; node++) {
+            int c, currents = 0;
+            if (!T->ic_given[node]) continue;
+            for (c = 1; c <= n; c++) {
+                if (c > T->nnodes) {
+                    if (AX(node, c) != 0.0) currents = 1;
+                } else {
+                    AX(node, c) = 0.0;
+                }
+            }
+            if (currents) {
+                T->rhs[node] = 1.0e10 * T->ic[node];
+                AX(node, node) += 1.0e10;
+            } else {
+                T->rhs[node] = T->ic[node];
+                AX(node, node) = 1.0;
+            }
+        }
+    }
+    return 0;
+}
+
+/* dense LU with partial pivoting on rows/cols 1..n; solution into rhs */
+static int solve(ORTran *T)
+{
+    const int n = T->n, ld = n + 1;
+    double *a = T->lu;
+    int i, j, k;
+    memcpy(a, T->A, sizeof(double) * (size_t)ld * ld);
+    for (k = 1; k <= n; k++) {
+        int p = k;
+        double mx = fabs(a[(size_t)k * ld + k]);
+        for (i = k + 1; i <= n; i++) {
+            double v = fabs(a[(size_t)i * ld + k]);
+            if (v > mx) { mx = v; p = i; }
+        }
+        if (mx == 0.0) return 102;
+        T->piv[k] = p;
+        if (p != k) {
+            for (j = 0; j <= n; j++) {
+                double t = a[(size_t)k * ld + j];
+                a[(size_t)k * ld + j] = a[(size_t)p * ld + j];
+                a[(size_t)p * ld + j] = t;
+            }
+            { double t = T->rhs[k]; T->rhs[k] = T->rhs[p]; T->rhs[p] = t; }
+        }
+        for (i = k + 1; i <= n; i++) {
+            double l = a[(size_t)i * ld + k] / a[(size_t)k * ld + k];
+            if (l == 0.0) continue;
+            a[(size_t)i * ld + k] = l;
+            for (j = k + 1; j <= n; j++) a[(size_t)i * ld + j] -= l * a[(size_t)k * ld + j];
+            T->rhs[i] -= l * T->rhs[k];
+        }
+    }
+    for (k = n; k >= 1; k--) {
+        double s = T->rhs[k];
+        for (j = k + 1; j <= n; j++) s -= a[(size_t)k * ld + j] * T->rhs[j];
+        T->rhs[k] = s / a[(size_t)k * ld + k];
+    }
+    T->rhs[0] = 0.0;
+    return 0;
+}
+
+/* niconv.c:21-90 */
+static int conv_test(const ORTran *T)
+{
+    int i;
+    for (i = 1; i <= T->n; i++) {
+        const double nw = T->rhs[i], old = T->rhsOld[i];
+        double tol;
+        if (isnan(nw)) return 1;
+        if (i <= T->nnodes) tol = T->reltol * fmax(fabs(old), fabs(nw)) + T->vntol;
+        else tol = T->reltol * fmax(fabs(old), fabs(nw)) + T->abstol;
+        if (fabs(nw - old) > tol) return 1;
+    }
+    return 0;
+}
+
+/* niiter.c:28-367 (no nodesets, no node damping) */
+static int ni_iter(ORTran *T, int maxIter)
+{
+    ORCkt *ck = &T->ck;
+    int iterno = 0;
+    if (maxIter < 100) maxIter = 100;
+    for (;;) {
+        int rc;
+        ck->noncon = 0;
+        if (ckt_load(T)) return 1;
+        iterno++;
+        rc = solve(T);
+        T->niter++;
+        if (rc) return rc;
+        if (iterno > maxIter) return 103;
+        if (ck->noncon == 0 && iterno != 1) ck->noncon = conv_test(T);
+        else ck->noncon = 1;
+        if (ck->mode & OR_MODEINITFLOAT) {
+            /* converged: CKTrhs holds the last solve, CKTrhsOld the iterate
+             * before it (niiter.c:300-305 return before the swap); CKTdump
+             * writes CKTrhsOld (cktdump.c:43) */
+            if (ck->noncon == 0) return 0;
+        } else if (ck->mode & OR_MODEINITJCT) {
+            ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFIX;
+        } else if (ck->mode & OR_MODEINITFIX) {
+            if (ck->noncon == 0) ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
+        } else if (ck->mode & OR_MODEINITSMSIG) {
+            ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
+        } else if (ck->mode & OR_MODEINITTRAN) {
+            ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
+        } else if (ck->mode & OR_MODEINITPRED) {
+            ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
+        } else {
+            return 1;
+        }
+        { double *t = T->rhs; T->rhs = T->rhsOld; T->rhsOld = t; }
+    }
+}
+
+/* nicomcof.c:17-130 */
+static int com_cof(ORTran *T)
+{
+    ORCkt *ck = &T->ck;
+    double mat[8][8], arg, arg1;
+    int i, j, k;
+    if (ck->method == OR_TRAPEZOIDAL) {
+        if (ck->order == 1) {
+            ck->ag[0] = 1 / ck->delta;
+            ck->ag[1] = -1 / ck->delta;
+        } else if (ck->order == 2) {
+            const double xmu = 0.5;
+            ck->ag[0] = 1.0 / ck->delta / (1.0 - xmu);
+            ck->ag[1] = xmu / (1.0 - xmu);
+        } else return -1;
+        return 0;
+    }
+    if (ck->order < 1 || ck->order > 6) return -1;
+    memset(ck->ag, 0, 7 * sizeof(double));
+    ck->ag[1] = -1 / ck->delta;
+    arg = 0;
+    for (i = 0; i <= ck->order; i++) mat[0][i] = 1;
+    for (i = 1; i <= ck->order; i++) mat[i][0] = 0;
+    for (i = 1; i <= ck->order; i++) {
+        arg += ck->deltaOld[i - 1];
+        arg1 = 1;
+        for (j = 1; j <= ck->order; j++) {
+            arg1 *= arg / ck->delta;
+            mat[j][i] = arg1;
+        }
+    }
+    for (i = 1; i <= ck->order; i++)
+        for (j = i + 1; j <= ck->order; j++) {
+            mat[j][i] /= mat[i][i];
+            for (k = i + 1; k <= ck->order; k++) mat[j][k] -= mat[j][i] * mat[i][k];
+        }
+    for (i = 1; i <= ck->order; i++)
+        for (j = i + 1; j <= ck->order; j++) ck->ag[j] = ck->ag[j] - mat[j][i] * ck->ag[i];
+    ck->ag[ck->order] /= mat[ck->order][ck->order];
+    for (i = ck->order - 1; i >= 0; i--) {
+        for (j = i + 1; j <= ck->order; j++) ck->ag[i] = ck->ag[i] - mat[i][j] * ck->ag[j];
+        ck->ag[i] /= mat[i][i];
+    }
+    return 0;
+}
+
+/* ckttrunc.c:20-56 with CAPtrunc and BSIM4trunc */
+static void ckt_trunc(ORTran *T, double *timeStep)
+{
+    double timetemp = 1e308;
+    int k;
+    for (k = 0; k < T->ncap; k++) or_terr(T->cap_q[k], &T->ck, &timetemp, T->trtol, T->chgtol);
+    for (k = 0; k < T->nmos; k++) or_bsim4_trunc(T->mos[k], &T->ck, &timetemp, T->trtol, T->chgtol);
+    *timeStep = fmin(2 * *timeStep, timetemp);
+}
+
+static int dump(ORTran *T)
+{
+    if (T->npts == T->cap_pts) {
+        int64_t nc = T->cap_pts ? 2 * T->cap_pts : 1024;
+        double *t = (double *)realloc(T->out_t, sizeof(double) * nc);
+        double *x = (double *)realloc(T->out_x, sizeof(double) * nc * (T->n + 1));
+        if (!t || !x) return -1;
+        T->out_t = t;
+        T->out_x = x;
+        T->cap_pts = nc;
+    }
+    T->out_t[T->npts] = T->time;
+    memcpy(T->out_x + T->npts * (T->n + 1), T->rhsOld, sizeof(double) * (T->n + 1));
+    T->npts++;
+    return 0;
+}
+
+static void rotate_states(ORTran *T)
+{
+    double *temp = T->ck.states[T->maxOrder + 1];
+    int i;
+    for (i = T->maxOrder; i >= 0; i--) T->ck.states[i + 1] = T->ck.states[i];
+    T->ck.states[0] = temp;
+}
+
+/* traninit.c + dctran.c (XSPICE build, no event instances) */
+int or_tran_run(ORTran *T, double tstep, double tstop, double tmax)
+{
+    ORCkt *ck = &T->ck;
+    double delta, olddelta, newdelta;
+    int firsttime, converged, i;
+    const size_t ns = (size_t)T->nstates;
+    T->finalTime = tstop;
+    T->step = tstep;
+    T->maxStep = tmax;
+    if (T->maxStep == 0) {
+        if (T->step < T->finalTime / 50.0) T->maxStep = T->step;
+        else T->maxStep = T->finalTime / 50.0;
+    }
+    T->delmin = 1e-11 * T->maxStep;
+    delta = fmin(T->finalTime / 100, T->step) / 10;
+    T->breaks[0] = 0;
+    T->breaks[1] = T->finalTime;
+    T->nbreaks = 2;
+    T->minBreak = 10.0 * T->delmin;
+    T->time = 0;
+    ck->delta = 0;
+    firsttime = 1;
+    ck->method = T->method;
+    ck->order = 1;
+    for (i = 0; i < T->nvs; i++) T->vs[i].break_time = -1.0;
+    /* CKTop: direct Newton from INITJCT */
+    ck->mode = (T->uic ? OR_MODEUIC : 0) | OR_MODETRANOP | OR_MODEINITJCT;
+    memset(T->rhsOld, 0, sizeof(double) * (T->n + 1));
+    converged = ni_iter(T, 100);
+    if (converged) { T->noncon_dc_fail = 1; return converged; }
+    ck->order = 1;
+    for (i = 0; i < 7; i++) ck->deltaOld[i] = T->maxStep;
+    ck->delta = delta;
+    T->delta = delta;
+    T->saveDelta = T->finalTime / 50;
+    ck->mode = (T->uic ? OR_MODEUIC : 0) | OR_MODETRAN | OR_MODEINITTRAN;
+    ck->ag[0] = ck->ag[1] = 0;
+    memcpy(ck->states[1], ck->states[0], ns * sizeof(double));
+    for (;;) {
+        /* nextTime: */
+        if (vsrc_accept(T, ck->mode)) return 1;
+        if (T->time > T->breaks[0]) clr_break(T);
+        T->accepted++;
+        if (dump(T)) return 8;
+        if (almost_equal_ulps(T->time, T->finalTime, 100)) return 0;
+        /* resume: */
+        T->delta = fmin(T->delta, T->maxStep);
+        if (almost_equal_ulps(T->time, T->breaks[0], 100) || T->breaks[0] - T->time <= T->delmin) {
+            ck->order = 1;
+            T->delta = fmin(T->delta, .1 * fmin(T->saveDelta, T->breaks[1] - T->breaks[0]));
+            if (firsttime) T->delta /= 10;
+        }
+        while ((T->breaks[0] <= T->time + T->minBreak ||
+                almost_equal_ulps(T->breaks[0], T->time, 100)) &&
+               T->breaks[0] < T->finalTime)
+            clr_break(T);
+        if (T->time + T->delta > T->breaks[0]) {
+            T->saveDelta = T->delta;
+            T->delta = T->breaks[0] - T->time;
+        } else if (T->time + 1.9 * T->delta > T->breaks[0]) {
+            T->saveDelta = T->delta;
+            T->delta = (T->breaks[0] - T->time) / 2.;
+        }
+        for (i = 5; i >= 0; i--) ck->deltaOld[i + 1] = ck->deltaOld[i];
+        ck->deltaOld[0] = T->delta;
+        rotate_states(T);
+        for (;;) {
+            int accept = 0;
+            olddelta = T->delta;
+            T->time += T->delta;
+            ck->delta = T->delta;
+            ck->deltaOld[0] = T->delta;
+            if (com_cof(T)) return 1;
+            converged = ni_iter(T, 10);
+            ck->mode = (T->uic ? OR_MODEUIC : 0) | OR_MODETRAN | OR_MODEINITPRED;
+            if (firsttime) {
+                memcpy(ck->states[2], ck->states[1], ns * sizeof(double));
+                memcpy(ck->states[3], ck->states[1], ns * sizeof(double));
+            }
+            if (converged != 0) {
+                T->time = T->time - T->delta;
+                T->rejected++;
+                T->delta = T->delta / 8;
+                if (firsttime) ck->mode = (T->uic ? OR_MODEUIC : 0) | OR_MODETRAN | OR_MODEINITTRAN;
+                ck->order = 1;
+            } else {
+                if (firsttime) {
+                    firsttime = 0;
+                    accept = 1;
+                } else {
+                    newdelta = T->delta;
+                    ckt_trunc(T, &newdelta);
+                    if (newdelta > .9 * T->delta) {
+                        if (ck->order == 1 && T->maxOrder > 1) {
+                            newdelta = T->delta;
+                            ck->order = 2;
+                            ckt_trunc(T, &newdelta);
+                            if (newdelta <= 1.05 * T->delta) ck->order = 1;
+                        }
+                        T->delta = newdelta;
+                        accept = 1;
+                    } else {
+                        T->time = T->time - T->delta;
+                        T->rejected++;
+                        T->delta = newdelta;
+                    }
+                }
+            }
+            if (accept) break;
+            if (T->delta <= T->delmin) {
+                if (olddelta > T->delmin) T->delta = T->delmin;
+                else return 106;
+            }
+        }
+    }
+}
+
+/* ---- construction API (ctypes) ---- */
+
+ORTran *or_tran_new(int nnodes, int nmos, int ncap, int nres, int nvs)
+{
+    ORTran *T = (ORTran *)calloc(1, sizeof(ORTran));
+    if (!T) return NULL;
+    T->nnodes = nnodes;
+    T->mos = (ORInst **)calloc((size_t)nmos + 1, sizeof(ORInst *));
+    T->cap_p = (int *)calloc((size_t)ncap + 1, sizeof(int));
+    T->cap_n = (int *)calloc((size_t)ncap + 1, sizeof(int));
+    T->cap_q = (int *)calloc((size_t)ncap + 1, sizeof(int));
+    T->cap_c = (double *)calloc((size_t)ncap + 1, sizeof(double));
+    T->res_p = (int *)calloc((size_t)nres + 1, sizeof(int));
+    T->res_n = (int *)calloc((size_t)nres + 1, sizeof(int));
+    T->res_g = (double *)calloc((size_t)nres + 1, sizeof(double));
+    T->vs = (ORVsrc *)calloc((size_t)nvs + 1, sizeof(ORVsrc));
+    T->ic_given = (int *)calloc((size_t)nnodes + 1, sizeof(int));
+    T->ic = (double *)calloc((size_t)nnodes + 1, sizeof(double));
+    T->reltol = 1e-3; T->abstol = 1e-12; T->vntol = 1e-6; T->chgtol = 1e-14;
+    T->trtol = 7; T->gmin = 1e-12; T->temp = 300.15; T->maxOrder = 2;
+    T->method = OR_TRAPEZOIDAL;
+    return T;
+}
+
+int or_tran_set_options(ORTran *T, double reltol, double abstol, double vntol, double chgtol,
+                        double trtol, double gmin, int method, int maxord, int uic)
+{
+    T->reltol = reltol; T->abstol = abstol; T->vntol = vntol; T->chgtol = chgtol;
+    T->trtol = trtol; T->gmin = gmin; T->method = method; T->maxOrder = maxord; T->uic = uic;
+    return 0;
+}
+
+/* the instance must be set up (defaults) but not yet temperature-resolved */
+int or_tran_add_mos(ORTran *T, ORInst *h, int d, int g, int s, int b)
+{
+    int bits = or_bsim4_internal_nodes(h);
+    if (bits) return -1;
+    h->node[N_d] = d; h->node[N_g] = g; h->node[N_s] = s; h->node[N_b] = b;
+    h->node[N_dp] = d; h->node[N_gp] = g; h->node[N_gm] = g; h->node[N_sp] = s;
+    h->node[N_db] = h->node[N_bp] = h->node[N_sb] = b;
+    h->node[N_q] = 0;
+    h->state = T->nstates;
+    T->nstates += S__COUNT;
+    T->mos[T->nmos++] = h;
+    return 0;
+}
+
+int or_tran_add_cap(ORTran *T, int p, int n, double c)
+{
+    T->cap_p[T->ncap] = p; T->cap_n[T->ncap] = n; T->cap_c[T->ncap] = c;
+    T->cap_q[T->ncap] = T->nstates;
+    T->nstates += 2;
+    T->ncap++;
+    return 0;
+}
+
+int or_tran_add_res(ORTran *T, int p, int n, double r)
+{
+    T->res_p[T->nres] = p; T->res_n[T->nres] = n; T->res_g[T->nres] = 1.0 / r;
+    T->nres++;
+    return 0;
+}
+
+/* pulse: NULL for DC, else npulse coefficients (v1 v2 td tr tf pw per) */
+int or_tran_add_vsrc(ORTran *T, int p, int n, double dc, const double *pulse, int npulse)
+{
+    ORVsrc *v = &T->vs[T->nvs];
+    int k;
+    v->p = p; v->n = n; v->dc = dc;
+    v->pulse = pulse != NULL;
+    v->order = npulse;
+    for (k = 0; k < npulse && k < 7; k++) v->c[k] = pulse[k];
+    v->br = -1;
+    T->nvs++;
+    return 0;
+}
+
+int or_tran_set_ic(ORTran *T, int node, double v)
+{
+    if (node < 1 || node > T->nnodes) return -1;
+    T->ic_given[node] = 1;
+    T->ic[node] = v;
+    return 0;
+}
+
+/* branch numbering, state arrays, instance temperature; returns 0 */
+int or_tran_setup(ORTran *T)
+{
+    int k;
+    T->nbr = T->nvs;
+    T->n = T->nnodes + T->nbr;
+    for (k = 0; k < T->nvs; k++) T->vs[k].br = T->nnodes + 1 + k;
+    T->stbuf = (double *)calloc((size_t)4 * (T->nstates + 1), sizeof(double));
+    for (k = 0; k < 4; k++) T->states[k] = T->stbuf + (size_t)k * (T->nstates + 1);
+    for (k = 0; k < 4; k++) T->ck.states[k] = T->states[k];
+    T->A = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
+    T->lu = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
+    T->piv = (int *)calloc((size_t)T->n + 1, sizeof(int));
+    T->rhs = (double *)calloc((size_t)T->n + 1, sizeof(double));
+    T->rhsOld = (double *)calloc((size_t)T->n + 1, sizeof(double));
+    T->ck.gmin = T->gmin; T->ck.reltol = T->reltol; T->ck.abstol = T->abstol;
+    T->ck.voltTol = T->vntol; T->ck.temp = T->temp;
+    for (k = 0; k < T->nmos; k++)
+        if (or_bsim4_inst_temp(T->mos[k], T->temp)) return -1;
+    return (T->A && T->lu && T->rhs && T->rhsOld && T->stbuf) ? 0 : -1;
+}
+
+int64_t or_tran_npts(const ORTran *T) { return T->npts; }
+int or_tran_n(const ORTran *T) { return T->n; }
+void or_tran_stats(const ORTran *T, int64_t *acc, int64_t *rej, int64_t *iters)
+{
+    *acc = T->accepted; *rej = T->rejected; *iters = T->niter;
+}
+void or_tran_output(const ORTran *T, double *t, double *x)
+{
+    memcpy(t, T->out_t, sizeof(double) * T->npts);
+    memcpy(x, T->out_x, sizeof(double) * T->npts * (T->n + 1));
+}
+
+void or_tran_free(ORTran *T)
+{
+    if (!T) return;
+    free(T->mos); free(T->cap_p); free(T->cap_n); free(T->cap_q); free(T->cap_c);
+    free(T->res_p); free(T->res_n); free(T->res_g); free(T->vs); free(T->ic_given);
+    free(T->ic); free(T->stbuf); free(T->A); free(T->lu); free(T->piv); free(T->rhs);
+    free(T->rhsOld); free(T->out_t); free(T->out_x);
+    free(T);
+}

@@ -1,0 +1,96 @@
+"""GPU parity of the batched transient engine against the oracle's DCtran
+restatement.
+
+Bar: the engine accepts the same timepoints as the oracle (times equal to
+1e-9 relative) and every recorded node voltage / branch current agrees
+within 1e-6 of the waveform's full scale (1 V for node voltages; the
+largest magnitude for branch currents), the tolerance stated for waveforms
+in the engine's parity contract.
+"""
+import numpy as np
+import pytest
+
+from engine_path import ngmi, workloads
+import sys
+import os
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), '..', 'oracle'))
+import pyoracle as po  # noqa: E402
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+WAVE_TOL = 1e-6
+
+
+def _oracle(desc, tstop):
+    T = po.Transient(desc)
+    rc, t, x = T.run(desc['tstep'], tstop)
+    assert rc == 0, rc
+    return t, x, T.stats()
+
+
+def _engine(descs, tstop):
+    circ = [ngmi.Circuit(d) for d in descs]
+    b = ngmi.Batch(circ, max_points=200000)
+    rc, passes = b.tran(descs[0]['tstep'], tstop)
+    assert rc == 0, (rc, ngmi.last_error())
+    n = b.info()['unknowns']
+    out = []
+    for c in range(len(descs)):
+        st = b.stats(c)
+        assert st['status'] == 0, st
+        t, x = b.waveform(c, n)
+        out.append((t, x, st))
+    return out, b.info(), passes
+
+
+def _compare(tag, eng, ora, nn):
+    te, xe, se = eng
+    to, xo, so = ora
+    print('%s: engine %d pts (%d rejected, %d iters), oracle %d pts (%d rejected, %d iters)'
+          % (tag, len(te), se['rejected'], se['iterations'], len(to), so['rejected'],
+             so['iterations']), flush=True)
+    assert len(te) == len(to), (len(te), len(to))
+    assert np.allclose(te, to, rtol=1e-9, atol=1e-21)
+    # oracle x has a ground column 0; engine records unknowns 1..n
+    xo = xo[:, 1:]
+    scale = np.ones(xo.shape[1])
+    scale[nn:] = np.maximum(np.abs(xo[:, nn:]).max(axis=0), 1e-30)
+    err = (np.abs(xe - xo) / scale).max()
+    print('%s: max scaled waveform error %.3e' % (tag, err), flush=True)
+    assert err <= WAVE_TOL
+
+
+def test_rc_step_against_oracle_and_closed_form():
+    desc = {'nnodes': 2, 'models': {}, 'mos': [], 'caps': [(2, 0, 1.0)], 'res': [(1, 2, 1.0)],
+            'vsrc': [(1, 0, 0.0, (0, 1, 0, 0.1, 0.1, 7, 7))], 'ic': {}, 'tstep': 0.1,
+            'tstop': 7.0}
+    print('case: RC step, .tran 0.1 7', flush=True)
+    (eng,), info, _ = _engine([desc], 7.0)
+    ora = _oracle(desc, 7.0)
+    _compare('rc', eng, ora, 2)
+    t, x, _ = eng
+    a = (0.1 - (1 - np.exp(-0.1))) / 0.1
+    ref = np.where(t <= 0.1, (t - (1 - np.exp(-t))) / 0.1, 1 - (1 - a) * np.exp(-(t - 0.1)))
+    assert np.abs(x[:, 1] - ref).max() < 1e-3
+
+
+def test_small_ring_oscillator():
+    desc = workloads.ring_oscillator(stages=11, tstop=2e-9)
+    print('case: 11-stage ring, .tran 10p 2n', flush=True)
+    (eng,), info, _ = _engine([desc], 2e-9)
+    print('lu', info, flush=True)
+    ora = _oracle(desc, 2e-9)
+    _compare('ring11', eng, ora, desc['nnodes'])
+
+
+def test_adder_batch():
+    descs = [workloads.mc_adder(s) for s in (0, 1, 2)]
+    tstop = 2.5e-9
+    print('case: 16-bit mirror adder, seeds 0..2, .tran 5p 2.5n', flush=True)
+    engs, info, passes = _engine(descs, tstop)
+    print('lu', info, 'passes', passes, flush=True)
+    for s, d in enumerate(descs):
+        ora = _oracle(d, tstop)
+        _compare('adder%d' % s, engs[s], ora, d['nnodes'])
