@@ -111,15 +111,124 @@ def bench_eval(args):
     }
 
 
+def cpu_baseline_mc(tstop, budget_s=12.0):
+    """The oracle's DCtran restatement on whole adder circuits (seeds 0, 1,
+    ...), one host core, solving with the reference's own KLU when
+    oracle/_ref was built; stops after the first circuit that ends past the
+    time budget."""
+    sys.path.insert(0, os.path.join(REPO, 'oracle'))
+    import pyoracle as po
+    import workloads
+    pts, secs, seeds, solver = 0, 0.0, 0, 'dense'
+    while secs < budget_s and seeds < 64:
+        d = workloads.mc_adder(seeds)
+        T = po.Transient(d)
+        solver = 'klu' if T.use_reference_klu() else 'dense'
+        t0 = time.perf_counter()
+        rc, t, _ = T.run(d['tstep'], tstop)
+        secs += time.perf_counter() - t0
+        if rc != 0:
+            break
+        pts += len(t)
+        seeds += 1
+    return {'value': pts / secs if secs > 0 else None, 'unit': 'timepoints x circuits/s',
+            'cores': 1, 'kind': 'port',
+            'sample': '%d adder circuit(s) (seeds 0..%d), full .tran 5p %gn, oracle DCtran '
+                      'restatement with %s linear solve, %d accepted points in %.1f s'
+                      % (seeds, seeds - 1, tstop * 1e9,
+                         'the reference KLU' if solver == 'klu' else 'dense LU', pts, secs)}
+
+
+def bench_mc(args):
+    import torch
+    import ngmi
+    import workloads
+
+    C = args.circuits
+    tstop = args.tstop
+    t0 = time.time()
+    descs = [workloads.mc_adder(s, tstop=tstop) for s in range(C)]
+    names = descs[0]['names']
+    probe = [names['s%d' % i] for i in range(workloads.ADDER_BITS)] + \
+        [names['co%d' % (workloads.ADDER_BITS - 1)]]
+    circ = [ngmi.Circuit(d) for d in descs]
+    del descs
+    b = ngmi.Batch(circ, probe=probe, max_points=args.max_points)
+    del circ
+    info = b.info()
+    icols, mcols = b.table_cols()
+    log('setup %.1fs: %d circuits, %s, table cols %d/%d' % (time.time() - t0, C, info, icols,
+                                                            mcols))
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(args.warmup):
+        rc, passes = b.tran(workloads.ADDER_TSTEP, tstop, stream=stream)
+        log('warmup rc %d passes %d' % (rc, passes))
+    b.set_profile(True)
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        rc, passes = b.tran(workloads.ADDER_TSTEP, tstop, stream=stream)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - w0)
+        if rc != 0:
+            raise RuntimeError('transient failed: %d %s' % (rc, ngmi.last_error()))
+    wall = float(np.mean(times))
+    st = [b.stats(c) for c in range(C)]
+    failed = sum(1 for s in st if s['status'] != 0)
+    acc = sum(s['accepted'] for s in st)
+    rej = sum(s['rejected'] for s in st)
+    its = sum(s['iterations'] for s in st)
+    mos_ms, step_ms, evals, npasses = b.kernel_times()
+    # algorithmic bytes of one BSIM4 evaluation in the transient kernel: the
+    # per-instance table columns, 4 node voltages, state reads (4 terminal
+    # voltages, 3 charges and 3 charge currents of the previous level), 21
+    # state writes, 38 cached values read and written, 16 + 4 outputs, the
+    # convergence vote and the model index
+    b_eval = 8 * (icols + 4 + 4 + 6 + 21 + 38 + 38 + 20) + 4 + 4
+    achieved = evals * b_eval / (mos_ms * 1e-3) / 1e9 if mos_ms > 0 else None
+    log('passes %d, accepted %d, rejected %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
+        'step %.1f ms, wall %.1f ms' % (passes, acc, rej, its, failed, mos_ms, step_ms,
+                                        wall * 1e3))
+    return {
+        'metric': 'MC transient timepoints x circuits/s at reltol 1e-3 (plain vs learned NR guess)',
+        'value': acc / wall,
+        'unit': 'timepoints x circuits/s',
+        'ms_per_step': wall * 1e3,
+        'dtype': 'f64',
+        'config': {'workload': '%d Monte Carlo 16-bit mirror-adder RCAs (448 BSIM4 each), '
+                               '.tran 5p %gn, reltol 1e-3, plain predictor' % (C, tstop * 1e9),
+                   'circuits': C, 'unknowns': info['unknowns'],
+                   'lu_nonzeros': info['lu_nonzeros']},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': None,
+                     'kernel': 'k_tran_mos', 'kernel_ms_total': mos_ms,
+                     'launches': passes, 'bytes_per_eval': b_eval, 'evals': evals},
+        'extra': {'accepted': acc, 'rejected': rej, 'newton_iterations': its,
+                  'iterations_per_point': its / max(acc, 1), 'passes': passes,
+                  'failed_circuits': failed, 'step_kernel_ms_total': step_ms,
+                  'bsim4_kernel_ms_total': mos_ms},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='eval', choices=['eval'])
+    ap.add_argument('--steps', type=int, default=None,
+                    help='timed steps (mc: whole .tran runs of the batch; default 1)')
+    ap.add_argument('--warmup', type=int, default=None,
+                    help='untimed steps before (mc default 1)')
+    ap.add_argument('--workload', default='mc', choices=['mc', 'eval'])
     ap.add_argument('--n', type=int, default=1 << 20)
+    ap.add_argument('--circuits', type=int, default=4096)
+    ap.add_argument('--tstop', type=float, default=20e-9)
+    ap.add_argument('--max-points', type=int, default=8192)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 1 if args.workload == 'mc' else 20
+    if args.warmup is None:
+        args.warmup = 1 if args.workload == 'mc' else 3
     if args.gpus != 1:
         print('bench.py runs on one GPU only (the engine has no multi-GPU path); '
               'got --gpus %d' % args.gpus, file=sys.stderr)
@@ -128,12 +237,17 @@ def main():
     if not torch.cuda.is_available():
         print('bench.py needs a GPU', file=sys.stderr)
         sys.exit(2)
-    res = bench_eval(args)
+    if args.workload == 'mc':
+        res = bench_mc(args)
+        cpu = None if args.no_cpu_baseline else cpu_baseline_mc(args.tstop)
+    else:
+        res = bench_eval(args)
+        cpu = None
     out = {'metric': res['metric'], 'value': res['value'], 'unit': res['unit'], 'n_gpus': 1,
            'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': res['ms_per_step'],
            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
            'dtype': res['dtype'], 'data': 'synthetic (seeded Philox)', 'config': res['config'],
-           'roofline': res['roofline'], 'cpu_baseline': None, 'extra': res['extra']}
+           'roofline': res['roofline'], 'cpu_baseline': cpu, 'extra': res['extra']}
     print(json.dumps(out))
 
 

@@ -148,6 +148,11 @@ void ngmi_circuit_free(ngmi_circuit *c);
 /* one BSIM4 instance (BSIM4param, b4par.c:22, for its instance parameters) */
 int ngmi_circuit_add_bsim4(ngmi_circuit *c, ngmi_bsim4_model *m, int d, int g, int s, int b,
                            int nparams, const char *const *names, const double *values);
+/* n instances at once: instance k uses models[model_idx[k]], terminals
+ * nodes[4k..4k+3] (d g s b) and parameter j value values[j*n + k] */
+int ngmi_circuit_add_bsim4_many(ngmi_circuit *c, ngmi_bsim4_model *const *models,
+                                const int *model_idx, const int *nodes, int n, int nparams,
+                                const char *const *names, const double *values);
 /* capacitor (CAPload, capload.c:19-86), resistor (RESload, resload.c:16-38) */
 int ngmi_circuit_add_cap(ngmi_circuit *c, int p, int n, double farad);
 int ngmi_circuit_add_res(ngmi_circuit *c, int p, int n, double ohm);
@@ -177,6 +182,14 @@ int ngmi_batch_info(const ngmi_batch *b, int *n_unknowns, int *nslots, int *nter
  * synchronised before return. */
 int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int64_t max_passes,
                     void *stream, int64_t *passes);
+/* per-instance and per-model BSIM4 table columns of the batch */
+int ngmi_batch_table_cols(const ngmi_batch *b, int *inst_cols, int *model_cols);
+/* record HIP events around every kernel of every pass of ngmi_batch_tran
+ * (on its stream) and report the summed kernel milliseconds, the BSIM4
+ * evaluations and the per-circuit Newton passes of the last run */
+int ngmi_batch_set_profile(ngmi_batch *b, int enable);
+int ngmi_batch_kernel_times(const ngmi_batch *b, double *mos_ms, double *step_ms,
+                            unsigned long long *mos_evals, unsigned long long *newton_passes);
 /* per circuit: status (0 finished, -1 running, else the error number),
  * recorded points, accepted / rejected timepoints, Newton iterations */
 int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int64_t *accepted,

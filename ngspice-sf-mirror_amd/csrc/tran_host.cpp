@@ -31,7 +31,8 @@
 #include "tran_types.h"
 
 namespace ngmi {
-int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s);
+int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s,
+                     hipEvent_t *ev);
 int set_tran_lds_limit(size_t bytes);
 const B4Tab *set_table(const ngmi_bsim4_set *s);
 }
@@ -75,6 +76,10 @@ struct ngmi_batch {
     std::vector<double> ic_host;
     std::vector<char> ic_nodes_host;
     int method = 1, maxord = 2;
+    /* kernel timing of the last run (events on the launch stream) */
+    bool profile = false;
+    double mos_ms = 0, step_ms = 0;
+    unsigned long long counters[2] = {0, 0};
     ~ngmi_batch()
     {
         for (void *p : allocs) (void)hipFree(p);
@@ -641,6 +646,9 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
     D.out_x = dp;
     if (alloc(b, 1, &ip) != NGMI_OK) return bail(nullptr);
     D.done_count = ip;
+    unsigned long long *cp;
+    if (alloc(b, 2, &cp) != NGMI_OK) return bail(nullptr);
+    D.counters = cp;
     b->lds = sizeof(double) * ((size_t)nslots + N + 2 * (size_t)NCAP + NVS);
     if (b->lds > 150 * 1024) return bail("circuit too large for the per-circuit LDS factorisation");
     if (b->lds > 64 * 1024 && set_tran_lds_limit(b->lds) != NGMI_OK) return bail(nullptr);
@@ -706,6 +714,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     NGMI_HIP(hipMemsetAsync(D.cstate, 0, sizeof(double) * NLEV * 2 * (size_t)cnc, s));
     NGMI_HIP(hipMemsetAsync(D.cache, 0, sizeof(double) * B4C__COUNT * (size_t)cnm, s));
     NGMI_HIP(hipMemsetAsync(D.done_count, 0, sizeof(int), s));
+    NGMI_HIP(hipMemsetAsync(D.counters, 0, 2 * sizeof(unsigned long long), s));
     {
         std::vector<double> brk((size_t)C * b->NVS, -1.0);
         NGMI_HIP(hipMemcpyAsync(D.vs_brk, brk.data(), brk.size() * sizeof(double),
@@ -716,17 +725,80 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     int64_t pass = 0;
     int done = 0;
     const int check_every = 32;
-    while (pass < max_passes) {
-        for (int k = 0; k < check_every && pass < max_passes; k++, pass++) {
-            const int rc = launch_tran_pass(*tab, D, b->lds, s);
-            if (rc != NGMI_OK) return rc;
+    std::vector<hipEvent_t> ev;
+    if (b->profile) {
+        ev.resize(3 * check_every);
+        for (auto &e : ev) NGMI_HIP(hipEventCreate(&e));
+    }
+    b->mos_ms = b->step_ms = 0;
+    int rc = NGMI_OK;
+    while (pass < max_passes && rc == NGMI_OK) {
+        int k = 0;
+        for (; k < check_every && pass < max_passes; k++, pass++) {
+            rc = launch_tran_pass(*tab, D, b->lds, s, b->profile ? &ev[3 * k] : nullptr);
+            if (rc != NGMI_OK) break;
         }
-        NGMI_HIP(hipMemcpyAsync(&done, D.done_count, sizeof(int), hipMemcpyDeviceToHost, s));
-        NGMI_HIP(hipStreamSynchronize(s));
+        if (hipMemcpyAsync(&done, D.done_count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            set_error("transient pass failed: %s", hipGetErrorString(hipGetLastError()));
+            rc = NGMI_E_DEVICE;
+        }
+        if (b->profile && rc == NGMI_OK) {
+            for (int q = 0; q < k; q++) {
+                float a = 0, c2 = 0;
+                (void)hipEventElapsedTime(&a, ev[3 * q], ev[3 * q + 1]);
+                (void)hipEventElapsedTime(&c2, ev[3 * q + 1], ev[3 * q + 2]);
+                b->mos_ms += a;
+                b->step_ms += c2;
+            }
+        }
         if (done >= C) break;
     }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    NGMI_HIP(hipMemcpy(b->counters, D.counters, sizeof(b->counters), hipMemcpyDeviceToHost));
     if (passes_out) *passes_out = pass;
+    if (rc != NGMI_OK) return rc;
     return done >= C ? NGMI_OK : NGMI_E_ITERLIM;
+}
+
+int ngmi_batch_table_cols(const ngmi_batch *b, int *inst_cols, int *model_cols)
+{
+    if (!b) return NGMI_E_PANIC;
+    return ngmi_bsim4_set_layout(b->set, inst_cols, model_cols, nullptr);
+}
+
+int ngmi_batch_set_profile(ngmi_batch *b, int enable)
+{
+    if (!b) return NGMI_E_PANIC;
+    b->profile = enable != 0;
+    return NGMI_OK;
+}
+
+int ngmi_batch_kernel_times(const ngmi_batch *b, double *mos_ms, double *step_ms,
+                            unsigned long long *mos_evals, unsigned long long *newton_passes)
+{
+    if (!b) return NGMI_E_PANIC;
+    if (mos_ms) *mos_ms = b->mos_ms;
+    if (step_ms) *step_ms = b->step_ms;
+    if (mos_evals) *mos_evals = b->counters[0];
+    if (newton_passes) *newton_passes = b->counters[1];
+    return NGMI_OK;
+}
+
+int ngmi_circuit_add_bsim4_many(ngmi_circuit *c, ngmi_bsim4_model *const *models,
+                                const int *model_idx, const int *nodes, int n, int nparams,
+                                const char *const *names, const double *values)
+{
+    if (!c || !models || (n > 0 && (!model_idx || !nodes))) return NGMI_E_PANIC;
+    for (int k = 0; k < n; k++) {
+        std::vector<double> v(nparams);
+        for (int p = 0; p < nparams; p++) v[p] = values[(size_t)p * n + k];
+        const int rc = ngmi_circuit_add_bsim4(c, models[model_idx[k]], nodes[4 * k], nodes[4 * k + 1],
+                                              nodes[4 * k + 2], nodes[4 * k + 3], nparams, names,
+                                              v.data());
+        if (rc != NGMI_OK) return rc;
+    }
+    return NGMI_OK;
 }
 
 /* per-circuit results: status (0 ok, else error number), counts */

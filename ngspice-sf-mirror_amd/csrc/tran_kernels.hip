@@ -409,6 +409,10 @@ k_tran_step(TranDev D)
     double *xo = D.x + ((int64_t)xs * D.C + c) * (N + 1);
     double *xn = D.x + ((int64_t)(xs ^ 1) * D.C + c) * (N + 1);
     if (tid < 4) s_flag[tid] = 0;
+    if (tid == 0) {
+        atomicAdd(D.counters, (unsigned long long)D.NM);
+        atomicAdd(D.counters + 1, 1ull);
+    }
 
     /* capload.c:19-86 */
     for (int k = tid; k < D.NCAP; k += nt) {
@@ -728,15 +732,21 @@ int set_tran_lds_limit(size_t bytes)
     return NGMI_OK;
 }
 
-int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s)
+/* one Newton pass; ev (optional) = 3 events recorded before the MOSFET
+ * kernel, between the two kernels and after the step kernel */
+int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s,
+                     hipEvent_t *ev)
 {
     const int64_t cnm = (int64_t)D.C * D.NM;
+    if (ev) NGMI_HIP(hipEventRecord(ev[0], s));
     if (cnm > 0) {
         hipLaunchKernelGGL(k_tran_mos, dim3((unsigned)((cnm + 255) / 256)), dim3(256), 0, s, tab, D);
         NGMI_HIP(hipGetLastError());
     }
+    if (ev) NGMI_HIP(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(k_tran_step, dim3((unsigned)D.C), dim3(256), lds_bytes, s, D);
     NGMI_HIP(hipGetLastError());
+    if (ev) NGMI_HIP(hipEventRecord(ev[2], s));
     return NGMI_OK;
 }
 

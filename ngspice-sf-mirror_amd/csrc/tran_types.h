@@ -96,6 +96,7 @@ struct TranDev {
     double *out_t;            /* [C][cap_pts] */
     double *out_x;            /* [C][cap_pts][nprobe] */
     int *done_count;          /* circuits finished or failed */
+    unsigned long long *counters;   /* [0] MOSFET evaluations, [1] circuit Newton passes */
 };
 
 }  // namespace ngmi

@@ -284,6 +284,14 @@ def _tran_protos(L):
                                    ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.ngmi_batch_waveform.argtypes = [vp, i, ctypes.POINTER(d), ctypes.POINTER(d), i64,
                                       ctypes.POINTER(i64)]
+    L.ngmi_circuit_add_bsim4_many.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(i),
+                                              ctypes.POINTER(i), i, i, ctypes.POINTER(cp),
+                                              ctypes.POINTER(d)]
+    L.ngmi_batch_set_profile.argtypes = [vp, i]
+    L.ngmi_batch_table_cols.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+    u64 = ctypes.c_ulonglong
+    L.ngmi_batch_kernel_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d),
+                                          ctypes.POINTER(u64), ctypes.POINTER(u64)]
     L._tran_ready = True
     return L
 
@@ -304,11 +312,26 @@ class Circuit:
         check(L.ngmi_circuit_set_options(self.h, o['reltol'], o['abstol'], o['vntol'],
                                          o['chgtol'], o['trtol'], o['gmin'], o['method'],
                                          o['maxord']), 'options')
-        for key, dn, gn, sn, bn, ip in desc['mos']:
-            names = (ctypes.c_char_p * max(len(ip), 1))(*[k.encode() for k in ip])
-            vals = (ctypes.c_double * max(len(ip), 1))(*[float(v) for v in ip.values()])
-            check(L.ngmi_circuit_add_bsim4(self.h, self.models[key].h, dn, gn, sn, bn, len(ip),
-                                           names, vals), 'add bsim4')
+        mos = desc['mos']
+        if mos:
+            keys = list(self.models)
+            pnames = sorted({k for m in mos for k in m[5]})
+            n = len(mos)
+            midx = np.array([keys.index(m[0]) for m in mos], dtype=np.int32)
+            nodes = np.array([m[1:5] for m in mos], dtype=np.int32).reshape(-1)
+            vals = np.zeros((len(pnames), n))
+            for k, m in enumerate(mos):
+                for j, p in enumerate(pnames):
+                    if p not in m[5]:
+                        raise ValueError('every MOSFET of a circuit must give the same '
+                                         'instance parameters (%s missing)' % p)
+                    vals[j, k] = m[5][p]
+            mh = (ctypes.c_void_p * len(keys))(*[self.models[k].h for k in keys])
+            names = (ctypes.c_char_p * max(len(pnames), 1))(*[p.encode() for p in pnames])
+            check(L.ngmi_circuit_add_bsim4_many(
+                self.h, mh, midx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n, len(pnames), names,
+                vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), 'add bsim4')
         for p, n, cv in desc['caps']:
             check(L.ngmi_circuit_add_cap(self.h, p, n, cv), 'add cap')
         for p, n, r in desc.get('res', []):
@@ -359,6 +382,23 @@ class Batch:
         rc = lib().ngmi_batch_tran(self.h, tstep, tstop, tmax, max_passes, stream,
                                    ctypes.byref(passes))
         return rc, passes.value
+
+    def table_cols(self):
+        a, b2 = ctypes.c_int(), ctypes.c_int()
+        check(lib().ngmi_batch_table_cols(self.h, ctypes.byref(a), ctypes.byref(b2)), 'cols')
+        return a.value, b2.value
+
+    def set_profile(self, on=True):
+        check(lib().ngmi_batch_set_profile(self.h, 1 if on else 0), 'profile')
+
+    def kernel_times(self):
+        """(BSIM4 kernel ms, step kernel ms, BSIM4 evaluations, circuit Newton
+        passes) summed over the last tran() run with profiling on."""
+        a, b2 = ctypes.c_double(), ctypes.c_double()
+        e, p = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(lib().ngmi_batch_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b2),
+                                            ctypes.byref(e), ctypes.byref(p)), 'kernel times')
+        return a.value, b2.value, e.value, p.value
 
     def stats(self, c):
         st = ctypes.c_int()

@@ -1,0 +1,71 @@
+/*
+ * ORACLE (test infrastructure only): a four-call wrapper around the
+ * reference's own KLU, compiled from /root/reference/src/maths/KLU into
+ * _ref/libklu_ref.so together with this file (see Makefile `ref`).  It lets
+ * the oracle's transient restatement solve with the reference's solver the
+ * way klusmp.c drives it: klu_analyze once (SMPpreOrder, klusmp.c:1255-1285),
+ * klu_factor on SMPreorder (klusmp.c:752-806), klu_refactor on SMPluFac
+ * (klusmp.c:573-624, falling back to a fresh klu_factor when the refactor
+ * reports a singular matrix, niiter.c:160-190), klu_solve on SMPsolve
+ * (klusmp.c:950-1011).
+ */
+#include <stdlib.h>
+#include "ngspice/klu.h"
+
+typedef struct {
+    int n;
+    int *Ap, *Ai;
+    klu_common common;
+    klu_symbolic *sym;
+    klu_numeric *num;
+} kg_t;
+
+void *kg_new(int n, const int *Ap, const int *Ai)
+{
+    kg_t *h = (kg_t *)calloc(1, sizeof(kg_t));
+    int k, nz = Ap[n];
+    if (!h) return NULL;
+    h->n = n;
+    h->Ap = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    h->Ai = (int *)malloc(sizeof(int) * (size_t)(nz > 0 ? nz : 1));
+    for (k = 0; k <= n; k++) h->Ap[k] = Ap[k];
+    for (k = 0; k < nz; k++) h->Ai[k] = Ai[k];
+    klu_defaults(&h->common);
+    h->sym = klu_analyze(n, h->Ap, h->Ai, &h->common);
+    if (!h->sym) {
+        free(h->Ap); free(h->Ai); free(h);
+        return NULL;
+    }
+    return h;
+}
+
+/* full = 1: klu_factor (pivoting); full = 0: klu_refactor, then klu_factor
+ * if that fails.  Returns 0, or 102 (E_SINGULAR) / 1 on failure. */
+int kg_factor(void *hp, double *Ax, int full, double pivrel)
+{
+    kg_t *h = (kg_t *)hp;
+    h->common.tol = pivrel;
+    if (!full && h->num) {
+        if (klu_refactor(h->Ap, h->Ai, Ax, h->sym, h->num, &h->common)) return 0;
+    }
+    if (h->num) klu_free_numeric(&h->num, &h->common);
+    h->num = klu_factor(h->Ap, h->Ai, Ax, h->sym, &h->common);
+    if (!h->num) return h->common.status == KLU_SINGULAR ? 102 : 1;
+    if (h->common.status == KLU_SINGULAR) return 102;
+    return 0;
+}
+
+int kg_solve(void *hp, double *b)
+{
+    kg_t *h = (kg_t *)hp;
+    return klu_solve(h->sym, h->num, h->n, 1, b, &h->common) ? 0 : 1;
+}
+
+void kg_free(void *hp)
+{
+    kg_t *h = (kg_t *)hp;
+    if (!h) return;
+    if (h->num) klu_free_numeric(&h->num, &h->common);
+    if (h->sym) klu_free_symbolic(&h->sym, &h->common);
+    free(h->Ap); free(h->Ai); free(h);
+}

@@ -183,6 +183,7 @@ def _tran_protos():
                                 ctypes.POINTER(ctypes.c_int64)]
     L.or_tran_output.argtypes = [vp, dp, dp]
     L.or_tran_free.argtypes = [vp]
+    L.or_tran_use_klu.argtypes = [vp, ctypes.c_char_p]
     L._tran_ready = True
     return L
 
@@ -229,6 +230,18 @@ class Transient:
             L.or_tran_set_ic(self.h, node, v)
         if L.or_tran_setup(self.h) != 0:
             raise RuntimeError('transient setup failed')
+        self.solver = 'dense'
+
+    def use_reference_klu(self):
+        """Solve with the reference's KLU (oracle/_ref) when it was built;
+        returns True when switched."""
+        path = os.path.join(HERE, '_ref', 'libklu_ref.so')
+        if not os.path.exists(path):
+            return False
+        if _tran_protos().or_tran_use_klu(self.h, path.encode()) != 0:
+            return False
+        self.solver = 'klu'
+        return True
 
     def run(self, tstep, tstop, tmax=0.0):
         """Returns (rc, t[npts], x[npts, n+1]) with x[:, 0] = ground."""

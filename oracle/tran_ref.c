@@ -32,6 +32,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <dlfcn.h>
 #include "bsim4_ref.h"
 
 int or_slot_rc(int slot, int *row_role, int *col_role);
@@ -79,6 +80,15 @@ typedef struct {
     double *out_t, *out_x;
     int64_t niter, accepted, rejected;
     int noncon_dc_fail;
+    int iterno;
+    /* optional: the reference's KLU (oracle/_ref) instead of the dense LU */
+    void *klu;
+    int *Ap, *Ai;
+    double *Ax, *kb;
+    void *(*kg_new)(int, const int *, const int *);
+    int (*kg_factor)(void *, double *, int, double);
+    int (*kg_solve)(void *, double *);
+    void (*kg_free)(void *);
 } ORTran;
 
 /* equality.c:22-60 */
@@ -298,9 +308,29 @@ static int ckt_load(ORTran *T)
     return 0;
 }
 
+/* the reference's KLU on the structural pattern; NIiter reorders (a fresh
+ * klu_factor) on INITJCT and on the first INITTRAN iteration (niiter.c:
+ * 110-125), refactors otherwise */
+static int solve_klu(ORTran *T)
+{
+    const int n = T->n;
+    int c, p, full;
+    for (c = 0; c < n; c++)
+        for (p = T->Ap[c]; p < T->Ap[c + 1]; p++) T->Ax[p] = AX(T->Ai[p] + 1, c + 1);
+    full = (T->ck.mode & OR_MODEINITJCT) || ((T->ck.mode & OR_MODEINITTRAN) && T->iterno == 1);
+    c = T->kg_factor(T->klu, T->Ax, full, 1e-3);
+    if (c) return c;
+    for (c = 0; c < n; c++) T->kb[c] = T->rhs[c + 1];
+    if (T->kg_solve(T->klu, T->kb)) return 1;
+    for (c = 0; c < n; c++) T->rhs[c + 1] = T->kb[c];
+    T->rhs[0] = 0.0;
+    return 0;
+}
+
 /* dense LU with partial pivoting on rows/cols 1..n; solution into rhs */
 static int solve(ORTran *T)
 {
+    if (T->klu) return solve_klu(T);
     const int n = T->n, ld = n + 1;
     double *a = T->lu;
     int i, j, k;
@@ -365,6 +395,7 @@ static int ni_iter(ORTran *T, int maxIter)
         ck->noncon = 0;
         if (ckt_load(T)) return 1;
         iterno++;
+        T->iterno = iterno;
         rc = solve(T);
         T->niter++;
         if (rc) return rc;
@@ -699,6 +730,60 @@ int or_tran_setup(ORTran *T)
     return (T->A && T->lu && T->rhs && T->rhsOld && T->stbuf) ? 0 : -1;
 }
 
+/* switch the linear solve to the reference's KLU in the library at path
+ * (oracle/_ref/libklu_ref.so); call after or_tran_setup.  Returns 0. */
+int or_tran_use_klu(ORTran *T, const char *path)
+{
+    void *dl = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    int n = T->n, r, c, k, s, nz = 0;
+    char *S;
+    if (!dl) return -1;
+    T->kg_new = (void *(*)(int, const int *, const int *))dlsym(dl, "kg_new");
+    T->kg_factor = (int (*)(void *, double *, int, double))dlsym(dl, "kg_factor");
+    T->kg_solve = (int (*)(void *, double *))dlsym(dl, "kg_solve");
+    T->kg_free = (void (*)(void *))dlsym(dl, "kg_free");
+    if (!T->kg_new || !T->kg_factor || !T->kg_solve || !T->kg_free) return -2;
+    S = (char *)calloc((size_t)(n + 1) * (n + 1), 1);
+#define MARK(a, b) do { if ((a) && (b)) S[(size_t)(a) * (n + 1) + (b)] = 1; } while (0)
+    for (k = 0; k < T->nmos; k++)
+        for (s = 0; s < P__COUNT; s++) {
+            int rr, cc;
+            or_slot_rc(s, &rr, &cc);
+            MARK(T->mos[k]->node[rr], T->mos[k]->node[cc]);
+        }
+    for (k = 0; k < T->ncap; k++) {
+        MARK(T->cap_p[k], T->cap_p[k]); MARK(T->cap_n[k], T->cap_n[k]);
+        MARK(T->cap_p[k], T->cap_n[k]); MARK(T->cap_n[k], T->cap_p[k]);
+    }
+    for (k = 0; k < T->nres; k++) {
+        MARK(T->res_p[k], T->res_p[k]); MARK(T->res_n[k], T->res_n[k]);
+        MARK(T->res_p[k], T->res_n[k]); MARK(T->res_n[k], T->res_p[k]);
+    }
+    for (k = 0; k < T->nvs; k++) {
+        MARK(T->vs[k].p, T->vs[k].br); MARK(T->vs[k].n, T->vs[k].br);
+        MARK(T->vs[k].br, T->vs[k].p); MARK(T->vs[k].br, T->vs[k].n);
+    }
+    for (k = 1; k <= T->nnodes; k++)
+        if (T->ic_given[k]) MARK(k, k);
+#undef MARK
+    T->Ap = (int *)calloc((size_t)n + 1, sizeof(int));
+    for (c = 1; c <= n; c++)
+        for (r = 1; r <= n; r++) nz += S[(size_t)r * (n + 1) + c];
+    T->Ai = (int *)calloc((size_t)nz + 1, sizeof(int));
+    T->Ax = (double *)calloc((size_t)nz + 1, sizeof(double));
+    T->kb = (double *)calloc((size_t)n + 1, sizeof(double));
+    nz = 0;
+    for (c = 1; c <= n; c++) {
+        T->Ap[c - 1] = nz;
+        for (r = 1; r <= n; r++)
+            if (S[(size_t)r * (n + 1) + c]) T->Ai[nz++] = r - 1;
+    }
+    T->Ap[n] = nz;
+    free(S);
+    T->klu = T->kg_new(n, T->Ap, T->Ai);
+    return T->klu ? 0 : -3;
+}
+
 int64_t or_tran_npts(const ORTran *T) { return T->npts; }
 int or_tran_n(const ORTran *T) { return T->n; }
 void or_tran_stats(const ORTran *T, int64_t *acc, int64_t *rej, int64_t *iters)
@@ -718,5 +803,7 @@ void or_tran_free(ORTran *T)
     free(T->res_p); free(T->res_n); free(T->res_g); free(T->vs); free(T->ic_given);
     free(T->ic); free(T->stbuf); free(T->A); free(T->lu); free(T->piv); free(T->rhs);
     free(T->rhsOld); free(T->out_t); free(T->out_x);
+    if (T->klu) T->kg_free(T->klu);
+    free(T->Ap); free(T->Ai); free(T->Ax); free(T->kb);
     free(T);
 }

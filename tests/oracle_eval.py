@@ -15,6 +15,14 @@ import pyoracle as po  # noqa: E402
 D = ctypes.c_double
 NODES = [1, 2, 3, 4, 1, 2, 2, 3, 4, 4, 4, 0]   # roles d g s b dp gp gm sp db bp sb q
 
+# values the load keeps per instance for the next call (b4ld.c:444-507), in
+# the engine's cache row order; the first 25 rows of its cache
+CACHE_FIELDS = ['cd', 'csub', 'cbd', 'cbs', 'gbd', 'gbs', 'gm', 'gds', 'gmbs', 'gbbs', 'gbgs',
+                'gbds', 'Igidl', 'Igisl', 'ggidld', 'ggidlg', 'ggidlb', 'ggisls', 'ggislg',
+                'ggislb', 'Igs', 'Igd', 'Igcs', 'Igcd', 'Igb']
+CURRENT_ROWS = [0, 1, 2, 3, 12, 13, 20, 21, 22, 23, 24]
+CONDUCTANCE_ROWS = [4, 5, 6, 7, 8, 9, 10, 11, 14, 15, 16, 17, 18, 19]
+
 
 class OracleBatch:
     """n oracle instances; states kept as [level][29][n] numpy arrays."""
@@ -43,6 +51,7 @@ class OracleBatch:
         stamp = np.zeros((16, n))
         rhs = np.zeros((4, n))
         nc = np.zeros(n, dtype=np.int32)
+        self.cache = np.zeros((len(CACHE_FIELDS), n))
         for k in range(n):
             sv = [(D * 29)(*states[lv][:, k]) for lv in range(3)] + [(D * 29)()]
             rv = (D * 16)(0.0, *vterm[:, k])
@@ -59,4 +68,6 @@ class OracleBatch:
             for lv in range(3):
                 states[lv][:, k] = np.frombuffer(sv[lv], dtype=np.float64)
             nc[k] = c
+            for f, name in enumerate(CACHE_FIELDS):
+                self.cache[f, k] = self.inst[k].get(name)
         return stamp, rhs, nc
