@@ -21,6 +21,13 @@
 
 #include "bsim4_fields.h"
 
+/* Every multiply and add is rounded on its own, as in the reference build:
+ * contracting a*b + c*d into a fused multiply-add changes results whenever
+ * the terms cancel (ag0 q0 + ag1 q1 right after INITTRAN is exactly 0 in
+ * the reference and a rounding residual with FMA).  The remaining difference
+ * to the reference is the last bit of exp/log/sqrt/pow. */
+#pragma clang fp contract(off)
+
 #ifdef __HIPCC__
 #define NGMI_HD __host__ __device__ __forceinline__
 #else
