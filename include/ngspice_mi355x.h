@@ -199,6 +199,26 @@ int ngmi_batch_waveform(const ngmi_batch *b, int c, double *t, double *x, int64_
                         int64_t *npts);
 
 /* ------------------------------------------------------------------ */
+/* Newton initial-guess network (no counterpart in the reference)      */
+/* ------------------------------------------------------------------ */
+/* 16 -> 256 -> 256 -> 256 -> 1, SiLU, BF16 weights on the matrix cores
+ * with FP32 accumulation; FP32 master weights and Adam.  Parameters are one
+ * FP32 array of ngmi_mlp_num_params() values (input layer stored 32 wide). */
+typedef struct ngmi_mlp ngmi_mlp;
+ngmi_mlp *ngmi_mlp_new(unsigned long long seed);
+void ngmi_mlp_free(ngmi_mlp *n);
+int64_t ngmi_mlp_num_params(void);
+/* copy parameters (grads = 0) or the last gradients (grads = 1) to host */
+int ngmi_mlp_get_params(const ngmi_mlp *n, float *host, int grads);
+int ngmi_mlp_set_params(ngmi_mlp *n, const float *host);
+/* y[M] = net(X[M][16]), device pointers, asynchronous on stream */
+int ngmi_mlp_forward(ngmi_mlp *n, const float *X, float *y, int M, void *stream);
+/* MSE gradients over M samples; lr > 0 also applies one Adam step; the loss
+ * before the step goes to *loss_host when given (synchronises) */
+int ngmi_mlp_train_step(ngmi_mlp *n, const float *X, const float *target, int M, float lr,
+                        float *loss_host, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Device memory helpers for callers without a HIP binding             */
 /* ------------------------------------------------------------------ */
 void *ngmi_dev_alloc(size_t bytes);

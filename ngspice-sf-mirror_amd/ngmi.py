@@ -296,6 +296,67 @@ def _tran_protos(L):
     return L
 
 
+class GuessNet:
+    """The Newton initial-guess MLP (16 -> 256 x 3 SiLU -> 1) on the GPU."""
+
+    IN, HID, IN_PAD = 16, 256, 32
+
+    def __init__(self, seed=0):
+        L = lib()
+        vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.ngmi_mlp_new.argtypes = [ctypes.c_ulonglong]
+        L.ngmi_mlp_new.restype = vp
+        L.ngmi_mlp_free.argtypes = [vp]
+        L.ngmi_mlp_num_params.restype = ctypes.c_int64
+        L.ngmi_mlp_get_params.argtypes = [vp, fp, i]
+        L.ngmi_mlp_set_params.argtypes = [vp, fp]
+        L.ngmi_mlp_forward.argtypes = [vp, vp, vp, i, vp]
+        L.ngmi_mlp_train_step.argtypes = [vp, vp, vp, i, ctypes.c_float, fp, vp]
+        self.h = L.ngmi_mlp_new(seed)
+        if not self.h:
+            raise EngineError(1, 'mlp')
+        self.n = L.ngmi_mlp_num_params()
+
+    def params(self, grads=False):
+        out = np.zeros(self.n, dtype=np.float32)
+        check(lib().ngmi_mlp_get_params(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        1 if grads else 0), 'get params')
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().ngmi_mlp_set_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+              'set params')
+
+    def unpack(self, p):
+        """split a parameter vector into (W1[256, 32], b1, W2, b2, W3, b3, w4[256], b4)"""
+        H, I = self.HID, self.IN_PAD
+        sizes = [H * I, H, H * H, H, H * H, H, H, 1]
+        out, o = [], 0
+        for s in sizes:
+            out.append(p[o:o + s])
+            o += s
+        W1, b1, W2, b2, W3, b3, w4, b4 = out
+        return W1.reshape(H, I), b1, W2.reshape(H, H), b2, W3.reshape(H, H), b3, w4, b4
+
+    def forward(self, X_dev, y_dev, M, stream=0):
+        check(lib().ngmi_mlp_forward(self.h, X_dev, y_dev, M, stream), 'mlp forward')
+
+    def train_step(self, X_dev, t_dev, M, lr, stream=0, want_loss=True):
+        loss = ctypes.c_float()
+        check(lib().ngmi_mlp_train_step(self.h, X_dev, t_dev, M, lr,
+                                        ctypes.byref(loss) if want_loss else None, stream),
+              'mlp train')
+        return loss.value
+
+    def __del__(self):
+        try:
+            lib().ngmi_mlp_free(self.h)
+        except Exception:
+            pass
+
+
 class Circuit:
     """A netlist for the engine from a circuit description dict (nnodes,
     models, mos, caps, res, vsrc, ic, options); see workloads.py."""
