@@ -210,7 +210,9 @@ __device__ int vsrc_accept(const TranDev &D, int c, CktCtl *K)
 }
 
 __device__ int com_cof(CktCtl *K)
-{   /* nicomcof.c:17-130 */
+{   /* nicomcof.c:17-130 for orders 1 and 2 (the engine keeps order <= 2);
+     * the Gear branch is the reference's small LU written out in its own
+     * operation order */
     if (K->method == 1) {
         if (K->order == 1) {
             K->ag[0] = 1 / K->delta;
@@ -223,60 +225,67 @@ __device__ int com_cof(CktCtl *K)
         return 0;
     }
     const int ord = K->order;
-    if (ord < 1 || ord > 6) return 104;
-    double mat[7][7];
+    if (ord < 1 || ord > 2) return 104;
     for (int k = 0; k < 7; k++) K->ag[k] = 0.0;
-    K->ag[1] = -1 / K->delta;
-    double arg = 0, arg1;
-    for (int i = 0; i <= ord; i++) mat[0][i] = 1;
-    for (int i = 1; i <= ord; i++) mat[i][0] = 0;
-    for (int i = 1; i <= ord; i++) {
-        arg += K->deltaOld[i - 1];
-        arg1 = 1;
-        for (int j = 1; j <= ord; j++) {
-            arg1 *= arg / K->delta;
-            mat[j][i] = arg1;
-        }
+    double ag0 = 0.0, ag1 = -1 / K->delta, ag2 = 0.0;
+    const double arg_1 = K->deltaOld[0];
+    const double m11 = arg_1 / K->delta;
+    if (ord == 1) {
+        ag1 /= m11;
+        ag0 = ag0 - 1.0 * ag1;
+        ag0 /= 1.0;
+    } else {
+        double m21 = m11 * (arg_1 / K->delta);
+        const double arg_2 = arg_1 + K->deltaOld[1];
+        const double m12 = arg_2 / K->delta;
+        double m22 = m12 * (arg_2 / K->delta);
+        m21 /= m11;
+        m22 -= m21 * m12;
+        ag2 = ag2 - m21 * ag1;
+        ag2 /= m22;
+        ag1 = ag1 - m12 * ag2;
+        ag1 /= m11;
+        ag0 = ag0 - 1.0 * ag1;
+        ag0 = ag0 - 1.0 * ag2;
+        ag0 /= 1.0;
     }
-    for (int i = 1; i <= ord; i++)
-        for (int j = i + 1; j <= ord; j++) {
-            mat[j][i] /= mat[i][i];
-            for (int k = i + 1; k <= ord; k++) mat[j][k] -= mat[j][i] * mat[i][k];
-        }
-    for (int i = 1; i <= ord; i++)
-        for (int j = i + 1; j <= ord; j++) K->ag[j] = K->ag[j] - mat[j][i] * K->ag[i];
-    K->ag[ord] /= mat[ord][ord];
-    for (int i = ord - 1; i >= 0; i--) {
-        for (int j = i + 1; j <= ord; j++) K->ag[i] = K->ag[i] - mat[i][j] * K->ag[j];
-        K->ag[i] /= mat[i][i];
-    }
+    K->ag[0] = ag0;
+    K->ag[1] = ag1;
+    K->ag[2] = ag2;
     return 0;
 }
 
-/* CKTterr (cktterr.c:13-78) on one charge state; q[l], cq[l] = level l */
-__device__ double terr_one(const double *q, const double *cq, const CktCtl *K, const TranDev &D)
+/* CKTterr (cktterr.c:13-78) on one charge state, orders 1 and 2 (the
+ * divided-difference loop of the reference unrolled in its own order);
+ * q0..q3 = charge at levels 0..3, c0, c1 = its current at levels 0, 1 */
+__device__ double terr_one(double q0, double q1, double q2, double q3, double c0, double c1,
+                           const CktCtl *K, const TranDev &D)
 {
-    const double trapCoeff[2] = {.5, .08333333333};
-    const double gearCoeff[6] = {.5, .2222222222, .1363636364, .096, .07299270073, .05830903790};
     const int order = K->order;
-    const double volttol = D.abstol + D.reltol * fmax(fabs(cq[0]), fabs(cq[1]));
-    double chargetol = fmax(fabs(q[0]), fabs(q[1]));
+    const double volttol = D.abstol + D.reltol * fmax(fabs(c0), fabs(c1));
+    double chargetol = fmax(fabs(q0), fabs(q1));
     chargetol = D.reltol * fmax(chargetol, D.chgtol) / K->delta;
     const double tol = fmax(volttol, chargetol);
-    double diff[8], deltmp[8];
-    for (int i = order + 1; i >= 0; i--) diff[i] = q[i];
-    for (int i = 0; i <= order; i++) deltmp[i] = K->deltaOld[i];
-    int j = order;
-    for (;;) {
-        for (int i = 0; i <= j; i++) diff[i] = (diff[i] - diff[i + 1]) / deltmp[i];
-        if (--j < 0) break;
-        for (int i = 0; i <= j; i++) deltmp[i] = deltmp[i + 1] + K->deltaOld[i];
+    const double d0 = K->deltaOld[0], d1 = K->deltaOld[1], d2 = K->deltaOld[2];
+    double diff0, factor;
+    if (order == 1) {
+        double a = (q0 - q1) / d0;
+        const double b = (q1 - q2) / d1;
+        const double t0 = d1 + d0;
+        diff0 = (a - b) / t0;
+        factor = .5;                      /* trapCoeff[0] = gearCoeff[0] */
+    } else {
+        double a = (q0 - q1) / d0, b = (q1 - q2) / d1;
+        const double c = (q2 - q3) / d2;
+        const double t0 = d1 + d0, t1 = d2 + d1;
+        a = (a - b) / t0;
+        b = (b - c) / t1;
+        const double u0 = t1 + d0;
+        diff0 = (a - b) / u0;
+        factor = (K->method == 2) ? .2222222222 : .08333333333;
     }
-    const double factor = (K->method == 2) ? gearCoeff[order - 1] : trapCoeff[order - 1];
-    double del = D.trtol * tol / fmax(D.abstol, factor * fabs(diff[0]));
+    double del = D.trtol * tol / fmax(D.abstol, factor * fabs(diff0));
     if (order == 2) del = sqrt(del);
-    else if (order == 3) del = cbrt(del);
-    else if (order > 3) del = exp(log(del) / order);
     return del;
 }
 
@@ -307,21 +316,24 @@ __device__ double circuit_terr(const TranDev &D, const CktCtl *K, int c, double 
     double m = 1e308;
     for (int k = threadIdx.x; k < D.NCAP; k += blockDim.x) {
         const int64_t ci = (int64_t)c * D.NCAP + k;
-        double q[4], cq[2];
-        for (int l = 0; l <= order + 1; l++) q[l] = D.cstate[((int64_t)lvl(rb, l) * 2) * cnc + ci];
-        for (int l = 0; l < 2; l++) cq[l] = D.cstate[((int64_t)lvl(rb, l) * 2 + 1) * cnc + ci];
-        m = fmin(m, terr_one(q, cq, K, D));
+        const double *Q = D.cstate + ci;
+        const double q0 = Q[(int64_t)lvl(rb, 0) * 2 * cnc], q1 = Q[(int64_t)lvl(rb, 1) * 2 * cnc];
+        const double q2 = Q[(int64_t)lvl(rb, 2) * 2 * cnc];
+        const double q3 = order > 1 ? Q[(int64_t)lvl(rb, 3) * 2 * cnc] : 0.0;
+        const double c0 = Q[((int64_t)lvl(rb, 0) * 2 + 1) * cnc];
+        const double c1 = Q[((int64_t)lvl(rb, 1) * 2 + 1) * cnc];
+        m = fmin(m, terr_one(q0, q1, q2, q3, c0, c1, K, D));
     }
-    const int qs[3] = {B4X_qb, B4X_qg, B4X_qd};
     for (int t = threadIdx.x; t < 3 * D.NM; t += blockDim.x) {
-        const int j = t / 3, which = qs[t - 3 * j];
+        const int j = t / 3, r = t - 3 * j;
+        const int which = r == 0 ? B4X_qb : (r == 1 ? B4X_qg : B4X_qd);
         const int64_t mi = (int64_t)c * D.NM + j;
-        double q[4], cq[2];
-        for (int l = 0; l <= order + 1; l++)
-            q[l] = D.mstate[((int64_t)lvl(rb, l) * B4X__COUNT + which) * cnm + mi];
-        for (int l = 0; l < 2; l++)
-            cq[l] = D.mstate[((int64_t)lvl(rb, l) * B4X__COUNT + which + 1) * cnm + mi];
-        m = fmin(m, terr_one(q, cq, K, D));
+        const double *Q = D.mstate + (int64_t)which * cnm + mi;
+        const int64_t lv = (int64_t)B4X__COUNT * cnm;
+        const double q0 = Q[lvl(rb, 0) * lv], q1 = Q[lvl(rb, 1) * lv], q2 = Q[lvl(rb, 2) * lv];
+        const double q3 = order > 1 ? Q[lvl(rb, 3) * lv] : 0.0;
+        const double c0 = Q[lvl(rb, 0) * lv + cnm], c1 = Q[lvl(rb, 1) * lv + cnm];
+        m = fmin(m, terr_one(q0, q1, q2, q3, c0, c1, K, D));
     }
     return block_min(m, red);
 }
@@ -385,11 +397,16 @@ __device__ int loop_top(CktCtl *K)
     return com_cof(K);
 }
 
-__global__ void __launch_bounds__(256)
+/* one wavefront per circuit: the level-scheduled LU program has a few
+ * dozen entries per level, and a one-wave workgroup makes each level's
+ * barrier nearly free while many circuits share a compute unit */
+constexpr int STEP_THREADS = 64;
+
+__global__ void __launch_bounds__(STEP_THREADS)
 k_tran_step(TranDev D)
 {
     extern __shared__ double sm[];
-    __shared__ double red[256];
+    __shared__ double red[STEP_THREADS];
     __shared__ int s_flag[4];
     __shared__ int s_act;
 
@@ -744,7 +761,7 @@ int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipSt
         NGMI_HIP(hipGetLastError());
     }
     if (ev) NGMI_HIP(hipEventRecord(ev[1], s));
-    hipLaunchKernelGGL(k_tran_step, dim3((unsigned)D.C), dim3(256), lds_bytes, s, D);
+    hipLaunchKernelGGL(k_tran_step, dim3((unsigned)D.C), dim3(STEP_THREADS), lds_bytes, s, D);
     NGMI_HIP(hipGetLastError());
     if (ev) NGMI_HIP(hipEventRecord(ev[2], s));
     return NGMI_OK;

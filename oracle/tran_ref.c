@@ -210,7 +210,9 @@ static int vsrc_accept(ORTran *T, int mode)
 
 #define AX(r, c) T->A[(size_t)(r) * (T->n + 1) + (c)]
 
-/* CKTload (cktload.c:31-180): CAP, BSIM4, RES, VSRC, then .ic rows */
+/* CKTload (cktload.c:31-180): device types in registry order BSIM4, CAP,
+ * RES, VSRC (dev.c:142-206), instances newest first as the instance lists
+ * are built by prepending (cktcrte.c:62-64), then the .ic rows */
 static int ckt_load(ORTran *T)
 {
     ORCkt *ck = &T->ck;
@@ -219,12 +221,28 @@ static int ckt_load(ORTran *T)
     memset(T->A, 0, sizeof(double) * (size_t)(n + 1) * (n + 1));
     memset(T->rhs, 0, sizeof(double) * (size_t)(n + 1));
     ck->rhsOld = T->rhsOld;
+    /* b4ld.c:69-5433 */
+    for (k = T->nmos - 1; k >= 0; k--) {   /* instances load newest first */
+        ORInst *h = T->mos[k];
+        if (or_bsim4_load(h, ck)) return -1;
+        for (s = 0; s < P__COUNT; s++) {
+            int rr, cc, r, c;
+            or_slot_rc(s, &rr, &cc);
+            r = h->node[rr];
+            c = h->node[cc];
+            if (r && c) AX(r, c) += h->stamp[s];
+        }
+        for (s = 0; s < N__COUNT; s++) {
+            int r = h->node[s];
+            if (r) T->rhs[r] += h->rhs[s];
+        }
+    }
     /* capload.c:19-86 */
     if (mode & (OR_MODETRAN | OR_MODEAC | OR_MODETRANOP)) {
         const int cond1 = ((mode & (OR_MODEDCOP | OR_MODETRANOP | OR_MODEDCTRANCURVE)) &&
                            (mode & OR_MODEINITJCT)) ||
                           ((mode & OR_MODEUIC) && (mode & OR_MODEINITTRAN));
-        for (k = 0; k < T->ncap; k++) {
+        for (k = T->ncap - 1; k >= 0; k--) {
             const int p = T->cap_p[k], q = T->cap_n[k], qc = T->cap_q[k];
             double vcap, geq, ceq;
             vcap = cond1 ? 0.0 : T->rhsOld[p] - T->rhsOld[q];
@@ -248,24 +266,8 @@ static int ckt_load(ORTran *T)
             }
         }
     }
-    /* b4ld.c:69-5433 */
-    for (k = 0; k < T->nmos; k++) {
-        ORInst *h = T->mos[k];
-        if (or_bsim4_load(h, ck)) return -1;
-        for (s = 0; s < P__COUNT; s++) {
-            int rr, cc, r, c;
-            or_slot_rc(s, &rr, &cc);
-            r = h->node[rr];
-            c = h->node[cc];
-            if (r && c) AX(r, c) += h->stamp[s];
-        }
-        for (s = 0; s < N__COUNT; s++) {
-            int r = h->node[s];
-            if (r) T->rhs[r] += h->rhs[s];
-        }
-    }
     /* resload.c:16-38 */
-    for (k = 0; k < T->nres; k++) {
+    for (k = T->nres - 1; k >= 0; k--) {
         const int p = T->res_p[k], q = T->res_n[k];
         const double g = T->res_g[k];
         AX(p, p) += g;
@@ -274,7 +276,7 @@ static int ckt_load(ORTran *T)
         AX(q, p) -= g;
     }
     /* vsrcload.c:40-60 + value */
-    for (k = 0; k < T->nvs; k++) {
+    for (k = T->nvs - 1; k >= 0; k--) {
         const ORVsrc *v = &T->vs[k];
         AX(v->p, v->br) += 1.0;
         AX(v->n, v->br) -= 1.0;
