@@ -138,6 +138,7 @@ int ngmi_gather_run(const ngmi_gather *g, const double *src, double *Ax, void *s
 /* ------------------------------------------------------------------ */
 typedef struct ngmi_circuit ngmi_circuit;
 typedef struct ngmi_batch ngmi_batch;
+typedef struct ngmi_mlp ngmi_mlp;
 
 /* A netlist with nodes 1..nnodes (0 = ground): the engine's counterpart of
  * building a CKTcircuit (CKTinit + the device setup calls of CKTsetup,
@@ -190,6 +191,20 @@ int ngmi_batch_table_cols(const ngmi_batch *b, int *inst_cols, int *model_cols);
 int ngmi_batch_set_profile(ngmi_batch *b, int enable);
 int ngmi_batch_kernel_times(const ngmi_batch *b, double *mos_ms, double *step_ms,
                             unsigned long long *mos_evals, unsigned long long *newton_passes);
+/* Learned Newton initial guess (no counterpart in the reference; it stands
+ * in for the device-level predictor of MODEINITPRED, b4ld.c:322-372 and
+ * capload.c:53-66): with a network attached, every Newton run of a step
+ * after the first four accepted points starts from node voltages
+ * v0 + r (v0 - v1) + net(features) instead of the predictor.  Convergence
+ * criteria are unchanged; a run that fails redoes the step from the
+ * predictor.  NULL detaches.  The network must outlive the batch's runs. */
+int ngmi_batch_set_guess(ngmi_batch *b, ngmi_mlp *net);
+/* the guess features' neighbour table: [nnodes+1][8] node numbers, -1 = none */
+int ngmi_batch_neighbours(const ngmi_batch *b, int *out);
+/* per circuit: steps started from the guess, fallbacks to the predictor;
+ * guess kernels' total milliseconds of the last profiled run */
+int ngmi_batch_guess_stats(const ngmi_batch *b, int c, int64_t *guessed, int64_t *fallbacks,
+                           double *guess_ms);
 /* per circuit: status (0 finished, -1 running, else the error number),
  * recorded points, accepted / rejected timepoints, Newton iterations */
 int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int64_t *accepted,

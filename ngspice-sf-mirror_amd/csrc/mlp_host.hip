@@ -15,7 +15,8 @@
 namespace ngmi {
 int launch_gemm_bf16(const unsigned short *A, int64_t as0, int64_t as1, const unsigned short *B,
                      int64_t bs0, int64_t bs1, const float *bias, float *Z, float *C,
-                     unsigned short *Cb, int ldc, int M, int N, int K, int epi, hipStream_t s);
+                     unsigned short *Cb, int ldc, int M, int N, int K, int epi, hipStream_t s,
+                     const int *Mdev = nullptr);
 __global__ void k_f32_to_bf16(const float *x, unsigned short *y, int64_t n);
 __global__ void k_silu_back(const float *dY, const float *Z, float *dZ, unsigned short *dZb,
                             int64_t n);
@@ -23,7 +24,7 @@ __global__ void k_colsum(const float *dZ, float *db, int M, int N);
 __global__ void k_adam(float *w, unsigned short *wb, const float *g, float *m, float *v, int64_t n,
                        float lr, float b1, float b2, float eps, float c1, float c2);
 __global__ void k_head(const unsigned short *H, const unsigned short *w, const float *b,
-                       const float *target, float *y, float *dy, int M, int K);
+                       const float *target, float *y, float *dy, int M, int K, const int *Mdev);
 __global__ void k_head_back(const unsigned short *H, const unsigned short *w, const float *dy,
                             float *dH, float *dw, float *db, int M, int K);
 }  // namespace ngmi
@@ -49,10 +50,24 @@ struct ngmi_mlp {
     float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *dh = nullptr, *dz = nullptr;
     float *y = nullptr, *dy = nullptr;
     int64_t step = 0;
+    /* inference-only buffers (BF16 activations, ping-pong) */
+    int icap = 0;
+    unsigned short *ixb = nullptr, *ia = nullptr, *ib = nullptr;
     ~ngmi_mlp()
     {
-        void *ps[] = {w, g, m, v, wb, xb, h1, h2, h3, dzb, z1, z2, z3, dh, dz, y, dy};
+        void *ps[] = {w, g, m, v, wb, xb, h1, h2, h3, dzb, z1, z2, z3, dh, dz, y, dy, ixb, ia, ib};
         for (void *p : ps) (void)hipFree(p);
+    }
+    int ensure_infer(int rows)
+    {
+        if (rows <= icap) return NGMI_OK;
+        void *ps[] = {ixb, ia, ib};
+        for (void *p : ps) (void)hipFree(p);
+        NGMI_HIP(hipMalloc(&ixb, (size_t)rows * 32 * 2));
+        NGMI_HIP(hipMalloc(&ia, (size_t)rows * MLP_HID * 2));
+        NGMI_HIP(hipMalloc(&ib, (size_t)rows * MLP_HID * 2));
+        icap = rows;
+        return NGMI_OK;
     }
     int ensure(int rows)
     {
@@ -83,9 +98,13 @@ int blocks(int64_t n) { return (int)((n + 255) / 256); }
 
 /* X (device fp32, M x 16, row-major) -> padded BF16 input rows */
 __global__ void k_pack_input(const float *__restrict__ X, unsigned short *__restrict__ xb, int M,
-                             int Mpad)
+                             int Mpad, const int *__restrict__ Mdev = nullptr)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (Mdev) {
+        M = min(M, *Mdev);
+        Mpad = min(Mpad, (M + 31) / 32 * 32);
+    }
     if (i >= (int64_t)Mpad * IN_PAD) return;
     const int r = (int)(i / IN_PAD), c = (int)(i % IN_PAD);
     float v = (r < M && c < MLP_IN) ? X[(int64_t)r * MLP_IN + c] : 0.0f;
@@ -115,6 +134,34 @@ int forward(ngmi_mlp *n, const float *X, int M, int Mpad, hipStream_t s)
 }
 
 }  // namespace
+
+namespace ngmi {
+/* forward over the first *dcount of at most maxRows rows (device-side count) */
+int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const int *dcount,
+                     hipStream_t s)
+{
+    const int Mpad = (maxRows + 31) / 32 * 32;
+    int rc = n->ensure_infer(Mpad);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks((int64_t)Mpad * IN_PAD)), dim3(256), 0, s, X, n->ixb,
+                       maxRows, Mpad, dcount);
+    NGMI_HIP(hipGetLastError());
+    const unsigned short *wb = n->wb;
+    if ((rc = launch_gemm_bf16(n->ixb, IN_PAD, 1, wb + OFF_W1, IN_PAD, 1, n->w + OFF_B1, nullptr,
+                               nullptr, n->ia, H, Mpad, H, IN_PAD, 2, s, dcount)))
+        return rc;
+    if ((rc = launch_gemm_bf16(n->ia, H, 1, wb + OFF_W2, H, 1, n->w + OFF_B2, nullptr, nullptr, n->ib,
+                               H, Mpad, H, H, 2, s, dcount)))
+        return rc;
+    if ((rc = launch_gemm_bf16(n->ib, H, 1, wb + OFF_W3, H, 1, n->w + OFF_B3, nullptr, nullptr, n->ia,
+                               H, Mpad, H, H, 2, s, dcount)))
+        return rc;
+    hipLaunchKernelGGL(k_head, dim3(blocks(maxRows)), dim3(256), 0, s, n->ia, wb + OFF_W4,
+                       n->w + OFF_B4, nullptr, y, nullptr, maxRows, H, dcount);
+    NGMI_HIP(hipGetLastError());
+    return NGMI_OK;
+}
+}  // namespace ngmi
 
 extern "C" {
 
@@ -189,7 +236,7 @@ int ngmi_mlp_forward(ngmi_mlp *n, const float *X, float *y, int M, void *stream)
     if (rc) return rc;
     if ((rc = forward(n, X, M, Mpad, s))) return rc;
     hipLaunchKernelGGL(k_head, dim3(blocks(M)), dim3(256), 0, s, n->h3, n->wb + OFF_W4,
-                       n->w + OFF_B4, nullptr, y, nullptr, M, H);
+                       n->w + OFF_B4, nullptr, y, nullptr, M, H, nullptr);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }
@@ -208,7 +255,7 @@ int ngmi_mlp_train_step(ngmi_mlp *n, const float *X, const float *target, int M,
     float *g = n->g;
     const unsigned short *wb = n->wb;
     hipLaunchKernelGGL(k_head, dim3(blocks(M)), dim3(256), 0, s, n->h3, wb + OFF_W4, n->w + OFF_B4,
-                       target, n->y, n->dy, M, H);
+                       target, n->y, n->dy, M, H, nullptr);
     NGMI_HIP(hipGetLastError());
     /* padded rows carry no gradient */
     if (Mpad > M) NGMI_HIP(hipMemsetAsync(n->dy + M, 0, (size_t)(Mpad - M) * 4, s));

@@ -42,10 +42,15 @@ __global__ void __launch_bounds__(64)
 k_gemm_bf16(const unsigned short *__restrict__ A, int64_t as0, int64_t as1,
             const unsigned short *__restrict__ B, int64_t bs0, int64_t bs1,
             const float *__restrict__ bias, float *__restrict__ Z, float *__restrict__ C,
-            unsigned short *__restrict__ Cb, int ldc, int M, int N, int K, int epi)
+            unsigned short *__restrict__ Cb, int ldc, int M, int N, int K, int epi,
+            const int *__restrict__ Mdev)
 {
     const int l = threadIdx.x;
     const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    if (Mdev) {
+        M = min(M, *Mdev);
+        if (i0 >= M) return;   /* whole wavefront: rows beyond the device-side count */
+    }
     const int r = l & 31, kq = (l >> 5) * 4;
     f32x16 acc;
 #pragma unroll
@@ -125,9 +130,11 @@ __global__ void k_adam(float *__restrict__ w, unsigned short *__restrict__ wb,
  * MSE gradient dy = 2 (y - t) / M */
 __global__ void k_head(const unsigned short *__restrict__ H, const unsigned short *__restrict__ w,
                        const float *__restrict__ b, const float *__restrict__ target,
-                       float *__restrict__ y, float *__restrict__ dy, int M, int K)
+                       float *__restrict__ y, float *__restrict__ dy, int M, int K,
+                       const int *__restrict__ Mdev)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (Mdev) M = min(M, *Mdev);
     if (i >= M) return;
     float s = b[0];
     for (int k = 0; k < K; k++) s += bf2f(H[(int64_t)i * K + k]) * bf2f(w[k]);
@@ -157,14 +164,15 @@ __global__ void k_head_back(const unsigned short *__restrict__ H, const unsigned
 
 int launch_gemm_bf16(const unsigned short *A, int64_t as0, int64_t as1, const unsigned short *B,
                      int64_t bs0, int64_t bs1, const float *bias, float *Z, float *C,
-                     unsigned short *Cb, int ldc, int M, int N, int K, int epi, hipStream_t s)
+                     unsigned short *Cb, int ldc, int M, int N, int K, int epi, hipStream_t s,
+                     const int *Mdev)
 {
     if (M % 32 || N % 32 || K % 8) {
         set_error("gemm shape %d x %d x %d not a multiple of 32 x 32 x 8", M, N, K);
         return NGMI_E_PARMVAL;
     }
     hipLaunchKernelGGL(k_gemm_bf16, dim3(N / 32, M / 32), dim3(64), 0, s, A, as0, as1, B, bs0, bs1,
-                       bias, Z, C, Cb, ldc, M, N, K, epi);
+                       bias, Z, C, Cb, ldc, M, N, K, epi, Mdev);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }

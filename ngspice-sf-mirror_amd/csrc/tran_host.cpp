@@ -34,6 +34,12 @@ namespace ngmi {
 int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s,
                      hipEvent_t *ev);
 int set_tran_lds_limit(size_t bytes);
+int launch_guess_features(const TranDev &D, float *F, int *rowmap, double *base, int *count,
+                          hipStream_t s);
+int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double *base,
+                       const int *count, int maxrows, hipStream_t s);
+int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const int *dcount,
+                     hipStream_t s);
 const B4Tab *set_table(const ngmi_bsim4_set *s);
 }
 
@@ -78,7 +84,13 @@ struct ngmi_batch {
     int method = 1, maxord = 2;
     /* kernel timing of the last run (events on the launch stream) */
     bool profile = false;
-    double mos_ms = 0, step_ms = 0;
+    double mos_ms = 0, step_ms = 0, guess_ms = 0;
+    /* learned initial guess */
+    ngmi_mlp *net = nullptr;
+    float *gF = nullptr, *gy = nullptr;
+    int *grow = nullptr, *gcount = nullptr;
+    double *gbase = nullptr;
+    std::vector<int> nbr_host;
     unsigned long long counters[2] = {0, 0};
     ~ngmi_batch()
     {
@@ -526,6 +538,30 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
     }
     r_ptr[N] = (int)r_code.size();
 
+    /* ---- neighbour table for the learned guess: up to 8 other non-ground
+     * nodes sharing an element with each node, ascending */
+    {
+        std::vector<std::vector<int>> nb(NN + 1);
+        auto link = [&](int a, int c2) {
+            if (a >= 1 && a <= NN && c2 >= 1 && c2 <= NN && a != c2) {
+                nb[a].push_back(c2);
+                nb[c2].push_back(a);
+            }
+        };
+        for (int j = 0; j < NM; j++)
+            for (int r1 = 0; r1 < 4; r1++)
+                for (int r2 = r1 + 1; r2 < 4; r2++) link(c0.mos[j].node[r1], c0.mos[j].node[r2]);
+        for (int k = 0; k < NCAP; k++) link((int)c0.caps[k][0], (int)c0.caps[k][1]);
+        for (int k = 0; k < NRES; k++) link((int)c0.res[k][0], (int)c0.res[k][1]);
+        for (int k = 0; k < NVS; k++) link(c0.vs[k].p, c0.vs[k].n);
+        b->nbr_host.assign((size_t)(NN + 1) * 8, -1);
+        for (int u = 1; u <= NN; u++) {
+            std::sort(nb[u].begin(), nb[u].end());
+            nb[u].erase(std::unique(nb[u].begin(), nb[u].end()), nb[u].end());
+            for (int k = 0; k < 8 && k < (int)nb[u].size(); k++) b->nbr_host[(size_t)u * 8 + k] = nb[u][k];
+        }
+    }
+
     /* ---- BSIM4 instances, circuit-major */
     b->set = ngmi_bsim4_set_new();
     for (int c = 0; c < C; c++) {
@@ -619,6 +655,7 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
     UP(bw_term, bw_term, int2, i2);
     UP(perm, perm_unknown, int, ip);
     UP(b->probe, probe, int, ip);
+    UP(b->nbr_host, nbr, int, ip);
 #undef UP
     const int64_t cnm = (int64_t)C * NM, cnc = (int64_t)C * NCAP;
     if (alloc(b, (size_t)C * NVS, &dp) != NGMI_OK) return bail(nullptr);
@@ -706,6 +743,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         k.maxOrder = b->maxord;
         k.phase = PH_OP;
         for (int i = 0; i < 7; i++) k.deltaOld[i] = maxStep;
+        k.learned = b->net ? 1 : 0;
     }
     NGMI_HIP(hipMemcpyAsync(D.ctl, ctl.data(), sizeof(CktCtl) * C, hipMemcpyHostToDevice, s));
     const int64_t cnm = (int64_t)C * b->NM, cnc = (int64_t)C * b->NCAP;
@@ -725,16 +763,31 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     int64_t pass = 0;
     int done = 0;
     const int check_every = 32;
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev, gev;
     if (b->profile) {
         ev.resize(3 * check_every);
         for (auto &e : ev) NGMI_HIP(hipEventCreate(&e));
+        if (b->net) {
+            gev.resize(2 * check_every);
+            for (auto &e : gev) NGMI_HIP(hipEventCreate(&e));
+        }
     }
-    b->mos_ms = b->step_ms = 0;
+    b->mos_ms = b->step_ms = b->guess_ms = 0;
     int rc = NGMI_OK;
     while (pass < max_passes && rc == NGMI_OK) {
         int k = 0;
         for (; k < check_every && pass < max_passes; k++, pass++) {
+            if (b->net) {
+                if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k], s));
+                const int maxrows = C * b->N;
+                NGMI_HIP(hipMemsetAsync(b->gcount, 0, sizeof(int), s));
+                rc = launch_guess_features(D, b->gF, b->grow, b->gbase, b->gcount, s);
+                if (rc == NGMI_OK) rc = mlp_forward_rows(b->net, b->gF, b->gy, maxrows, b->gcount, s);
+                if (rc == NGMI_OK)
+                    rc = launch_guess_apply(D, b->gy, b->grow, b->gbase, b->gcount, maxrows, s);
+                if (rc != NGMI_OK) break;
+                if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k + 1], s));
+            }
             rc = launch_tran_pass(*tab, D, b->lds, s, b->profile ? &ev[3 * k] : nullptr);
             if (rc != NGMI_OK) break;
         }
@@ -750,15 +803,58 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
                 (void)hipEventElapsedTime(&c2, ev[3 * q + 1], ev[3 * q + 2]);
                 b->mos_ms += a;
                 b->step_ms += c2;
+                if (b->net) {
+                    float g2 = 0;
+                    (void)hipEventElapsedTime(&g2, gev[2 * q], gev[2 * q + 1]);
+                    b->guess_ms += g2;
+                }
             }
         }
         if (done >= C) break;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
+    for (auto &e : gev) (void)hipEventDestroy(e);
     NGMI_HIP(hipMemcpy(b->counters, D.counters, sizeof(b->counters), hipMemcpyDeviceToHost));
     if (passes_out) *passes_out = pass;
     if (rc != NGMI_OK) return rc;
     return done >= C ? NGMI_OK : NGMI_E_ITERLIM;
+}
+
+int ngmi_batch_set_guess(ngmi_batch *b, ngmi_mlp *net)
+{
+    if (!b) return NGMI_E_PANIC;
+    b->net = net;
+    if (!net) return NGMI_OK;
+    TranDev &D = b->D;
+    const size_t rows = (size_t)b->C * b->N;
+    if (!b->gF) {
+        if (alloc(b, rows * 16, &b->gF) || alloc(b, rows, &b->gy) || alloc(b, rows, &b->grow) ||
+            alloc(b, rows, &b->gbase) || alloc(b, 1, &b->gcount))
+            return NGMI_E_DEVICE;
+        double *h;
+        if (alloc(b, (size_t)4 * b->C * (b->N + 1), &h)) return NGMI_E_DEVICE;
+        D.hist = h;
+    }
+    return NGMI_OK;
+}
+
+int ngmi_batch_neighbours(const ngmi_batch *b, int *out)
+{
+    if (!b || !out) return NGMI_E_PANIC;
+    std::memcpy(out, b->nbr_host.data(), b->nbr_host.size() * sizeof(int));
+    return NGMI_OK;
+}
+
+int ngmi_batch_guess_stats(const ngmi_batch *b, int c, int64_t *guessed, int64_t *fallbacks,
+                           double *guess_ms)
+{
+    if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
+    CktCtl k;
+    NGMI_HIP(hipMemcpy(&k, b->D.ctl + c, sizeof(k), hipMemcpyDeviceToHost));
+    if (guessed) *guessed = k.guessed;
+    if (fallbacks) *fallbacks = k.fallbacks;
+    if (guess_ms) *guess_ms = b->guess_ms;
+    return NGMI_OK;
 }
 
 int ngmi_batch_table_cols(const ngmi_batch *b, int *inst_cols, int *model_cols)

@@ -397,6 +397,16 @@ __device__ int loop_top(CktCtl *K)
     return com_cof(K);
 }
 
+/* start the next Newton run from the learned guess when the circuit has
+ * four accepted points of history */
+__device__ void request_guess(CktCtl *K)
+{
+    if (K->learned && K->nhist >= 4 && !K->firsttime && K->phase == PH_TRAN) {
+        K->guess = 1;
+        K->mode = M_TRAN | M_INITFLOAT;
+    }
+}
+
 /* one wavefront per circuit: the level-scheduled LU program has a few
  * dozen entries per level, and a one-wave workgroup makes each level's
  * barrier nearly free while many circuits share a compute unit */
@@ -429,6 +439,7 @@ k_tran_step(TranDev D)
     if (tid == 0) {
         atomicAdd(D.counters, (unsigned long long)D.NM);
         atomicAdd(D.counters + 1, 1ull);
+        if (K->guess == 1) K->guess = 2;   /* this iteration starts from the learned guess */
     }
 
     /* capload.c:19-86 */
@@ -628,7 +639,19 @@ k_tran_step(TranDev D)
                     K->ag[0] = K->ag[1] = 0;
                     s_act = 10;   /* copy state0 -> state1, then accept */
                 }
+            } else if (rc != 0 && K->guess == 2) {
+                /* the Newton run from the learned guess failed: redo the same
+                 * step from the plain predictor */
+                K->guess = 0;
+                K->fallbacks++;
+                K->mode = M_TRAN | M_INITPRED;
+                K->iterno = 0;
+                s_act = ACT_STOP;
             } else {
+                if (K->guess == 2) {
+                    K->guess = 0;
+                    K->guessed++;
+                }
                 K->mode = M_TRAN | M_INITPRED;
                 s_act = K->firsttime ? 20 : 21;   /* 20: copy state1 -> state2, state3 first */
                 s_flag[3] = rc;
@@ -717,15 +740,25 @@ k_tran_step(TranDev D)
             for (int k = tid; k < D.nprobe; k += nt) row[k] = xold[D.probe[k]];
             if (tid == 0) D.out_t[(int64_t)c * D.cap_pts + np] = K->time;
         }
+        if (K->learned) {
+            double *hrow = D.hist + ((int64_t)K->hpos * D.C + c) * (N + 1);
+            for (int u = tid; u <= N; u += nt) hrow[u] = xold[u];
+        }
         __syncthreads();
         if (tid == 0) {
             K->npts = np + 1;
+            if (K->learned) {
+                K->ht[K->hpos] = K->time;
+                K->hpos = (K->hpos + 1) & 3;
+                K->nhist++;
+            }
             if (almost_equal_ulps(K->time, K->finalTime, 100)) {
                 K->phase = PH_DONE;
                 atomicAdd(D.done_count, 1);
             } else {
                 resume_block(K);
                 if (loop_top(K)) fail(D, K, 104);
+                else request_guess(K);
             }
         }
         return;
@@ -739,7 +772,123 @@ k_tran_step(TranDev D)
             }
         }
         if (loop_top(K)) fail(D, K, 104);
+        else request_guess(K);
     }
+}
+
+/* ---------------- learned initial guess ---------------- */
+
+constexpr double GUESS_RATIO_MAX = 4.0;    /* step ratios are clamped to [0, 4] */
+constexpr double GUESS_OUT_SCALE = 0.01;   /* network output unit: 10 mV */
+
+__device__ __forceinline__ double clamp_ratio(double r)
+{
+    return r != r ? 0.0 : fmin(fmax(r, 0.0), GUESS_RATIO_MAX);
+}
+
+/*
+ * Features of node u of circuit c for the step the circuit is starting
+ * (t_n -> t_n + h), from the last four accepted points v0..v3 at t0 > t1 >
+ * t2 > t3:  v0, v0 - v1, v1 - v2, v2 - v3, h / (t0 - t1), (t0 - t1) /
+ * (t1 - t2), (t1 - t2) / (t2 - t3), the voltages of up to 8 neighbour nodes
+ * minus v0 (0 where absent), 1.  The guess is v0 + r (v0 - v1) + 10 mV * net.
+ * Branch currents start from their last accepted value.
+ */
+__global__ void __launch_bounds__(256)
+k_guess_features(TranDev D, float *F, int *rowmap, double *base, int *count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)D.C * D.N) return;
+    const int c = (int)(i / D.N), u = (int)(i - (int64_t)c * D.N) + 1;
+    const CktCtl *K = D.ctl + c;
+    if (K->phase != PH_TRAN || K->guess != 1) return;
+    const int s0 = (K->hpos + 3) & 3, s1 = (K->hpos + 2) & 3, s2 = (K->hpos + 1) & 3, s3 = K->hpos;
+    const int64_t stride = (int64_t)D.C * (D.N + 1);
+    const double *h0p = D.hist + ((int64_t)s0 * D.C + c) * (D.N + 1);
+    double *xo = D.x + ((int64_t)K->xsel * D.C + c) * (D.N + 1);
+    if (u > D.NN) {
+        xo[u] = h0p[u];
+        return;
+    }
+    const double v0 = h0p[u];
+    const double v1 = h0p[u + ((s1 - s0) * stride)];
+    const double v2 = h0p[u + ((s2 - s0) * stride)];
+    const double v3 = h0p[u + ((s3 - s0) * stride)];
+    const double t0 = K->ht[s0], t1 = K->ht[s1], t2 = K->ht[s2], t3 = K->ht[s3];
+    const double r0 = clamp_ratio(K->delta / (t0 - t1));
+    const double r1 = clamp_ratio((t0 - t1) / (t1 - t2));
+    const double r2 = clamp_ratio((t1 - t2) / (t2 - t3));
+    const int row = atomicAdd(count, 1);
+    float *f = F + (int64_t)row * 16;
+    f[0] = (float)v0;
+    f[1] = (float)(v0 - v1);
+    f[2] = (float)(v1 - v2);
+    f[3] = (float)(v2 - v3);
+    f[4] = (float)r0;
+    f[5] = (float)r1;
+    f[6] = (float)r2;
+    for (int k = 0; k < 8; k++) {
+        const int nb = D.nbr[u * 8 + k];
+        f[7 + k] = nb >= 0 ? (float)(h0p[nb] - v0) : 0.0f;
+    }
+    f[15] = 1.0f;
+    rowmap[row] = c * (D.N + 1) + u;
+    base[row] = v0 + r0 * (v0 - v1);
+}
+
+__global__ void __launch_bounds__(256)
+k_guess_apply(TranDev D, const float *y, const int *rowmap, const double *base, const int *count)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= *count) return;
+    const int m = rowmap[row];
+    const int c = m / (D.N + 1), u = m - c * (D.N + 1);
+    const CktCtl *K = D.ctl + c;
+    D.x[((int64_t)K->xsel * D.C + c) * (D.N + 1) + u] = base[row] + GUESS_OUT_SCALE * (double)y[row];
+}
+
+/* the first iteration from a guess limits against the last accepted terminal
+ * voltages, as the predictor step does (b4ld.c:322-372 copies state1 into
+ * state0 before extrapolating) */
+__global__ void __launch_bounds__(256)
+k_guess_states(TranDev D)
+{
+    const int64_t cnm = (int64_t)D.C * D.NM;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnm) return;
+    const int c = (int)(i / D.NM);
+    const CktCtl *K = D.ctl + c;
+    if (K->phase != PH_TRAN || K->guess != 1) return;
+    const int p0 = lvl(K->rb, 0), p1 = lvl(K->rb, 1);
+    const int ks[6] = {B4X_vds, B4X_vgs, B4X_vbs, B4X_vbd, B4X_vdbs, B4X_vdbd};
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+        D.mstate[((int64_t)p0 * B4X__COUNT + ks[q]) * cnm + i] =
+            D.mstate[((int64_t)p1 * B4X__COUNT + ks[q]) * cnm + i];
+}
+
+int launch_guess_features(const TranDev &D, float *F, int *rowmap, double *base, int *count,
+                          hipStream_t s)
+{
+    const int64_t n = (int64_t)D.C * D.N;
+    hipLaunchKernelGGL(k_guess_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, F,
+                       rowmap, base, count);
+    NGMI_HIP(hipGetLastError());
+    const int64_t cnm = (int64_t)D.C * D.NM;
+    if (cnm > 0) {
+        hipLaunchKernelGGL(k_guess_states, dim3((unsigned)((cnm + 255) / 256)), dim3(256), 0, s, D);
+        NGMI_HIP(hipGetLastError());
+    }
+    return NGMI_OK;
+}
+
+int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double *base,
+                       const int *count, int maxrows, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_guess_apply, dim3((unsigned)((maxrows + 255) / 256)), dim3(256), 0, s, D, y,
+                       rowmap, base, count);
+    NGMI_HIP(hipGetLastError());
+    return NGMI_OK;
 }
 
 int set_tran_lds_limit(size_t bytes)

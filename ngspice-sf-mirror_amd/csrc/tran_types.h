@@ -36,6 +36,13 @@ struct CktCtl {
     int phase, status, firsttime, iterno;
     int nbreaks, rb, xsel, pad;
     int64_t npts, accepted, rejected, iters;
+    /* learned initial guess: learned = enabled for this circuit; guess = 0
+     * none, 1 features requested for the step being started, 2 guess applied
+     * (first Newton iteration runs from it); history ring of the last 4
+     * accepted points (times here, node vectors in TranDev::hist) */
+    int learned, guess, nhist, hpos;
+    double ht[4];
+    int64_t guessed, fallbacks;
 };
 
 /* contribution codes of the assembly plans */
@@ -96,6 +103,8 @@ struct TranDev {
     double *out_t;            /* [C][cap_pts] */
     double *out_x;            /* [C][cap_pts][nprobe] */
     int *done_count;          /* circuits finished or failed */
+    double *hist;             /* [4][C][N+1] accepted node vectors (learned guess) */
+    const int *nbr;           /* [NN+1][8] neighbour nodes (0 = none) */
     unsigned long long *counters;   /* [0] MOSFET evaluations, [1] circuit Newton passes */
 };
 

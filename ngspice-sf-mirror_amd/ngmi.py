@@ -449,6 +449,30 @@ class Batch:
         check(lib().ngmi_batch_table_cols(self.h, ctypes.byref(a), ctypes.byref(b2)), 'cols')
         return a.value, b2.value
 
+    def set_guess(self, net):
+        """attach a GuessNet (None detaches) for the learned initial guess"""
+        L = lib()
+        L.ngmi_batch_set_guess.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.net = net
+        check(L.ngmi_batch_set_guess(self.h, net.h if net is not None else None), 'set guess')
+
+    def neighbours(self, nnodes):
+        L = lib()
+        L.ngmi_batch_neighbours.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        out = np.zeros((nnodes + 1) * 8, dtype=np.int32)
+        check(L.ngmi_batch_neighbours(self.h, out.ctypes.data), 'neighbours')
+        return out
+
+    def guess_stats(self, c):
+        L = lib()
+        i64 = ctypes.c_int64
+        L.ngmi_batch_guess_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(i64),
+                                             ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
+        g, f, ms = i64(), i64(), ctypes.c_double()
+        check(L.ngmi_batch_guess_stats(self.h, c, ctypes.byref(g), ctypes.byref(f),
+                                       ctypes.byref(ms)), 'guess stats')
+        return dict(guessed=g.value, fallbacks=f.value, guess_ms=ms.value)
+
     def set_profile(self, on=True):
         check(lib().ngmi_batch_set_profile(self.h, 1 if on else 0), 'profile')
 
