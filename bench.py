@@ -1,6 +1,11 @@
 """Benchmark driver (one JSON line on stdout).
 
 Workloads
+  mc   : (default) 4096 Monte Carlo 16-bit mirror-adder ripple-carry adders,
+         .tran 5p 20n at reltol 1e-3 on the batched engine; a step is one
+         whole transient of the batch.  Then, unless --no-guess, the guess
+         network is trained on engine waveforms of held-out seeds and the
+         same batch runs again from the learned Newton guess.
   eval : 2**20 isolated BSIM4 instances of the 45 nm base card, one
          backward-Euler transient load each plus deterministic assembly of
          their 16 Jacobian entries into a random 2**21-row CSC matrix.  A step
@@ -190,6 +195,9 @@ def bench_mc(args):
     log('passes %d, accepted %d, rejected %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
         'step %.1f ms, wall %.1f ms' % (passes, acc, rej, its, failed, mos_ms, step_ms,
                                         wall * 1e3))
+    learned = None
+    if not args.no_guess:
+        learned = learned_guess_run(args, b, probe, stream, acc, its)
     return {
         'metric': 'MC transient timepoints x circuits/s at reltol 1e-3 (plain vs learned NR guess)',
         'value': acc / wall,
@@ -207,8 +215,115 @@ def bench_mc(args):
         'extra': {'accepted': acc, 'rejected': rej, 'newton_iterations': its,
                   'iterations_per_point': its / max(acc, 1), 'passes': passes,
                   'failed_circuits': failed, 'step_kernel_ms_total': step_ms,
-                  'bsim4_kernel_ms_total': mos_ms},
+                  'bsim4_kernel_ms_total': mos_ms, 'learned_guess': learned},
     }
+
+
+def learned_guess_run(args, b, probe, stream, plain_acc, plain_its):
+    """Train the guess network on engine waveforms of held-out seeds, then
+    run the same batch with the learned guess; returns its figures and the
+    waveform error of circuit 0 (plain and learned) against the oracle."""
+    import torch
+    import ngmi
+    import workloads
+    import guess
+
+    t0 = time.time()
+    seeds = range(4096, 4096 + args.train_circuits)
+    descs = [workloads.mc_adder(s, tstop=args.tstop) for s in seeds]
+    nn = descs[0]['nnodes']
+    tb = ngmi.Batch([ngmi.Circuit(d) for d in descs], max_points=args.max_points)
+    rc, _ = tb.tran(workloads.ADDER_TSTEP, args.tstop, stream=stream)
+    if rc != 0:
+        raise RuntimeError('training-data run failed: %s' % ngmi.last_error())
+    nbr = tb.neighbours(nn)
+    n = tb.info()['unknowns']
+    Fs, Ts = [], []
+    for c in range(len(descs)):
+        t, x = tb.waveform(c, n)
+        F, T = guess.features(t, x, nbr, nn, every=2)
+        Fs.append(F)
+        Ts.append(T)
+    del tb
+    F = np.concatenate(Fs)
+    T = np.concatenate(Ts)
+    g = np.random.default_rng(4096)
+    sel = g.permutation(len(F))[:min(len(F), 2_000_000)]
+    F, T = F[sel], T[sel]
+    base_ms = float(np.mean(T.astype(np.float64) ** 2))
+    t1 = time.time()
+    net = ngmi.GuessNet(seed=4096)
+    dF = torch.from_numpy(F).cuda()
+    dT = torch.from_numpy(T).cuda()
+    M = 8192
+    for it in range(args.train_steps):
+        k = (it * M) % max(len(F) - M, 1)
+        net.train_step(dF[k:].data_ptr(), dT[k:].data_ptr(), M, 1e-3 if it < args.train_steps // 2
+                       else 3e-4, want_loss=False, stream=stream)
+    loss = net.train_step(dF.data_ptr(), dT.data_ptr(), min(len(F), 262144), 0.0, stream=stream)
+    torch.cuda.synchronize()
+    t2 = time.time()
+    log('guess training: %d rows, baseline %.4e -> loss %.4e, data %.1fs, train %.1fs'
+        % (len(F), base_ms, loss, t1 - t0, t2 - t1))
+    ora = None if args.no_cpu_baseline else oracle_waveform(args)
+    plain_err = waveform_error(b, probe, ora) if ora else None
+    b.set_guess(net)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    rc, passes = b.tran(workloads.ADDER_TSTEP, args.tstop, stream=stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    if rc != 0:
+        raise RuntimeError('learned run failed: %s' % ngmi.last_error())
+    C = b.C
+    st = [b.stats(c) for c in range(C)]
+    gs = [b.guess_stats(c) for c in range(C)]
+    acc = sum(s['accepted'] for s in st)
+    its = sum(s['iterations'] for s in st)
+    mos_ms, step_ms, evals, _ = b.kernel_times()
+    out = {'value': acc / wall, 'unit': 'timepoints x circuits/s', 'ms': wall * 1e3,
+           'accepted': acc, 'newton_iterations': its, 'iterations_per_point': its / max(acc, 1),
+           'plain_iterations_per_point': plain_its / max(plain_acc, 1),
+           'steps_from_guess': sum(x['guessed'] for x in gs),
+           'fallbacks_to_predictor': sum(x['fallbacks'] for x in gs),
+           'guess_kernels_ms_total': gs[0]['guess_ms'], 'bsim4_kernel_ms_total': mos_ms,
+           'step_kernel_ms_total': step_ms, 'failed_circuits': sum(1 for s in st if s['status']),
+           'training': {'seeds': '%d..%d' % (seeds[0], seeds[-1]), 'rows': int(len(F)),
+                        'steps': args.train_steps, 'target_mean_square': base_ms,
+                        'final_loss': loss, 'seconds': t2 - t0}}
+    log('learned: %.0f tp*c/s, %.3f it/pt (plain %.3f), %d fallbacks, guess kernels %.1f ms'
+        % (out['value'], out['iterations_per_point'], out['plain_iterations_per_point'],
+           out['fallbacks_to_predictor'], out['guess_kernels_ms_total']))
+    if ora:
+        out['waveform_error_vs_oracle'] = {'plain': plain_err,
+                                           'learned': waveform_error(b, probe, ora)}
+    b.set_guess(None)
+    return out
+
+
+def oracle_waveform(args):
+    """the oracle's DCtran restatement (reference KLU) on circuit 0 (seed 0)"""
+    sys.path.insert(0, os.path.join(REPO, 'oracle'))
+    import pyoracle as po
+    import workloads
+    d = workloads.mc_adder(0, tstop=args.tstop)
+    T = po.Transient(d)
+    T.use_reference_klu()
+    rc, to, xo = T.run(d['tstep'], args.tstop)
+    return (to, xo) if rc == 0 else None
+
+
+def waveform_error(b, probe, ora):
+    """max |v_engine - v_oracle| over the probed outputs of circuit 0 at the
+    oracle's timepoints (engine waveform interpolated where the timepoints
+    differ)"""
+    to, xo = ora
+    te, xe = b.waveform(0, len(probe))
+    err = max(float(np.abs(np.interp(to, te, xe[:, k]) - xo[:, u]).max())
+              for k, u in enumerate(probe))
+    same = len(te) == len(to) and bool(np.allclose(te, to, rtol=1e-9, atol=0))
+    return {'circuit': 0, 'max_abs_V': err, 'oracle_points': int(len(to)),
+            'engine_points': int(len(te)), 'same_timepoints': same}
 
 
 def main():
@@ -224,6 +339,9 @@ def main():
     ap.add_argument('--tstop', type=float, default=20e-9)
     ap.add_argument('--max-points', type=int, default=8192)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess run')
+    ap.add_argument('--train-circuits', type=int, default=32)
+    ap.add_argument('--train-steps', type=int, default=1500)
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 1 if args.workload == 'mc' else 20
