@@ -116,6 +116,30 @@ def bench_eval(args):
     }
 
 
+def bench_copy(args):
+    """A plain device-to-device copy of a known size: the reference point for
+    the HBM byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
+    import torch
+    n = args.copy_bytes // 8
+    a = torch.ones(n, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - w0) / args.steps * 1e3
+    gbs = 2 * n * 8 / (ms * 1e-3) / 1e9
+    return {'metric': 'device copy bandwidth (counter calibration)', 'value': gbs, 'unit': 'GB/s',
+            'ms_per_step': ms, 'dtype': 'f64',
+            'config': {'workload': 'copy of %d bytes' % (n * 8)},
+            'roofline': {'bound': 'hbm', 'achieved': gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                         'frac': gbs / HBM_PEAK_GBS, 'traffic': 2 * n * 8},
+            'extra': {'bytes_read': n * 8, 'bytes_written': n * 8}}
+
+
 def cpu_baseline_mc(tstop, budget_s=12.0):
     """The oracle's DCtran restatement on whole adder circuits (seeds 0, 1,
     ...), one host core, solving with the reference's own KLU when
@@ -333,7 +357,8 @@ def main():
                     help='timed steps (mc: whole .tran runs of the batch; default 1)')
     ap.add_argument('--warmup', type=int, default=None,
                     help='untimed steps before (mc default 1)')
-    ap.add_argument('--workload', default='mc', choices=['mc', 'eval'])
+    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy'])
+    ap.add_argument('--copy-bytes', type=int, default=1 << 30)
     ap.add_argument('--n', type=int, default=1 << 20)
     ap.add_argument('--circuits', type=int, default=4096)
     ap.add_argument('--tstop', type=float, default=20e-9)
@@ -358,6 +383,9 @@ def main():
     if args.workload == 'mc':
         res = bench_mc(args)
         cpu = None if args.no_cpu_baseline else cpu_baseline_mc(args.tstop)
+    elif args.workload == 'copy':
+        res = bench_copy(args)
+        cpu = None
     else:
         res = bench_eval(args)
         cpu = None
