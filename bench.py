@@ -365,6 +365,8 @@ def main():
     ap.add_argument('--max-points', type=int, default=8192)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess run')
+    ap.add_argument('--eval-waves', type=int, default=None, help='BSIM4 eval kernel occupancy')
+    ap.add_argument('--mos-waves', type=int, default=None, help='transient BSIM4 kernel occupancy')
     ap.add_argument('--train-circuits', type=int, default=32)
     ap.add_argument('--train-steps', type=int, default=1500)
     args = ap.parse_args()
@@ -380,6 +382,11 @@ def main():
     if not torch.cuda.is_available():
         print('bench.py needs a GPU', file=sys.stderr)
         sys.exit(2)
+    import ngmi
+    if args.eval_waves:
+        ngmi.tune('eval_waves', args.eval_waves)
+    if args.mos_waves:
+        ngmi.tune('mos_waves', args.mos_waves)
     if args.workload == 'mc':
         res = bench_mc(args)
         cpu = None if args.no_cpu_baseline else cpu_baseline_mc(args.tstop)

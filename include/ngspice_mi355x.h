@@ -51,6 +51,12 @@ extern "C" {
 /* last error text of the calling thread (never NULL) */
 const char *ngmi_last_error(void);
 
+/* Launch-shape knobs (no counterpart in the reference): "eval_waves" and
+ * "mos_waves" (1 or 2) set the occupancy the BSIM4 kernels' register
+ * allocation targets for ngmi_bsim4_eval and the batched transient.
+ * Process-wide; defaults are the measured best on MI355X. */
+int ngmi_tune(const char *knob, int value);
+
 /* ------------------------------------------------------------------ */
 /* BSIM4 model cards                                                   */
 /* ------------------------------------------------------------------ */

@@ -10,6 +10,10 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
+static int g_tune[TUNE__COUNT] = {1, 1};
+
+int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
+
 void set_error(const char *fmt, ...)
 {
     va_list ap;
@@ -25,6 +29,24 @@ using namespace ngmi;
 extern "C" {
 
 const char *ngmi_last_error(void) { return g_err; }
+
+int ngmi_tune(const char *knob, int value)
+{
+    if (!knob) return NGMI_E_PANIC;
+    int k = -1;
+    if (!std::strcmp(knob, "eval_waves")) k = TUNE_EVAL_WAVES;
+    else if (!std::strcmp(knob, "mos_waves")) k = TUNE_MOS_WAVES;
+    if (k < 0) {
+        set_error("unknown tuning knob '%s'", knob);
+        return NGMI_E_BADPARM;
+    }
+    if (value != 1 && value != 2) {
+        set_error("knob '%s' takes 1 or 2", knob);
+        return NGMI_E_PARMVAL;
+    }
+    g_tune[k] = value;
+    return NGMI_OK;
+}
 
 void *ngmi_dev_alloc(size_t bytes)
 {

@@ -45,7 +45,11 @@ struct EvalIO {
     int *noncon;
 };
 
-__global__ void __launch_bounds__(256)
+/* WAVES = wavefronts per SIMD the register allocation aims for: 1 gives the
+ * whole evaluation its registers, 2 trades some spills for latency hiding
+ * (chosen per workload, see ngmi_tune) */
+template <int WAVES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 k_bsim4_eval(B4Tab tab, B4Ckt ck, EvalIO io)
 {
     const int64_t n = tab.n;
@@ -75,7 +79,10 @@ int launch_bsim4_eval(const B4Tab &tab, const B4Ckt &ck, const double *vterm,
     EvalIO io{vterm, state[0], state[1], state[2], cache, stamp, rhs, noncon};
     const int bs = 256;
     const int64_t nb = (tab.n + bs - 1) / bs;
-    hipLaunchKernelGGL(k_bsim4_eval, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io);
+    if (tune_get(TUNE_EVAL_WAVES) == 2)
+        hipLaunchKernelGGL(k_bsim4_eval<2>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io);
+    else
+        hipLaunchKernelGGL(k_bsim4_eval<1>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }

@@ -61,7 +61,8 @@ struct TranCache {
     __device__ __forceinline__ void setc(int k, double v) { c_[k * cnm + i] = v; }
 };
 
-__global__ void __launch_bounds__(256)
+template <int WAVES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 k_tran_mos(B4Tab tab, TranDev D)
 {
     const int64_t cnm = (int64_t)D.C * D.NM;
@@ -906,7 +907,11 @@ int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipSt
     const int64_t cnm = (int64_t)D.C * D.NM;
     if (ev) NGMI_HIP(hipEventRecord(ev[0], s));
     if (cnm > 0) {
-        hipLaunchKernelGGL(k_tran_mos, dim3((unsigned)((cnm + 255) / 256)), dim3(256), 0, s, tab, D);
+        const dim3 g((unsigned)((cnm + 255) / 256));
+        if (tune_get(TUNE_MOS_WAVES) == 2)
+            hipLaunchKernelGGL(k_tran_mos<2>, g, dim3(256), 0, s, tab, D);
+        else
+            hipLaunchKernelGGL(k_tran_mos<1>, g, dim3(256), 0, s, tab, D);
         NGMI_HIP(hipGetLastError());
     }
     if (ev) NGMI_HIP(hipEventRecord(ev[1], s));

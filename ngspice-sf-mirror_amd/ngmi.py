@@ -125,6 +125,13 @@ def sync():
     check(lib().ngmi_sync(), 'sync')
 
 
+def tune(knob, value):
+    """launch-shape knob ('eval_waves' / 'mos_waves', 1 or 2)"""
+    L = lib()
+    L.ngmi_tune.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    check(L.ngmi_tune(knob.encode(), int(value)), 'tune %s' % knob)
+
+
 class Model:
     """One BSIM4 .model card at one temperature."""
 
