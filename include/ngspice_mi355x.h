@@ -172,6 +172,22 @@ int ngmi_circuit_set_ic(ngmi_circuit *c, int node, double v);
 /* .options reltol abstol vntol chgtol trtol gmin method(1 trap, 2 gear) maxord */
 int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, double vntol,
                              double chgtol, double trtol, double gmin, int method, int maxord);
+/* Pivot sequence for the static LU: step k eliminates row rows[k] and
+ * column cols[k] (1-based MNA unknowns, n = nodes + voltage sources).  This
+ * is where KLU's own ordering enters: after SMPpreOrder's klu_analyze
+ * (klusmp.c:1255-1285) and SMPreorder's klu_factor (klusmp.c:752-806) the
+ * caller passes Numeric->Pnum[k] + 1 and Symbolic->Q[k] + 1, and every later
+ * refactor runs on that sequence as klu_refactor does (klu_refactor.c:
+ * 18-482).  n = 0 restores the engine's own ordering (source rows paired
+ * with their node rows, then minimum degree).  A batch uses the sequence of
+ * its first circuit. */
+int ngmi_circuit_set_pivot_order(ngmi_circuit *c, int n, const int *rows, const int *cols);
+/* Host-only symbolic analysis of the circuit's topology (no device work):
+ * unknowns, L+U nonzeros, factor terms, strictly-lower L nonzeros, levels
+ * [factor, forward, backward] and the pivot sequence used (rows/cols,
+ * n_unknowns entries each; any output may be NULL). */
+int ngmi_circuit_analyze(const ngmi_circuit *c, int *n_unknowns, int *nslots, int *nterms,
+                         int *nnz_l, int *levels, int *rows, int *cols);
 
 /* A batch of C circuits on the current device: SMPpreOrder/SMPreorder
  * (klusmp.c:752-806, 1255-1285) once for the shared topology, device tables,

@@ -45,6 +45,11 @@ const B4Tab *set_table(const ngmi_bsim4_set *s);
 
 using namespace ngmi;
 
+/* assembly leaves sum at most ASM_LEAF contributions unless a heavy entry
+ * would need more than ASM_MAXPART partial sums */
+constexpr int ASM_LEAF = 16;
+constexpr int ASM_MAXPART = 48;
+
 struct ngmi_circuit {
     int nnodes = 0;
     struct Mos {
@@ -67,6 +72,8 @@ struct ngmi_circuit {
     std::vector<char> ic_given;
     double reltol = 1e-3, abstol = 1e-12, vntol = 1e-6, chgtol = 1e-14, trtol = 7.0, gmin = 1e-12;
     int method = 1, maxord = 2;
+    /* optional pivot sequence (1-based unknowns); empty = the engine's own */
+    std::vector<int> order_rows, order_cols;
 };
 
 struct ngmi_batch {
@@ -187,6 +194,338 @@ std::vector<int> min_degree(int n, std::vector<std::vector<uint64_t>> adj)
     return order;
 }
 
+/* host-side symbolic analysis of one topology, shared by every circuit of
+ * a batch: contributions, pivot sequence, filled L+U pattern, the factor and
+ * solve level programs, assembly plans and the neighbour table */
+struct Plan {
+    std::vector<char> ic_nodes;
+    std::vector<int> rowp, colq;   /* step k pivots on (rowp[k], colq[k]), 1-based */
+    int nslots = 0, nnz_l = 0, nterms = 0, nlev_f = 0, nlev_fw = 0, nlev_bw = 0;
+    std::vector<int> f_lev, f_ent, fw_lev, fw_ent, bw_lev, bw_ent;
+    std::vector<int2> f_term, fw_term, bw_term;
+    std::vector<uint32_t> a_code, r_code;
+    std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
+    int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
+    std::vector<int> slot_flags, perm, ic_row, nbr;
+};
+
+int analyze(const ngmi_circuit &c0, Plan &P)
+{
+    auto fail = [](const char *what) {
+        set_error("%s", what);
+        return NGMI_E_UNSUPP;
+    };
+    const int NN = c0.nnodes, NVS = (int)c0.vs.size(), N = NN + NVS;
+    const int NM = (int)c0.mos.size(), NCAP = (int)c0.caps.size(), NRES = (int)c0.res.size();
+    /* ---- matrix and RHS contributions (original rows/cols, 1-based) */
+    std::vector<Elt> elts;
+    std::vector<std::vector<uint32_t>> rhs_codes(N + 1);
+    static const int rr[16] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3};
+    static const int cc[16] = {0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3};
+    for (int j = 0; j < NM; j++) {
+        const int *nd = c0.mos[j].node;
+        for (int s = 0; s < 16; s++) {
+            const int r = nd[rr[s]], col = nd[cc[s]];
+            if (r && col) elts.push_back({r, col, mk(CT_MOS, (uint32_t)(s * NM + j), false)});
+        }
+        for (int r = 0; r < 4; r++)
+            if (nd[r]) rhs_codes[nd[r]].push_back(mk(CT_MOS, (uint32_t)(r * NM + j), false));
+    }
+    for (int k = 0; k < NCAP; k++) {
+        const int p = (int)c0.caps[k][0], q = (int)c0.caps[k][1];
+        if (p) elts.push_back({p, p, mk(CT_CAP, k, false)});
+        if (q) elts.push_back({q, q, mk(CT_CAP, k, false)});
+        if (p && q) {
+            elts.push_back({p, q, mk(CT_CAP, k, true)});
+            elts.push_back({q, p, mk(CT_CAP, k, true)});
+        }
+        if (p) rhs_codes[p].push_back(mk(CT_CAP, k, true));
+        if (q) rhs_codes[q].push_back(mk(CT_CAP, k, false));
+    }
+    for (int k = 0; k < NRES; k++) {
+        const int p = (int)c0.res[k][0], q = (int)c0.res[k][1];
+        if (p) elts.push_back({p, p, mk(CT_RES, k, false)});
+        if (q) elts.push_back({q, q, mk(CT_RES, k, false)});
+        if (p && q) {
+            elts.push_back({p, q, mk(CT_RES, k, true)});
+            elts.push_back({q, p, mk(CT_RES, k, true)});
+        }
+    }
+    for (int k = 0; k < NVS; k++) {
+        const int p = c0.vs[k].p, q = c0.vs[k].n, br = NN + 1 + k;
+        if (p) { elts.push_back({p, br, mk(CT_ONE, 0, false)}); elts.push_back({br, p, mk(CT_ONE, 0, false)}); }
+        if (q) { elts.push_back({q, br, mk(CT_ONE, 0, true)}); elts.push_back({br, q, mk(CT_ONE, 0, true)}); }
+        rhs_codes[br].push_back(mk(CT_VSRC, k, false));
+    }
+    /* .ic rows: voltage columns are zeroed in the transient operating point
+     * (cktload.c:146-173); the row must hold no branch-current column */
+    P.ic_nodes.assign(NN + 1, 0);
+    for (int u = 1; u <= NN; u++) P.ic_nodes[u] = c0.ic_given[u];
+    for (Elt &e : elts) {
+        if (e.r <= NN && P.ic_nodes[e.r]) {
+            if (e.c > NN) return fail(".ic on a node with a voltage source is not supported");
+            e.code |= CT_ICZ;
+        }
+    }
+
+    /* ---- pivot sequence: step k eliminates row rowp[k], column colq[k]
+     * (1-based unknowns).  Given by the caller (the order klu_factor chose,
+     * Numeric->Pnum / Symbolic->Q), or built here: each source's branch row
+     * is paired with the row of its positive node so both diagonals are the
+     * exact +-1 entries of the source stamp, then a minimum-degree symmetric
+     * ordering of the row-paired pattern limits fill. */
+    std::vector<int> rowp(N), colq(N);
+    if (!c0.order_rows.empty()) {
+        if ((int)c0.order_rows.size() != N) return fail("pivot order length differs from the unknown count");
+        rowp = c0.order_rows;
+        colq = c0.order_cols;
+    } else {
+        std::vector<int> prow(N + 1), prow_inv(N + 1);
+        for (int u = 0; u <= N; u++) prow[u] = u;
+        for (int k = 0; k < NVS; k++) {
+            const int br = NN + 1 + k;
+            const int node = c0.vs[k].p ? c0.vs[k].p : c0.vs[k].n;
+            if (prow[node] != node) return fail("two voltage sources share a node row");
+            prow[node] = br;
+            prow[br] = node;
+        }
+        for (int u = 0; u <= N; u++) prow_inv[prow[u]] = u;
+        std::vector<std::vector<char>> Bp(N, std::vector<char>(N, 0));   /* B = P A, 0-based */
+        for (const Elt &e : elts) Bp[prow_inv[e.r] - 1][e.c - 1] = 1;
+        for (int a = 0; a < N; a++) Bp[a][a] = 1;
+        const int W = (N + 63) / 64;
+        std::vector<std::vector<uint64_t>> adj(N, std::vector<uint64_t>(W, 0));
+        for (int a = 0; a < N; a++)
+            for (int c = 0; c < N; c++)
+                if (a != c && (Bp[a][c] || Bp[c][a])) adj[a][c >> 6] |= 1ull << (c & 63);
+        const std::vector<int> order = min_degree(N, adj);   /* B index eliminated q-th */
+        for (int q = 0; q < N; q++) {
+            rowp[q] = prow[order[q] + 1];
+            colq[q] = order[q] + 1;
+        }
+    }
+    std::vector<int> rinv(N + 1, -1), cinv(N + 1, -1);   /* unknown -> step */
+    for (int k = 0; k < N; k++) {
+        if (rowp[k] < 1 || rowp[k] > N || colq[k] < 1 || colq[k] > N || rinv[rowp[k]] >= 0 ||
+            cinv[colq[k]] >= 0)
+            return fail("pivot order is not a pair of permutations of 1..N");
+        rinv[rowp[k]] = k;
+        cinv[colq[k]] = k;
+    }
+    P.rowp = rowp;
+    P.colq = colq;
+
+    /* ---- filled pattern of M = A(rowp, colq): the pivots, every element
+     * position and the .ic diagonals are structural */
+    std::vector<std::vector<char>> S(N, std::vector<char>(N, 0));
+    for (const Elt &e : elts) S[rinv[e.r]][cinv[e.c]] = 1;
+    for (int k = 0; k < N; k++) S[k][k] = 1;
+    for (int u = 1; u <= NN; u++)
+        if (P.ic_nodes[u]) S[rinv[u]][cinv[u]] = 1;
+    for (int k = 0; k < N; k++)
+        for (int i = k + 1; i < N; i++)
+            if (S[i][k])
+                for (int j = k + 1; j < N; j++)
+                    if (S[k][j]) S[i][j] = 1;
+    std::vector<int> slot((size_t)N * N, -1);
+    int nslots = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++)
+            if (S[i][j]) {
+                slot[(size_t)i * N + j] = nslots++;
+                if (i > j) P.nnz_l++;
+            }
+    P.nslots = nslots;
+
+    /* ---- factor program: entries ordered by m = min(i, j), diagonal first */
+    std::vector<int> lev(nslots, 0);
+    struct Ent { int t, piv, lev; std::vector<int2> terms; };
+    std::vector<Ent> ents;
+    ents.reserve(nslots);
+    for (int m = 0; m < N; m++) {
+        std::vector<std::pair<int, int>> list;
+        list.push_back({m, m});
+        for (int j = m + 1; j < N; j++) if (S[m][j]) list.push_back({m, j});
+        for (int i = m + 1; i < N; i++) if (S[i][m]) list.push_back({i, m});
+        for (auto [i, j] : list) {
+            Ent e;
+            e.t = slot[(size_t)i * N + j];
+            e.piv = (i > j) ? slot[(size_t)j * N + j] : -1;
+            int L = 0;
+            for (int k = 0; k < m; k++) {
+                if (S[i][k] && S[k][j]) {
+                    const int a = slot[(size_t)i * N + k], c2 = slot[(size_t)k * N + j];
+                    e.terms.push_back(make_int2(a, c2));
+                    L = std::max(L, std::max(lev[a], lev[c2]) + 1);
+                }
+            }
+            if (e.piv >= 0) L = std::max(L, lev[e.piv] + 1);
+            lev[e.t] = L;
+            e.lev = L;
+            ents.push_back(std::move(e));
+        }
+    }
+    int nlev = 0;
+    for (const Ent &e : ents) nlev = std::max(nlev, e.lev + 1);
+    std::vector<int> f_lev(nlev + 1, 0), f_ent;
+    std::vector<int2> f_term;
+    {
+        std::vector<std::vector<int>> by(nlev);
+        for (int k = 0; k < (int)ents.size(); k++) by[ents[k].lev].push_back(k);
+        for (int L = 0; L < nlev; L++) {
+            f_lev[L] = (int)(f_ent.size() / 4);
+            for (int k : by[L]) {
+                const Ent &e = ents[k];
+                f_ent.push_back(e.t);
+                f_ent.push_back(e.piv);
+                f_ent.push_back((int)f_term.size());
+                f_term.insert(f_term.end(), e.terms.begin(), e.terms.end());
+                f_ent.push_back((int)f_term.size());
+            }
+        }
+        f_lev[nlev] = (int)(f_ent.size() / 4);
+    }
+    P.nlev_f = nlev;
+    P.nterms = (int)f_term.size();
+
+    /* ---- forward (unit L) and backward programs */
+    std::vector<int> fw_lev, fw_ent, bw_lev, bw_ent;
+    std::vector<int2> fw_term, bw_term;
+    {
+        std::vector<int> lv(N, 0);
+        int nl = 0;
+        for (int a = 0; a < N; a++) {
+            int L = 0;
+            for (int k = 0; k < a; k++) if (S[a][k]) L = std::max(L, lv[k] + 1);
+            lv[a] = L;
+            nl = std::max(nl, L + 1);
+        }
+        std::vector<std::vector<int>> by(nl);
+        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
+        for (int L = 0; L < nl; L++) {
+            fw_lev.push_back((int)(fw_ent.size() / 3));
+            for (int a : by[L]) {
+                fw_ent.push_back(a);
+                fw_ent.push_back((int)fw_term.size());
+                for (int k = 0; k < a; k++)
+                    if (S[a][k]) fw_term.push_back(make_int2(slot[(size_t)a * N + k], k));
+                fw_ent.push_back((int)fw_term.size());
+            }
+        }
+        fw_lev.push_back((int)(fw_ent.size() / 3));
+        P.nlev_fw = nl;
+    }
+    {
+        std::vector<int> lv(N, 0);
+        int nl = 0;
+        for (int a = N - 1; a >= 0; a--) {
+            int L = 0;
+            for (int j = a + 1; j < N; j++) if (S[a][j]) L = std::max(L, lv[j] + 1);
+            lv[a] = L;
+            nl = std::max(nl, L + 1);
+        }
+        std::vector<std::vector<int>> by(nl);
+        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
+        for (int L = 0; L < nl; L++) {
+            bw_lev.push_back((int)(bw_ent.size() / 4));
+            for (int a : by[L]) {
+                bw_ent.push_back(a);
+                bw_ent.push_back(slot[(size_t)a * N + a]);
+                bw_ent.push_back((int)bw_term.size());
+                for (int j = a + 1; j < N; j++)
+                    if (S[a][j]) bw_term.push_back(make_int2(slot[(size_t)a * N + j], j));
+                bw_ent.push_back((int)bw_term.size());
+            }
+        }
+        bw_lev.push_back((int)(bw_ent.size() / 4));
+        P.nlev_bw = nl;
+    }
+
+    /* ---- assembly plans in permuted coordinates */
+    std::vector<std::vector<uint32_t>> per_slot(nslots);
+    std::vector<int> slot_flags(nslots, 0);
+    for (const Elt &e : elts) {
+        const int i = rinv[e.r], j = cinv[e.c];
+        per_slot[slot[(size_t)i * N + j]].push_back(e.code);
+    }
+    for (int u = 1; u <= NN; u++) {
+        if (!P.ic_nodes[u]) continue;
+        const int i = rinv[u], j = cinv[u];
+        slot_flags[slot[(size_t)i * N + j]] |= 1;
+    }
+    /* a slot or row with more than a few contributions is split into
+     * leaves that write partial sums, plus one combine step (the supply
+     * node's diagonal collects one contribution per connected device) */
+    auto split = [](const std::vector<uint32_t> &codes, int dest, int partial_dest0, int &npart,
+                    std::vector<uint32_t> &code, std::vector<int4> &leaf, std::vector<int4> &comb) {
+        const int n = (int)codes.size();
+        const int begin = (int)code.size();
+        code.insert(code.end(), codes.begin(), codes.end());
+        if (n <= ASM_LEAF) {
+            leaf.push_back(make_int4(dest, begin, begin + n, 0));
+            return;
+        }
+        const int chunk = std::max(ASM_LEAF, (n + ASM_MAXPART - 1) / ASM_MAXPART);
+        const int p0 = npart;
+        for (int q = 0; q < n; q += chunk)
+            leaf.push_back(make_int4(partial_dest0 + npart++, begin + q, begin + std::min(n, q + chunk), 0));
+        comb.push_back(make_int4(dest, p0, npart, 0));
+    };
+    std::vector<uint32_t> a_code, r_code;
+    std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
+    int n_apart = 0, n_rpart = 0;
+    for (int s = 0; s < nslots; s++) split(per_slot[s], s, nslots, n_apart, a_code, a_leaf, a_comb);
+    std::vector<int> perm(2 * N), ic_row;
+    for (int a = 0; a < N; a++) {
+        perm[a] = colq[a];                    /* unknown solved in column a */
+        perm[N + a] = rowp[a];                /* original row assembled in row a */
+        split(rhs_codes[rowp[a]], a, N, n_rpart, r_code, r_leaf, r_comb);
+        if (perm[N + a] <= NN && P.ic_nodes[perm[N + a]]) ic_row.push_back(a);
+    }
+    P.n_aleaf = (int)a_leaf.size(); P.n_acomb = (int)a_comb.size();
+    P.n_rleaf = (int)r_leaf.size(); P.n_rcomb = (int)r_comb.size(); P.n_icrow = (int)ic_row.size();
+    P.n_apart = n_apart; P.n_rpart = n_rpart;
+    /* keep every device array non-empty */
+    if (a_comb.empty()) a_comb.push_back(make_int4(0, 0, 0, 0));
+    if (r_comb.empty()) r_comb.push_back(make_int4(0, 0, 0, 0));
+    if (r_leaf.empty()) r_leaf.push_back(make_int4(0, 0, 0, 0));
+    if (r_code.empty()) r_code.push_back(0);
+    if (ic_row.empty()) ic_row.push_back(0);
+
+    /* ---- neighbour table for the learned guess: up to 8 other non-ground
+     * nodes sharing an element with each node, ascending */
+    {
+        std::vector<std::vector<int>> nb(NN + 1);
+        auto link = [&](int a, int c2) {
+            if (a >= 1 && a <= NN && c2 >= 1 && c2 <= NN && a != c2) {
+                nb[a].push_back(c2);
+                nb[c2].push_back(a);
+            }
+        };
+        for (int j = 0; j < NM; j++)
+            for (int r1 = 0; r1 < 4; r1++)
+                for (int r2 = r1 + 1; r2 < 4; r2++) link(c0.mos[j].node[r1], c0.mos[j].node[r2]);
+        for (int k = 0; k < NCAP; k++) link((int)c0.caps[k][0], (int)c0.caps[k][1]);
+        for (int k = 0; k < NRES; k++) link((int)c0.res[k][0], (int)c0.res[k][1]);
+        for (int k = 0; k < NVS; k++) link(c0.vs[k].p, c0.vs[k].n);
+        P.nbr.assign((size_t)(NN + 1) * 8, -1);
+        for (int u = 1; u <= NN; u++) {
+            std::sort(nb[u].begin(), nb[u].end());
+            nb[u].erase(std::unique(nb[u].begin(), nb[u].end()), nb[u].end());
+            for (int k = 0; k < 8 && k < (int)nb[u].size(); k++) P.nbr[(size_t)u * 8 + k] = nb[u][k];
+        }
+    }
+
+    P.f_lev = std::move(f_lev); P.f_ent = std::move(f_ent); P.f_term = std::move(f_term);
+    P.fw_lev = std::move(fw_lev); P.fw_ent = std::move(fw_ent); P.fw_term = std::move(fw_term);
+    P.bw_lev = std::move(bw_lev); P.bw_ent = std::move(bw_ent); P.bw_term = std::move(bw_term);
+    P.a_code = std::move(a_code); P.r_code = std::move(r_code);
+    P.a_leaf = std::move(a_leaf); P.a_comb = std::move(a_comb);
+    P.r_leaf = std::move(r_leaf); P.r_comb = std::move(r_comb);
+    P.slot_flags = std::move(slot_flags); P.perm = std::move(perm); P.ic_row = std::move(ic_row);
+    return NGMI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -281,6 +620,54 @@ int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, doub
     return NGMI_OK;
 }
 
+int ngmi_circuit_set_pivot_order(ngmi_circuit *c, int n, const int *rows, const int *cols)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (n == 0 || !rows || !cols) {
+        c->order_rows.clear();
+        c->order_cols.clear();
+        return NGMI_OK;
+    }
+    const int N = c->nnodes + (int)c->vs.size();
+    if (n != N) {
+        set_error("pivot order has %d steps, the circuit %d unknowns", n, N);
+        return NGMI_E_PARMVAL;
+    }
+    std::vector<char> sr(N + 1, 0), sc(N + 1, 0);
+    for (int k = 0; k < n; k++) {
+        if (rows[k] < 1 || rows[k] > N || cols[k] < 1 || cols[k] > N || sr[rows[k]] || sc[cols[k]]) {
+            set_error("pivot order is not a pair of permutations of 1..%d", N);
+            return NGMI_E_PARMVAL;
+        }
+        sr[rows[k]] = sc[cols[k]] = 1;
+    }
+    c->order_rows.assign(rows, rows + n);
+    c->order_cols.assign(cols, cols + n);
+    return NGMI_OK;
+}
+
+int ngmi_circuit_analyze(const ngmi_circuit *c, int *n_unknowns, int *nslots, int *nterms,
+                         int *nnz_l, int *levels, int *rows, int *cols)
+{
+    if (!c) return NGMI_E_PANIC;
+    Plan P;
+    const int rc = analyze(*c, P);
+    if (rc != NGMI_OK) return rc;
+    const int N = (int)P.rowp.size();
+    if (n_unknowns) *n_unknowns = N;
+    if (nslots) *nslots = P.nslots;
+    if (nterms) *nterms = P.nterms;
+    if (nnz_l) *nnz_l = P.nnz_l;
+    if (levels) {
+        levels[0] = P.nlev_f;
+        levels[1] = P.nlev_fw;
+        levels[2] = P.nlev_bw;
+    }
+    if (rows) std::memcpy(rows, P.rowp.data(), sizeof(int) * N);
+    if (cols) std::memcpy(cols, P.colq.data(), sizeof(int) * N);
+    return NGMI_OK;
+}
+
 ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, int nprobe,
                            int64_t max_points)
 {
@@ -310,257 +697,12 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
     b->NRES = (int)c0.res.size();
     const int N = b->N, NN = b->NN, NM = b->NM, NCAP = b->NCAP, NRES = b->NRES, NVS = b->NVS;
 
-    /* ---- matrix and RHS contributions (original rows/cols, 1-based) */
-    std::vector<Elt> elts;
-    std::vector<std::vector<uint32_t>> rhs_codes(N + 1);
-    static const int rr[16] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3};
-    static const int cc[16] = {0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3};
-    for (int j = 0; j < NM; j++) {
-        const int *nd = c0.mos[j].node;
-        for (int s = 0; s < 16; s++) {
-            const int r = nd[rr[s]], col = nd[cc[s]];
-            if (r && col) elts.push_back({r, col, mk(CT_MOS, (uint32_t)(s * NM + j), false)});
-        }
-        for (int r = 0; r < 4; r++)
-            if (nd[r]) rhs_codes[nd[r]].push_back(mk(CT_MOS, (uint32_t)(r * NM + j), false));
-    }
-    for (int k = 0; k < NCAP; k++) {
-        const int p = (int)c0.caps[k][0], q = (int)c0.caps[k][1];
-        if (p) elts.push_back({p, p, mk(CT_CAP, k, false)});
-        if (q) elts.push_back({q, q, mk(CT_CAP, k, false)});
-        if (p && q) {
-            elts.push_back({p, q, mk(CT_CAP, k, true)});
-            elts.push_back({q, p, mk(CT_CAP, k, true)});
-        }
-        if (p) rhs_codes[p].push_back(mk(CT_CAP, k, true));
-        if (q) rhs_codes[q].push_back(mk(CT_CAP, k, false));
-    }
-    for (int k = 0; k < NRES; k++) {
-        const int p = (int)c0.res[k][0], q = (int)c0.res[k][1];
-        if (p) elts.push_back({p, p, mk(CT_RES, k, false)});
-        if (q) elts.push_back({q, q, mk(CT_RES, k, false)});
-        if (p && q) {
-            elts.push_back({p, q, mk(CT_RES, k, true)});
-            elts.push_back({q, p, mk(CT_RES, k, true)});
-        }
-    }
-    for (int k = 0; k < NVS; k++) {
-        const int p = c0.vs[k].p, q = c0.vs[k].n, br = NN + 1 + k;
-        if (p) { elts.push_back({p, br, mk(CT_ONE, 0, false)}); elts.push_back({br, p, mk(CT_ONE, 0, false)}); }
-        if (q) { elts.push_back({q, br, mk(CT_ONE, 0, true)}); elts.push_back({br, q, mk(CT_ONE, 0, true)}); }
-        rhs_codes[br].push_back(mk(CT_VSRC, k, false));
-    }
-    /* .ic rows: voltage columns are zeroed in the transient operating point
-     * (cktload.c:146-173); the row must hold no branch-current column */
-    b->ic_nodes_host.assign(NN + 1, 0);
-    for (int u = 1; u <= NN; u++) b->ic_nodes_host[u] = c0.ic_given[u];
-    for (Elt &e : elts) {
-        if (e.r <= NN && b->ic_nodes_host[e.r]) {
-            if (e.c > NN) return bail(".ic on a node with a voltage source is not supported");
-            e.code |= CT_ICZ;
-        }
-    }
-
-    /* ---- static row permutation: branch row <-> its positive node row */
-    std::vector<int> prow(N + 1), prow_inv(N + 1);
-    for (int u = 0; u <= N; u++) prow[u] = u;
-    for (int k = 0; k < NVS; k++) {
-        const int br = NN + 1 + k;
-        const int node = c0.vs[k].p ? c0.vs[k].p : c0.vs[k].n;
-        if (prow[node] != node) return bail("two voltage sources share a node row");
-        prow[node] = br;
-        prow[br] = node;
-    }
-    for (int u = 0; u <= N; u++) prow_inv[prow[u]] = u;
-
-    /* ---- pattern of B = P A (0-based), minimum-degree ordering of B + B^T */
-    std::vector<std::vector<char>> Bp(N, std::vector<char>(N, 0));
-    for (const Elt &e : elts) Bp[prow_inv[e.r] - 1][e.c - 1] = 1;
-    for (int a = 0; a < N; a++) Bp[a][a] = 1;
-    const int W = (N + 63) / 64;
-    std::vector<std::vector<uint64_t>> adj(N, std::vector<uint64_t>(W, 0));
-    for (int a = 0; a < N; a++)
-        for (int c = 0; c < N; c++)
-            if (a != c && (Bp[a][c] || Bp[c][a])) adj[a][c >> 6] |= 1ull << (c & 63);
-    const std::vector<int> order = min_degree(N, adj);   /* order[q] = B index eliminated q-th */
-    std::vector<int> oinv(N);
-    for (int q = 0; q < N; q++) oinv[order[q]] = q;
-
-    /* ---- filled pattern of M = Q^T B Q */
-    std::vector<std::vector<char>> S(N, std::vector<char>(N, 0));
-    for (int a = 0; a < N; a++)
-        for (int c = 0; c < N; c++)
-            if (Bp[a][c]) S[oinv[a]][oinv[c]] = 1;
-    for (int k = 0; k < N; k++)
-        for (int i = k + 1; i < N; i++)
-            if (S[i][k])
-                for (int j = k + 1; j < N; j++)
-                    if (S[k][j]) S[i][j] = 1;
-    std::vector<int> slot((size_t)N * N, -1);
-    int nslots = 0;
-    for (int i = 0; i < N; i++)
-        for (int j = 0; j < N; j++)
-            if (S[i][j]) slot[(size_t)i * N + j] = nslots++;
-    b->nslots = nslots;
-
-    /* ---- factor program: entries ordered by m = min(i, j), diagonal first */
-    std::vector<int> lev(nslots, 0);
-    struct Ent { int t, piv, lev; std::vector<int2> terms; };
-    std::vector<Ent> ents;
-    ents.reserve(nslots);
-    for (int m = 0; m < N; m++) {
-        std::vector<std::pair<int, int>> list;
-        list.push_back({m, m});
-        for (int j = m + 1; j < N; j++) if (S[m][j]) list.push_back({m, j});
-        for (int i = m + 1; i < N; i++) if (S[i][m]) list.push_back({i, m});
-        for (auto [i, j] : list) {
-            Ent e;
-            e.t = slot[(size_t)i * N + j];
-            e.piv = (i > j) ? slot[(size_t)j * N + j] : -1;
-            int L = 0;
-            for (int k = 0; k < m; k++) {
-                if (S[i][k] && S[k][j]) {
-                    const int a = slot[(size_t)i * N + k], c2 = slot[(size_t)k * N + j];
-                    e.terms.push_back(make_int2(a, c2));
-                    L = std::max(L, std::max(lev[a], lev[c2]) + 1);
-                }
-            }
-            if (e.piv >= 0) L = std::max(L, lev[e.piv] + 1);
-            lev[e.t] = L;
-            e.lev = L;
-            ents.push_back(std::move(e));
-        }
-    }
-    int nlev = 0;
-    for (const Ent &e : ents) nlev = std::max(nlev, e.lev + 1);
-    std::vector<int> f_lev(nlev + 1, 0), f_ent;
-    std::vector<int2> f_term;
-    {
-        std::vector<std::vector<int>> by(nlev);
-        for (int k = 0; k < (int)ents.size(); k++) by[ents[k].lev].push_back(k);
-        for (int L = 0; L < nlev; L++) {
-            f_lev[L] = (int)(f_ent.size() / 4);
-            for (int k : by[L]) {
-                const Ent &e = ents[k];
-                f_ent.push_back(e.t);
-                f_ent.push_back(e.piv);
-                f_ent.push_back((int)f_term.size());
-                f_term.insert(f_term.end(), e.terms.begin(), e.terms.end());
-                f_ent.push_back((int)f_term.size());
-            }
-        }
-        f_lev[nlev] = (int)(f_ent.size() / 4);
-    }
-    b->nlev_f = nlev;
-    b->nterms = (int)f_term.size();
-
-    /* ---- forward (unit L) and backward programs */
-    std::vector<int> fw_lev, fw_ent, bw_lev, bw_ent;
-    std::vector<int2> fw_term, bw_term;
-    {
-        std::vector<int> lv(N, 0);
-        int nl = 0;
-        for (int a = 0; a < N; a++) {
-            int L = 0;
-            for (int k = 0; k < a; k++) if (S[a][k]) L = std::max(L, lv[k] + 1);
-            lv[a] = L;
-            nl = std::max(nl, L + 1);
-        }
-        std::vector<std::vector<int>> by(nl);
-        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
-        for (int L = 0; L < nl; L++) {
-            fw_lev.push_back((int)(fw_ent.size() / 3));
-            for (int a : by[L]) {
-                fw_ent.push_back(a);
-                fw_ent.push_back((int)fw_term.size());
-                for (int k = 0; k < a; k++)
-                    if (S[a][k]) fw_term.push_back(make_int2(slot[(size_t)a * N + k], k));
-                fw_ent.push_back((int)fw_term.size());
-            }
-        }
-        fw_lev.push_back((int)(fw_ent.size() / 3));
-        b->nlev_fw = nl;
-    }
-    {
-        std::vector<int> lv(N, 0);
-        int nl = 0;
-        for (int a = N - 1; a >= 0; a--) {
-            int L = 0;
-            for (int j = a + 1; j < N; j++) if (S[a][j]) L = std::max(L, lv[j] + 1);
-            lv[a] = L;
-            nl = std::max(nl, L + 1);
-        }
-        std::vector<std::vector<int>> by(nl);
-        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
-        for (int L = 0; L < nl; L++) {
-            bw_lev.push_back((int)(bw_ent.size() / 4));
-            for (int a : by[L]) {
-                bw_ent.push_back(a);
-                bw_ent.push_back(slot[(size_t)a * N + a]);
-                bw_ent.push_back((int)bw_term.size());
-                for (int j = a + 1; j < N; j++)
-                    if (S[a][j]) bw_term.push_back(make_int2(slot[(size_t)a * N + j], j));
-                bw_ent.push_back((int)bw_term.size());
-            }
-        }
-        bw_lev.push_back((int)(bw_ent.size() / 4));
-        b->nlev_bw = nl;
-    }
-
-    /* ---- assembly plans in permuted coordinates */
-    std::vector<std::vector<uint32_t>> per_slot(nslots);
-    std::vector<int> slot_flags(nslots, 0);
-    for (const Elt &e : elts) {
-        const int i = oinv[prow_inv[e.r] - 1], j = oinv[e.c - 1];
-        per_slot[slot[(size_t)i * N + j]].push_back(e.code);
-    }
-    for (int u = 1; u <= NN; u++) {
-        if (!b->ic_nodes_host[u]) continue;
-        const int i = oinv[prow_inv[u] - 1], j = oinv[u - 1];
-        slot_flags[slot[(size_t)i * N + j]] |= 1;
-    }
-    std::vector<int> a_ptr(nslots + 1, 0);
-    std::vector<uint32_t> a_code;
-    for (int s = 0; s < nslots; s++) {
-        a_ptr[s] = (int)a_code.size();
-        a_code.insert(a_code.end(), per_slot[s].begin(), per_slot[s].end());
-    }
-    a_ptr[nslots] = (int)a_code.size();
-    std::vector<int> perm(2 * N), r_ptr(N + 1, 0);
-    std::vector<uint32_t> r_code;
-    for (int a = 0; a < N; a++) {
-        const int bidx = order[a];            /* B index in permuted position a */
-        perm[a] = bidx + 1;                   /* unknown solved in column a */
-        perm[N + a] = prow[bidx + 1];         /* original row assembled in row a */
-        r_ptr[a] = (int)r_code.size();
-        const auto &rc = rhs_codes[prow[bidx + 1]];
-        r_code.insert(r_code.end(), rc.begin(), rc.end());
-    }
-    r_ptr[N] = (int)r_code.size();
-
-    /* ---- neighbour table for the learned guess: up to 8 other non-ground
-     * nodes sharing an element with each node, ascending */
-    {
-        std::vector<std::vector<int>> nb(NN + 1);
-        auto link = [&](int a, int c2) {
-            if (a >= 1 && a <= NN && c2 >= 1 && c2 <= NN && a != c2) {
-                nb[a].push_back(c2);
-                nb[c2].push_back(a);
-            }
-        };
-        for (int j = 0; j < NM; j++)
-            for (int r1 = 0; r1 < 4; r1++)
-                for (int r2 = r1 + 1; r2 < 4; r2++) link(c0.mos[j].node[r1], c0.mos[j].node[r2]);
-        for (int k = 0; k < NCAP; k++) link((int)c0.caps[k][0], (int)c0.caps[k][1]);
-        for (int k = 0; k < NRES; k++) link((int)c0.res[k][0], (int)c0.res[k][1]);
-        for (int k = 0; k < NVS; k++) link(c0.vs[k].p, c0.vs[k].n);
-        b->nbr_host.assign((size_t)(NN + 1) * 8, -1);
-        for (int u = 1; u <= NN; u++) {
-            std::sort(nb[u].begin(), nb[u].end());
-            nb[u].erase(std::unique(nb[u].begin(), nb[u].end()), nb[u].end());
-            for (int k = 0; k < 8 && k < (int)nb[u].size(); k++) b->nbr_host[(size_t)u * 8 + k] = nb[u][k];
-        }
-    }
+    Plan P;
+    if (analyze(c0, P) != NGMI_OK) return bail(nullptr);
+    b->ic_nodes_host = P.ic_nodes;
+    b->nbr_host = P.nbr;
+    b->nslots = P.nslots; b->nterms = P.nterms;
+    b->nlev_f = P.nlev_f; b->nlev_fw = P.nlev_fw; b->nlev_bw = P.nlev_bw;
 
     /* ---- BSIM4 instances, circuit-major */
     b->set = ngmi_bsim4_set_new();
@@ -620,7 +762,9 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
 
     TranDev &D = b->D;
     D.C = C; D.N = N; D.NN = NN; D.NM = NM; D.NCAP = NCAP; D.NRES = NRES; D.NVS = NVS;
-    D.nslots = nslots; D.nlev_f = b->nlev_f; D.nlev_fw = b->nlev_fw; D.nlev_bw = b->nlev_bw;
+    D.nslots = P.nslots; D.nlev_f = b->nlev_f; D.nlev_fw = b->nlev_fw; D.nlev_bw = b->nlev_bw;
+    D.n_aleaf = P.n_aleaf; D.n_acomb = P.n_acomb; D.n_rleaf = P.n_rleaf; D.n_rcomb = P.n_rcomb;
+    D.n_apart = P.n_apart; D.n_rpart = P.n_rpart; D.n_icrow = P.n_icrow;
     D.nprobe = (int)b->probe.size();
     D.cap_pts = b->cap_pts;
     D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
@@ -629,6 +773,7 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
     double *dp;
     uint32_t *up;
     int2 *i2;
+    int4 *i4;
 #define UP(vec, field, T, tmp) do { if (upload<T>(b, vec, &tmp) != NGMI_OK) return bail(nullptr); D.field = tmp; } while (0)
     UP(mos_nodes, mos_nodes, int, ip);
     UP(cap_nodes, cap_nodes, int, ip);
@@ -639,21 +784,24 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
     UP(res_g, res_g, double, dp);
     UP(vs_par, vs_par, double, dp);
     UP(ic_val, ic_val, double, dp);
-    UP(a_ptr, a_ptr, int, ip);
-    UP(a_code, a_code, uint32_t, up);
-    UP(slot_flags, slot_flags, int, ip);
-    UP(r_ptr, r_ptr, int, ip);
-    UP(r_code, r_code, uint32_t, up);
-    UP(f_lev, f_lev, int, ip);
-    UP(f_ent, f_ent, int, ip);
-    UP(f_term, f_term, int2, i2);
-    UP(fw_lev, fw_lev, int, ip);
-    UP(fw_ent, fw_ent, int, ip);
-    UP(fw_term, fw_term, int2, i2);
-    UP(bw_lev, bw_lev, int, ip);
-    UP(bw_ent, bw_ent, int, ip);
-    UP(bw_term, bw_term, int2, i2);
-    UP(perm, perm_unknown, int, ip);
+    UP(P.a_code, a_code, uint32_t, up);
+    UP(P.slot_flags, slot_flags, int, ip);
+    UP(P.r_code, r_code, uint32_t, up);
+    UP(P.a_leaf, a_leaf, int4, i4);
+    UP(P.a_comb, a_comb, int4, i4);
+    UP(P.r_leaf, r_leaf, int4, i4);
+    UP(P.r_comb, r_comb, int4, i4);
+    UP(P.ic_row, ic_row, int, ip);
+    UP(P.f_lev, f_lev, int, ip);
+    UP(P.f_ent, f_ent, int, ip);
+    UP(P.f_term, f_term, int2, i2);
+    UP(P.fw_lev, fw_lev, int, ip);
+    UP(P.fw_ent, fw_ent, int, ip);
+    UP(P.fw_term, fw_term, int2, i2);
+    UP(P.bw_lev, bw_lev, int, ip);
+    UP(P.bw_ent, bw_ent, int, ip);
+    UP(P.bw_term, bw_term, int2, i2);
+    UP(P.perm, perm_unknown, int, ip);
     UP(b->probe, probe, int, ip);
     UP(b->nbr_host, nbr, int, ip);
 #undef UP
@@ -686,7 +834,7 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
     unsigned long long *cp;
     if (alloc(b, 2, &cp) != NGMI_OK) return bail(nullptr);
     D.counters = cp;
-    b->lds = sizeof(double) * ((size_t)nslots + N + 2 * (size_t)NCAP + NVS);
+    b->lds = sizeof(double) * ((size_t)P.nslots + N + 2 * (size_t)NCAP + NVS + P.n_apart + P.n_rpart);
     if (b->lds > 150 * 1024) return bail("circuit too large for the per-circuit LDS factorisation");
     if (b->lds > 64 * 1024 && set_tran_lds_limit(b->lds) != NGMI_OK) return bail(nullptr);
     b->method = c0.method;

@@ -490,12 +490,18 @@ k_tran_step(TranDev D)
         vsv[k] = vsrc_value(D.vs_par + ((int64_t)c * D.NVS + k) * VSP, mode, tnow, K->step);
     __syncthreads();
 
-    /* assembly into the permuted L+U slots (CKTload order is not kept: the
-     * sum per entry runs in element order of the plan) */
+    /* assembly into the permuted L+U slots and RHS rows.  Each leaf sums at
+     * most a few contributions in element order; entries fed by many
+     * elements (a supply node's diagonal) are split into leaves writing
+     * partial sums that one combine step adds in order, so no lane walks
+     * hundreds of contributions alone.  (CKTload's summation order is not
+     * kept; the order here is fixed, so results repeat bit for bit.) */
     const bool icz = (mode & M_TRANOP) && !(mode & M_UIC);
-    for (int s = tid; s < D.nslots; s += nt) {
+    double *part = vsv + D.NVS;
+    for (int e = tid; e < D.n_aleaf; e += nt) {
+        const int4 lf = D.a_leaf[e];
         double acc = 0.0;
-        for (int q = D.a_ptr[s]; q < D.a_ptr[s + 1]; q++) {
+        for (int q = lf.y; q < lf.z; q++) {
             const uint32_t code = D.a_code[q];
             if ((code & CT_ICZ) && icz) continue;
             const uint32_t idx = code & CT_INDEX_MASK;
@@ -512,33 +518,56 @@ k_tran_step(TranDev D)
             }
             acc += (code & CT_NEG) ? -v : v;
         }
-        if ((D.slot_flags[s] & 1) && icz) acc = 1.0;
-        lu[s] = acc;
-    }
-    for (int a = tid; a < N; a += nt) {
-        double acc = 0.0;
-        const int u = D.perm_unknown[N + a];   /* original row of permuted row a */
-        if (icz && u <= D.NN && D.ic_nodes[u]) {
-            acc = D.ic_val[(int64_t)c * (D.NN + 1) + u];
+        if (lf.x < D.nslots) {
+            if ((D.slot_flags[lf.x] & 1) && icz) acc = 1.0;
+            lu[lf.x] = acc;
         } else {
-            for (int q = D.r_ptr[a]; q < D.r_ptr[a + 1]; q++) {
-                const uint32_t code = D.r_code[q];
-                const uint32_t idx = code & CT_INDEX_MASK;
-                double v;
-                switch (code >> CT_TYPE_SHIFT) {
-                case CT_MOS: {
-                    const int r = idx / D.NM, j = idx - r * D.NM;
-                    v = D.mrhs[r * cnm + (int64_t)c * D.NM + j];
-                    break;
-                }
-                case CT_CAP: v = capq[idx]; break;
-                case CT_VSRC: v = vsv[idx]; break;
-                default: v = 0.0; break;
-                }
-                acc += (code & CT_NEG) ? -v : v;
-            }
+            part[lf.x - D.nslots] = acc;
         }
-        z[a] = acc;
+    }
+    for (int e = tid; e < D.n_rleaf; e += nt) {
+        const int4 lf = D.r_leaf[e];
+        double acc = 0.0;
+        for (int q = lf.y; q < lf.z; q++) {
+            const uint32_t code = D.r_code[q];
+            const uint32_t idx = code & CT_INDEX_MASK;
+            double v;
+            switch (code >> CT_TYPE_SHIFT) {
+            case CT_MOS: {
+                const int r = idx / D.NM, j = idx - r * D.NM;
+                v = D.mrhs[r * cnm + (int64_t)c * D.NM + j];
+                break;
+            }
+            case CT_CAP: v = capq[idx]; break;
+            case CT_VSRC: v = vsv[idx]; break;
+            default: v = 0.0; break;
+            }
+            acc += (code & CT_NEG) ? -v : v;
+        }
+        if (lf.x < N) z[lf.x] = acc;
+        else part[D.n_apart + lf.x - N] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < D.n_acomb; e += nt) {
+        const int4 cb = D.a_comb[e];
+        double acc = 0.0;
+        for (int p = cb.y; p < cb.z; p++) acc += part[p];
+        if ((D.slot_flags[cb.x] & 1) && icz) acc = 1.0;
+        lu[cb.x] = acc;
+    }
+    for (int e = tid; e < D.n_rcomb; e += nt) {
+        const int4 cb = D.r_comb[e];
+        double acc = 0.0;
+        for (int p = cb.y; p < cb.z; p++) acc += part[D.n_apart + p];
+        z[cb.x] = acc;
+    }
+    if (icz && D.n_icrow) {
+        __syncthreads();
+        for (int e = tid; e < D.n_icrow; e += nt) {
+            const int a = D.ic_row[e];                 /* permuted row of an .ic node */
+            const int u = D.perm_unknown[N + a];
+            z[a] = D.ic_val[(int64_t)c * (D.NN + 1) + u];
+        }
     }
     __syncthreads();
 

@@ -76,11 +76,17 @@ struct TranDev {
     double *vs_brk;           /* [C][NVS] break_time */
     const double *ic_val;     /* [C][NN+1] */
     /* LU program */
-    const int *a_ptr;         /* [nslots+1] */
+    /* assembly: leaves (dest, begin, end) sum contributions a_code[begin,
+     * end) into L+U slot dest (< nslots) or partial dest - nslots; combines
+     * (slot, pbegin, pend) add partials into a slot; likewise for the RHS
+     * rows (dest < N a row, else partial n_apart + dest - N) */
+    const int4 *a_leaf, *a_comb, *r_leaf, *r_comb;
+    int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
     const uint32_t *a_code;   /* contributions */
     const int *slot_flags;    /* [nslots] bit0: .ic diagonal */
-    const int *r_ptr;         /* [N+1] RHS gather per permuted row */
     const uint32_t *r_code;
+    const int *ic_row;        /* permuted rows of .ic nodes */
+    int n_icrow;
     const int *f_lev;         /* [nlev_f+1] -> entries */
     const int *f_ent;         /* [ne][4]: target, pivot (-1 for U), term begin, term end */
     const int2 *f_term;       /* (a, b) slot pairs */

@@ -281,6 +281,9 @@ def _tran_protos(L):
     L.ngmi_circuit_add_vsrc.argtypes = [vp, i, i, d, ctypes.POINTER(d), i]
     L.ngmi_circuit_set_ic.argtypes = [vp, i, d]
     L.ngmi_circuit_set_options.argtypes = [vp, d, d, d, d, d, d, i, i]
+    ip = ctypes.POINTER(i)
+    L.ngmi_circuit_set_pivot_order.argtypes = [vp, i, ip, ip]
+    L.ngmi_circuit_analyze.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
     L.ngmi_batch_new.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(i), i, i64]
     L.ngmi_batch_new.restype = vp
     L.ngmi_batch_free.argtypes = [vp]
@@ -412,6 +415,37 @@ class Circuit:
                 check(L.ngmi_circuit_add_vsrc(self.h, p, n, dc, arr, len(pulse)), 'add vsrc')
         for node, v in desc.get('ic', {}).items():
             check(L.ngmi_circuit_set_ic(self.h, node, v), 'set ic')
+
+    def set_pivot_order(self, rows, cols):
+        """Pivot sequence of the static LU (1-based unknowns), e.g. KLU's
+        Pnum + 1 / Q + 1; None restores the engine's own ordering."""
+        L = _tran_protos(lib())
+        if rows is None:
+            check(L.ngmi_circuit_set_pivot_order(self.h, 0, None, None), 'pivot order')
+            return
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(L.ngmi_circuit_set_pivot_order(self.h, len(r), r.ctypes.data_as(ip),
+                                             c.ctypes.data_as(ip)), 'pivot order')
+
+    def analyze(self):
+        """Host-only symbolic analysis: dict with unknowns, lu_nonzeros,
+        l_nonzeros (strictly lower), factor_terms, levels and the pivot
+        sequence rows/cols (1-based)."""
+        L = _tran_protos(lib())
+        i = ctypes.c_int
+        n, ns, nt, nl = i(), i(), i(), i()
+        lev = (i * 3)()
+        check(L.ngmi_circuit_analyze(self.h, ctypes.byref(n), ctypes.byref(ns), ctypes.byref(nt),
+                                     ctypes.byref(nl), lev, None, None), 'analyze')
+        rows = np.zeros(n.value, dtype=np.int32)
+        cols = np.zeros(n.value, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(L.ngmi_circuit_analyze(self.h, None, None, None, None, None, rows.ctypes.data_as(ip),
+                                     cols.ctypes.data_as(ip)), 'analyze')
+        return dict(unknowns=n.value, lu_nonzeros=ns.value, l_nonzeros=nl.value,
+                    factor_terms=nt.value, levels=tuple(lev), rows=rows, cols=cols)
 
     def __del__(self):
         try:

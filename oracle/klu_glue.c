@@ -69,3 +69,23 @@ void kg_free(void *hp)
     if (h->sym) klu_free_symbolic(&h->sym, &h->common);
     free(h->Ap); free(h->Ai); free(h);
 }
+
+/* The pivot sequence of the last klu_factor: rows[k] = Numeric->Pnum[k],
+ * cols[k] = Symbolic->Q[k] (0-based), and counts = {lnz, unz, nzoff,
+ * nblocks} (klu.h: nnz of L and U including their diagonals, entries of
+ * the off-diagonal blocks, BTF blocks).  Returns 0, or 1 before a factor. */
+int kg_order(void *hp, int *rows, int *cols, int *counts)
+{
+    kg_t *h = (kg_t *)hp;
+    int k;
+    if (!h || !h->num) return 1;
+    for (k = 0; k < h->n; k++) {
+        rows[k] = h->num->Pnum[k];
+        cols[k] = h->sym->Q[k];
+    }
+    counts[0] = (int)h->num->lnz;
+    counts[1] = (int)h->num->unz;
+    counts[2] = (int)h->num->nzoff;
+    counts[3] = (int)h->sym->nblocks;
+    return 0;
+}

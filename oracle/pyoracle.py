@@ -184,6 +184,8 @@ def _tran_protos():
     L.or_tran_output.argtypes = [vp, dp, dp]
     L.or_tran_free.argtypes = [vp]
     L.or_tran_use_klu.argtypes = [vp, ctypes.c_char_p]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.or_tran_klu_order.argtypes = [vp, ip, ip, ip]
     L._tran_ready = True
     return L
 
@@ -256,6 +258,23 @@ class Transient:
             L.or_tran_output(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                              x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return rc, t[:npts], x[:npts]
+
+    def klu_order(self):
+        """The reference KLU's pivot sequence from its last klu_factor as
+        1-based unknowns: (rows = Pnum + 1, cols = Q + 1, counts {lnz, unz,
+        nzoff, nblocks}); None before a factor or without the reference KLU."""
+        L = _tran_protos()
+        np = self.np
+        n = L.or_tran_n(self.h)
+        rows = np.zeros(n, dtype=np.int32)
+        cols = np.zeros(n, dtype=np.int32)
+        cnt = np.zeros(4, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        if L.or_tran_klu_order(self.h, rows.ctypes.data_as(ip), cols.ctypes.data_as(ip),
+                               cnt.ctypes.data_as(ip)) != 0:
+            return None
+        return rows, cols, dict(lnz=int(cnt[0]), unz=int(cnt[1]), nzoff=int(cnt[2]),
+                                nblocks=int(cnt[3]))
 
     def stats(self):
         L = _tran_protos()

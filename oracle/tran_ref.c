@@ -89,6 +89,7 @@ typedef struct {
     int (*kg_factor)(void *, double *, int, double);
     int (*kg_solve)(void *, double *);
     void (*kg_free)(void *);
+    int (*kg_order)(void *, int *, int *, int *);
 } ORTran;
 
 /* equality.c:22-60 */
@@ -744,6 +745,7 @@ int or_tran_use_klu(ORTran *T, const char *path)
     T->kg_factor = (int (*)(void *, double *, int, double))dlsym(dl, "kg_factor");
     T->kg_solve = (int (*)(void *, double *))dlsym(dl, "kg_solve");
     T->kg_free = (void (*)(void *))dlsym(dl, "kg_free");
+    T->kg_order = (int (*)(void *, int *, int *, int *))dlsym(dl, "kg_order");
     if (!T->kg_new || !T->kg_factor || !T->kg_solve || !T->kg_free) return -2;
     S = (char *)calloc((size_t)(n + 1) * (n + 1), 1);
 #define MARK(a, b) do { if ((a) && (b)) S[(size_t)(a) * (n + 1) + (b)] = 1; } while (0)
@@ -784,6 +786,22 @@ int or_tran_use_klu(ORTran *T, const char *path)
     free(S);
     T->klu = T->kg_new(n, T->Ap, T->Ai);
     return T->klu ? 0 : -3;
+}
+
+/* The reference KLU's pivot sequence from its last klu_factor, as 1-based
+ * unknowns (rows[k] = Pnum[k] + 1, cols[k] = Q[k] + 1), and counts = {lnz,
+ * unz, nzoff, nblocks}.  Returns 0, or nonzero without a KLU factor. */
+int or_tran_klu_order(const ORTran *T, int *rows, int *cols, int *counts)
+{
+    int k, rc;
+    if (!T->klu || !T->kg_order) return -1;
+    rc = T->kg_order(T->klu, rows, cols, counts);
+    if (rc) return rc;
+    for (k = 0; k < T->n; k++) {
+        rows[k] += 1;
+        cols[k] += 1;
+    }
+    return 0;
 }
 
 int64_t or_tran_npts(const ORTran *T) { return T->npts; }

@@ -94,3 +94,29 @@ def test_adder_batch():
     for s, d in enumerate(descs):
         ora = _oracle(d, tstop)
         _compare('adder%d' % s, engs[s], ora, d['nnodes'])
+
+
+def test_klu_pivot_order():
+    """The engine on KLU's own pivot sequence (Pnum/Q of the reference KLU,
+    compiled into oracle/_ref) against the oracle solving with that KLU."""
+    print('case: KLU pivot order, 11-stage ring .tran 10p 2n and adder seed 0 .tran 5p 2.5n',
+          flush=True)
+    for tag, desc, tstop in [('ring11-klu', workloads.ring_oscillator(stages=11, tstop=2e-9), 2e-9),
+                             ('adder0-klu', workloads.mc_adder(0), 2.5e-9)]:
+        T = po.Transient(desc)
+        if not T.use_reference_klu():
+            pytest.skip('reference KLU (oracle/_ref) not built')
+        rc, t, x = T.run(desc['tstep'], tstop)
+        assert rc == 0, rc
+        rows, cols, cnt = T.klu_order()
+        circ = ngmi.Circuit(desc)
+        circ.set_pivot_order(rows, cols)
+        b = ngmi.Batch([circ], max_points=200000)
+        rc, _ = b.tran(desc['tstep'], tstop)
+        assert rc == 0, (rc, ngmi.last_error())
+        info = b.info()
+        print(tag, 'lu', info, 'klu', cnt, flush=True)
+        st = b.stats(0)
+        assert st['status'] == 0, st
+        te, xe = b.waveform(0, info['unknowns'])
+        _compare(tag, (te, xe, st), (t, x, T.stats()), desc['nnodes'])
