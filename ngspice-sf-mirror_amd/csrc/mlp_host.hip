@@ -5,6 +5,7 @@
  * The input layer is 32 wide on the device: 16 features and 16 zero
  * columns, so every matrix-core tile is full.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -20,13 +21,13 @@ int launch_gemm_bf16(const unsigned short *A, int64_t as0, int64_t as1, const un
 __global__ void k_f32_to_bf16(const float *x, unsigned short *y, int64_t n);
 __global__ void k_silu_back(const float *dY, const float *Z, float *dZ, unsigned short *dZb,
                             int64_t n);
-__global__ void k_colsum(const float *dZ, float *db, int M, int N);
+int launch_colsum(const float *Xf, const unsigned short *Xb, const float *wt, float *part, float *out,
+                  int M, int N, hipStream_t s);
 __global__ void k_adam(float *w, unsigned short *wb, const float *g, float *m, float *v, int64_t n,
                        float lr, float b1, float b2, float eps, float c1, float c2);
 __global__ void k_head(const unsigned short *H, const unsigned short *w, const float *b,
                        const float *target, float *y, float *dy, int M, int K, const int *Mdev);
-__global__ void k_head_back(const unsigned short *H, const unsigned short *w, const float *dy,
-                            float *dH, float *dw, float *db, int M, int K);
+__global__ void k_head_back(const unsigned short *w, const float *dy, float *dH, int M, int K);
 }  // namespace ngmi
 
 using namespace ngmi;
@@ -49,13 +50,14 @@ struct ngmi_mlp {
     unsigned short *xb = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *dzb = nullptr;
     float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *dh = nullptr, *dz = nullptr;
     float *y = nullptr, *dy = nullptr;
+    float *part = nullptr;                                          /* column-sum partials */
     int64_t step = 0;
     /* inference-only buffers (BF16 activations, ping-pong) */
     int icap = 0;
     unsigned short *ixb = nullptr, *ia = nullptr, *ib = nullptr;
     ~ngmi_mlp()
     {
-        void *ps[] = {w, g, m, v, wb, xb, h1, h2, h3, dzb, z1, z2, z3, dh, dz, y, dy, ixb, ia, ib};
+        void *ps[] = {w, g, m, v, wb, xb, h1, h2, h3, dzb, z1, z2, z3, dh, dz, y, dy, part, ixb, ia, ib};
         for (void *p : ps) (void)hipFree(p);
     }
     int ensure_infer(int rows)
@@ -87,6 +89,7 @@ struct ngmi_mlp {
         NGMI_HIP(hipMalloc(&dz, r * H * 4));
         NGMI_HIP(hipMalloc(&y, r * 4));
         NGMI_HIP(hipMalloc(&dy, r * 4));
+        if (!part) NGMI_HIP(hipMalloc(&part, (size_t)64 * H * 4));
         cap = rows;
         return NGMI_OK;
     }
@@ -100,17 +103,19 @@ int blocks(int64_t n) { return (int)((n + 255) / 256); }
 __global__ void k_pack_input(const float *__restrict__ X, unsigned short *__restrict__ xb, int M,
                              int Mpad, const int *__restrict__ Mdev = nullptr)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (Mdev) {
         M = min(M, *Mdev);
         Mpad = min(Mpad, (M + 31) / 32 * 32);
     }
-    if (i >= (int64_t)Mpad * IN_PAD) return;
-    const int r = (int)(i / IN_PAD), c = (int)(i % IN_PAD);
-    float v = (r < M && c < MLP_IN) ? X[(int64_t)r * MLP_IN + c] : 0.0f;
-    unsigned int u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    xb[i] = (unsigned short)(u >> 16);
+    const int64_t n = (int64_t)Mpad * IN_PAD;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / IN_PAD), c = (int)(i % IN_PAD);
+        float v = (r < M && c < MLP_IN) ? X[(int64_t)r * MLP_IN + c] : 0.0f;
+        unsigned int u = __float_as_uint(v);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        xb[i] = (unsigned short)(u >> 16);
+    }
 }
 
 int forward(ngmi_mlp *n, const float *X, int M, int Mpad, hipStream_t s)
@@ -143,8 +148,8 @@ int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const i
     const int Mpad = (maxRows + 31) / 32 * 32;
     int rc = n->ensure_infer(Mpad);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pack_input, dim3(blocks((int64_t)Mpad * IN_PAD)), dim3(256), 0, s, X, n->ixb,
-                       maxRows, Mpad, dcount);
+    hipLaunchKernelGGL(k_pack_input, dim3(std::min(blocks((int64_t)Mpad * IN_PAD), 2048)), dim3(256), 0, s,
+                       X, n->ixb, maxRows, Mpad, dcount);
     NGMI_HIP(hipGetLastError());
     const unsigned short *wb = n->wb;
     if ((rc = launch_gemm_bf16(n->ixb, IN_PAD, 1, wb + OFF_W1, IN_PAD, 1, n->w + OFF_B1, nullptr,
@@ -156,7 +161,7 @@ int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const i
     if ((rc = launch_gemm_bf16(n->ib, H, 1, wb + OFF_W3, H, 1, n->w + OFF_B3, nullptr, nullptr, n->ia,
                                H, Mpad, H, H, 2, s, dcount)))
         return rc;
-    hipLaunchKernelGGL(k_head, dim3(blocks(maxRows)), dim3(256), 0, s, n->ia, wb + OFF_W4,
+    hipLaunchKernelGGL(k_head, dim3(std::min(blocks(maxRows), 1024)), dim3(256), 0, s, n->ia, wb + OFF_W4,
                        n->w + OFF_B4, nullptr, y, nullptr, maxRows, H, dcount);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
@@ -259,8 +264,10 @@ int ngmi_mlp_train_step(ngmi_mlp *n, const float *X, const float *target, int M,
     NGMI_HIP(hipGetLastError());
     /* padded rows carry no gradient */
     if (Mpad > M) NGMI_HIP(hipMemsetAsync(n->dy + M, 0, (size_t)(Mpad - M) * 4, s));
-    hipLaunchKernelGGL(k_head_back, dim3(blocks(H)), dim3(256), 0, s, n->h3, wb + OFF_W4, n->dy,
-                       n->dh, g + OFF_W4, g + OFF_B4, Mpad, H);
+    if ((rc = launch_colsum(nullptr, n->h3, n->dy, n->part, g + OFF_W4, Mpad, H, s))) return rc;
+    if ((rc = launch_colsum(n->dy, nullptr, nullptr, n->part, g + OFF_B4, Mpad, 1, s))) return rc;
+    hipLaunchKernelGGL(k_head_back, dim3(blocks((int64_t)Mpad * H)), dim3(256), 0, s, wb + OFF_W4, n->dy,
+                       n->dh, Mpad, H);
     NGMI_HIP(hipGetLastError());
     struct L { const unsigned short *hin; int kin; float *z; int64_t ow, ob; };
     const L layers[3] = {{n->xb, IN_PAD, n->z1, OFF_W1, OFF_B1},
@@ -276,8 +283,7 @@ int ngmi_mlp_train_step(ngmi_mlp *n, const float *X, const float *target, int M,
         if ((rc = launch_gemm_bf16(n->dzb, 1, H, Ly.hin, 1, Ly.kin, nullptr, nullptr, g + Ly.ow,
                                    nullptr, Ly.kin, H, Ly.kin, Mpad, 0, s)))
             return rc;
-        hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, n->dz, g + Ly.ob, Mpad, H);
-        NGMI_HIP(hipGetLastError());
+        if ((rc = launch_colsum(n->dz, nullptr, nullptr, n->part, g + Ly.ob, Mpad, H, s))) return rc;
         if (li > 0) {
             /* dHin[m][i] = sum_o dZ[m][o] W[o][i] */
             if ((rc = launch_gemm_bf16(n->dzb, H, 1, wb + Ly.ow, 1, Ly.kin, nullptr, nullptr, n->dh,

@@ -10,6 +10,7 @@
  * the k quad (l / 32) * 4 .. +3 of every 8-deep step, and receives
  * C rows (v / 4) * 8 + (l / 32) * 4 + v % 4, column l % 32, v = 0..15.
  */
+#include <algorithm>
 #include "ngmi_internal.h"
 #include "mlp_types.h"
 
@@ -34,10 +35,15 @@ __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z));
 enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2 };
 
 /*
- * grid: (N/32, M/32) wavefronts of 64 lanes; M, N multiples of 32, K a
- * multiple of 8.  Outputs: C (fp32, ldc) and, when Cb is set, a BF16 copy of
- * the activated values (ldcb); Z (fp32) receives the pre-activation.
+ * grid: (N/32, tiles) wavefronts of 64 lanes; M, N multiples of 32, K a
+ * multiple of 8.  Row tiles are visited grid-stride, so with Mdev (a device
+ * row count below M) the launch can be sized to the machine and the work
+ * follows the rows actually present.  Outputs: C (fp32, ldc) and, when Cb
+ * is set, a BF16 copy of the activated values; Z (fp32) receives the
+ * pre-activation.  CONTIG: as1 == bs1 == 1, so each lane's k quad is one
+ * 8-byte load per operand.
  */
+template <bool CONTIG>
 __global__ void __launch_bounds__(64)
 k_gemm_bf16(const unsigned short *__restrict__ A, int64_t as0, int64_t as1,
             const unsigned short *__restrict__ B, int64_t bs0, int64_t bs1,
@@ -46,37 +52,41 @@ k_gemm_bf16(const unsigned short *__restrict__ A, int64_t as0, int64_t as1,
             const int *__restrict__ Mdev)
 {
     const int l = threadIdx.x;
-    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
-    if (Mdev) {
-        M = min(M, *Mdev);
-        if (i0 >= M) return;   /* whole wavefront: rows beyond the device-side count */
-    }
+    const int j0 = blockIdx.x * 32;
+    if (Mdev) M = min(M, *Mdev);
     const int r = l & 31, kq = (l >> 5) * 4;
-    f32x16 acc;
+    for (int i0 = blockIdx.y * 32; i0 < M; i0 += gridDim.y * 32) {
+        f32x16 acc;
 #pragma unroll
-    for (int v = 0; v < 16; v++) acc[v] = 0.0f;
-    const unsigned short *pa = A + (int64_t)(i0 + r) * as0;
-    const unsigned short *pb = B + (int64_t)(j0 + r) * bs0;
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        bf16x4 a, b;
+        for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+        const unsigned short *pa = A + (int64_t)(i0 + r) * as0;
+        const unsigned short *pb = B + (int64_t)(j0 + r) * bs0;
+        for (int k0 = 0; k0 < K; k0 += 8) {
+            bf16x4 a, b;
+            if (CONTIG) {
+                a = *reinterpret_cast<const bf16x4 *>(pa + k0 + kq);
+                b = *reinterpret_cast<const bf16x4 *>(pb + k0 + kq);
+            } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            a[q] = (short)pa[(int64_t)(k0 + kq + q) * as1];
-            b[q] = (short)pb[(int64_t)(k0 + kq + q) * bs1];
+                for (int q = 0; q < 4; q++) {
+                    a[q] = (short)pa[(int64_t)(k0 + kq + q) * as1];
+                    b[q] = (short)pb[(int64_t)(k0 + kq + q) * bs1];
+                }
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-    }
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int i = i0 + (v >> 2) * 8 + (l >> 5) * 4 + (v & 3);
-        const int j = j0 + r;
-        if (i >= M || j >= N) continue;
-        float z = acc[v];
-        if (epi != EPI_NONE) z += bias[j];
-        if (Z) Z[(int64_t)i * ldc + j] = z;
-        const float y = (epi == EPI_BIAS_SILU) ? silu(z) : z;
-        if (C) C[(int64_t)i * ldc + j] = y;
-        if (Cb) Cb[(int64_t)i * ldc + j] = f2bf(y);
+        for (int v = 0; v < 16; v++) {
+            const int i = i0 + (v >> 2) * 8 + (l >> 5) * 4 + (v & 3);
+            const int j = j0 + r;
+            if (i >= M || j >= N) continue;
+            float z = acc[v];
+            if (epi != EPI_NONE) z += bias[j];
+            if (Z) Z[(int64_t)i * ldc + j] = z;
+            const float y = (epi == EPI_BIAS_SILU) ? silu(z) : z;
+            if (C) C[(int64_t)i * ldc + j] = y;
+            if (Cb) Cb[(int64_t)i * ldc + j] = f2bf(y);
+        }
     }
 }
 
@@ -99,14 +109,55 @@ __global__ void k_silu_back(const float *__restrict__ dY, const float *__restric
     dZb[i] = f2bf(g);
 }
 
-/* column sums (bias gradients): db[j] = sum_i dZ[i][j] */
-__global__ void k_colsum(const float *__restrict__ dZ, float *__restrict__ db, int M, int N)
+/* two-stage column sums, deterministic: part[p][j] = sum over the rows of
+ * chunk p of wt[i] * X[i][j] (X fp32 or BF16, wt optional), block = 64
+ * columns x 4 row lanes; then out[j] = sum_p part[p][j] in order */
+__global__ void __launch_bounds__(256)
+k_colsum_part(const float *__restrict__ Xf, const unsigned short *__restrict__ Xb,
+              const float *__restrict__ wt, float *__restrict__ part, int M, int N, int chunk)
+{
+    __shared__ float red[4][64];
+    const int cx = threadIdx.x & 63, ry = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + cx;
+    const int i0 = blockIdx.y * chunk, i1 = min(M, i0 + chunk);
+    float s = 0.0f;
+    if (j < N) {
+        for (int i = i0 + ry; i < i1; i += 4) {
+            const float x = Xf ? Xf[(int64_t)i * N + j] : bf2f(Xb[(int64_t)i * N + j]);
+            s += wt ? wt[i] * x : x;
+        }
+    }
+    red[ry][cx] = s;
+    __syncthreads();
+    if (ry == 0 && j < N)
+        part[(int64_t)blockIdx.y * N + j] = ((red[0][cx] + red[1][cx]) + red[2][cx]) + red[3][cx];
+}
+
+__global__ void k_colsum_final(const float *__restrict__ part, float *__restrict__ out, int P, int N)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     float s = 0.0f;
-    for (int i = 0; i < M; i++) s += dZ[(int64_t)i * N + j];
-    db[j] = s;
+    for (int p = 0; p < P; p++) s += part[(int64_t)p * N + j];
+    out[j] = s;
+}
+
+constexpr int COLSUM_PARTS = 64;
+
+/* out[j] = sum_i wt[i] X[i][j] for an M x N matrix; part holds
+ * COLSUM_PARTS x N floats */
+int launch_colsum(const float *Xf, const unsigned short *Xb, const float *wt, float *part, float *out,
+                  int M, int N, hipStream_t s)
+{
+    const int P = std::max(1, std::min(COLSUM_PARTS, (M + 127) / 128));
+    const int chunk = (M + P - 1) / P;
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((N + 63) / 64), (unsigned)P), dim3(256), 0, s,
+                       Xf, Xb, wt, part, M, N, chunk);
+    NGMI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, part, out, P,
+                       N);
+    NGMI_HIP(hipGetLastError());
+    return NGMI_OK;
 }
 
 /* Adam on FP32 master weights, refreshing the BF16 copy */
@@ -133,33 +184,23 @@ __global__ void k_head(const unsigned short *__restrict__ H, const unsigned shor
                        float *__restrict__ y, float *__restrict__ dy, int M, int K,
                        const int *__restrict__ Mdev)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (Mdev) M = min(M, *Mdev);
-    if (i >= M) return;
-    float s = b[0];
-    for (int k = 0; k < K; k++) s += bf2f(H[(int64_t)i * K + k]) * bf2f(w[k]);
-    y[i] = s;
-    if (target) dy[i] = 2.0f * (s - target[i]) / (float)M;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        float s = b[0];
+        for (int k = 0; k < K; k++) s += bf2f(H[(int64_t)i * K + k]) * bf2f(w[k]);
+        y[i] = s;
+        if (target) dy[i] = 2.0f * (s - target[i]) / (float)M;
+    }
 }
 
-/* output layer backward: dH[i][k] = dy[i] w[k] (fp32), dw[k] = sum_i dy[i] h[i][k], db */
-__global__ void k_head_back(const unsigned short *__restrict__ H, const unsigned short *__restrict__ w,
-                            const float *__restrict__ dy, float *__restrict__ dH,
-                            float *__restrict__ dw, float *__restrict__ db, int M, int K)
+/* output layer backward, input gradient: dH[i][k] = dy[i] w[k] (fp32) */
+__global__ void k_head_back(const unsigned short *__restrict__ w, const float *__restrict__ dy,
+                            float *__restrict__ dH, int M, int K)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < K) {
-        float s = 0.0f;
-        for (int i = 0; i < M; i++) s += dy[i] * bf2f(H[(int64_t)i * K + k]);
-        dw[k] = s;
-        const float wk = bf2f(w[k]);
-        for (int i = 0; i < M; i++) dH[(int64_t)i * K + k] = dy[i] * wk;
-    }
-    if (k == 0) {
-        float s = 0.0f;
-        for (int i = 0; i < M; i++) s += dy[i];
-        db[0] = s;
-    }
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)M * K) return;
+    const int i = (int)(e / K), k = (int)(e - (int64_t)i * K);
+    dH[e] = dy[i] * bf2f(w[k]);
 }
 
 int launch_gemm_bf16(const unsigned short *A, int64_t as0, int64_t as1, const unsigned short *B,
@@ -171,8 +212,15 @@ int launch_gemm_bf16(const unsigned short *A, int64_t as0, int64_t as1, const un
         set_error("gemm shape %d x %d x %d not a multiple of 32 x 32 x 8", M, N, K);
         return NGMI_E_PARMVAL;
     }
-    hipLaunchKernelGGL(k_gemm_bf16, dim3(N / 32, M / 32), dim3(64), 0, s, A, as0, as1, B, bs0, bs1,
-                       bias, Z, C, Cb, ldc, M, N, K, epi, Mdev);
+    /* with a device row count the grid is sized to the machine (8 XCDs x
+     * 32 CUs x a few waves per tile column) and strides over the tiles */
+    const int tiles = Mdev ? std::min(M / 32, 2048) : M / 32;
+    if (as1 == 1 && bs1 == 1)
+        hipLaunchKernelGGL(k_gemm_bf16<true>, dim3(N / 32, tiles), dim3(64), 0, s, A, as0, as1, B, bs0,
+                           bs1, bias, Z, C, Cb, ldc, M, N, K, epi, Mdev);
+    else
+        hipLaunchKernelGGL(k_gemm_bf16<false>, dim3(N / 32, tiles), dim3(64), 0, s, A, as0, as1, B,
+                           bs0, bs1, bias, Z, C, Cb, ldc, M, N, K, epi, Mdev);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }

@@ -34,9 +34,9 @@ namespace ngmi {
 int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s,
                      hipEvent_t *ev);
 int set_tran_lds_limit(size_t bytes);
-int launch_guess_features(const TranDev &D, float *F, int *rowmap, double *base, int *count,
+int launch_guess_features(const TranDev &D, float *F, int *rowmap, double2 *base, int *count,
                           hipStream_t s);
-int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double *base,
+int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double2 *base,
                        const int *count, int maxrows, hipStream_t s);
 int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const int *dcount,
                      hipStream_t s);
@@ -96,7 +96,7 @@ struct ngmi_batch {
     ngmi_mlp *net = nullptr;
     float *gF = nullptr, *gy = nullptr;
     int *grow = nullptr, *gcount = nullptr;
-    double *gbase = nullptr;
+    double2 *gbase = nullptr;   /* per row: linear extrapolation, activity */
     std::vector<int> nbr_host;
     unsigned long long counters[2] = {0, 0};
     ~ngmi_batch()

@@ -18,6 +18,7 @@
  * vsrcacct.c:20-160, cktsetbk.c, cktclrbk.c, equality.c:22-60,
  * cktload.c:146-173 (.ic rows in the transient operating point).
  */
+#include <algorithm>
 #include "ngmi_internal.h"
 #include "bsim4_eval.h"
 #include "tran_types.h"
@@ -808,8 +809,9 @@ k_tran_step(TranDev D)
 
 /* ---------------- learned initial guess ---------------- */
 
-constexpr double GUESS_RATIO_MAX = 4.0;    /* step ratios are clamped to [0, 4] */
-constexpr double GUESS_OUT_SCALE = 0.01;   /* network output unit: 10 mV */
+constexpr double GUESS_RATIO_MAX = 4.0;       /* step ratios are clamped to [0, 4] */
+constexpr double GUESS_MIN_ACTIVITY = 1e-5;   /* V: quieter nodes keep the linear extrapolation */
+constexpr double GUESS_CLAMP = 4.0;           /* |network output| limit, in units of activity */
 
 __device__ __forceinline__ double clamp_ratio(double r)
 {
@@ -819,13 +821,18 @@ __device__ __forceinline__ double clamp_ratio(double r)
 /*
  * Features of node u of circuit c for the step the circuit is starting
  * (t_n -> t_n + h), from the last four accepted points v0..v3 at t0 > t1 >
- * t2 > t3:  v0, v0 - v1, v1 - v2, v2 - v3, h / (t0 - t1), (t0 - t1) /
- * (t1 - t2), (t1 - t2) / (t2 - t3), the voltages of up to 8 neighbour nodes
- * minus v0 (0 where absent), 1.  The guess is v0 + r (v0 - v1) + 10 mV * net.
- * Branch currents start from their last accepted value.
+ * t2 > t3, with the node's activity a = |v0 - v1| + |v1 - v2| + |v2 - v3|:
+ *   v0, (v0 - v1) / a, (v1 - v2) / a, (v2 - v3) / a, h / (t0 - t1),
+ *   (t0 - t1) / (t1 - t2), (t1 - t2) / (t2 - t3), the voltages of up to 8
+ *   neighbour nodes minus v0 (0 where absent), log10(a) / 5.
+ * The guess is v0 + r (v0 - v1) + a * clamp(net, +-4): the network predicts
+ * the linear extrapolation's error in units of the node's own recent
+ * motion, so a node at rest keeps the extrapolation.  Nodes with a below
+ * 10 uV take the extrapolation without a network row; branch currents
+ * start from their last accepted value.
  */
 __global__ void __launch_bounds__(256)
-k_guess_features(TranDev D, float *F, int *rowmap, double *base, int *count)
+k_guess_features(TranDev D, float *F, int *rowmap, double2 *base, int *count)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)D.C * D.N) return;
@@ -848,12 +855,19 @@ k_guess_features(TranDev D, float *F, int *rowmap, double *base, int *count)
     const double r0 = clamp_ratio(K->delta / (t0 - t1));
     const double r1 = clamp_ratio((t0 - t1) / (t1 - t2));
     const double r2 = clamp_ratio((t1 - t2) / (t2 - t3));
+    const double d0 = v0 - v1, d1 = v1 - v2, d2 = v2 - v3;
+    const double act = fabs(d0) + fabs(d1) + fabs(d2);
+    const double lin = v0 + r0 * d0;
+    if (!(act >= GUESS_MIN_ACTIVITY)) {
+        xo[u] = lin;
+        return;
+    }
     const int row = atomicAdd(count, 1);
     float *f = F + (int64_t)row * 16;
     f[0] = (float)v0;
-    f[1] = (float)(v0 - v1);
-    f[2] = (float)(v1 - v2);
-    f[3] = (float)(v2 - v3);
+    f[1] = (float)(d0 / act);
+    f[2] = (float)(d1 / act);
+    f[3] = (float)(d2 / act);
     f[4] = (float)r0;
     f[5] = (float)r1;
     f[6] = (float)r2;
@@ -861,20 +875,24 @@ k_guess_features(TranDev D, float *F, int *rowmap, double *base, int *count)
         const int nb = D.nbr[u * 8 + k];
         f[7 + k] = nb >= 0 ? (float)(h0p[nb] - v0) : 0.0f;
     }
-    f[15] = 1.0f;
+    f[15] = (float)(log10(act) / 5.0);
     rowmap[row] = c * (D.N + 1) + u;
-    base[row] = v0 + r0 * (v0 - v1);
+    base[row] = make_double2(lin, act);
 }
 
+/* grid-stride over the rows the features kernel produced */
 __global__ void __launch_bounds__(256)
-k_guess_apply(TranDev D, const float *y, const int *rowmap, const double *base, const int *count)
+k_guess_apply(TranDev D, const float *y, const int *rowmap, const double2 *base, const int *count)
 {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= *count) return;
-    const int m = rowmap[row];
-    const int c = m / (D.N + 1), u = m - c * (D.N + 1);
-    const CktCtl *K = D.ctl + c;
-    D.x[((int64_t)K->xsel * D.C + c) * (D.N + 1) + u] = base[row] + GUESS_OUT_SCALE * (double)y[row];
+    const int n = *count;
+    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+        const int m = rowmap[row];
+        const int c = m / (D.N + 1), u = m - c * (D.N + 1);
+        const CktCtl *K = D.ctl + c;
+        const double2 b = base[row];
+        const double g = fmin(fmax((double)y[row], -GUESS_CLAMP), GUESS_CLAMP);
+        D.x[((int64_t)K->xsel * D.C + c) * (D.N + 1) + u] = b.x + b.y * g;
+    }
 }
 
 /* the first iteration from a guess limits against the last accepted terminal
@@ -897,7 +915,7 @@ k_guess_states(TranDev D)
             D.mstate[((int64_t)p1 * B4X__COUNT + ks[q]) * cnm + i];
 }
 
-int launch_guess_features(const TranDev &D, float *F, int *rowmap, double *base, int *count,
+int launch_guess_features(const TranDev &D, float *F, int *rowmap, double2 *base, int *count,
                           hipStream_t s)
 {
     const int64_t n = (int64_t)D.C * D.N;
@@ -912,10 +930,11 @@ int launch_guess_features(const TranDev &D, float *F, int *rowmap, double *base,
     return NGMI_OK;
 }
 
-int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double *base,
+int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double2 *base,
                        const int *count, int maxrows, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_guess_apply, dim3((unsigned)((maxrows + 255) / 256)), dim3(256), 0, s, D, y,
+    const int blocks = std::max(1, std::min((maxrows + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_guess_apply, dim3((unsigned)blocks), dim3(256), 0, s, D, y,
                        rowmap, base, count);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
