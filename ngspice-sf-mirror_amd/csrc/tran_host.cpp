@@ -611,10 +611,10 @@ int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, doub
 {
     if (!c) return NGMI_E_PANIC;
     if (method != 1 && method != 2) return NGMI_E_PARMVAL;
-    if (maxord < 1 || maxord > (method == 1 ? 2 : 2)) {
-        set_error("maximum order %d not supported (state ring keeps order <= 2)", maxord);
-        return NGMI_E_UNSUPP;
-    }
+    /* cktsopt.c:123-133 clamps maxord to 1..6.  DCtran raises the order
+     * from 1 to 2 only (dctran.c:820-830), so states beyond level 3 are
+     * never read and the engine's four-level ring serves any maxord. */
+    maxord = std::min(std::max(maxord, 1), 6);
     c->reltol = reltol; c->abstol = abstol; c->vntol = vntol; c->chgtol = chgtol;
     c->trtol = trtol; c->gmin = gmin; c->method = method; c->maxord = maxord;
     return NGMI_OK;

@@ -184,10 +184,25 @@ def _tran_protos():
     L.or_tran_output.argtypes = [vp, dp, dp]
     L.or_tran_free.argtypes = [vp]
     L.or_tran_use_klu.argtypes = [vp, ctypes.c_char_p]
+    L.or_comcof.argtypes = [i, i, d, dp, dp]
     ip = ctypes.POINTER(ctypes.c_int)
     L.or_tran_klu_order.argtypes = [vp, ip, ip, ip]
     L._tran_ready = True
     return L
+
+
+def comcof(method, order, delta, delta_old):
+    """NIcomCof restatement: ag[0..6] for method (1 trapezoidal, 2 Gear)."""
+    import numpy as np
+    L = _tran_protos()
+    d = np.zeros(7)
+    d[:len(delta_old)] = delta_old
+    ag = np.zeros(7)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.or_comcof(method, order, delta, d.ctypes.data_as(dp), ag.ctypes.data_as(dp))
+    if rc != 0:
+        raise ValueError('order %d not available for method %d' % (order, method))
+    return ag
 
 
 class Transient:

@@ -65,8 +65,8 @@ typedef struct {
     int maxOrder, method, uic;
     /* state */
     int nstates;
-    double *stbuf;            /* 4 x nstates */
-    double *states[4];
+    double *stbuf;            /* OR_NSTATE x nstates */
+    double *states[8];
     double *A;                /* (n+1) x (n+1), row 0 / col 0 are trash */
     double *rhs, *rhsOld;
     double *lu;
@@ -473,6 +473,24 @@ static int com_cof(ORTran *T)
     return 0;
 }
 
+/* NIcomCof alone: ag[0..6] for method (OR_TRAPEZOIDAL / OR_GEAR), order,
+ * the step delta and deltaOld[0..6].  Returns 0, or -1 for an order the
+ * method does not have. */
+int or_comcof(int method, int order, double delta, const double *deltaOld, double *ag)
+{
+    ORTran *T = (ORTran *)calloc(1, sizeof(ORTran));
+    int k, rc;
+    if (!T) return -1;
+    T->ck.method = method;
+    T->ck.order = order;
+    T->ck.delta = delta;
+    for (k = 0; k < 7; k++) T->ck.deltaOld[k] = deltaOld[k];
+    rc = com_cof(T);
+    for (k = 0; k < 7; k++) ag[k] = T->ck.ag[k];
+    free(T);
+    return rc;
+}
+
 /* ckttrunc.c:20-56 with CAPtrunc and BSIM4trunc */
 static void ckt_trunc(ORTran *T, double *timeStep)
 {
@@ -654,6 +672,9 @@ int or_tran_set_options(ORTran *T, double reltol, double abstol, double vntol, d
                         double trtol, double gmin, int method, int maxord, int uic)
 {
     T->reltol = reltol; T->abstol = abstol; T->vntol = vntol; T->chgtol = chgtol;
+    /* cktsopt.c:123-133: maxord within 1..6 */
+    if (maxord < 1) maxord = 1;
+    if (maxord > 6) maxord = 6;
     T->trtol = trtol; T->gmin = gmin; T->method = method; T->maxOrder = maxord; T->uic = uic;
     return 0;
 }
@@ -718,9 +739,10 @@ int or_tran_setup(ORTran *T)
     T->nbr = T->nvs;
     T->n = T->nnodes + T->nbr;
     for (k = 0; k < T->nvs; k++) T->vs[k].br = T->nnodes + 1 + k;
-    T->stbuf = (double *)calloc((size_t)4 * (T->nstates + 1), sizeof(double));
-    for (k = 0; k < 4; k++) T->states[k] = T->stbuf + (size_t)k * (T->nstates + 1);
-    for (k = 0; k < 4; k++) T->ck.states[k] = T->states[k];
+    /* CKTmaxOrder + 2 state vectors (cktsetup.c:192-194), at most 8 */
+    T->stbuf = (double *)calloc((size_t)8 * (T->nstates + 1), sizeof(double));
+    for (k = 0; k < 8; k++) T->states[k] = T->stbuf + (size_t)k * (T->nstates + 1);
+    for (k = 0; k < 8; k++) T->ck.states[k] = T->states[k];
     T->A = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
     T->lu = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
     T->piv = (int *)calloc((size_t)T->n + 1, sizeof(int));

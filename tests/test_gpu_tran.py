@@ -120,3 +120,20 @@ def test_klu_pivot_order():
         assert st['status'] == 0, st
         te, xe = b.waveform(0, info['unknowns'])
         _compare(tag, (te, xe, st), (t, x, T.stats()), desc['nnodes'])
+
+
+def test_gear_method():
+    """method=gear (NIcomCof's Gear branch, NIintegrate's Gear sum and
+    CKTterr's Gear factor) on the engine against the oracle."""
+    print('case: Gear integration, RC .tran 0.1 7, 11-stage ring .tran 10p 2n, adder seed 0 '
+          '.tran 5p 2.5n', flush=True)
+    rc_desc = {'nnodes': 2, 'models': {}, 'mos': [], 'caps': [(2, 0, 1.0)], 'res': [(1, 2, 1.0)],
+               'vsrc': [(1, 0, 0.0, (0, 1, 0, 0.1, 0.1, 7, 7))], 'ic': {}, 'tstep': 0.1}
+    cases = [('rc-gear', rc_desc, 7.0), ('ring11-gear', workloads.ring_oscillator(stages=11), 2e-9),
+             ('adder0-gear', workloads.mc_adder(0), 2.5e-9)]
+    for tag, desc, tstop in cases:
+        desc = dict(desc)
+        desc['options'] = dict(desc.get('options', {}), method=2, maxord=6)
+        (eng,), info, _ = _engine([desc], tstop)
+        ora = _oracle(desc, tstop)
+        _compare(tag, eng, ora, desc['nnodes'])

@@ -97,3 +97,57 @@ def test_ring_oscillates():
     v1 = x[:, 1]
     late = t > 0.5e-9
     assert v1[late].max() > 0.9 and v1[late].min() < 0.1
+
+
+@pytest.mark.parametrize('order', [1, 2])
+def test_gear_coefficients_differentiate_polynomials_exactly(order):
+    """NIcomCof (nicomcof.c:60-115) for Gear: sum_k ag[k] q(t_k) is the
+    derivative at the new point t_0 of every polynomial of degree <= order
+    through the points t_0, t_0 - h, t_0 - h - h1 (variable-step BDF), and
+    for order 2 equals the closed-form BDF2 weights."""
+    rng = np.random.default_rng(order)
+    for _ in range(50):
+        h, h1, h2 = rng.uniform(1e-13, 1e-9, size=3)
+        ag = po.comcof(2, order, h, [h, h1, h2])
+        tk = np.array([0.0, -h, -h - h1])[:order + 1]
+        w = ag[:order + 1]
+        assert np.all(ag[order + 1:] == 0)
+        for deg in range(order + 1):
+            got = np.sum(w * tk ** deg)
+            want = 1.0 if deg == 1 else 0.0
+            scale = np.sum(np.abs(w * tk ** deg)) + 1.0
+            assert abs(got - want) <= 1e-12 * scale, (deg, got, want)
+        if order == 2:
+            bdf2 = [(2 * h + h1) / (h * (h + h1)), -(h + h1) / (h * h1), h / (h1 * (h + h1))]
+            assert np.allclose(w, bdf2, rtol=1e-12, atol=0)
+
+
+def test_trapezoidal_coefficients():
+    """NIcomCof trapezoidal (nicomcof.c:33-46): order 1 is backward Euler;
+    order 2 with xmu = 0.5 makes NIintegrate's i0 = -i1 ag1 + ag0 (q0 - q1)
+    the trapezoidal rule, exact for a quadratic charge."""
+    h = 3.7e-11
+    ag = po.comcof(1, 1, h, [h])
+    assert ag[0] == 1 / h and ag[1] == -1 / h
+    ag = po.comcof(1, 2, h, [h, 2 * h])
+    t1, t0 = 1.3e-9, 1.3e-9 + h
+    q = lambda t: 4.0 * t * t + 2.0 * t
+    i = lambda t: 8.0 * t + 2.0
+    i0 = -i(t1) * ag[1] + ag[0] * (q(t0) - q(t1))
+    assert abs(i0 - i(t0)) <= 1e-9 * abs(i(t0))
+    with pytest.raises(ValueError):
+        po.comcof(1, 3, h, [h, h, h])
+
+
+def test_gear_rc_transient():
+    """Gear integration (method=gear) on the RC step against its closed form,
+    and against trapezoidal integration of the same circuit."""
+    desc = {'nnodes': 2, 'models': {}, 'mos': [], 'caps': [(2, 0, 1.0)], 'res': [(1, 2, 1.0)],
+            'vsrc': [(1, 0, 0.0, (0, 1, 0, 0.1, 0.1, 7, 7))], 'ic': {}, 'tstep': 0.1,
+            'options': {'method': 2, 'maxord': 6}}
+    T = po.Transient(desc)
+    rc, t, x = T.run(0.1, 7.0)
+    assert rc == 0
+    a = (0.1 - (1 - np.exp(-0.1))) / 0.1
+    ref = np.where(t <= 0.1, (t - (1 - np.exp(-t))) / 0.1, 1 - (1 - a) * np.exp(-(t - 0.1)))
+    assert np.abs(x[:, 2] - ref).max() < 2e-3
