@@ -243,6 +243,60 @@ def bench_mc(args):
     }
 
 
+def bench_rings(args):
+    """The uneven batch: ring oscillators of 11..301 stages (odd, uniform),
+    one topology group per stage count, advanced together; plain predictor."""
+    import torch
+    import ngmi
+    import workloads
+
+    t0 = time.time()
+    descs = workloads.uneven_rings(count=args.circuits, tstop=args.tstop)
+    b = ngmi.Batch([ngmi.Circuit(d) for d in descs], probe=[1], max_points=args.max_points)
+    C = len(descs)
+    unknowns = sum(d['nnodes'] + 1 for d in descs)
+    mosfets = sum(len(d['mos']) for d in descs)
+    del descs
+    log('setup %.1fs: %d rings in %d topology groups, %d unknowns, %d MOSFETs'
+        % (time.time() - t0, C, b.groups(), unknowns, mosfets))
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(args.warmup):
+        rc, passes = b.tran(10e-12, args.tstop, stream=stream)
+        log('warmup rc %d passes %d' % (rc, passes))
+    b.set_profile(True)
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        rc, passes = b.tran(10e-12, args.tstop, stream=stream)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - w0)
+        if rc != 0:
+            raise RuntimeError('transient failed: %d %s' % (rc, ngmi.last_error()))
+    wall = float(np.mean(times))
+    st = [b.stats(c) for c in range(C)]
+    acc = sum(s['accepted'] for s in st)
+    its = sum(s['iterations'] for s in st)
+    mos_ms, step_ms, evals, npasses = b.kernel_times()
+    log('rings: passes %d, accepted %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
+        'step %.1f ms, wall %.1f ms' % (passes, acc, its, sum(1 for s in st if s['status']),
+                                        mos_ms, step_ms, wall * 1e3))
+    return {
+        'metric': 'uneven-batch transient timepoints x circuits/s at reltol 1e-3',
+        'value': acc / wall, 'unit': 'timepoints x circuits/s', 'ms_per_step': wall * 1e3,
+        'dtype': 'f64',
+        'config': {'workload': '%d ring oscillators of 11..301 stages (odd, uniform, seed 3), '
+                               '.tran 10p %gn, reltol 1e-3' % (C, args.tstop * 1e9),
+                   'circuits': C, 'topology_groups': b.groups(), 'unknowns': unknowns,
+                   'mosfets': mosfets},
+        'roofline': None,
+        'extra': {'accepted': acc, 'newton_iterations': its, 'passes': passes,
+                  'iterations_per_point': its / max(acc, 1), 'bsim4_kernel_ms_total': mos_ms,
+                  'step_kernel_ms_total': step_ms,
+                  'failed_circuits': sum(1 for s in st if s['status'])},
+    }
+
+
 def learned_guess_run(args, b, probe, stream, plain_acc, plain_its):
     """Train the guess network on engine waveforms of held-out seeds, then
     run the same batch with the learned guess; returns its figures and the
@@ -357,11 +411,13 @@ def main():
                     help='timed steps (mc: whole .tran runs of the batch; default 1)')
     ap.add_argument('--warmup', type=int, default=None,
                     help='untimed steps before (mc default 1)')
-    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy'])
+    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings'])
     ap.add_argument('--copy-bytes', type=int, default=1 << 30)
     ap.add_argument('--n', type=int, default=1 << 20)
-    ap.add_argument('--circuits', type=int, default=4096)
-    ap.add_argument('--tstop', type=float, default=20e-9)
+    ap.add_argument('--circuits', type=int, default=None,
+                    help='adders (mc, default 4096) or rings (rings, default 2048)')
+    ap.add_argument('--tstop', type=float, default=None,
+                    help='mc default 20 ns, rings default 10 ns')
     ap.add_argument('--max-points', type=int, default=8192)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess run')
@@ -370,10 +426,14 @@ def main():
     ap.add_argument('--train-circuits', type=int, default=32)
     ap.add_argument('--train-steps', type=int, default=1500)
     args = ap.parse_args()
+    if args.circuits is None:
+        args.circuits = 2048 if args.workload == 'rings' else 4096
+    if args.tstop is None:
+        args.tstop = 10e-9 if args.workload == 'rings' else 20e-9
     if args.steps is None:
-        args.steps = 1 if args.workload == 'mc' else 20
+        args.steps = 1 if args.workload in ('mc', 'rings') else 20
     if args.warmup is None:
-        args.warmup = 1 if args.workload == 'mc' else 3
+        args.warmup = 1 if args.workload in ('mc', 'rings') else 3
     if args.gpus != 1:
         print('bench.py runs on one GPU only (the engine has no multi-GPU path); '
               'got --gpus %d' % args.gpus, file=sys.stderr)
@@ -392,6 +452,9 @@ def main():
         cpu = None if args.no_cpu_baseline else cpu_baseline_mc(args.tstop)
     elif args.workload == 'copy':
         res = bench_copy(args)
+        cpu = None
+    elif args.workload == 'rings':
+        res = bench_rings(args)
         cpu = None
     else:
         res = bench_eval(args)

@@ -148,8 +148,8 @@ typedef struct ngmi_mlp ngmi_mlp;
 
 /* A netlist with nodes 1..nnodes (0 = ground): the engine's counterpart of
  * building a CKTcircuit (CKTinit + the device setup calls of CKTsetup,
- * cktsetup.c:33-210).  Element values are per circuit; a batch needs the
- * same topology in every circuit. */
+ * cktsetup.c:33-210).  Element values are per circuit; circuits of equal
+ * topology in one batch share their symbolic analysis. */
 ngmi_circuit *ngmi_circuit_new(int nnodes);
 void ngmi_circuit_free(ngmi_circuit *c);
 /* one BSIM4 instance (BSIM4param, b4par.c:22, for its instance parameters) */
@@ -190,14 +190,23 @@ int ngmi_circuit_analyze(const ngmi_circuit *c, int *n_unknowns, int *nslots, in
                          int *nnz_l, int *levels, int *rows, int *cols);
 
 /* A batch of C circuits on the current device: SMPpreOrder/SMPreorder
- * (klusmp.c:752-806, 1255-1285) once for the shared topology, device tables,
+ * (klusmp.c:752-806, 1255-1285) once per distinct topology, device tables,
  * state and waveform buffers (probe: unknowns recorded at every accepted
- * point, NULL = all; max_points per circuit). */
+ * point, NULL = all of each circuit; max_points per circuit).  Circuits may
+ * differ in topology and size: circuits with equal topology form a group
+ * sharing one symbolic analysis, and every Newton pass is one launch over
+ * all groups. */
 ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, int nprobe,
                            int64_t max_points);
 void ngmi_batch_free(ngmi_batch *b);
-/* unknowns, L+U nonzeros, factor terms, and levels [factor, forward, backward] */
+/* unknowns, L+U nonzeros, factor terms, and levels [factor, forward,
+ * backward] of the group of circuit 0 */
 int ngmi_batch_info(const ngmi_batch *b, int *n_unknowns, int *nslots, int *nterms, int *levels);
+/* number of topology groups */
+int ngmi_batch_groups(const ngmi_batch *b, int *ngroups);
+/* circuit c's group, unknowns, recorded unknowns per point and L+U nonzeros */
+int ngmi_batch_circuit_info(const ngmi_batch *b, int c, int *group, int *n_unknowns, int *nprobe,
+                            int *nslots);
 /* .tran tstep tstop [tmax] for every circuit: CKTop then DCtran
  * (dctran.c:66-971) with NIiter (niiter.c:28-367) per circuit, all on the
  * device; returns when every circuit has finished or failed, or after

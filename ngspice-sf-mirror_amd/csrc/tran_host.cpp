@@ -31,8 +31,9 @@
 #include "tran_types.h"
 
 namespace ngmi {
-int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s,
-                     hipEvent_t *ev);
+int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, const B4Tab *tabs, const TranDev *Ds,
+                     const int2 *mos_blk, int n_mos_blk, const int2 *step_blk, int n_step_blk,
+                     size_t lds_bytes, hipStream_t s, hipEvent_t *ev);
 int set_tran_lds_limit(size_t bytes);
 int launch_guess_features(const TranDev &D, float *F, int *rowmap, double2 *base, int *count,
                           hipStream_t s);
@@ -76,34 +77,56 @@ struct ngmi_circuit {
     std::vector<int> order_rows, order_cols;
 };
 
-struct ngmi_batch {
+/* circuits of one topology inside a batch: shared symbolic analysis and
+ * BSIM4 table, per-circuit device arrays */
+struct TopoGroup {
     int C = 0, N = 0, NN = 0, NM = 0, NCAP = 0, NRES = 0, NVS = 0;
+    std::vector<int> ids;               /* batch index of each circuit of the group */
     ngmi_bsim4_set *set = nullptr;
     TranDev D{};
     size_t lds = 0;
-    std::vector<void *> allocs;
     std::vector<int> probe;
-    int64_t cap_pts = 0;
     /* host copies of the program statistics */
     int nslots = 0, nterms = 0, nlev_f = 0, nlev_fw = 0, nlev_bw = 0;
-    std::vector<double> ic_host;
     std::vector<char> ic_nodes_host;
+    std::vector<int> nbr_host;
     int method = 1, maxord = 2;
+    ~TopoGroup()
+    {
+        if (set) ngmi_bsim4_set_free(set);
+    }
+};
+
+struct ngmi_batch {
+    int C = 0;
+    std::vector<TopoGroup *> groups;
+    std::vector<int2> where;            /* circuit -> (group, index in the group) */
+    std::vector<void *> allocs;
+    int64_t cap_pts = 0;
+    /* one fused launch per pass: group descriptors and block -> group maps */
+    TranDev *dD = nullptr;
+    B4Tab *dTab = nullptr;
+    int2 *mos_blk = nullptr, *step_blk = nullptr;
+    int n_mos_blk = 0, n_step_blk = 0;
+    size_t lds = 0;
+    int *done_count = nullptr;
+    unsigned long long *dcounters = nullptr;
     /* kernel timing of the last run (events on the launch stream) */
     bool profile = false;
     double mos_ms = 0, step_ms = 0, guess_ms = 0;
-    /* learned initial guess */
+    /* learned initial guess (single-topology batches) */
     ngmi_mlp *net = nullptr;
     float *gF = nullptr, *gy = nullptr;
     int *grow = nullptr, *gcount = nullptr;
     double2 *gbase = nullptr;   /* per row: linear extrapolation, activity */
-    std::vector<int> nbr_host;
     unsigned long long counters[2] = {0, 0};
     ~ngmi_batch()
     {
         for (void *p : allocs) (void)hipFree(p);
-        if (set) ngmi_bsim4_set_free(set);
+        for (TopoGroup *g : groups) delete g;
     }
+    const TopoGroup &group_of(int c) const { return *groups[where[c].x]; }
+    const CktCtl *ctl_of(int c) const { return groups[where[c].x]->D.ctl + where[c].y; }
 };
 
 namespace {
@@ -202,7 +225,7 @@ struct Plan {
     std::vector<int> rowp, colq;   /* step k pivots on (rowp[k], colq[k]), 1-based */
     int nslots = 0, nnz_l = 0, nterms = 0, nlev_f = 0, nlev_fw = 0, nlev_bw = 0;
     std::vector<int> f_lev, f_ent, fw_lev, fw_ent, bw_lev, bw_ent;
-    std::vector<int2> f_term, fw_term, bw_term;
+    std::vector<uint32_t> f_term, fw_term, bw_term;   /* packed index pairs, whole uint4 groups */
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
@@ -367,8 +390,21 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     }
     int nlev = 0;
     for (const Ent &e : ents) nlev = std::max(nlev, e.lev + 1);
+    if (nslots + 1 > 65535 || N + 1 > 65535) return fail("circuit too large for 16-bit LU indices");
+    /* term lists: two 16-bit indices per word, padded to whole uint4 groups
+     * with a pair whose product is exactly zero (lu[nslots] and z[N] hold 0);
+     * begin/end are in uint4 units */
+    auto put_terms = [](std::vector<uint32_t> &dst, const std::vector<int2> &t, uint32_t pad) {
+        const int b = (int)(dst.size() / 4);
+        for (const int2 &x : t) dst.push_back((uint32_t)x.x | ((uint32_t)x.y << 16));
+        while (dst.size() % 4) dst.push_back(pad);
+        return make_int2(b, (int)(dst.size() / 4));
+    };
+    const uint32_t pad_lu = (uint32_t)nslots | ((uint32_t)nslots << 16);
+    const uint32_t pad_z = (uint32_t)nslots | ((uint32_t)N << 16);
     std::vector<int> f_lev(nlev + 1, 0), f_ent;
-    std::vector<int2> f_term;
+    std::vector<uint32_t> f_term;
+    int nterms = 0;
     {
         std::vector<std::vector<int>> by(nlev);
         for (int k = 0; k < (int)ents.size(); k++) by[ents[k].lev].push_back(k);
@@ -378,19 +414,20 @@ int analyze(const ngmi_circuit &c0, Plan &P)
                 const Ent &e = ents[k];
                 f_ent.push_back(e.t);
                 f_ent.push_back(e.piv);
-                f_ent.push_back((int)f_term.size());
-                f_term.insert(f_term.end(), e.terms.begin(), e.terms.end());
-                f_ent.push_back((int)f_term.size());
+                const int2 be = put_terms(f_term, e.terms, pad_lu);
+                f_ent.push_back(be.x);
+                f_ent.push_back(be.y);
+                nterms += (int)e.terms.size();
             }
         }
         f_lev[nlev] = (int)(f_ent.size() / 4);
     }
     P.nlev_f = nlev;
-    P.nterms = (int)f_term.size();
+    P.nterms = nterms;
 
     /* ---- forward (unit L) and backward programs */
     std::vector<int> fw_lev, fw_ent, bw_lev, bw_ent;
-    std::vector<int2> fw_term, bw_term;
+    std::vector<uint32_t> fw_term, bw_term;
     {
         std::vector<int> lv(N, 0);
         int nl = 0;
@@ -403,16 +440,19 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         std::vector<std::vector<int>> by(nl);
         for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
         for (int L = 0; L < nl; L++) {
-            fw_lev.push_back((int)(fw_ent.size() / 3));
+            fw_lev.push_back((int)(fw_ent.size() / 4));
             for (int a : by[L]) {
-                fw_ent.push_back(a);
-                fw_ent.push_back((int)fw_term.size());
+                std::vector<int2> t;
                 for (int k = 0; k < a; k++)
-                    if (S[a][k]) fw_term.push_back(make_int2(slot[(size_t)a * N + k], k));
-                fw_ent.push_back((int)fw_term.size());
+                    if (S[a][k]) t.push_back(make_int2(slot[(size_t)a * N + k], k));
+                const int2 be = put_terms(fw_term, t, pad_z);
+                fw_ent.push_back(a);
+                fw_ent.push_back(-1);
+                fw_ent.push_back(be.x);
+                fw_ent.push_back(be.y);
             }
         }
-        fw_lev.push_back((int)(fw_ent.size() / 3));
+        fw_lev.push_back((int)(fw_ent.size() / 4));
         P.nlev_fw = nl;
     }
     {
@@ -429,12 +469,14 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         for (int L = 0; L < nl; L++) {
             bw_lev.push_back((int)(bw_ent.size() / 4));
             for (int a : by[L]) {
+                std::vector<int2> t;
+                for (int j = a + 1; j < N; j++)
+                    if (S[a][j]) t.push_back(make_int2(slot[(size_t)a * N + j], j));
+                const int2 be = put_terms(bw_term, t, pad_z);
                 bw_ent.push_back(a);
                 bw_ent.push_back(slot[(size_t)a * N + a]);
-                bw_ent.push_back((int)bw_term.size());
-                for (int j = a + 1; j < N; j++)
-                    if (S[a][j]) bw_term.push_back(make_int2(slot[(size_t)a * N + j], j));
-                bw_ent.push_back((int)bw_term.size());
+                bw_ent.push_back(be.x);
+                bw_ent.push_back(be.y);
             }
         }
         bw_lev.push_back((int)(bw_ent.size() / 4));
@@ -458,17 +500,26 @@ int analyze(const ngmi_circuit &c0, Plan &P)
      * node's diagonal collects one contribution per connected device) */
     auto split = [](const std::vector<uint32_t> &codes, int dest, int partial_dest0, int &npart,
                     std::vector<uint32_t> &code, std::vector<int4> &leaf, std::vector<int4> &comb) {
+        /* contributions in whole uint4 groups, padded with zero codes;
+         * leaf ranges in uint4 units */
         const int n = (int)codes.size();
-        const int begin = (int)code.size();
-        code.insert(code.end(), codes.begin(), codes.end());
+        auto put = [&](int q0, int q1) {
+            const int b = (int)(code.size() / 4);
+            code.insert(code.end(), codes.begin() + q0, codes.begin() + q1);
+            while (code.size() % 4) code.push_back(CT_ZERO << CT_TYPE_SHIFT);
+            return make_int2(b, (int)(code.size() / 4));
+        };
         if (n <= ASM_LEAF) {
-            leaf.push_back(make_int4(dest, begin, begin + n, 0));
+            const int2 be = put(0, n);
+            leaf.push_back(make_int4(dest, be.x, be.y, 0));
             return;
         }
-        const int chunk = std::max(ASM_LEAF, (n + ASM_MAXPART - 1) / ASM_MAXPART);
+        const int chunk = (std::max(ASM_LEAF, (n + ASM_MAXPART - 1) / ASM_MAXPART) + 3) / 4 * 4;
         const int p0 = npart;
-        for (int q = 0; q < n; q += chunk)
-            leaf.push_back(make_int4(partial_dest0 + npart++, begin + q, begin + std::min(n, q + chunk), 0));
+        for (int q = 0; q < n; q += chunk) {
+            const int2 be = put(q, std::min(n, q + chunk));
+            leaf.push_back(make_int4(partial_dest0 + npart++, be.x, be.y, 0));
+        }
         comb.push_back(make_int4(dest, p0, npart, 0));
     };
     std::vector<uint32_t> a_code, r_code;
@@ -489,7 +540,7 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     if (a_comb.empty()) a_comb.push_back(make_int4(0, 0, 0, 0));
     if (r_comb.empty()) r_comb.push_back(make_int4(0, 0, 0, 0));
     if (r_leaf.empty()) r_leaf.push_back(make_int4(0, 0, 0, 0));
-    if (r_code.empty()) r_code.push_back(0);
+    while (r_code.size() < 4) r_code.push_back(CT_ZERO << CT_TYPE_SHIFT);
     if (ic_row.empty()) ic_row.push_back(0);
 
     /* ---- neighbour table for the learned guess: up to 8 other non-ground
@@ -523,6 +574,167 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     P.a_leaf = std::move(a_leaf); P.a_comb = std::move(a_comb);
     P.r_leaf = std::move(r_leaf); P.r_comb = std::move(r_comb);
     P.slot_flags = std::move(slot_flags); P.perm = std::move(perm); P.ic_row = std::move(ic_row);
+    return NGMI_OK;
+}
+
+/* one topology group of a batch: symbolic analysis, BSIM4 table and the
+ * device arrays of its circuits */
+int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circuit *> &ckts,
+                const int *probe, int nprobe)
+{
+    auto fail = [](const char *what) {
+        if (what) set_error("%s", what);
+        return NGMI_E_UNSUPP;
+    };
+    const ngmi_circuit &c0 = *ckts[0];
+    const int C = (int)ckts.size();
+    G->C = C;
+    G->NN = c0.nnodes;
+    G->NVS = (int)c0.vs.size();
+    G->N = G->NN + G->NVS;
+    G->NM = (int)c0.mos.size();
+    G->NCAP = (int)c0.caps.size();
+    G->NRES = (int)c0.res.size();
+    const int N = G->N, NN = G->NN, NM = G->NM, NCAP = G->NCAP, NRES = G->NRES, NVS = G->NVS;
+
+    Plan P;
+    if (analyze(c0, P) != NGMI_OK) return fail(nullptr);
+    G->ic_nodes_host = P.ic_nodes;
+    G->nbr_host = P.nbr;
+    G->nslots = P.nslots; G->nterms = P.nterms;
+    G->nlev_f = P.nlev_f; G->nlev_fw = P.nlev_fw; G->nlev_bw = P.nlev_bw;
+
+    /* ---- BSIM4 instances, circuit-major */
+    G->set = ngmi_bsim4_set_new();
+    for (int c = 0; c < C; c++) {
+        for (int j = 0; j < NM; j++) {
+            const ngmi_circuit::Mos &mo = ckts[c]->mos[j];
+            std::vector<const char *> nm;
+            for (const auto &s : mo.names) nm.push_back(s.c_str());
+            ngmi_bsim4_model *mp = mo.model;
+            const int zero = 0;
+            const int rc = ngmi_bsim4_set_add(G->set, &mp, 1, &zero, 1, nm.data(), (int)nm.size(),
+                                              mo.values.data());
+            if (rc < 0) return fail(nullptr);
+        }
+    }
+    if (ngmi_bsim4_set_finalize(G->set) != NGMI_OK) return fail(nullptr);
+
+    /* ---- per-circuit element values */
+    std::vector<double> cap_c((size_t)C * NCAP), res_g((size_t)C * NRES), vs_par((size_t)C * NVS * VSP);
+    std::vector<double> ic_val((size_t)C * (NN + 1), 0.0);
+    for (int c = 0; c < C; c++) {
+        const ngmi_circuit &ck = *ckts[c];
+        for (int k = 0; k < NCAP; k++) cap_c[(size_t)c * NCAP + k] = ck.caps[k][2];
+        for (int k = 0; k < NRES; k++) res_g[(size_t)c * NRES + k] = 1.0 / ck.res[k][2];
+        for (int k = 0; k < NVS; k++) {
+            double *v = &vs_par[((size_t)c * NVS + k) * VSP];
+            v[0] = ck.vs[k].dc;
+            v[1] = ck.vs[k].npulse > 0 ? 1.0 : 0.0;
+            v[2] = ck.vs[k].npulse;
+            for (int q = 0; q < 7; q++) v[3 + q] = q < ck.vs[k].npulse ? ck.vs[k].pulse[q] : 0.0;
+        }
+        for (int u = 1; u <= NN; u++) ic_val[(size_t)c * (NN + 1) + u] = ck.ic[u];
+    }
+    std::vector<int> mos_nodes(4 * (size_t)NM), cap_nodes(2 * (size_t)NCAP), res_nodes(2 * (size_t)NRES),
+        vs_nodes(3 * (size_t)NVS), ic_nodes(NN + 1);
+    for (int j = 0; j < NM; j++)
+        for (int r = 0; r < 4; r++) mos_nodes[(size_t)r * NM + j] = c0.mos[j].node[r];
+    for (int k = 0; k < NCAP; k++) {
+        cap_nodes[k] = (int)c0.caps[k][0];
+        cap_nodes[NCAP + k] = (int)c0.caps[k][1];
+    }
+    for (int k = 0; k < NRES; k++) {
+        res_nodes[k] = (int)c0.res[k][0];
+        res_nodes[NRES + k] = (int)c0.res[k][1];
+    }
+    for (int k = 0; k < NVS; k++) {
+        vs_nodes[k] = c0.vs[k].p;
+        vs_nodes[NVS + k] = c0.vs[k].n;
+        vs_nodes[2 * NVS + k] = NN + 1 + k;
+    }
+    for (int u = 0; u <= NN; u++) ic_nodes[u] = G->ic_nodes_host[u];
+    if (probe && nprobe > 0) G->probe.assign(probe, probe + nprobe);
+    else for (int u = 1; u <= N; u++) G->probe.push_back(u);
+    for (int u : G->probe)
+        if (u < 0 || u > N) return fail("probe outside the unknowns");
+    
+    TranDev &D = G->D;
+    D.C = C; D.N = N; D.NN = NN; D.NM = NM; D.NCAP = NCAP; D.NRES = NRES; D.NVS = NVS;
+    D.nslots = P.nslots; D.nlev_f = G->nlev_f; D.nlev_fw = G->nlev_fw; D.nlev_bw = G->nlev_bw;
+    D.n_aleaf = P.n_aleaf; D.n_acomb = P.n_acomb; D.n_rleaf = P.n_rleaf; D.n_rcomb = P.n_rcomb;
+    D.n_apart = P.n_apart; D.n_rpart = P.n_rpart; D.n_icrow = P.n_icrow;
+    D.nprobe = (int)G->probe.size();
+    D.cap_pts = b->cap_pts;
+    D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
+    D.trtol = c0.trtol; D.gmin = c0.gmin;
+    int *ip;
+    double *dp;
+    uint32_t *up;
+    int2 *i2;
+    int4 *i4;
+#define UP(vec, field, T, tmp) do { if (upload<T>(b, vec, &tmp) != NGMI_OK) return fail(nullptr); D.field = tmp; } while (0)
+    UP(mos_nodes, mos_nodes, int, ip);
+    UP(cap_nodes, cap_nodes, int, ip);
+    UP(res_nodes, res_nodes, int, ip);
+    UP(vs_nodes, vs_nodes, int, ip);
+    UP(ic_nodes, ic_nodes, int, ip);
+    UP(cap_c, cap_c, double, dp);
+    UP(res_g, res_g, double, dp);
+    UP(vs_par, vs_par, double, dp);
+    UP(ic_val, ic_val, double, dp);
+    UP(P.a_code, a_code, uint32_t, up);
+    UP(P.slot_flags, slot_flags, int, ip);
+    UP(P.r_code, r_code, uint32_t, up);
+    UP(P.a_leaf, a_leaf, int4, i4);
+    UP(P.a_comb, a_comb, int4, i4);
+    UP(P.r_leaf, r_leaf, int4, i4);
+    UP(P.r_comb, r_comb, int4, i4);
+    UP(P.ic_row, ic_row, int, ip);
+    UP(P.f_lev, f_lev, int, ip);
+    UP(P.f_ent, f_ent, int, ip);
+    UP(P.f_term, f_term, uint32_t, up);
+    UP(P.fw_lev, fw_lev, int, ip);
+    UP(P.fw_ent, fw_ent, int, ip);
+    UP(P.fw_term, fw_term, uint32_t, up);
+    UP(P.bw_lev, bw_lev, int, ip);
+    UP(P.bw_ent, bw_ent, int, ip);
+    UP(P.bw_term, bw_term, uint32_t, up);
+    UP(P.perm, perm_unknown, int, ip);
+    UP(G->probe, probe, int, ip);
+    UP(G->nbr_host, nbr, int, ip);
+#undef UP
+    const int64_t cnm = (int64_t)C * NM, cnc = (int64_t)C * NCAP;
+    if (alloc(b, (size_t)C * NVS, &dp) != NGMI_OK) return fail(nullptr);
+    D.vs_brk = dp;
+    CktCtl *ctl;
+    if (alloc(b, (size_t)C, &ctl) != NGMI_OK) return fail(nullptr);
+    D.ctl = ctl;
+    if (alloc(b, (size_t)2 * C * (N + 1), &dp) != NGMI_OK) return fail(nullptr);
+    D.x = dp;
+    if (alloc(b, (size_t)NLEV * B4X__COUNT * cnm, &dp) != NGMI_OK) return fail(nullptr);
+    D.mstate = dp;
+    if (alloc(b, (size_t)NLEV * 2 * cnc, &dp) != NGMI_OK) return fail(nullptr);
+    D.cstate = dp;
+    if (alloc(b, (size_t)B4C__COUNT * cnm, &dp) != NGMI_OK) return fail(nullptr);
+    D.cache = dp;
+    if (alloc(b, (size_t)B4S__COUNT * cnm, &dp) != NGMI_OK) return fail(nullptr);
+    D.stamp = dp;
+    if (alloc(b, (size_t)B4R__COUNT * cnm, &dp) != NGMI_OK) return fail(nullptr);
+    D.mrhs = dp;
+    if (alloc(b, (size_t)cnm, &ip) != NGMI_OK) return fail(nullptr);
+    D.noncon = ip;
+    if (alloc(b, (size_t)C * b->cap_pts, &dp) != NGMI_OK) return fail(nullptr);
+    D.out_t = dp;
+    if (alloc(b, (size_t)C * b->cap_pts * D.nprobe, &dp) != NGMI_OK) return fail(nullptr);
+    D.out_x = dp;
+    D.done_count = b->done_count;
+    D.counters = b->dcounters;
+    G->lds = sizeof(double) * ((size_t)P.nslots + 1 + N + 1 + 2 * (size_t)NCAP + NVS + P.n_apart +
+                              P.n_rpart);
+    if (G->lds > 150 * 1024) return fail("circuit too large for the per-circuit LDS factorisation");
+    G->method = c0.method;
+    G->maxord = c0.maxord;
     return NGMI_OK;
 }
 
@@ -675,13 +887,6 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
         set_error("empty batch");
         return nullptr;
     }
-    const ngmi_circuit &c0 = *ckts[0];
-    for (int c = 1; c < C; c++) {
-        if (!same_topology(c0, *ckts[c])) {
-            set_error("circuit %d has a different topology than circuit 0", c);
-            return nullptr;
-        }
-    }
     ngmi_batch *b = new ngmi_batch;
     auto bail = [&](const char *what) -> ngmi_batch * {
         if (what) set_error("%s", what);
@@ -689,156 +894,50 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
         return nullptr;
     };
     b->C = C;
-    b->NN = c0.nnodes;
-    b->NVS = (int)c0.vs.size();
-    b->N = b->NN + b->NVS;
-    b->NM = (int)c0.mos.size();
-    b->NCAP = (int)c0.caps.size();
-    b->NRES = (int)c0.res.size();
-    const int N = b->N, NN = b->NN, NM = b->NM, NCAP = b->NCAP, NRES = b->NRES, NVS = b->NVS;
-
-    Plan P;
-    if (analyze(c0, P) != NGMI_OK) return bail(nullptr);
-    b->ic_nodes_host = P.ic_nodes;
-    b->nbr_host = P.nbr;
-    b->nslots = P.nslots; b->nterms = P.nterms;
-    b->nlev_f = P.nlev_f; b->nlev_fw = P.nlev_fw; b->nlev_bw = P.nlev_bw;
-
-    /* ---- BSIM4 instances, circuit-major */
-    b->set = ngmi_bsim4_set_new();
-    for (int c = 0; c < C; c++) {
-        for (int j = 0; j < NM; j++) {
-            const ngmi_circuit::Mos &mo = ckts[c]->mos[j];
-            std::vector<const char *> nm;
-            for (const auto &s : mo.names) nm.push_back(s.c_str());
-            ngmi_bsim4_model *mp = mo.model;
-            const int zero = 0;
-            const int rc = ngmi_bsim4_set_add(b->set, &mp, 1, &zero, 1, nm.data(), (int)nm.size(),
-                                              mo.values.data());
-            if (rc < 0) return bail(nullptr);
-        }
-    }
-    if (ngmi_bsim4_set_finalize(b->set) != NGMI_OK) return bail(nullptr);
-
-    /* ---- per-circuit element values */
-    std::vector<double> cap_c((size_t)C * NCAP), res_g((size_t)C * NRES), vs_par((size_t)C * NVS * VSP);
-    std::vector<double> ic_val((size_t)C * (NN + 1), 0.0);
-    for (int c = 0; c < C; c++) {
-        const ngmi_circuit &ck = *ckts[c];
-        for (int k = 0; k < NCAP; k++) cap_c[(size_t)c * NCAP + k] = ck.caps[k][2];
-        for (int k = 0; k < NRES; k++) res_g[(size_t)c * NRES + k] = 1.0 / ck.res[k][2];
-        for (int k = 0; k < NVS; k++) {
-            double *v = &vs_par[((size_t)c * NVS + k) * VSP];
-            v[0] = ck.vs[k].dc;
-            v[1] = ck.vs[k].npulse > 0 ? 1.0 : 0.0;
-            v[2] = ck.vs[k].npulse;
-            for (int q = 0; q < 7; q++) v[3 + q] = q < ck.vs[k].npulse ? ck.vs[k].pulse[q] : 0.0;
-        }
-        for (int u = 1; u <= NN; u++) ic_val[(size_t)c * (NN + 1) + u] = ck.ic[u];
-    }
-    std::vector<int> mos_nodes(4 * (size_t)NM), cap_nodes(2 * (size_t)NCAP), res_nodes(2 * (size_t)NRES),
-        vs_nodes(3 * (size_t)NVS), ic_nodes(NN + 1);
-    for (int j = 0; j < NM; j++)
-        for (int r = 0; r < 4; r++) mos_nodes[(size_t)r * NM + j] = c0.mos[j].node[r];
-    for (int k = 0; k < NCAP; k++) {
-        cap_nodes[k] = (int)c0.caps[k][0];
-        cap_nodes[NCAP + k] = (int)c0.caps[k][1];
-    }
-    for (int k = 0; k < NRES; k++) {
-        res_nodes[k] = (int)c0.res[k][0];
-        res_nodes[NRES + k] = (int)c0.res[k][1];
-    }
-    for (int k = 0; k < NVS; k++) {
-        vs_nodes[k] = c0.vs[k].p;
-        vs_nodes[NVS + k] = c0.vs[k].n;
-        vs_nodes[2 * NVS + k] = NN + 1 + k;
-    }
-    for (int u = 0; u <= NN; u++) ic_nodes[u] = b->ic_nodes_host[u];
-    if (probe && nprobe > 0) b->probe.assign(probe, probe + nprobe);
-    else for (int u = 1; u <= N; u++) b->probe.push_back(u);
-    for (int u : b->probe)
-        if (u < 0 || u > N) return bail("probe outside the unknowns");
     b->cap_pts = max_points > 0 ? max_points : 1;
-
-    TranDev &D = b->D;
-    D.C = C; D.N = N; D.NN = NN; D.NM = NM; D.NCAP = NCAP; D.NRES = NRES; D.NVS = NVS;
-    D.nslots = P.nslots; D.nlev_f = b->nlev_f; D.nlev_fw = b->nlev_fw; D.nlev_bw = b->nlev_bw;
-    D.n_aleaf = P.n_aleaf; D.n_acomb = P.n_acomb; D.n_rleaf = P.n_rleaf; D.n_rcomb = P.n_rcomb;
-    D.n_apart = P.n_apart; D.n_rpart = P.n_rpart; D.n_icrow = P.n_icrow;
-    D.nprobe = (int)b->probe.size();
-    D.cap_pts = b->cap_pts;
-    D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
-    D.trtol = c0.trtol; D.gmin = c0.gmin;
-    int *ip;
-    double *dp;
-    uint32_t *up;
-    int2 *i2;
-    int4 *i4;
-#define UP(vec, field, T, tmp) do { if (upload<T>(b, vec, &tmp) != NGMI_OK) return bail(nullptr); D.field = tmp; } while (0)
-    UP(mos_nodes, mos_nodes, int, ip);
-    UP(cap_nodes, cap_nodes, int, ip);
-    UP(res_nodes, res_nodes, int, ip);
-    UP(vs_nodes, vs_nodes, int, ip);
-    UP(ic_nodes, ic_nodes, int, ip);
-    UP(cap_c, cap_c, double, dp);
-    UP(res_g, res_g, double, dp);
-    UP(vs_par, vs_par, double, dp);
-    UP(ic_val, ic_val, double, dp);
-    UP(P.a_code, a_code, uint32_t, up);
-    UP(P.slot_flags, slot_flags, int, ip);
-    UP(P.r_code, r_code, uint32_t, up);
-    UP(P.a_leaf, a_leaf, int4, i4);
-    UP(P.a_comb, a_comb, int4, i4);
-    UP(P.r_leaf, r_leaf, int4, i4);
-    UP(P.r_comb, r_comb, int4, i4);
-    UP(P.ic_row, ic_row, int, ip);
-    UP(P.f_lev, f_lev, int, ip);
-    UP(P.f_ent, f_ent, int, ip);
-    UP(P.f_term, f_term, int2, i2);
-    UP(P.fw_lev, fw_lev, int, ip);
-    UP(P.fw_ent, fw_ent, int, ip);
-    UP(P.fw_term, fw_term, int2, i2);
-    UP(P.bw_lev, bw_lev, int, ip);
-    UP(P.bw_ent, bw_ent, int, ip);
-    UP(P.bw_term, bw_term, int2, i2);
-    UP(P.perm, perm_unknown, int, ip);
-    UP(b->probe, probe, int, ip);
-    UP(b->nbr_host, nbr, int, ip);
-#undef UP
-    const int64_t cnm = (int64_t)C * NM, cnc = (int64_t)C * NCAP;
-    if (alloc(b, (size_t)C * NVS, &dp) != NGMI_OK) return bail(nullptr);
-    D.vs_brk = dp;
-    CktCtl *ctl;
-    if (alloc(b, (size_t)C, &ctl) != NGMI_OK) return bail(nullptr);
-    D.ctl = ctl;
-    if (alloc(b, (size_t)2 * C * (N + 1), &dp) != NGMI_OK) return bail(nullptr);
-    D.x = dp;
-    if (alloc(b, (size_t)NLEV * B4X__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
-    D.mstate = dp;
-    if (alloc(b, (size_t)NLEV * 2 * cnc, &dp) != NGMI_OK) return bail(nullptr);
-    D.cstate = dp;
-    if (alloc(b, (size_t)B4C__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
-    D.cache = dp;
-    if (alloc(b, (size_t)B4S__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
-    D.stamp = dp;
-    if (alloc(b, (size_t)B4R__COUNT * cnm, &dp) != NGMI_OK) return bail(nullptr);
-    D.mrhs = dp;
-    if (alloc(b, (size_t)cnm, &ip) != NGMI_OK) return bail(nullptr);
-    D.noncon = ip;
-    if (alloc(b, (size_t)C * b->cap_pts, &dp) != NGMI_OK) return bail(nullptr);
-    D.out_t = dp;
-    if (alloc(b, (size_t)C * b->cap_pts * D.nprobe, &dp) != NGMI_OK) return bail(nullptr);
-    D.out_x = dp;
-    if (alloc(b, 1, &ip) != NGMI_OK) return bail(nullptr);
-    D.done_count = ip;
-    unsigned long long *cp;
-    if (alloc(b, 2, &cp) != NGMI_OK) return bail(nullptr);
-    D.counters = cp;
-    b->lds = sizeof(double) * ((size_t)P.nslots + N + 2 * (size_t)NCAP + NVS + P.n_apart + P.n_rpart);
-    if (b->lds > 150 * 1024) return bail("circuit too large for the per-circuit LDS factorisation");
+    /* circuits sharing a topology form one group, groups in order of first
+     * appearance */
+    std::vector<std::vector<int>> members;
+    b->where.resize(C);
+    for (int c = 0; c < C; c++) {
+        int g = -1;
+        for (size_t q = 0; q < members.size() && g < 0; q++)
+            if (same_topology(*ckts[members[q][0]], *ckts[c])) g = (int)q;
+        if (g < 0) {
+            g = (int)members.size();
+            members.emplace_back();
+        }
+        b->where[c] = make_int2(g, (int)members[g].size());
+        members[g].push_back(c);
+    }
+    if (alloc(b, 1, &b->done_count) != NGMI_OK || alloc(b, 2, &b->dcounters) != NGMI_OK)
+        return bail(nullptr);
+    std::vector<TranDev> Ds;
+    std::vector<B4Tab> tabs;
+    std::vector<int2> mb, sb;
+    for (size_t g = 0; g < members.size(); g++) {
+        TopoGroup *G = new TopoGroup;
+        b->groups.push_back(G);
+        G->ids = members[g];
+        std::vector<const ngmi_circuit *> cs;
+        for (int c : members[g]) cs.push_back(ckts[c]);
+        if (build_group(b, G, cs, probe, nprobe) != NGMI_OK) return bail(nullptr);
+        Ds.push_back(G->D);
+        tabs.push_back(*set_table(G->set));
+        /* MOSFET kernel: whole blocks per group; (group, first block) */
+        const int64_t cnm = (int64_t)G->C * G->NM;
+        const int first = (int)mb.size();
+        for (int64_t k = 0; k < (cnm + 255) / 256; k++) mb.push_back(make_int2((int)g, first));
+        /* step kernel: one block per circuit; (group, circuit) */
+        for (int c = 0; c < G->C; c++) sb.push_back(make_int2((int)g, c));
+        b->lds = std::max(b->lds, G->lds);
+    }
+    b->n_mos_blk = (int)mb.size();
+    b->n_step_blk = (int)sb.size();
+    if (upload<TranDev>(b, Ds, &b->dD) != NGMI_OK || upload<B4Tab>(b, tabs, &b->dTab) != NGMI_OK ||
+        upload<int2>(b, mb, &b->mos_blk) != NGMI_OK || upload<int2>(b, sb, &b->step_blk) != NGMI_OK)
+        return bail(nullptr);
     if (b->lds > 64 * 1024 && set_tran_lds_limit(b->lds) != NGMI_OK) return bail(nullptr);
-    b->method = c0.method;
-    b->maxord = c0.maxord;
     return b;
 }
 
@@ -847,14 +946,34 @@ void ngmi_batch_free(ngmi_batch *b) { delete b; }
 int ngmi_batch_info(const ngmi_batch *b, int *n_unknowns, int *nslots, int *nterms, int *levels)
 {
     if (!b) return NGMI_E_PANIC;
-    if (n_unknowns) *n_unknowns = b->N;
-    if (nslots) *nslots = b->nslots;
-    if (nterms) *nterms = b->nterms;
+    const TopoGroup &G = *b->groups[0];
+    if (n_unknowns) *n_unknowns = G.N;
+    if (nslots) *nslots = G.nslots;
+    if (nterms) *nterms = G.nterms;
     if (levels) {
-        levels[0] = b->nlev_f;
-        levels[1] = b->nlev_fw;
-        levels[2] = b->nlev_bw;
+        levels[0] = G.nlev_f;
+        levels[1] = G.nlev_fw;
+        levels[2] = G.nlev_bw;
     }
+    return NGMI_OK;
+}
+
+int ngmi_batch_groups(const ngmi_batch *b, int *ngroups)
+{
+    if (!b || !ngroups) return NGMI_E_PANIC;
+    *ngroups = (int)b->groups.size();
+    return NGMI_OK;
+}
+
+int ngmi_batch_circuit_info(const ngmi_batch *b, int c, int *group, int *n_unknowns, int *nprobe,
+                            int *nslots)
+{
+    if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
+    const TopoGroup &G = b->group_of(c);
+    if (group) *group = b->where[c].x;
+    if (n_unknowns) *n_unknowns = G.N;
+    if (nprobe) *nprobe = (int)G.probe.size();
+    if (nslots) *nslots = G.nslots;
     return NGMI_OK;
 }
 
@@ -865,49 +984,51 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
 {
     if (!b) return NGMI_E_PANIC;
     if (!(tstep > 0) || !(tstop > 0)) return NGMI_E_PARMVAL;
-    TranDev &D = b->D;
     const hipStream_t s = (hipStream_t)stream;
     const int C = b->C;
-    std::vector<CktCtl> ctl(C);
     double maxStep = tmax;
     if (maxStep == 0) maxStep = (tstep < tstop / 50.0) ? tstep : tstop / 50.0;
-    for (int c = 0; c < C; c++) {
-        CktCtl &k = ctl[c];
-        std::memset(&k, 0, sizeof(k));
-        k.finalTime = tstop;
-        k.step = tstep;
-        k.maxStep = maxStep;
-        k.delmin = 1e-11 * maxStep;
-        k.minBreak = 10.0 * k.delmin;
-        k.delta = std::min(tstop / 100, tstep) / 10;
-        k.breaks[0] = 0;
-        k.breaks[1] = tstop;
-        k.nbreaks = 2;
-        k.time = 0;
-        k.firsttime = 1;
-        k.mode = NGMI_MODETRANOP | NGMI_MODEINITJCT;
-        k.order = 1;
-        k.method = b->method;
-        k.maxOrder = b->maxord;
-        k.phase = PH_OP;
-        for (int i = 0; i < 7; i++) k.deltaOld[i] = maxStep;
-        k.learned = b->net ? 1 : 0;
-    }
-    NGMI_HIP(hipMemcpyAsync(D.ctl, ctl.data(), sizeof(CktCtl) * C, hipMemcpyHostToDevice, s));
-    const int64_t cnm = (int64_t)C * b->NM, cnc = (int64_t)C * b->NCAP;
-    NGMI_HIP(hipMemsetAsync(D.x, 0, sizeof(double) * 2 * C * (size_t)(b->N + 1), s));
-    NGMI_HIP(hipMemsetAsync(D.mstate, 0, sizeof(double) * NLEV * B4X__COUNT * (size_t)cnm, s));
-    NGMI_HIP(hipMemsetAsync(D.cstate, 0, sizeof(double) * NLEV * 2 * (size_t)cnc, s));
-    NGMI_HIP(hipMemsetAsync(D.cache, 0, sizeof(double) * B4C__COUNT * (size_t)cnm, s));
-    NGMI_HIP(hipMemsetAsync(D.done_count, 0, sizeof(int), s));
-    NGMI_HIP(hipMemsetAsync(D.counters, 0, 2 * sizeof(unsigned long long), s));
-    {
-        std::vector<double> brk((size_t)C * b->NVS, -1.0);
+    for (TopoGroup *G : b->groups) {
+        const TranDev &D = G->D;
+        std::vector<CktCtl> ctl(G->C);
+        for (int c = 0; c < G->C; c++) {
+            CktCtl &k = ctl[c];
+            std::memset(&k, 0, sizeof(k));
+            k.finalTime = tstop;
+            k.step = tstep;
+            k.maxStep = maxStep;
+            k.delmin = 1e-11 * maxStep;
+            k.minBreak = 10.0 * k.delmin;
+            k.delta = std::min(tstop / 100, tstep) / 10;
+            k.breaks[0] = 0;
+            k.breaks[1] = tstop;
+            k.nbreaks = 2;
+            k.time = 0;
+            k.firsttime = 1;
+            k.mode = NGMI_MODETRANOP | NGMI_MODEINITJCT;
+            k.order = 1;
+            k.method = G->method;
+            k.maxOrder = G->maxord;
+            k.phase = PH_OP;
+            for (int i = 0; i < 7; i++) k.deltaOld[i] = maxStep;
+            k.learned = b->net ? 1 : 0;
+        }
+        NGMI_HIP(hipMemcpyAsync(D.ctl, ctl.data(), sizeof(CktCtl) * G->C, hipMemcpyHostToDevice, s));
+        const int64_t cnm = (int64_t)G->C * G->NM, cnc = (int64_t)G->C * G->NCAP;
+        NGMI_HIP(hipMemsetAsync(D.x, 0, sizeof(double) * 2 * G->C * (size_t)(G->N + 1), s));
+        NGMI_HIP(hipMemsetAsync(D.mstate, 0, sizeof(double) * NLEV * B4X__COUNT * (size_t)cnm, s));
+        NGMI_HIP(hipMemsetAsync(D.cstate, 0, sizeof(double) * NLEV * 2 * (size_t)cnc, s));
+        NGMI_HIP(hipMemsetAsync(D.cache, 0, sizeof(double) * B4C__COUNT * (size_t)cnm, s));
+        std::vector<double> brk((size_t)G->C * G->NVS, -1.0);
         NGMI_HIP(hipMemcpyAsync(D.vs_brk, brk.data(), brk.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
         NGMI_HIP(hipStreamSynchronize(s));
     }
-    const B4Tab *tab = set_table(b->set);
+    NGMI_HIP(hipMemsetAsync(b->done_count, 0, sizeof(int), s));
+    NGMI_HIP(hipMemsetAsync(b->dcounters, 0, 2 * sizeof(unsigned long long), s));
+    const TopoGroup &G0 = *b->groups[0];
+    const bool one = b->groups.size() == 1;
+    const B4Tab *tab0 = set_table(G0.set);
     int64_t pass = 0;
     int done = 0;
     const int check_every = 32;
@@ -927,19 +1048,21 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         for (; k < check_every && pass < max_passes; k++, pass++) {
             if (b->net) {
                 if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k], s));
-                const int maxrows = C * b->N;
+                const int maxrows = G0.C * G0.N;
                 NGMI_HIP(hipMemsetAsync(b->gcount, 0, sizeof(int), s));
-                rc = launch_guess_features(D, b->gF, b->grow, b->gbase, b->gcount, s);
+                rc = launch_guess_features(G0.D, b->gF, b->grow, b->gbase, b->gcount, s);
                 if (rc == NGMI_OK) rc = mlp_forward_rows(b->net, b->gF, b->gy, maxrows, b->gcount, s);
                 if (rc == NGMI_OK)
-                    rc = launch_guess_apply(D, b->gy, b->grow, b->gbase, b->gcount, maxrows, s);
+                    rc = launch_guess_apply(G0.D, b->gy, b->grow, b->gbase, b->gcount, maxrows, s);
                 if (rc != NGMI_OK) break;
                 if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k + 1], s));
             }
-            rc = launch_tran_pass(*tab, D, b->lds, s, b->profile ? &ev[3 * k] : nullptr);
+            rc = launch_tran_pass(one ? tab0 : nullptr, one ? &G0.D : nullptr, b->dTab, b->dD,
+                                  b->mos_blk, b->n_mos_blk, b->step_blk, b->n_step_blk, b->lds, s,
+                                  b->profile ? &ev[3 * k] : nullptr);
             if (rc != NGMI_OK) break;
         }
-        if (hipMemcpyAsync(&done, D.done_count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        if (hipMemcpyAsync(&done, b->done_count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess) {
             set_error("transient pass failed: %s", hipGetErrorString(hipGetLastError()));
             rc = NGMI_E_DEVICE;
@@ -962,7 +1085,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     for (auto &e : gev) (void)hipEventDestroy(e);
-    NGMI_HIP(hipMemcpy(b->counters, D.counters, sizeof(b->counters), hipMemcpyDeviceToHost));
+    NGMI_HIP(hipMemcpy(b->counters, b->dcounters, sizeof(b->counters), hipMemcpyDeviceToHost));
     if (passes_out) *passes_out = pass;
     if (rc != NGMI_OK) return rc;
     return done >= C ? NGMI_OK : NGMI_E_ITERLIM;
@@ -971,17 +1094,23 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
 int ngmi_batch_set_guess(ngmi_batch *b, ngmi_mlp *net)
 {
     if (!b) return NGMI_E_PANIC;
+    if (net && b->groups.size() != 1) {
+        set_error("the learned guess needs a batch of one topology");
+        return NGMI_E_UNSUPP;
+    }
     b->net = net;
     if (!net) return NGMI_OK;
-    TranDev &D = b->D;
-    const size_t rows = (size_t)b->C * b->N;
+    TopoGroup &G = *b->groups[0];
+    TranDev &D = G.D;
+    const size_t rows = (size_t)G.C * G.N;
     if (!b->gF) {
         if (alloc(b, rows * 16, &b->gF) || alloc(b, rows, &b->gy) || alloc(b, rows, &b->grow) ||
             alloc(b, rows, &b->gbase) || alloc(b, 1, &b->gcount))
             return NGMI_E_DEVICE;
         double *h;
-        if (alloc(b, (size_t)4 * b->C * (b->N + 1), &h)) return NGMI_E_DEVICE;
+        if (alloc(b, (size_t)4 * G.C * (G.N + 1), &h)) return NGMI_E_DEVICE;
         D.hist = h;
+        NGMI_HIP(hipMemcpy(b->dD, &D, sizeof(TranDev), hipMemcpyHostToDevice));
     }
     return NGMI_OK;
 }
@@ -989,7 +1118,8 @@ int ngmi_batch_set_guess(ngmi_batch *b, ngmi_mlp *net)
 int ngmi_batch_neighbours(const ngmi_batch *b, int *out)
 {
     if (!b || !out) return NGMI_E_PANIC;
-    std::memcpy(out, b->nbr_host.data(), b->nbr_host.size() * sizeof(int));
+    const TopoGroup &G = *b->groups[0];
+    std::memcpy(out, G.nbr_host.data(), G.nbr_host.size() * sizeof(int));
     return NGMI_OK;
 }
 
@@ -998,7 +1128,7 @@ int ngmi_batch_guess_stats(const ngmi_batch *b, int c, int64_t *guessed, int64_t
 {
     if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
     CktCtl k;
-    NGMI_HIP(hipMemcpy(&k, b->D.ctl + c, sizeof(k), hipMemcpyDeviceToHost));
+    NGMI_HIP(hipMemcpy(&k, b->ctl_of(c), sizeof(k), hipMemcpyDeviceToHost));
     if (guessed) *guessed = k.guessed;
     if (fallbacks) *fallbacks = k.fallbacks;
     if (guess_ms) *guess_ms = b->guess_ms;
@@ -1008,7 +1138,7 @@ int ngmi_batch_guess_stats(const ngmi_batch *b, int c, int64_t *guessed, int64_t
 int ngmi_batch_table_cols(const ngmi_batch *b, int *inst_cols, int *model_cols)
 {
     if (!b) return NGMI_E_PANIC;
-    return ngmi_bsim4_set_layout(b->set, inst_cols, model_cols, nullptr);
+    return ngmi_bsim4_set_layout(b->groups[0]->set, inst_cols, model_cols, nullptr);
 }
 
 int ngmi_batch_set_profile(ngmi_batch *b, int enable)
@@ -1051,7 +1181,7 @@ int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int
 {
     if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
     CktCtl k;
-    NGMI_HIP(hipMemcpy(&k, b->D.ctl + c, sizeof(k), hipMemcpyDeviceToHost));
+    NGMI_HIP(hipMemcpy(&k, b->ctl_of(c), sizeof(k), hipMemcpyDeviceToHost));
     if (status) *status = k.phase == PH_DONE ? 0 : (k.phase == PH_FAIL ? k.status : -1);
     if (npts) *npts = k.npts;
     if (accepted) *accepted = k.accepted;
@@ -1067,13 +1197,14 @@ int ngmi_batch_waveform(const ngmi_batch *b, int c, double *t, double *x, int64_
     if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
     int64_t n = 0;
     CktCtl k;
-    NGMI_HIP(hipMemcpy(&k, b->D.ctl + c, sizeof(k), hipMemcpyDeviceToHost));
+    NGMI_HIP(hipMemcpy(&k, b->ctl_of(c), sizeof(k), hipMemcpyDeviceToHost));
+    const TranDev &D = b->group_of(c).D;
+    const size_t lc = (size_t)b->where[c].y;
     n = std::min<int64_t>(std::min<int64_t>(k.npts, b->cap_pts), maxpts);
     if (n > 0) {
-        NGMI_HIP(hipMemcpy(t, b->D.out_t + (size_t)c * b->cap_pts, n * sizeof(double),
+        NGMI_HIP(hipMemcpy(t, D.out_t + lc * b->cap_pts, n * sizeof(double), hipMemcpyDeviceToHost));
+        NGMI_HIP(hipMemcpy(x, D.out_x + lc * b->cap_pts * D.nprobe, n * D.nprobe * sizeof(double),
                            hipMemcpyDeviceToHost));
-        NGMI_HIP(hipMemcpy(x, b->D.out_x + (size_t)c * b->cap_pts * b->D.nprobe,
-                           n * b->D.nprobe * sizeof(double), hipMemcpyDeviceToHost));
     }
     if (npts) *npts = n;
     return NGMI_OK;

@@ -62,12 +62,11 @@ struct TranCache {
     __device__ __forceinline__ void setc(int k, double v) { c_[k * cnm + i] = v; }
 };
 
-template <int WAVES>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
-k_tran_mos(B4Tab tab, TranDev D)
+/* one thread per (circuit, MOSFET) of a topology group, circuit-major
+ * instances i = c * NM + j */
+__device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D, int64_t i)
 {
     const int64_t cnm = (int64_t)D.C * D.NM;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cnm) return;
     const int c = (int)(i / D.NM);
     const int j = (int)(i - (int64_t)c * D.NM);
@@ -97,6 +96,26 @@ k_tran_mos(B4Tab tab, TranDev D)
 #pragma unroll
     for (int k = 0; k < B4R__COUNT; k++) D.mrhs[k * cnm + i] = R[k];
     D.noncon[i] = nc;
+}
+
+/* one topology: the descriptor is a kernel argument, so every table pointer
+ * is known to address global memory */
+template <int WAVES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
+k_tran_mos(B4Tab tab, TranDev D)
+{
+    tran_mos_body(tab, D, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+/* several topologies in one launch: blk[b] = (group, first block of the
+ * group) */
+template <int WAVES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
+k_tran_mos_groups(const B4Tab *__restrict__ tabs, const TranDev *__restrict__ Ds,
+                  const int2 *__restrict__ blk)
+{
+    const int2 bg = blk[blockIdx.x];
+    tran_mos_body(tabs[bg.x], Ds[bg.x], (int64_t)(blockIdx.x - bg.y) * blockDim.x + threadIdx.x);
 }
 
 /* ---------------- scalar helpers (thread 0 of a circuit) ---------------- */
@@ -409,26 +428,101 @@ __device__ void request_guess(CktCtl *K)
     }
 }
 
+/* a read-only table load through the global address space (the tables'
+ * pointers come from the group descriptor, so the compiler cannot infer
+ * the space and would emit flat loads) */
+template <class T>
+__device__ __forceinline__ T gload(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+__device__ __forceinline__ int4 gload(const int4 *p)
+{
+    typedef int v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const __attribute__((address_space(1))) v4 *)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ uint4 gload(const uint4 *p)
+{
+    typedef unsigned int v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const __attribute__((address_space(1))) v4 *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+/* acc - x[a] y[b] for the four packed index pairs of w, in order */
+__device__ __forceinline__ double sub_terms(double acc, uint4 w, const double *x, const double *y)
+{
+    acc -= x[w.x & 0xffffu] * y[w.x >> 16];
+    acc -= x[w.y & 0xffffu] * y[w.y >> 16];
+    acc -= x[w.z & 0xffffu] * y[w.z >> 16];
+    acc -= x[w.w & 0xffffu] * y[w.w >> 16];
+    return acc;
+}
+
+/* value of one matrix contribution code (MOS stamp, capacitor companion
+ * conductance, resistor conductance, source +-1, padding 0) */
+__device__ __forceinline__ double mat_value(const TranDev &D, uint32_t code, int c, int64_t cnm,
+                                            const double *capg, bool icz)
+{
+    if ((code & CT_ICZ) && icz) return 0.0;
+    const uint32_t idx = code & CT_INDEX_MASK;
+    double v;
+    switch (code >> CT_TYPE_SHIFT) {
+    case CT_MOS: {
+        const int sl = idx / D.NM, j = idx - sl * D.NM;
+        v = gload(D.stamp + sl * cnm + (int64_t)c * D.NM + j);
+        break;
+    }
+    case CT_CAP: v = capg[idx]; break;
+    case CT_RES: v = gload(D.res_g + (int64_t)c * D.NRES + idx); break;
+    case CT_ONE: v = 1.0; break;
+    default: v = 0.0; break;
+    }
+    return (code & CT_NEG) ? -v : v;
+}
+
+/* value of one RHS contribution code (MOS current, capacitor companion
+ * current, source value, padding 0) */
+__device__ __forceinline__ double rhs_value(const TranDev &D, uint32_t code, int c, int64_t cnm,
+                                            const double *capq, const double *vsv)
+{
+    const uint32_t idx = code & CT_INDEX_MASK;
+    double v;
+    switch (code >> CT_TYPE_SHIFT) {
+    case CT_MOS: {
+        const int r = idx / D.NM, j = idx - r * D.NM;
+        v = gload(D.mrhs + r * cnm + (int64_t)c * D.NM + j);
+        break;
+    }
+    case CT_CAP: v = capq[idx]; break;
+    case CT_VSRC: v = vsv[idx]; break;
+    default: v = 0.0; break;
+    }
+    return (code & CT_NEG) ? -v : v;
+}
+
 /* one wavefront per circuit: the level-scheduled LU program has a few
  * dozen entries per level, and a one-wave workgroup makes each level's
  * barrier nearly free while many circuits share a compute unit */
 constexpr int STEP_THREADS = 64;
 
-__global__ void __launch_bounds__(STEP_THREADS)
-k_tran_step(TranDev D)
+/* circuit c of the topology D, one workgroup */
+__device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
 {
     extern __shared__ double sm[];
     __shared__ double red[STEP_THREADS];
     __shared__ int s_flag[4];
     __shared__ int s_act;
 
-    const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     CktCtl *K = D.ctl + c;
     if (K->phase >= PH_DONE) return;
     const int N = D.N;
-    double *lu = sm;
-    double *z = lu + D.nslots;
-    double *capg = z + N;
+    double *lu = sm;                 /* nslots + 1: lu[nslots] = 0 pads term groups */
+    double *z = lu + D.nslots + 1;   /* N + 1: z[N] = 0 */
+    double *capg = z + N + 1;
     double *capq = capg + D.NCAP;
     double *vsv = capq + D.NCAP;
 
@@ -438,6 +532,10 @@ k_tran_step(TranDev D)
     double *xo = D.x + ((int64_t)xs * D.C + c) * (N + 1);
     double *xn = D.x + ((int64_t)(xs ^ 1) * D.C + c) * (N + 1);
     if (tid < 4) s_flag[tid] = 0;
+    if (tid == 0) {
+        lu[D.nslots] = 0.0;
+        z[N] = 0.0;
+    }
     if (tid == 0) {
         atomicAdd(D.counters, (unsigned long long)D.NM);
         atomicAdd(D.counters + 1, 1ull);
@@ -492,72 +590,63 @@ k_tran_step(TranDev D)
     __syncthreads();
 
     /* assembly into the permuted L+U slots and RHS rows.  Each leaf sums at
-     * most a few contributions in element order; entries fed by many
-     * elements (a supply node's diagonal) are split into leaves writing
-     * partial sums that one combine step adds in order, so no lane walks
-     * hundreds of contributions alone.  (CKTload's summation order is not
-     * kept; the order here is fixed, so results repeat bit for bit.) */
+     * most a few groups of four contributions in element order; entries fed
+     * by many elements (a supply node's diagonal) are split into leaves
+     * writing partial sums that one combine step adds in order, so no lane
+     * walks hundreds of contributions alone.  (CKTload's summation order is
+     * not kept; the order here is fixed, so results repeat bit for bit.) */
     const bool icz = (mode & M_TRANOP) && !(mode & M_UIC);
     double *part = vsv + D.NVS;
+    const uint4 *AC = reinterpret_cast<const uint4 *>(D.a_code);
+    const uint4 *RC = reinterpret_cast<const uint4 *>(D.r_code);
     for (int e = tid; e < D.n_aleaf; e += nt) {
-        const int4 lf = D.a_leaf[e];
+        const int4 lf = gload(D.a_leaf + e);
         double acc = 0.0;
         for (int q = lf.y; q < lf.z; q++) {
-            const uint32_t code = D.a_code[q];
-            if ((code & CT_ICZ) && icz) continue;
-            const uint32_t idx = code & CT_INDEX_MASK;
-            double v;
-            switch (code >> CT_TYPE_SHIFT) {
-            case CT_MOS: {
-                const int sl = idx / D.NM, j = idx - sl * D.NM;
-                v = D.stamp[sl * cnm + (int64_t)c * D.NM + j];
-                break;
-            }
-            case CT_CAP: v = capg[idx]; break;
-            case CT_RES: v = D.res_g[(int64_t)c * D.NRES + idx]; break;
-            default: v = 1.0; break;
-            }
-            acc += (code & CT_NEG) ? -v : v;
+            const uint4 w = gload(AC + q);
+            const double v0 = mat_value(D, w.x, c, cnm, capg, icz);
+            const double v1 = mat_value(D, w.y, c, cnm, capg, icz);
+            const double v2 = mat_value(D, w.z, c, cnm, capg, icz);
+            const double v3 = mat_value(D, w.w, c, cnm, capg, icz);
+            acc += v0;
+            acc += v1;
+            acc += v2;
+            acc += v3;
         }
         if (lf.x < D.nslots) {
-            if ((D.slot_flags[lf.x] & 1) && icz) acc = 1.0;
+            if ((gload(D.slot_flags + lf.x) & 1) && icz) acc = 1.0;
             lu[lf.x] = acc;
         } else {
             part[lf.x - D.nslots] = acc;
         }
     }
     for (int e = tid; e < D.n_rleaf; e += nt) {
-        const int4 lf = D.r_leaf[e];
+        const int4 lf = gload(D.r_leaf + e);
         double acc = 0.0;
         for (int q = lf.y; q < lf.z; q++) {
-            const uint32_t code = D.r_code[q];
-            const uint32_t idx = code & CT_INDEX_MASK;
-            double v;
-            switch (code >> CT_TYPE_SHIFT) {
-            case CT_MOS: {
-                const int r = idx / D.NM, j = idx - r * D.NM;
-                v = D.mrhs[r * cnm + (int64_t)c * D.NM + j];
-                break;
-            }
-            case CT_CAP: v = capq[idx]; break;
-            case CT_VSRC: v = vsv[idx]; break;
-            default: v = 0.0; break;
-            }
-            acc += (code & CT_NEG) ? -v : v;
+            const uint4 w = gload(RC + q);
+            const double v0 = rhs_value(D, w.x, c, cnm, capq, vsv);
+            const double v1 = rhs_value(D, w.y, c, cnm, capq, vsv);
+            const double v2 = rhs_value(D, w.z, c, cnm, capq, vsv);
+            const double v3 = rhs_value(D, w.w, c, cnm, capq, vsv);
+            acc += v0;
+            acc += v1;
+            acc += v2;
+            acc += v3;
         }
         if (lf.x < N) z[lf.x] = acc;
         else part[D.n_apart + lf.x - N] = acc;
     }
     __syncthreads();
     for (int e = tid; e < D.n_acomb; e += nt) {
-        const int4 cb = D.a_comb[e];
+        const int4 cb = gload(D.a_comb + e);
         double acc = 0.0;
         for (int p = cb.y; p < cb.z; p++) acc += part[p];
-        if ((D.slot_flags[cb.x] & 1) && icz) acc = 1.0;
+        if ((gload(D.slot_flags + cb.x) & 1) && icz) acc = 1.0;
         lu[cb.x] = acc;
     }
     for (int e = tid; e < D.n_rcomb; e += nt) {
-        const int4 cb = D.r_comb[e];
+        const int4 cb = gload(D.r_comb + e);
         double acc = 0.0;
         for (int p = cb.y; p < cb.z; p++) acc += part[D.n_apart + p];
         z[cb.x] = acc;
@@ -565,21 +654,31 @@ k_tran_step(TranDev D)
     if (icz && D.n_icrow) {
         __syncthreads();
         for (int e = tid; e < D.n_icrow; e += nt) {
-            const int a = D.ic_row[e];                 /* permuted row of an .ic node */
-            const int u = D.perm_unknown[N + a];
-            z[a] = D.ic_val[(int64_t)c * (D.NN + 1) + u];
+            const int a = gload(D.ic_row + e);            /* permuted row of an .ic node */
+            const int u = gload(D.perm_unknown + N + a);
+            z[a] = gload(D.ic_val + (int64_t)c * (D.NN + 1) + u);
         }
     }
     __syncthreads();
 
-    /* LU factorisation, entry-wise Doolittle by dependency levels */
+    /* LU factorisation, entry-wise Doolittle by dependency levels; each
+     * lane keeps the next term group's load in flight while it uses the
+     * current one */
+    const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
     for (int L = 0; L < D.nlev_f; L++) {
-        for (int e = D.f_lev[L] + tid; e < D.f_lev[L + 1]; e += nt) {
-            const int4 en = reinterpret_cast<const int4 *>(D.f_ent)[e];
+        const int e1 = gload(D.f_lev + L + 1);
+        for (int e = gload(D.f_lev + L) + tid; e < e1; e += nt) {
+            const int4 en = gload(reinterpret_cast<const int4 *>(D.f_ent) + e);
             double acc = lu[en.x];
-            for (int q = en.z; q < en.w; q++) {
-                const int2 t = D.f_term[q];
-                acc -= lu[t.x] * lu[t.y];
+            int q = en.z;
+            if (q < en.w) {
+                uint4 w = gload(FT + q);
+                while (++q < en.w) {
+                    const uint4 wn = gload(FT + q);
+                    acc = sub_terms(acc, w, lu, lu);
+                    w = wn;
+                }
+                acc = sub_terms(acc, w, lu, lu);
             }
             if (en.y >= 0) acc /= lu[en.y];
             lu[en.x] = acc;
@@ -587,26 +686,42 @@ k_tran_step(TranDev D)
         __syncthreads();
     }
     /* forward substitution (unit L) */
+    const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
     for (int L = 0; L < D.nlev_fw; L++) {
-        for (int e = D.fw_lev[L] + tid; e < D.fw_lev[L + 1]; e += nt) {
-            const int row = D.fw_ent[3 * e], tb = D.fw_ent[3 * e + 1], te = D.fw_ent[3 * e + 2];
-            double acc = z[row];
-            for (int q = tb; q < te; q++) {
-                const int2 t = D.fw_term[q];
-                acc -= lu[t.x] * z[t.y];
+        const int e1 = gload(D.fw_lev + L + 1);
+        for (int e = gload(D.fw_lev + L) + tid; e < e1; e += nt) {
+            const int4 en = gload(reinterpret_cast<const int4 *>(D.fw_ent) + e);
+            double acc = z[en.x];
+            int q = en.z;
+            if (q < en.w) {
+                uint4 w = gload(WT + q);
+                while (++q < en.w) {
+                    const uint4 wn = gload(WT + q);
+                    acc = sub_terms(acc, w, lu, z);
+                    w = wn;
+                }
+                acc = sub_terms(acc, w, lu, z);
             }
-            z[row] = acc;
+            z[en.x] = acc;
         }
         __syncthreads();
     }
     /* back substitution */
+    const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
     for (int L = 0; L < D.nlev_bw; L++) {
-        for (int e = D.bw_lev[L] + tid; e < D.bw_lev[L + 1]; e += nt) {
-            const int4 en = reinterpret_cast<const int4 *>(D.bw_ent)[e];
+        const int e1 = gload(D.bw_lev + L + 1);
+        for (int e = gload(D.bw_lev + L) + tid; e < e1; e += nt) {
+            const int4 en = gload(reinterpret_cast<const int4 *>(D.bw_ent) + e);
             double acc = z[en.x];
-            for (int q = en.z; q < en.w; q++) {
-                const int2 t = D.bw_term[q];
-                acc -= lu[t.x] * z[t.y];
+            int q = en.z;
+            if (q < en.w) {
+                uint4 w = gload(BT + q);
+                while (++q < en.w) {
+                    const uint4 wn = gload(BT + q);
+                    acc = sub_terms(acc, w, lu, z);
+                    w = wn;
+                }
+                acc = sub_terms(acc, w, lu, z);
             }
             const double piv = lu[en.y];
             if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
@@ -807,6 +922,21 @@ k_tran_step(TranDev D)
     }
 }
 
+/* one topology, one workgroup per circuit */
+__global__ void __launch_bounds__(STEP_THREADS)
+k_tran_step(TranDev D)
+{
+    tran_step_body(D, blockIdx.x);
+}
+
+/* several topologies: blk[b] = (group, circuit) */
+__global__ void __launch_bounds__(STEP_THREADS)
+k_tran_step_groups(const TranDev *__restrict__ Ds, const int2 *__restrict__ blk)
+{
+    const int2 bg = blk[blockIdx.x];
+    tran_step_body(Ds[bg.x], bg.y);
+}
+
 /* ---------------- learned initial guess ---------------- */
 
 constexpr double GUESS_RATIO_MAX = 4.0;       /* step ratios are clamped to [0, 4] */
@@ -944,26 +1074,39 @@ int set_tran_lds_limit(size_t bytes)
 {
     NGMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tran_step),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    NGMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tran_step_groups),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return NGMI_OK;
 }
 
-/* one Newton pass; ev (optional) = 3 events recorded before the MOSFET
- * kernel, between the two kernels and after the step kernel */
-int launch_tran_pass(const B4Tab &tab, const TranDev &D, size_t lds_bytes, hipStream_t s,
-                     hipEvent_t *ev)
+/* one Newton pass over every circuit of every topology group; with one
+ * group (tab0, D0 set) the descriptor goes by value.  ev (optional) = 3
+ * events recorded before the MOSFET kernel, between the two kernels and
+ * after the step kernel. */
+int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, const B4Tab *tabs, const TranDev *Ds,
+                     const int2 *mos_blk, int n_mos_blk, const int2 *step_blk, int n_step_blk,
+                     size_t lds_bytes, hipStream_t s, hipEvent_t *ev)
 {
-    const int64_t cnm = (int64_t)D.C * D.NM;
+    const bool two = tune_get(TUNE_MOS_WAVES) == 2;
     if (ev) NGMI_HIP(hipEventRecord(ev[0], s));
-    if (cnm > 0) {
-        const dim3 g((unsigned)((cnm + 255) / 256));
-        if (tune_get(TUNE_MOS_WAVES) == 2)
-            hipLaunchKernelGGL(k_tran_mos<2>, g, dim3(256), 0, s, tab, D);
-        else
-            hipLaunchKernelGGL(k_tran_mos<1>, g, dim3(256), 0, s, tab, D);
+    if (n_mos_blk > 0) {
+        const dim3 g((unsigned)n_mos_blk);
+        if (D0) {
+            if (two) hipLaunchKernelGGL(k_tran_mos<2>, g, dim3(256), 0, s, *tab0, *D0);
+            else hipLaunchKernelGGL(k_tran_mos<1>, g, dim3(256), 0, s, *tab0, *D0);
+        } else {
+            if (two) hipLaunchKernelGGL(k_tran_mos_groups<2>, g, dim3(256), 0, s, tabs, Ds, mos_blk);
+            else hipLaunchKernelGGL(k_tran_mos_groups<1>, g, dim3(256), 0, s, tabs, Ds, mos_blk);
+        }
         NGMI_HIP(hipGetLastError());
     }
     if (ev) NGMI_HIP(hipEventRecord(ev[1], s));
-    hipLaunchKernelGGL(k_tran_step, dim3((unsigned)D.C), dim3(STEP_THREADS), lds_bytes, s, D);
+    if (D0)
+        hipLaunchKernelGGL(k_tran_step, dim3((unsigned)n_step_blk), dim3(STEP_THREADS), lds_bytes, s,
+                           *D0);
+    else
+        hipLaunchKernelGGL(k_tran_step_groups, dim3((unsigned)n_step_blk), dim3(STEP_THREADS),
+                           lds_bytes, s, Ds, step_blk);
     NGMI_HIP(hipGetLastError());
     if (ev) NGMI_HIP(hipEventRecord(ev[2], s));
     return NGMI_OK;

@@ -47,7 +47,7 @@ struct CktCtl {
 
 /* contribution codes of the assembly plans */
 enum : uint32_t {
-    CT_MOS = 0u, CT_CAP = 1u, CT_RES = 2u, CT_ONE = 3u, CT_VSRC = 4u,
+    CT_MOS = 0u, CT_CAP = 1u, CT_RES = 2u, CT_ONE = 3u, CT_VSRC = 4u, CT_ZERO = 5u,
     CT_TYPE_SHIFT = 28u, CT_NEG = 1u << 27, CT_ICZ = 1u << 26,
     CT_INDEX_MASK = (1u << 26) - 1u
 };
@@ -76,10 +76,11 @@ struct TranDev {
     double *vs_brk;           /* [C][NVS] break_time */
     const double *ic_val;     /* [C][NN+1] */
     /* LU program */
-    /* assembly: leaves (dest, begin, end) sum contributions a_code[begin,
-     * end) into L+U slot dest (< nslots) or partial dest - nslots; combines
-     * (slot, pbegin, pend) add partials into a slot; likewise for the RHS
-     * rows (dest < N a row, else partial n_apart + dest - N) */
+    /* assembly: leaves (dest, begin, end) sum the contribution groups
+     * [begin, end) (uint4 units of a_code, padded with zero codes) into L+U
+     * slot dest (< nslots) or partial dest - nslots; combines (slot, pbegin,
+     * pend) add partials into a slot; likewise for the RHS rows (dest < N a
+     * row, else partial n_apart + dest - N) */
     const int4 *a_leaf, *a_comb, *r_leaf, *r_comb;
     int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
     const uint32_t *a_code;   /* contributions */
@@ -87,13 +88,17 @@ struct TranDev {
     const uint32_t *r_code;
     const int *ic_row;        /* permuted rows of .ic nodes */
     int n_icrow;
+    /* LU program.  Terms are 16-bit index pairs packed in one word (low |
+     * high << 16), grouped four to a uint4 and padded with pairs whose
+     * product is exactly zero (LDS lu[nslots] = z[N] = 0); entry term ranges
+     * are in uint4 units. */
     const int *f_lev;         /* [nlev_f+1] -> entries */
     const int *f_ent;         /* [ne][4]: target, pivot (-1 for U), term begin, term end */
-    const int2 *f_term;       /* (a, b) slot pairs */
-    const int *fw_lev, *fw_ent;   /* forward solve: [ne][3] row, term begin, end */
-    const int2 *fw_term;          /* (L slot, z index) */
+    const uint32_t *f_term;   /* (a, b) slot pairs: lu[t] -= lu[a] lu[b] */
+    const int *fw_lev, *fw_ent;   /* forward solve: [ne][4] row, -1, term begin, end */
+    const uint32_t *fw_term;      /* (L slot, z index) */
     const int *bw_lev, *bw_ent;   /* backward solve: [ne][4] row, diag slot, term begin, end */
-    const int2 *bw_term;          /* (U slot, z index) */
+    const uint32_t *bw_term;      /* (U slot, z index) */
     const int *perm_unknown;  /* [2N]: [a] unknown (1-based) solved in permuted column a;
                                  [N + a] original row assembled into permuted row a */
     const int *probe;         /* [nprobe] unknowns written at each accepted point */

@@ -297,6 +297,9 @@ def _tran_protos(L):
     L.ngmi_circuit_add_bsim4_many.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(i),
                                               ctypes.POINTER(i), i, i, ctypes.POINTER(cp),
                                               ctypes.POINTER(d)]
+    L.ngmi_batch_groups.argtypes = [vp, ctypes.POINTER(i)]
+    L.ngmi_batch_circuit_info.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i),
+                                          ctypes.POINTER(i), ctypes.POINTER(i)]
     L.ngmi_batch_set_profile.argtypes = [vp, i]
     L.ngmi_batch_table_cols.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
     u64 = ctypes.c_ulonglong
@@ -455,7 +458,8 @@ class Circuit:
 
 
 class Batch:
-    """C circuits of one topology advanced together on the GPU."""
+    """C circuits advanced together on the GPU; circuits of equal topology
+    share one symbolic analysis (a group), one launch per pass covers all."""
 
     def __init__(self, circuits, probe=None, max_points=20000):
         L = _tran_protos(lib())
@@ -478,6 +482,17 @@ class Batch:
                                     lv), 'info')
         return dict(unknowns=a.value, lu_nonzeros=b.value, factor_terms=c.value,
                     levels=tuple(lv))
+
+    def groups(self):
+        g = ctypes.c_int()
+        check(lib().ngmi_batch_groups(self.h, ctypes.byref(g)), 'groups')
+        return g.value
+
+    def circuit_info(self, c):
+        g, n, p, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().ngmi_batch_circuit_info(self.h, c, ctypes.byref(g), ctypes.byref(n),
+                                            ctypes.byref(p), ctypes.byref(s)), 'circuit info')
+        return dict(group=g.value, unknowns=n.value, nprobe=p.value, lu_nonzeros=s.value)
 
     def tran(self, tstep, tstop, tmax=0.0, max_passes=10 ** 7, stream=0):
         passes = ctypes.c_int64()
@@ -534,7 +549,9 @@ class Batch:
         return dict(status=st.value, npts=a[0].value, accepted=a[1].value, rejected=a[2].value,
                     iterations=a[3].value)
 
-    def waveform(self, c, nprobe):
+    def waveform(self, c, nprobe=None):
+        if nprobe is None:
+            nprobe = self.circuit_info(c)['nprobe']
         s = self.stats(c)
         n = max(s['npts'], 1)
         t = np.zeros(n)

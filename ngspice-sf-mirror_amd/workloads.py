@@ -111,6 +111,19 @@ def ring_oscillator(stages=RING_STAGES, cload=1e-15, ic_node1=0.51, tstep=10e-12
             'tstep': tstep, 'tstop': tstop}
 
 
+UNEVEN_RINGS = 2048
+UNEVEN_TSTOP = 10e-9
+
+
+def uneven_rings(count=UNEVEN_RINGS, seed=3, tstop=UNEVEN_TSTOP):
+    """The uneven batch: ring oscillators whose stage counts are drawn
+    uniformly from the odd numbers 11..301 (Philox stream (3, seed)), each
+    with its own sparsity pattern; .tran 10p 10n."""
+    g = rng(3, seed)
+    stages = 11 + 2 * g.integers(0, (301 - 11) // 2 + 1, size=count)
+    return [ring_oscillator(stages=int(k), tstop=tstop) for k in stages]
+
+
 def mirror_adder_topology(bits=16):
     """Node map and transistor list of a ripple-carry adder of 28-transistor
     mirror full adders (carry: 10 T, sum: 14 T, two output inverters).

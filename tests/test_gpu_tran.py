@@ -137,3 +137,26 @@ def test_gear_method():
         (eng,), info, _ = _engine([desc], tstop)
         ora = _oracle(desc, tstop)
         _compare(tag, eng, ora, desc['nnodes'])
+
+
+def test_uneven_batch_rings():
+    """Rings of 11, 21, 13 and 31 stages, two of each with different loads,
+    interleaved in one batch: four topology groups advanced by one launch
+    per pass; every circuit against the oracle."""
+    print('case: uneven batch, rings of 11/21/13/31 stages x 2, .tran 10p 1n', flush=True)
+    stages = [11, 21, 13, 31, 11, 13, 21, 31]
+    loads = [1e-15, 1e-15, 1e-15, 1e-15, 1.5e-15, 1.5e-15, 1.5e-15, 1.5e-15]
+    tstop = 1e-9
+    descs = [workloads.ring_oscillator(stages=k, cload=cl, tstop=tstop)
+             for k, cl in zip(stages, loads)]
+    b = ngmi.Batch([ngmi.Circuit(d) for d in descs], max_points=200000)
+    assert b.groups() == 4
+    rc, passes = b.tran(descs[0]['tstep'], tstop)
+    assert rc == 0, (rc, ngmi.last_error())
+    for c, d in enumerate(descs):
+        ci = b.circuit_info(c)
+        assert ci['unknowns'] == d['nnodes'] + 1
+        st = b.stats(c)
+        assert st['status'] == 0, st
+        te, xe = b.waveform(c)
+        _compare('ring%d-c%d' % (stages[c], c), (te, xe, st), _oracle(d, tstop), d['nnodes'])
