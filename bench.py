@@ -423,6 +423,8 @@ def main():
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess run')
     ap.add_argument('--eval-waves', type=int, default=None, help='BSIM4 eval kernel occupancy')
     ap.add_argument('--mos-waves', type=int, default=None, help='transient BSIM4 kernel occupancy')
+    ap.add_argument('--streams', type=int, default=None,
+                    help='2: the plain transient alternates circuit halves on two streams')
     ap.add_argument('--train-circuits', type=int, default=32)
     ap.add_argument('--train-steps', type=int, default=1500)
     args = ap.parse_args()
@@ -447,6 +449,8 @@ def main():
         ngmi.tune('eval_waves', args.eval_waves)
     if args.mos_waves:
         ngmi.tune('mos_waves', args.mos_waves)
+    if args.streams:
+        ngmi.tune('streams', args.streams)
     if args.workload == 'mc':
         res = bench_mc(args)
         cpu = None if args.no_cpu_baseline else cpu_baseline_mc(args.tstop)

@@ -10,7 +10,7 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {1, 1};
+static int g_tune[TUNE__COUNT] = {2, 2, 1};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
 
@@ -36,12 +36,14 @@ int ngmi_tune(const char *knob, int value)
     int k = -1;
     if (!std::strcmp(knob, "eval_waves")) k = TUNE_EVAL_WAVES;
     else if (!std::strcmp(knob, "mos_waves")) k = TUNE_MOS_WAVES;
+    else if (!std::strcmp(knob, "streams")) k = TUNE_STREAMS;
     if (k < 0) {
         set_error("unknown tuning knob '%s'", knob);
         return NGMI_E_BADPARM;
     }
-    if (value != 1 && value != 2) {
-        set_error("knob '%s' takes 1 or 2", knob);
+    const int vmax = (k == TUNE_STREAMS) ? 2 : 4;
+    if (value < 1 || value > vmax) {
+        set_error("knob '%s' takes 1..%d", knob, vmax);
         return NGMI_E_PARMVAL;
     }
     g_tune[k] = value;

@@ -79,10 +79,12 @@ int launch_bsim4_eval(const B4Tab &tab, const B4Ckt &ck, const double *vterm,
     EvalIO io{vterm, state[0], state[1], state[2], cache, stamp, rhs, noncon};
     const int bs = 256;
     const int64_t nb = (tab.n + bs - 1) / bs;
-    if (tune_get(TUNE_EVAL_WAVES) == 2)
-        hipLaunchKernelGGL(k_bsim4_eval<2>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io);
-    else
-        hipLaunchKernelGGL(k_bsim4_eval<1>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io);
+    switch (tune_get(TUNE_EVAL_WAVES)) {
+    case 1: hipLaunchKernelGGL(k_bsim4_eval<1>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
+    case 3: hipLaunchKernelGGL(k_bsim4_eval<3>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
+    case 4: hipLaunchKernelGGL(k_bsim4_eval<4>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
+    default: hipLaunchKernelGGL(k_bsim4_eval<2>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
+    }
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }

@@ -31,9 +31,9 @@
 #include "tran_types.h"
 
 namespace ngmi {
-int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, const B4Tab *tabs, const TranDev *Ds,
-                     const int2 *mos_blk, int n_mos_blk, const int2 *step_blk, int n_step_blk,
-                     size_t lds_bytes, hipStream_t s, hipEvent_t *ev);
+int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, int c_begin, int c_end, const B4Tab *tabs,
+                     const TranDev *Ds, const int2 *mos_blk, int n_mos_blk, const int2 *step_blk,
+                     int n_step_blk, size_t lds_bytes, hipStream_t s, hipEvent_t *ev);
 int set_tran_lds_limit(size_t bytes);
 int launch_guess_features(const TranDev &D, float *F, int *rowmap, double2 *base, int *count,
                           hipStream_t s);
@@ -1029,13 +1029,32 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     const TopoGroup &G0 = *b->groups[0];
     const bool one = b->groups.size() == 1;
     const B4Tab *tab0 = set_table(G0.set);
+    /* two streams: the circuits' halves alternate, so one half's latency-
+     * bound step kernel runs beside the other half's BSIM4 kernel (single
+     * topology, no learned guess) */
+    const bool split = one && !b->net && C >= 2 && tune_get(TUNE_STREAMS) == 2;
+    hipStream_t s2 = nullptr;
+    hipEvent_t join = nullptr;
+    if (split) {
+        NGMI_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        NGMI_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    }
+    const int half = C / 2;
+    if (split) {   /* the second stream starts after the resets above */
+        NGMI_HIP(hipEventRecord(join, s));
+        NGMI_HIP(hipStreamWaitEvent(s2, join, 0));
+    }
     int64_t pass = 0;
     int done = 0;
     const int check_every = 32;
-    std::vector<hipEvent_t> ev, gev;
+    std::vector<hipEvent_t> ev, ev2, gev;
     if (b->profile) {
         ev.resize(3 * check_every);
         for (auto &e : ev) NGMI_HIP(hipEventCreate(&e));
+        if (split) {
+            ev2.resize(3 * check_every);
+            for (auto &e : ev2) NGMI_HIP(hipEventCreate(&e));
+        }
         if (b->net) {
             gev.resize(2 * check_every);
             for (auto &e : gev) NGMI_HIP(hipEventCreate(&e));
@@ -1057,10 +1076,23 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
                 if (rc != NGMI_OK) break;
                 if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k + 1], s));
             }
-            rc = launch_tran_pass(one ? tab0 : nullptr, one ? &G0.D : nullptr, b->dTab, b->dD,
-                                  b->mos_blk, b->n_mos_blk, b->step_blk, b->n_step_blk, b->lds, s,
-                                  b->profile ? &ev[3 * k] : nullptr);
+            if (split) {
+                rc = launch_tran_pass(tab0, &G0.D, 0, half, nullptr, nullptr, nullptr, 1, nullptr,
+                                      half, b->lds, s, b->profile ? &ev[3 * k] : nullptr);
+                if (rc == NGMI_OK)
+                    rc = launch_tran_pass(tab0, &G0.D, half, C, nullptr, nullptr, nullptr, 1, nullptr,
+                                          C - half, b->lds, s2, b->profile ? &ev2[3 * k] : nullptr);
+            } else {
+                rc = launch_tran_pass(one ? tab0 : nullptr, one ? &G0.D : nullptr, 0, C, b->dTab,
+                                      b->dD, b->mos_blk, b->n_mos_blk, b->step_blk, b->n_step_blk,
+                                      b->lds, s, b->profile ? &ev[3 * k] : nullptr);
+            }
             if (rc != NGMI_OK) break;
+        }
+        if (split && (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess)) {
+            set_error("stream join failed: %s", hipGetErrorString(hipGetLastError()));
+            rc = NGMI_E_DEVICE;
+            break;
         }
         if (hipMemcpyAsync(&done, b->done_count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess) {
@@ -1074,6 +1106,12 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
                 (void)hipEventElapsedTime(&c2, ev[3 * q + 1], ev[3 * q + 2]);
                 b->mos_ms += a;
                 b->step_ms += c2;
+                if (split) {
+                    (void)hipEventElapsedTime(&a, ev2[3 * q], ev2[3 * q + 1]);
+                    (void)hipEventElapsedTime(&c2, ev2[3 * q + 1], ev2[3 * q + 2]);
+                    b->mos_ms += a;
+                    b->step_ms += c2;
+                }
                 if (b->net) {
                     float g2 = 0;
                     (void)hipEventElapsedTime(&g2, gev[2 * q], gev[2 * q + 1]);
@@ -1084,7 +1122,13 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         if (done >= C) break;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
+    for (auto &e : ev2) (void)hipEventDestroy(e);
     for (auto &e : gev) (void)hipEventDestroy(e);
+    if (split) {
+        (void)hipStreamSynchronize(s2);
+        (void)hipEventDestroy(join);
+        (void)hipStreamDestroy(s2);
+    }
     NGMI_HIP(hipMemcpy(b->counters, b->dcounters, sizeof(b->counters), hipMemcpyDeviceToHost));
     if (passes_out) *passes_out = pass;
     if (rc != NGMI_OK) return rc;

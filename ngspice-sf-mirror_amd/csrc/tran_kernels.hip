@@ -102,9 +102,10 @@ __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D
  * is known to address global memory */
 template <int WAVES>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
-k_tran_mos(B4Tab tab, TranDev D)
+k_tran_mos(B4Tab tab, TranDev D, int64_t i0, int64_t i1)
 {
-    tran_mos_body(tab, D, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < i1) tran_mos_body(tab, D, i);
 }
 
 /* several topologies in one launch: blk[b] = (group, first block of the
@@ -922,11 +923,11 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     }
 }
 
-/* one topology, one workgroup per circuit */
+/* one topology, one workgroup per circuit, circuits c0.. */
 __global__ void __launch_bounds__(STEP_THREADS)
-k_tran_step(TranDev D)
+k_tran_step(TranDev D, int c0)
 {
-    tran_step_body(D, blockIdx.x);
+    tran_step_body(D, c0 + blockIdx.x);
 }
 
 /* several topologies: blk[b] = (group, circuit) */
@@ -1080,30 +1081,39 @@ int set_tran_lds_limit(size_t bytes)
 }
 
 /* one Newton pass over every circuit of every topology group; with one
- * group (tab0, D0 set) the descriptor goes by value.  ev (optional) = 3
+ * group (tab0, D0 set) the descriptor goes by value and the pass covers
+ * circuits [c_begin, c_end).  ev (optional) = 3
  * events recorded before the MOSFET kernel, between the two kernels and
  * after the step kernel. */
-int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, const B4Tab *tabs, const TranDev *Ds,
-                     const int2 *mos_blk, int n_mos_blk, const int2 *step_blk, int n_step_blk,
-                     size_t lds_bytes, hipStream_t s, hipEvent_t *ev)
+int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, int c_begin, int c_end, const B4Tab *tabs,
+                     const TranDev *Ds, const int2 *mos_blk, int n_mos_blk, const int2 *step_blk,
+                     int n_step_blk, size_t lds_bytes, hipStream_t s, hipEvent_t *ev)
 {
-    const bool two = tune_get(TUNE_MOS_WAVES) == 2;
+    const int waves = tune_get(TUNE_MOS_WAVES);
     if (ev) NGMI_HIP(hipEventRecord(ev[0], s));
     if (n_mos_blk > 0) {
         const dim3 g((unsigned)n_mos_blk);
         if (D0) {
-            if (two) hipLaunchKernelGGL(k_tran_mos<2>, g, dim3(256), 0, s, *tab0, *D0);
-            else hipLaunchKernelGGL(k_tran_mos<1>, g, dim3(256), 0, s, *tab0, *D0);
+            const int64_t i0 = (int64_t)c_begin * D0->NM, i1 = (int64_t)c_end * D0->NM;
+            const dim3 gh((unsigned)((i1 - i0 + 255) / 256));
+            if (i1 > i0) {
+                switch (waves) {
+                case 1: hipLaunchKernelGGL(k_tran_mos<1>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
+                case 3: hipLaunchKernelGGL(k_tran_mos<3>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
+                case 4: hipLaunchKernelGGL(k_tran_mos<4>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
+                default: hipLaunchKernelGGL(k_tran_mos<2>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
+                }
+            }
         } else {
-            if (two) hipLaunchKernelGGL(k_tran_mos_groups<2>, g, dim3(256), 0, s, tabs, Ds, mos_blk);
-            else hipLaunchKernelGGL(k_tran_mos_groups<1>, g, dim3(256), 0, s, tabs, Ds, mos_blk);
+            if (waves == 1) hipLaunchKernelGGL(k_tran_mos_groups<1>, g, dim3(256), 0, s, tabs, Ds, mos_blk);
+            else hipLaunchKernelGGL(k_tran_mos_groups<2>, g, dim3(256), 0, s, tabs, Ds, mos_blk);
         }
         NGMI_HIP(hipGetLastError());
     }
     if (ev) NGMI_HIP(hipEventRecord(ev[1], s));
     if (D0)
-        hipLaunchKernelGGL(k_tran_step, dim3((unsigned)n_step_blk), dim3(STEP_THREADS), lds_bytes, s,
-                           *D0);
+        hipLaunchKernelGGL(k_tran_step, dim3((unsigned)(c_end - c_begin)), dim3(STEP_THREADS),
+                           lds_bytes, s, *D0, c_begin);
     else
         hipLaunchKernelGGL(k_tran_step_groups, dim3((unsigned)n_step_blk), dim3(STEP_THREADS),
                            lds_bytes, s, Ds, step_blk);
