@@ -391,14 +391,20 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     int nlev = 0;
     for (const Ent &e : ents) nlev = std::max(nlev, e.lev + 1);
     if (nslots + 1 > 65535 || N + 1 > 65535) return fail("circuit too large for 16-bit LU indices");
-    /* term lists: two 16-bit indices per word, padded to whole uint4 groups
-     * with a pair whose product is exactly zero (lu[nslots] and z[N] hold 0);
-     * begin/end are in uint4 units */
-    auto put_terms = [](std::vector<uint32_t> &dst, const std::vector<int2> &t, uint32_t pad) {
-        const int b = (int)(dst.size() / 4);
-        for (const int2 &x : t) dst.push_back((uint32_t)x.x | ((uint32_t)x.y << 16));
-        while (dst.size() % 4) dst.push_back(pad);
-        return make_int2(b, (int)(dst.size() / 4));
+    /* term lists: two 16-bit indices per word, padded to whole groups of
+     * four with a pair whose product is exactly zero (lu[nslots] and z[N]
+     * hold 0).  The first group sits in the entry's own 32-byte record
+     * {x, y, begin, end, group 0}; [begin, end) are the further groups, in
+     * uint4 units of dst. */
+    auto put_terms = [](std::vector<uint32_t> &dst, const std::vector<int2> &t, uint32_t pad,
+                        std::vector<int> &rec) {
+        std::vector<uint32_t> w;
+        for (const int2 &x : t) w.push_back((uint32_t)x.x | ((uint32_t)x.y << 16));
+        while (w.size() < 4 || w.size() % 4) w.push_back(pad);
+        rec.push_back((int)(dst.size() / 4));
+        dst.insert(dst.end(), w.begin() + 4, w.end());
+        rec.push_back((int)(dst.size() / 4));
+        for (int k = 0; k < 4; k++) rec.push_back((int)w[k]);
     };
     const uint32_t pad_lu = (uint32_t)nslots | ((uint32_t)nslots << 16);
     const uint32_t pad_z = (uint32_t)nslots | ((uint32_t)N << 16);
@@ -409,18 +415,16 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         std::vector<std::vector<int>> by(nlev);
         for (int k = 0; k < (int)ents.size(); k++) by[ents[k].lev].push_back(k);
         for (int L = 0; L < nlev; L++) {
-            f_lev[L] = (int)(f_ent.size() / 4);
+            f_lev[L] = (int)(f_ent.size() / 8);
             for (int k : by[L]) {
                 const Ent &e = ents[k];
                 f_ent.push_back(e.t);
                 f_ent.push_back(e.piv);
-                const int2 be = put_terms(f_term, e.terms, pad_lu);
-                f_ent.push_back(be.x);
-                f_ent.push_back(be.y);
+                put_terms(f_term, e.terms, pad_lu, f_ent);
                 nterms += (int)e.terms.size();
             }
         }
-        f_lev[nlev] = (int)(f_ent.size() / 4);
+        f_lev[nlev] = (int)(f_ent.size() / 8);
     }
     P.nlev_f = nlev;
     P.nterms = nterms;
@@ -440,19 +444,17 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         std::vector<std::vector<int>> by(nl);
         for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
         for (int L = 0; L < nl; L++) {
-            fw_lev.push_back((int)(fw_ent.size() / 4));
+            fw_lev.push_back((int)(fw_ent.size() / 8));
             for (int a : by[L]) {
                 std::vector<int2> t;
                 for (int k = 0; k < a; k++)
                     if (S[a][k]) t.push_back(make_int2(slot[(size_t)a * N + k], k));
-                const int2 be = put_terms(fw_term, t, pad_z);
                 fw_ent.push_back(a);
                 fw_ent.push_back(-1);
-                fw_ent.push_back(be.x);
-                fw_ent.push_back(be.y);
+                put_terms(fw_term, t, pad_z, fw_ent);
             }
         }
-        fw_lev.push_back((int)(fw_ent.size() / 4));
+        fw_lev.push_back((int)(fw_ent.size() / 8));
         P.nlev_fw = nl;
     }
     {
@@ -467,19 +469,17 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         std::vector<std::vector<int>> by(nl);
         for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
         for (int L = 0; L < nl; L++) {
-            bw_lev.push_back((int)(bw_ent.size() / 4));
+            bw_lev.push_back((int)(bw_ent.size() / 8));
             for (int a : by[L]) {
                 std::vector<int2> t;
                 for (int j = a + 1; j < N; j++)
                     if (S[a][j]) t.push_back(make_int2(slot[(size_t)a * N + j], j));
-                const int2 be = put_terms(bw_term, t, pad_z);
                 bw_ent.push_back(a);
                 bw_ent.push_back(slot[(size_t)a * N + a]);
-                bw_ent.push_back(be.x);
-                bw_ent.push_back(be.y);
+                put_terms(bw_term, t, pad_z, bw_ent);
             }
         }
-        bw_lev.push_back((int)(bw_ent.size() / 4));
+        bw_lev.push_back((int)(bw_ent.size() / 8));
         P.nlev_bw = nl;
     }
 
@@ -500,26 +500,25 @@ int analyze(const ngmi_circuit &c0, Plan &P)
      * node's diagonal collects one contribution per connected device) */
     auto split = [](const std::vector<uint32_t> &codes, int dest, int partial_dest0, int &npart,
                     std::vector<uint32_t> &code, std::vector<int4> &leaf, std::vector<int4> &comb) {
-        /* contributions in whole uint4 groups, padded with zero codes;
-         * leaf ranges in uint4 units */
+        /* contributions in whole groups of four, padded with zero codes;
+         * a leaf is a 32-byte record {dest, begin, end, 0, group 0} with
+         * [begin, end) its further groups in uint4 units of code */
         const int n = (int)codes.size();
-        auto put = [&](int q0, int q1) {
+        auto put = [&](int d, int q0, int q1) {
+            std::vector<uint32_t> w(codes.begin() + q0, codes.begin() + q1);
+            while (w.size() < 4 || w.size() % 4) w.push_back(CT_ZERO << CT_TYPE_SHIFT);
             const int b = (int)(code.size() / 4);
-            code.insert(code.end(), codes.begin() + q0, codes.begin() + q1);
-            while (code.size() % 4) code.push_back(CT_ZERO << CT_TYPE_SHIFT);
-            return make_int2(b, (int)(code.size() / 4));
+            code.insert(code.end(), w.begin() + 4, w.end());
+            leaf.push_back(make_int4(d, b, (int)(code.size() / 4), 0));
+            leaf.push_back(make_int4((int)w[0], (int)w[1], (int)w[2], (int)w[3]));
         };
         if (n <= ASM_LEAF) {
-            const int2 be = put(0, n);
-            leaf.push_back(make_int4(dest, be.x, be.y, 0));
+            put(dest, 0, n);
             return;
         }
         const int chunk = (std::max(ASM_LEAF, (n + ASM_MAXPART - 1) / ASM_MAXPART) + 3) / 4 * 4;
         const int p0 = npart;
-        for (int q = 0; q < n; q += chunk) {
-            const int2 be = put(q, std::min(n, q + chunk));
-            leaf.push_back(make_int4(partial_dest0 + npart++, be.x, be.y, 0));
-        }
+        for (int q = 0; q < n; q += chunk) put(partial_dest0 + npart++, q, std::min(n, q + chunk));
         comb.push_back(make_int4(dest, p0, npart, 0));
     };
     std::vector<uint32_t> a_code, r_code;
@@ -533,14 +532,15 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         split(rhs_codes[rowp[a]], a, N, n_rpart, r_code, r_leaf, r_comb);
         if (perm[N + a] <= NN && P.ic_nodes[perm[N + a]]) ic_row.push_back(a);
     }
-    P.n_aleaf = (int)a_leaf.size(); P.n_acomb = (int)a_comb.size();
-    P.n_rleaf = (int)r_leaf.size(); P.n_rcomb = (int)r_comb.size(); P.n_icrow = (int)ic_row.size();
+    P.n_aleaf = (int)a_leaf.size() / 2; P.n_acomb = (int)a_comb.size();
+    P.n_rleaf = (int)r_leaf.size() / 2; P.n_rcomb = (int)r_comb.size(); P.n_icrow = (int)ic_row.size();
     P.n_apart = n_apart; P.n_rpart = n_rpart;
     /* keep every device array non-empty */
     if (a_comb.empty()) a_comb.push_back(make_int4(0, 0, 0, 0));
     if (r_comb.empty()) r_comb.push_back(make_int4(0, 0, 0, 0));
-    if (r_leaf.empty()) r_leaf.push_back(make_int4(0, 0, 0, 0));
+    if (r_leaf.empty()) r_leaf.assign(2, make_int4(0, 0, 0, 0));
     while (r_code.size() < 4) r_code.push_back(CT_ZERO << CT_TYPE_SHIFT);
+    while (a_code.size() < 4) a_code.push_back(CT_ZERO << CT_TYPE_SHIFT);
     if (ic_row.empty()) ic_row.push_back(0);
 
     /* ---- neighbour table for the learned guess: up to 8 other non-ground

@@ -452,6 +452,21 @@ __device__ __forceinline__ uint4 gload(const uint4 *p)
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
+/* a 32-byte program record: four ints, then one group of four packed
+ * words (terms or contribution codes) */
+struct Rec {
+    int4 h;
+    uint4 g;
+};
+
+__device__ __forceinline__ Rec load_rec(const int4 *base, int e)
+{
+    Rec r;
+    r.h = gload(base + 2 * e);
+    r.g = gload(reinterpret_cast<const uint4 *>(base + 2 * e + 1));
+    return r;
+}
+
 /* acc - x[a] y[b] for the four packed index pairs of w, in order */
 __device__ __forceinline__ double sub_terms(double acc, uint4 w, const double *x, const double *y)
 {
@@ -460,6 +475,50 @@ __device__ __forceinline__ double sub_terms(double acc, uint4 w, const double *x
     acc -= x[w.z & 0xffffu] * y[w.z >> 16];
     acc -= x[w.w & 0xffffu] * y[w.w >> 16];
     return acc;
+}
+
+/* acc minus the record's term groups: the inline first group, then the
+ * further groups [h.z, h.w) with the next one's load in flight */
+__device__ __forceinline__ double lu_row_terms(double acc, const Rec &r, const uint4 *T,
+                                               const double *x, const double *y)
+{
+    acc = sub_terms(acc, r.g, x, y);
+    int q = r.h.z;
+    if (q < r.h.w) {
+        uint4 w = gload(T + q);
+        while (++q < r.h.w) {
+            const uint4 wn = gload(T + q);
+            acc = sub_terms(acc, w, x, y);
+            w = wn;
+        }
+        acc = sub_terms(acc, w, x, y);
+    }
+    return acc;
+}
+
+/* Level-scheduled program: entries of level L (lev[L] .. lev[L+1]) are
+ * independent; a barrier separates levels.  Each lane's first record of the
+ * next level is loaded before the current level runs, so its latency hides
+ * behind the current level's work and barrier. */
+template <class F>
+__device__ __forceinline__ void run_levels(const int *lev, const int4 *ent, int nlev, int tid, int nt,
+                                           F body)
+{
+    if (nlev <= 0) return;
+    int e0 = gload(lev) + tid, e1 = gload(lev + 1);
+    Rec nx;
+    if (e0 < e1) nx = load_rec(ent, e0);
+    for (int L = 0; L < nlev; L++) {
+        const int ce0 = e0, ce1 = e1;
+        const Rec cur = nx;
+        if (L + 1 < nlev) {
+            e0 = gload(lev + L + 1) + tid;
+            e1 = gload(lev + L + 2);
+            if (e0 < e1) nx = load_rec(ent, e0);
+        }
+        for (int e = ce0; e < ce1; e += nt) body(e == ce0 ? cur : load_rec(ent, e));
+        __syncthreads();
+    }
 }
 
 /* value of one matrix contribution code (MOS stamp, capacitor companion
@@ -600,43 +659,58 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     double *part = vsv + D.NVS;
     const uint4 *AC = reinterpret_cast<const uint4 *>(D.a_code);
     const uint4 *RC = reinterpret_cast<const uint4 *>(D.r_code);
-    for (int e = tid; e < D.n_aleaf; e += nt) {
-        const int4 lf = gload(D.a_leaf + e);
-        double acc = 0.0;
-        for (int q = lf.y; q < lf.z; q++) {
-            const uint4 w = gload(AC + q);
-            const double v0 = mat_value(D, w.x, c, cnm, capg, icz);
-            const double v1 = mat_value(D, w.y, c, cnm, capg, icz);
-            const double v2 = mat_value(D, w.z, c, cnm, capg, icz);
-            const double v3 = mat_value(D, w.w, c, cnm, capg, icz);
-            acc += v0;
-            acc += v1;
-            acc += v2;
-            acc += v3;
-        }
-        if (lf.x < D.nslots) {
-            if ((gload(D.slot_flags + lf.x) & 1) && icz) acc = 1.0;
-            lu[lf.x] = acc;
-        } else {
-            part[lf.x - D.nslots] = acc;
+    /* each lane walks its leaves with the next leaf's record in flight */
+    {
+        Rec nx;
+        if (tid < D.n_aleaf) nx = load_rec(D.a_leaf, tid);
+        for (int e = tid; e < D.n_aleaf; e += nt) {
+            const Rec lf = nx;
+            if (e + nt < D.n_aleaf) nx = load_rec(D.a_leaf, e + nt);
+            double acc = 0.0;
+            uint4 w = lf.g;
+            for (int q = lf.h.y;; q++) {
+                const double v0 = mat_value(D, w.x, c, cnm, capg, icz);
+                const double v1 = mat_value(D, w.y, c, cnm, capg, icz);
+                const double v2 = mat_value(D, w.z, c, cnm, capg, icz);
+                const double v3 = mat_value(D, w.w, c, cnm, capg, icz);
+                acc += v0;
+                acc += v1;
+                acc += v2;
+                acc += v3;
+                if (q >= lf.h.z) break;
+                w = gload(AC + q);
+            }
+            if (lf.h.x < D.nslots) {
+                if ((gload(D.slot_flags + lf.h.x) & 1) && icz) acc = 1.0;
+                lu[lf.h.x] = acc;
+            } else {
+                part[lf.h.x - D.nslots] = acc;
+            }
         }
     }
-    for (int e = tid; e < D.n_rleaf; e += nt) {
-        const int4 lf = gload(D.r_leaf + e);
-        double acc = 0.0;
-        for (int q = lf.y; q < lf.z; q++) {
-            const uint4 w = gload(RC + q);
-            const double v0 = rhs_value(D, w.x, c, cnm, capq, vsv);
-            const double v1 = rhs_value(D, w.y, c, cnm, capq, vsv);
-            const double v2 = rhs_value(D, w.z, c, cnm, capq, vsv);
-            const double v3 = rhs_value(D, w.w, c, cnm, capq, vsv);
-            acc += v0;
-            acc += v1;
-            acc += v2;
-            acc += v3;
+    {
+        Rec nx;
+        if (tid < D.n_rleaf) nx = load_rec(D.r_leaf, tid);
+        for (int e = tid; e < D.n_rleaf; e += nt) {
+            const Rec lf = nx;
+            if (e + nt < D.n_rleaf) nx = load_rec(D.r_leaf, e + nt);
+            double acc = 0.0;
+            uint4 w = lf.g;
+            for (int q = lf.h.y;; q++) {
+                const double v0 = rhs_value(D, w.x, c, cnm, capq, vsv);
+                const double v1 = rhs_value(D, w.y, c, cnm, capq, vsv);
+                const double v2 = rhs_value(D, w.z, c, cnm, capq, vsv);
+                const double v3 = rhs_value(D, w.w, c, cnm, capq, vsv);
+                acc += v0;
+                acc += v1;
+                acc += v2;
+                acc += v3;
+                if (q >= lf.h.z) break;
+                w = gload(RC + q);
+            }
+            if (lf.h.x < N) z[lf.h.x] = acc;
+            else part[D.n_apart + lf.h.x - N] = acc;
         }
-        if (lf.x < N) z[lf.x] = acc;
-        else part[D.n_apart + lf.x - N] = acc;
     }
     __syncthreads();
     for (int e = tid; e < D.n_acomb; e += nt) {
@@ -662,74 +736,27 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     }
     __syncthreads();
 
-    /* LU factorisation, entry-wise Doolittle by dependency levels; each
-     * lane keeps the next term group's load in flight while it uses the
-     * current one */
+    /* LU factorisation, entry-wise Doolittle by dependency levels, then the
+     * two substitutions; see run_levels for the load schedule */
     const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
-    for (int L = 0; L < D.nlev_f; L++) {
-        const int e1 = gload(D.f_lev + L + 1);
-        for (int e = gload(D.f_lev + L) + tid; e < e1; e += nt) {
-            const int4 en = gload(reinterpret_cast<const int4 *>(D.f_ent) + e);
-            double acc = lu[en.x];
-            int q = en.z;
-            if (q < en.w) {
-                uint4 w = gload(FT + q);
-                while (++q < en.w) {
-                    const uint4 wn = gload(FT + q);
-                    acc = sub_terms(acc, w, lu, lu);
-                    w = wn;
-                }
-                acc = sub_terms(acc, w, lu, lu);
-            }
-            if (en.y >= 0) acc /= lu[en.y];
-            lu[en.x] = acc;
-        }
-        __syncthreads();
-    }
+    run_levels(D.f_lev, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
+               [&](const Rec &r) {
+                   const double acc = lu_row_terms(lu[r.h.x], r, FT, lu, lu);
+                   lu[r.h.x] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
+               });
     /* forward substitution (unit L) */
     const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
-    for (int L = 0; L < D.nlev_fw; L++) {
-        const int e1 = gload(D.fw_lev + L + 1);
-        for (int e = gload(D.fw_lev + L) + tid; e < e1; e += nt) {
-            const int4 en = gload(reinterpret_cast<const int4 *>(D.fw_ent) + e);
-            double acc = z[en.x];
-            int q = en.z;
-            if (q < en.w) {
-                uint4 w = gload(WT + q);
-                while (++q < en.w) {
-                    const uint4 wn = gload(WT + q);
-                    acc = sub_terms(acc, w, lu, z);
-                    w = wn;
-                }
-                acc = sub_terms(acc, w, lu, z);
-            }
-            z[en.x] = acc;
-        }
-        __syncthreads();
-    }
+    run_levels(D.fw_lev, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
+               [&](const Rec &r) { z[r.h.x] = lu_row_terms(z[r.h.x], r, WT, lu, z); });
     /* back substitution */
     const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
-    for (int L = 0; L < D.nlev_bw; L++) {
-        const int e1 = gload(D.bw_lev + L + 1);
-        for (int e = gload(D.bw_lev + L) + tid; e < e1; e += nt) {
-            const int4 en = gload(reinterpret_cast<const int4 *>(D.bw_ent) + e);
-            double acc = z[en.x];
-            int q = en.z;
-            if (q < en.w) {
-                uint4 w = gload(BT + q);
-                while (++q < en.w) {
-                    const uint4 wn = gload(BT + q);
-                    acc = sub_terms(acc, w, lu, z);
-                    w = wn;
-                }
-                acc = sub_terms(acc, w, lu, z);
-            }
-            const double piv = lu[en.y];
-            if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
-            z[en.x] = acc / piv;
-        }
-        __syncthreads();
-    }
+    run_levels(D.bw_lev, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
+               [&](const Rec &r) {
+                   const double acc = lu_row_terms(z[r.h.x], r, BT, lu, z);
+                   const double piv = lu[r.h.y];
+                   if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
+                   z[r.h.x] = acc / piv;
+               });
     /* new iterate; NIconvTest (niconv.c:21-90) and the device votes */
     for (int a = tid; a < N; a += nt) xn[D.perm_unknown[a]] = z[a];
     if (tid == 0) xn[0] = 0.0;

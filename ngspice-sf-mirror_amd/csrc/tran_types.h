@@ -76,11 +76,12 @@ struct TranDev {
     double *vs_brk;           /* [C][NVS] break_time */
     const double *ic_val;     /* [C][NN+1] */
     /* LU program */
-    /* assembly: leaves (dest, begin, end) sum the contribution groups
-     * [begin, end) (uint4 units of a_code, padded with zero codes) into L+U
-     * slot dest (< nslots) or partial dest - nslots; combines (slot, pbegin,
-     * pend) add partials into a slot; likewise for the RHS rows (dest < N a
-     * row, else partial n_apart + dest - N) */
+    /* assembly: a leaf is a 32-byte record {dest, begin, end, 0} + its
+     * first group of four contribution codes; [begin, end) are its further
+     * groups (uint4 units of a_code, padded with zero codes).  It sums into
+     * L+U slot dest (< nslots) or partial dest - nslots; combines (slot,
+     * pbegin, pend) add partials into a slot; likewise for the RHS rows
+     * (dest < N a row, else partial n_apart + dest - N) */
     const int4 *a_leaf, *a_comb, *r_leaf, *r_comb;
     int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
     const uint32_t *a_code;   /* contributions */
@@ -90,14 +91,15 @@ struct TranDev {
     int n_icrow;
     /* LU program.  Terms are 16-bit index pairs packed in one word (low |
      * high << 16), grouped four to a uint4 and padded with pairs whose
-     * product is exactly zero (LDS lu[nslots] = z[N] = 0); entry term ranges
-     * are in uint4 units. */
+     * product is exactly zero (LDS lu[nslots] = z[N] = 0).  An entry is a
+     * 32-byte record: four ints, then its first term group; the ints' term
+     * range [begin, end) holds the further groups, in uint4 units. */
     const int *f_lev;         /* [nlev_f+1] -> entries */
-    const int *f_ent;         /* [ne][4]: target, pivot (-1 for U), term begin, term end */
+    const int *f_ent;         /* [ne][8]: target, pivot (-1 for U), term begin, term end, group 0 */
     const uint32_t *f_term;   /* (a, b) slot pairs: lu[t] -= lu[a] lu[b] */
-    const int *fw_lev, *fw_ent;   /* forward solve: [ne][4] row, -1, term begin, end */
+    const int *fw_lev, *fw_ent;   /* forward solve: [ne][8] row, -1, term begin, end, group 0 */
     const uint32_t *fw_term;      /* (L slot, z index) */
-    const int *bw_lev, *bw_ent;   /* backward solve: [ne][4] row, diag slot, term begin, end */
+    const int *bw_lev, *bw_ent;   /* backward solve: [ne][8] row, diag slot, term begin, end, group 0 */
     const uint32_t *bw_term;      /* (U slot, z index) */
     const int *perm_unknown;  /* [2N]: [a] unknown (1-based) solved in permuted column a;
                                  [N + a] original row assembled into permuted row a */
