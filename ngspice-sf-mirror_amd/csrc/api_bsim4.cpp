@@ -243,6 +243,11 @@ int ngmi_bsim4_set_finalize(ngmi_bsim4_set *s)
             for (int m = 0; m < nm; m++) dst[m] = s->rows[firstOf[m] * F + f];
         }
     }
+    /* one selector word for the whole set lets the kernels compile the
+     * selectors as constants */
+    s->tab.flags = n ? (int)s->rows[B4F_flags] : -1;
+    for (size_t k = 1; k < n && s->tab.flags >= 0; k++)
+        if ((int)s->rows[k * F + B4F_flags] != s->tab.flags) s->tab.flags = -1;
     s->release();
     const size_t bi = std::max<size_t>(itab.size(), 1) * sizeof(double);
     const size_t bm = std::max<size_t>(mtab.size(), 1) * sizeof(double);

@@ -10,7 +10,7 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {2, 2, 1};
+static int g_tune[TUNE__COUNT] = {2, 2, 1, 1};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
 
@@ -37,13 +37,15 @@ int ngmi_tune(const char *knob, int value)
     if (!std::strcmp(knob, "eval_waves")) k = TUNE_EVAL_WAVES;
     else if (!std::strcmp(knob, "mos_waves")) k = TUNE_MOS_WAVES;
     else if (!std::strcmp(knob, "streams")) k = TUNE_STREAMS;
+    else if (!std::strcmp(knob, "specialise")) k = TUNE_SPECIALISE;
     if (k < 0) {
         set_error("unknown tuning knob '%s'", knob);
         return NGMI_E_BADPARM;
     }
-    const int vmax = (k == TUNE_STREAMS) ? 2 : 4;
-    if (value < 1 || value > vmax) {
-        set_error("knob '%s' takes 1..%d", knob, vmax);
+    const int vmin = (k == TUNE_SPECIALISE) ? 0 : 1;
+    const int vmax = (k == TUNE_STREAMS) ? 2 : (k == TUNE_SPECIALISE) ? 1 : 4;
+    if (value < vmin || value > vmax) {
+        set_error("knob '%s' takes %d..%d", knob, vmin, vmax);
         return NGMI_E_PARMVAL;
     }
     g_tune[k] = value;

@@ -292,7 +292,9 @@ NGMI_HD void b4_polydep(double phi, double ngate, double coxe, double Vgs, doubl
  * level 0 (and level 1 in INITTRAN), the cache, and returns the number of
  * non-convergence votes (0 or 1) this instance adds to CKTnoncon.
  */
-template <class P, class S, class C>
+/* FL >= 0: every instance has selector word FL, compiled as a constant so
+ * the selector branches fold away; FL = -1 reads it per instance */
+template <int FL = -1, class P, class S, class C>
 NGMI_HD int b4_load(const P &prm, S &st, C &ca, const B4Ckt &ck, const double *vnode,
                     double *stamp, double *rhs)
 {
@@ -301,7 +303,7 @@ NGMI_HD int b4_load(const P &prm, S &st, C &ca, const B4Ckt &ck, const double *v
 #define CA(f) ca.c(B4C_##f)
 #define SETC(f, v) ca.setc(B4C_##f, (v))
     const double type = PR(type);
-    const int flags = (int)PR(flags);
+    const int flags = (FL >= 0) ? FL : (int)PR(flags);
     const int mobMod = flags & B4FL_MOBMOD_MASK;
     const int dioMod = (flags >> B4FL_DIOMOD_SHIFT) & 3;
     const int igcMod = (flags >> B4FL_IGCMOD_SHIFT) & 3;

@@ -70,6 +70,8 @@ enum NgmiB4Field {
 #define B4FL_PIGCD_GIVEN   0x1000
 #define B4FL_LAMBDA        0x2000   /* lambda given and > 0 */
 #define B4FL_VTL           0x4000   /* vtl given and > 0 */
+/* the default card's selector word: dioMod 1, every other selector 0 */
+#define B4FL_DEFAULT       (1 << B4FL_DIOMOD_SHIFT)
 
 /* 16 Jacobian entries of one instance, row-major over (dp, gp, sp, bp) */
 enum { B4S_DD, B4S_DG, B4S_DS, B4S_DB, B4S_GD, B4S_GG, B4S_GS, B4S_GB,

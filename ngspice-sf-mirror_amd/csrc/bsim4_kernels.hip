@@ -46,9 +46,10 @@ struct EvalIO {
 };
 
 /* WAVES = wavefronts per SIMD the register allocation aims for: 1 gives the
- * whole evaluation its registers, 2 trades some spills for latency hiding
- * (chosen per workload, see ngmi_tune) */
-template <int WAVES>
+ * whole evaluation its registers, 2..4 trade spills for latency hiding
+ * (chosen per workload, see ngmi_tune).  FL >= 0: the selector word every
+ * instance shares, compiled as a constant. */
+template <int WAVES, int FL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 k_bsim4_eval(B4Tab tab, B4Ckt ck, EvalIO io)
 {
@@ -63,7 +64,7 @@ k_bsim4_eval(B4Tab tab, B4Ckt ck, EvalIO io)
 #pragma unroll
     for (int k = 0; k < 4; k++) v[k] = io.vterm[k * n + i];
     double J[B4S__COUNT], R[B4R__COUNT];
-    const int nc = b4_load(p, st, ca, ck, v, J, R);
+    const int nc = b4_load<FL>(p, st, ca, ck, v, J, R);
 #pragma unroll
     for (int k = 0; k < B4S__COUNT; k++) io.stamp[k * n + i] = J[k];
 #pragma unroll
@@ -79,12 +80,16 @@ int launch_bsim4_eval(const B4Tab &tab, const B4Ckt &ck, const double *vterm,
     EvalIO io{vterm, state[0], state[1], state[2], cache, stamp, rhs, noncon};
     const int bs = 256;
     const int64_t nb = (tab.n + bs - 1) / bs;
+    const bool dflt = tab.flags == B4FL_DEFAULT && tune_get(TUNE_SPECIALISE);
+#define NGMI_EVAL(W, F) \
+    hipLaunchKernelGGL((k_bsim4_eval<W, F>), dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io)
     switch (tune_get(TUNE_EVAL_WAVES)) {
-    case 1: hipLaunchKernelGGL(k_bsim4_eval<1>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
-    case 3: hipLaunchKernelGGL(k_bsim4_eval<3>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
-    case 4: hipLaunchKernelGGL(k_bsim4_eval<4>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
-    default: hipLaunchKernelGGL(k_bsim4_eval<2>, dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io); break;
+    case 1: if (dflt) NGMI_EVAL(1, B4FL_DEFAULT); else NGMI_EVAL(1, -1); break;
+    case 3: if (dflt) NGMI_EVAL(3, B4FL_DEFAULT); else NGMI_EVAL(3, -1); break;
+    case 4: if (dflt) NGMI_EVAL(4, B4FL_DEFAULT); else NGMI_EVAL(4, -1); break;
+    default: if (dflt) NGMI_EVAL(2, B4FL_DEFAULT); else NGMI_EVAL(2, -1); break;
     }
+#undef NGMI_EVAL
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }

@@ -16,7 +16,7 @@ namespace ngmi {
 void set_error(const char *fmt, ...);
 
 /* launch-shape knobs (ngmi_tune); defaults are the measured best */
-enum { TUNE_EVAL_WAVES = 0, TUNE_MOS_WAVES = 1, TUNE_STREAMS = 2, TUNE__COUNT };
+enum { TUNE_EVAL_WAVES = 0, TUNE_MOS_WAVES = 1, TUNE_STREAMS = 2, TUNE_SPECIALISE = 3, TUNE__COUNT };
 int tune_get(int knob);
 
 #define NGMI_HIP(call)                                                          \
@@ -45,6 +45,7 @@ struct B4Tab {
     int64_t n;       /* instances (row stride of itab) */
     int nm;          /* models (row stride of mtab) */
     const int *code;
+    int flags;       /* the selector word when every instance shares it, else -1 */
 };
 
 }  // namespace ngmi

@@ -64,6 +64,7 @@ struct TranCache {
 
 /* one thread per (circuit, MOSFET) of a topology group, circuit-major
  * instances i = c * NM + j */
+template <int FL>
 __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D, int64_t i)
 {
     const int64_t cnm = (int64_t)D.C * D.NM;
@@ -90,7 +91,7 @@ __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D
     TranState st{D.mstate, cnm, i, K->rb};
     TranCache ca{D.cache, cnm, i};
     double J[B4S__COUNT], R[B4R__COUNT];
-    const int nc = b4_load(p, st, ca, ck, v, J, R);
+    const int nc = b4_load<FL>(p, st, ca, ck, v, J, R);
 #pragma unroll
     for (int k = 0; k < B4S__COUNT; k++) D.stamp[k * cnm + i] = J[k];
 #pragma unroll
@@ -100,12 +101,12 @@ __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D
 
 /* one topology: the descriptor is a kernel argument, so every table pointer
  * is known to address global memory */
-template <int WAVES>
+template <int WAVES, int FL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 k_tran_mos(B4Tab tab, TranDev D, int64_t i0, int64_t i1)
 {
     const int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < i1) tran_mos_body(tab, D, i);
+    if (i < i1) tran_mos_body<FL>(tab, D, i);
 }
 
 /* several topologies in one launch: blk[b] = (group, first block of the
@@ -116,7 +117,7 @@ k_tran_mos_groups(const B4Tab *__restrict__ tabs, const TranDev *__restrict__ Ds
                   const int2 *__restrict__ blk)
 {
     const int2 bg = blk[blockIdx.x];
-    tran_mos_body(tabs[bg.x], Ds[bg.x], (int64_t)(blockIdx.x - bg.y) * blockDim.x + threadIdx.x);
+    tran_mos_body<-1>(tabs[bg.x], Ds[bg.x], (int64_t)(blockIdx.x - bg.y) * blockDim.x + threadIdx.x);
 }
 
 /* ---------------- scalar helpers (thread 0 of a circuit) ---------------- */
@@ -1123,13 +1124,18 @@ int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, int c_begin, int c_en
         if (D0) {
             const int64_t i0 = (int64_t)c_begin * D0->NM, i1 = (int64_t)c_end * D0->NM;
             const dim3 gh((unsigned)((i1 - i0 + 255) / 256));
+            /* the default card's selectors compiled as constants when the
+             * whole set shares them */
+            const bool dflt = tab0->flags == B4FL_DEFAULT && tune_get(TUNE_SPECIALISE);
             if (i1 > i0) {
+#define NGMI_MOS(W, F) hipLaunchKernelGGL((k_tran_mos<W, F>), gh, dim3(256), 0, s, *tab0, *D0, i0, i1)
                 switch (waves) {
-                case 1: hipLaunchKernelGGL(k_tran_mos<1>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
-                case 3: hipLaunchKernelGGL(k_tran_mos<3>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
-                case 4: hipLaunchKernelGGL(k_tran_mos<4>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
-                default: hipLaunchKernelGGL(k_tran_mos<2>, gh, dim3(256), 0, s, *tab0, *D0, i0, i1); break;
+                case 1: if (dflt) NGMI_MOS(1, B4FL_DEFAULT); else NGMI_MOS(1, -1); break;
+                case 3: if (dflt) NGMI_MOS(3, B4FL_DEFAULT); else NGMI_MOS(3, -1); break;
+                case 4: if (dflt) NGMI_MOS(4, B4FL_DEFAULT); else NGMI_MOS(4, -1); break;
+                default: if (dflt) NGMI_MOS(2, B4FL_DEFAULT); else NGMI_MOS(2, -1); break;
                 }
+#undef NGMI_MOS
             }
         } else {
             if (waves == 1) hipLaunchKernelGGL(k_tran_mos_groups<1>, g, dim3(256), 0, s, tabs, Ds, mos_blk);
