@@ -28,6 +28,17 @@ sys.path.insert(0, os.path.join(REPO, 'ngspice-sf-mirror_amd'))
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per BSIM4 evaluation that the hardware counters read for the
+# default (2-wave, default-card) kernels: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE summed over the kernel's dispatches, scaled by what the same
+# counters read for a device copy of known size (FETCH_SIZE x 2.00, WRITE_SIZE
+# x 1.00), divided by the evaluations the engine counted.  None where not
+# measured for the selected variant.
+MEASURED_HBM_BYTES_PER_EVAL = {('k_tran_mos', 2): 1823, ('k_bsim4_eval', 2): 1749}
+# FP64 FLOP per evaluation from SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes
+# (FMA counted twice), same runs
+MEASURED_FLOP_PER_EVAL = {'k_tran_mos': 6630, 'k_bsim4_eval': 6583}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -101,6 +112,8 @@ def bench_eval(args):
     # 1 int vote, 1 int model index
     per_inst = 8 * (icols + 4 + 9 + 19 + 76 + 20) + 4 + 4
     achieved = per_inst * n / (ev * 1e-3) / 1e9
+    mb = MEASURED_HBM_BYTES_PER_EVAL.get(('k_bsim4_eval', args.eval_waves or 2))
+    traffic = mb * n if mb else None
     return {
         'metric': 'BSIM4 device evaluations/s (1M instances, with CSC assembly)',
         'value': n / (ms_step * 1e-3),
@@ -110,8 +123,10 @@ def bench_eval(args):
         'config': {'workload': 'device-eval: 2^20 BSIM4 instances, 45nm base card, '
                                'random 2^21-row CSC (~8 nnz/row)', 'nnz': nnz},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'k_bsim4_eval', 'kernel_ms': ev, 'bytes_per_instance': per_inst},
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+                     'kernel': 'k_bsim4_eval', 'kernel_ms': ev, 'bytes_per_instance': per_inst,
+                     'fp64_tflops': MEASURED_FLOP_PER_EVAL['k_bsim4_eval'] * n / (ev * 1e-3) / 1e12,
+                     'fp64_peak_tflops': FP64_PEAK_TFLOPS},
         'extra': {'gather_ms': ga, 'eval_ms': ev, 'inst_cols': icols, 'model_cols': mcols},
     }
 
@@ -140,17 +155,17 @@ def bench_copy(args):
             'extra': {'bytes_read': n * 8, 'bytes_written': n * 8}}
 
 
-def cpu_baseline_mc(tstop, budget_s=12.0):
-    """The oracle's DCtran restatement on whole adder circuits (seeds 0, 1,
-    ...), one host core, solving with the reference's own KLU when
-    oracle/_ref was built; stops after the first circuit that ends past the
-    time budget."""
+def _cpu_worker(job):
+    """one host core: adder circuits seeds w, w + P, ... through the oracle's
+    DCtran restatement until the time budget is spent; returns (accepted
+    points, seconds, circuits, solver)"""
+    w, procs, tstop, budget_s = job
     sys.path.insert(0, os.path.join(REPO, 'oracle'))
     import pyoracle as po
     import workloads
-    pts, secs, seeds, solver = 0, 0.0, 0, 'dense'
-    while secs < budget_s and seeds < 64:
-        d = workloads.mc_adder(seeds)
+    pts, secs, n, solver, seed = 0, 0.0, 0, 'dense', w
+    while secs < budget_s and n < 64:
+        d = workloads.mc_adder(seed)
         T = po.Transient(d)
         solver = 'klu' if T.use_reference_klu() else 'dense'
         t0 = time.perf_counter()
@@ -159,12 +174,36 @@ def cpu_baseline_mc(tstop, budget_s=12.0):
         if rc != 0:
             break
         pts += len(t)
-        seeds += 1
+        n += 1
+        seed += procs
+    return pts, secs, n, solver
+
+
+def cpu_baseline_mc(tstop, budget_s=12.0, procs=1):
+    """The oracle's DCtran restatement on whole adder circuits, solving with
+    the reference's own KLU when oracle/_ref was built: `procs` independent
+    processes (one per core) each run whole circuits until about budget_s of
+    CPU time; throughput = all accepted points / the slowest process's time.
+    Runs before the GPU is touched, so the worker processes fork from a
+    process without a GPU context."""
+    import multiprocessing as mp
+    jobs = [(w, procs, tstop, budget_s) for w in range(procs)]
+    if procs == 1:
+        res = [_cpu_worker(jobs[0])]
+    else:
+        with mp.get_context('fork').Pool(procs) as pool:
+            res = pool.map(_cpu_worker, jobs)
+    pts = sum(r[0] for r in res)
+    secs = max(r[1] for r in res)
+    n = sum(r[2] for r in res)
+    solver = res[0][3]
+    busy = sum(r[1] for r in res)
     return {'value': pts / secs if secs > 0 else None, 'unit': 'timepoints x circuits/s',
-            'cores': 1, 'kind': 'port',
-            'sample': '%d adder circuit(s) (seeds 0..%d), full .tran 5p %gn, oracle DCtran '
-                      'restatement with %s linear solve, %d accepted points in %.1f s'
-                      % (seeds, seeds - 1, tstop * 1e9,
+            'cores': procs, 'kind': 'port', 'per_core': pts / busy if busy > 0 else None,
+            'sample': '%d adder circuit(s) on %d process(es), one per core, full .tran 5p %gn, '
+                      'oracle DCtran restatement with %s linear solve, %d accepted points, '
+                      'slowest process %.1f s'
+                      % (n, procs, tstop * 1e9,
                          'the reference KLU' if solver == 'klu' else 'dense LU', pts, secs)}
 
 
@@ -216,6 +255,10 @@ def bench_mc(args):
     # convergence vote and the model index
     b_eval = 8 * (icols + 4 + 4 + 6 + 21 + 38 + 38 + 20) + 4 + 4
     achieved = evals * b_eval / (mos_ms * 1e-3) / 1e9 if mos_ms > 0 else None
+    mb = MEASURED_HBM_BYTES_PER_EVAL.get(('k_tran_mos', args.mos_waves or 2))
+    traffic = mb * evals / passes if (mb and passes) else None
+    fp64_tflops = (MEASURED_FLOP_PER_EVAL['k_tran_mos'] * evals / (mos_ms * 1e-3) / 1e12
+                   if mos_ms > 0 else None)
     log('passes %d, accepted %d, rejected %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
         'step %.1f ms, wall %.1f ms' % (passes, acc, rej, its, failed, mos_ms, step_ms,
                                         wall * 1e3))
@@ -233,9 +276,10 @@ def bench_mc(args):
                    'circuits': C, 'unknowns': info['unknowns'],
                    'lu_nonzeros': info['lu_nonzeros']},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': None,
+                     'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': traffic,
                      'kernel': 'k_tran_mos', 'kernel_ms_total': mos_ms,
-                     'launches': passes, 'bytes_per_eval': b_eval, 'evals': evals},
+                     'launches': passes, 'bytes_per_eval': b_eval, 'evals': evals,
+                     'fp64_tflops': fp64_tflops, 'fp64_peak_tflops': FP64_PEAK_TFLOPS},
         'extra': {'accepted': acc, 'rejected': rej, 'newton_iterations': its,
                   'iterations_per_point': its / max(acc, 1), 'passes': passes,
                   'failed_circuits': failed, 'step_kernel_ms_total': step_ms,
@@ -420,9 +464,13 @@ def main():
                     help='mc default 20 ns, rings default 10 ns')
     ap.add_argument('--max-points', type=int, default=8192)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-procs', type=int, default=min(16, os.cpu_count() or 1),
+                    help='host processes for the CPU baseline (one per core)')
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess run')
     ap.add_argument('--eval-waves', type=int, default=None, help='BSIM4 eval kernel occupancy')
     ap.add_argument('--mos-waves', type=int, default=None, help='transient BSIM4 kernel occupancy')
+    ap.add_argument('--specialise', type=int, default=None,
+                    help='0: BSIM4 kernels read the selector word per instance')
     ap.add_argument('--streams', type=int, default=None,
                     help='2: the plain transient alternates circuit halves on two streams')
     ap.add_argument('--train-circuits', type=int, default=32)
@@ -440,6 +488,10 @@ def main():
         print('bench.py runs on one GPU only (the engine has no multi-GPU path); '
               'got --gpus %d' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    cpu = None
+    if args.workload == 'mc' and not args.no_cpu_baseline:
+        cpu = cpu_baseline_mc(args.tstop, procs=args.cpu_procs)
+        log('cpu baseline: %s' % cpu)
     import torch
     if not torch.cuda.is_available():
         print('bench.py needs a GPU', file=sys.stderr)
@@ -451,9 +503,10 @@ def main():
         ngmi.tune('mos_waves', args.mos_waves)
     if args.streams:
         ngmi.tune('streams', args.streams)
+    if args.specialise is not None:
+        ngmi.tune('specialise', args.specialise)
     if args.workload == 'mc':
         res = bench_mc(args)
-        cpu = None if args.no_cpu_baseline else cpu_baseline_mc(args.tstop)
     elif args.workload == 'copy':
         res = bench_copy(args)
         cpu = None

@@ -64,6 +64,12 @@ def lib():
     """The engine library; raises when it has not been built."""
     global _lib
     if _lib is None:
+        # a process that uses torch must load torch's HIP runtime first, so
+        # the engine binds to the same runtime instead of a second copy
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError('engine library %s not built (run make -C %s)' % (LIB_PATH, HERE))
         L = ctypes.CDLL(LIB_PATH)
