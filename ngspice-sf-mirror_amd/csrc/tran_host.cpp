@@ -230,6 +230,8 @@ struct Plan {
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
     std::vector<int> slot_flags, perm, ic_row, nbr;
+    std::vector<int> mos_pos, rmos_pos;
+    int NA = 0, NR = 0;
 };
 
 int analyze(const ngmi_circuit &c0, Plan &P)
@@ -501,14 +503,14 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     auto split = [](const std::vector<uint32_t> &codes, int dest, int partial_dest0, int &npart,
                     std::vector<uint32_t> &code, std::vector<int4> &leaf, std::vector<int4> &comb) {
         /* contributions in whole groups of four, padded with zero codes;
-         * a leaf is a 32-byte record {dest, begin, end, 0, group 0} with
-         * [begin, end) its further groups in uint4 units of code */
+         * a leaf is a 32-byte record {dest, begin, end, 0, copy of group 0}
+         * with [begin, end) all its groups in uint4 units of code */
         const int n = (int)codes.size();
         auto put = [&](int d, int q0, int q1) {
             std::vector<uint32_t> w(codes.begin() + q0, codes.begin() + q1);
             while (w.size() < 4 || w.size() % 4) w.push_back(CT_ZERO << CT_TYPE_SHIFT);
             const int b = (int)(code.size() / 4);
-            code.insert(code.end(), w.begin() + 4, w.end());
+            code.insert(code.end(), w.begin(), w.end());
             leaf.push_back(make_int4(d, b, (int)(code.size() / 4), 0));
             leaf.push_back(make_int4((int)w[0], (int)w[1], (int)w[2], (int)w[3]));
         };
@@ -541,6 +543,15 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     if (r_leaf.empty()) r_leaf.assign(2, make_int4(0, 0, 0, 0));
     while (r_code.size() < 4) r_code.push_back(CT_ZERO << CT_TYPE_SHIFT);
     while (a_code.size() < 4) a_code.push_back(CT_ZERO << CT_TYPE_SHIFT);
+    /* where the BSIM4 kernel writes each stamp and RHS value */
+    P.mos_pos.assign((size_t)B4S__COUNT * NM, -1);
+    P.rmos_pos.assign((size_t)B4R__COUNT * NM, -1);
+    for (size_t p = 0; p < a_code.size(); p++)
+        if ((a_code[p] >> CT_TYPE_SHIFT) == CT_MOS) P.mos_pos[a_code[p] & CT_INDEX_MASK] = (int)p;
+    for (size_t p = 0; p < r_code.size(); p++)
+        if ((r_code[p] >> CT_TYPE_SHIFT) == CT_MOS) P.rmos_pos[r_code[p] & CT_INDEX_MASK] = (int)p;
+    P.NA = (int)a_code.size();
+    P.NR = (int)r_code.size();
     if (ic_row.empty()) ic_row.push_back(0);
 
     /* ---- neighbour table for the learned guess: up to 8 other non-ground
@@ -701,6 +712,8 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     UP(P.bw_ent, bw_ent, int, ip);
     UP(P.bw_term, bw_term, uint32_t, up);
     UP(P.perm, perm_unknown, int, ip);
+    UP(P.mos_pos, mos_pos, int, ip);
+    UP(P.rmos_pos, rmos_pos, int, ip);
     UP(G->probe, probe, int, ip);
     UP(G->nbr_host, nbr, int, ip);
 #undef UP
@@ -718,10 +731,12 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.cstate = dp;
     if (alloc(b, (size_t)B4C__COUNT * cnm, &dp) != NGMI_OK) return fail(nullptr);
     D.cache = dp;
-    if (alloc(b, (size_t)B4S__COUNT * cnm, &dp) != NGMI_OK) return fail(nullptr);
-    D.stamp = dp;
-    if (alloc(b, (size_t)B4R__COUNT * cnm, &dp) != NGMI_OK) return fail(nullptr);
-    D.mrhs = dp;
+    D.NA = P.NA;
+    D.NR = P.NR;
+    if (alloc(b, (size_t)C * P.NA, &dp) != NGMI_OK) return fail(nullptr);
+    D.contrib = dp;
+    if (alloc(b, (size_t)C * P.NR, &dp) != NGMI_OK) return fail(nullptr);
+    D.rcontrib = dp;
     if (alloc(b, (size_t)cnm, &ip) != NGMI_OK) return fail(nullptr);
     D.noncon = ip;
     if (alloc(b, (size_t)C * b->cap_pts, &dp) != NGMI_OK) return fail(nullptr);

@@ -92,10 +92,19 @@ __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D
     TranCache ca{D.cache, cnm, i};
     double J[B4S__COUNT], R[B4R__COUNT];
     const int nc = b4_load<FL>(p, st, ca, ck, v, J, R);
+    /* straight into the circuit's assembly-ordered contribution arrays */
+    double *ca_ = D.contrib + (int64_t)c * D.NA;
+    double *cr_ = D.rcontrib + (int64_t)c * D.NR;
 #pragma unroll
-    for (int k = 0; k < B4S__COUNT; k++) D.stamp[k * cnm + i] = J[k];
+    for (int k = 0; k < B4S__COUNT; k++) {
+        const int p = D.mos_pos[k * D.NM + j];
+        if (p >= 0) ca_[p] = J[k];
+    }
 #pragma unroll
-    for (int k = 0; k < B4R__COUNT; k++) D.mrhs[k * cnm + i] = R[k];
+    for (int k = 0; k < B4R__COUNT; k++) {
+        const int p = D.rmos_pos[k * D.NM + j];
+        if (p >= 0) cr_[p] = R[k];
+    }
     D.noncon[i] = nc;
 }
 
@@ -446,6 +455,14 @@ __device__ __forceinline__ int4 gload(const int4 *p)
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+__device__ __forceinline__ double4 gload4(const double4 *p)
+{
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const __attribute__((address_space(1))) v2 *q = (const __attribute__((address_space(1))) v2 *)p;
+    const v2 a = q[0], b = q[1];
+    return make_double4(a.x, a.y, b.x, b.y);
+}
+
 __device__ __forceinline__ uint4 gload(const uint4 *p)
 {
     typedef unsigned int v4 __attribute__((ext_vector_type(4)));
@@ -522,20 +539,17 @@ __device__ __forceinline__ void run_levels(const int *lev, const int4 *ent, int 
     }
 }
 
-/* value of one matrix contribution code (MOS stamp, capacitor companion
+/* value of one matrix contribution code (MOS stamp = mv, the value the
+ * BSIM4 kernel left at this code's position; capacitor companion
  * conductance, resistor conductance, source +-1, padding 0) */
-__device__ __forceinline__ double mat_value(const TranDev &D, uint32_t code, int c, int64_t cnm,
+__device__ __forceinline__ double mat_value(const TranDev &D, uint32_t code, double mv, int c,
                                             const double *capg, bool icz)
 {
     if ((code & CT_ICZ) && icz) return 0.0;
     const uint32_t idx = code & CT_INDEX_MASK;
     double v;
     switch (code >> CT_TYPE_SHIFT) {
-    case CT_MOS: {
-        const int sl = idx / D.NM, j = idx - sl * D.NM;
-        v = gload(D.stamp + sl * cnm + (int64_t)c * D.NM + j);
-        break;
-    }
+    case CT_MOS: v = mv; break;
     case CT_CAP: v = capg[idx]; break;
     case CT_RES: v = gload(D.res_g + (int64_t)c * D.NRES + idx); break;
     case CT_ONE: v = 1.0; break;
@@ -544,19 +558,15 @@ __device__ __forceinline__ double mat_value(const TranDev &D, uint32_t code, int
     return (code & CT_NEG) ? -v : v;
 }
 
-/* value of one RHS contribution code (MOS current, capacitor companion
- * current, source value, padding 0) */
-__device__ __forceinline__ double rhs_value(const TranDev &D, uint32_t code, int c, int64_t cnm,
-                                            const double *capq, const double *vsv)
+/* value of one RHS contribution code (MOS current = mv as above,
+ * capacitor companion current, source value, padding 0) */
+__device__ __forceinline__ double rhs_value(uint32_t code, double mv, const double *capq,
+                                            const double *vsv)
 {
     const uint32_t idx = code & CT_INDEX_MASK;
     double v;
     switch (code >> CT_TYPE_SHIFT) {
-    case CT_MOS: {
-        const int r = idx / D.NM, j = idx - r * D.NM;
-        v = gload(D.mrhs + r * cnm + (int64_t)c * D.NM + j);
-        break;
-    }
+    case CT_MOS: v = mv; break;
     case CT_CAP: v = capq[idx]; break;
     case CT_VSRC: v = vsv[idx]; break;
     default: v = 0.0; break;
@@ -660,6 +670,10 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     double *part = vsv + D.NVS;
     const uint4 *AC = reinterpret_cast<const uint4 *>(D.a_code);
     const uint4 *RC = reinterpret_cast<const uint4 *>(D.r_code);
+    /* the circuit's contribution values, four per group: consecutive leaves
+     * read consecutive groups, so a wavefront's loads are contiguous */
+    const double4 *AV = reinterpret_cast<const double4 *>(D.contrib + (int64_t)c * D.NA);
+    const double4 *RV = reinterpret_cast<const double4 *>(D.rcontrib + (int64_t)c * D.NR);
     /* each lane walks its leaves with the next leaf's record in flight */
     {
         Rec nx;
@@ -669,16 +683,17 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
             if (e + nt < D.n_aleaf) nx = load_rec(D.a_leaf, e + nt);
             double acc = 0.0;
             uint4 w = lf.g;
-            for (int q = lf.h.y;; q++) {
-                const double v0 = mat_value(D, w.x, c, cnm, capg, icz);
-                const double v1 = mat_value(D, w.y, c, cnm, capg, icz);
-                const double v2 = mat_value(D, w.z, c, cnm, capg, icz);
-                const double v3 = mat_value(D, w.w, c, cnm, capg, icz);
+            for (int q = lf.h.y;;) {
+                const double4 m = gload4(AV + q);
+                const double v0 = mat_value(D, w.x, m.x, c, capg, icz);
+                const double v1 = mat_value(D, w.y, m.y, c, capg, icz);
+                const double v2 = mat_value(D, w.z, m.z, c, capg, icz);
+                const double v3 = mat_value(D, w.w, m.w, c, capg, icz);
                 acc += v0;
                 acc += v1;
                 acc += v2;
                 acc += v3;
-                if (q >= lf.h.z) break;
+                if (++q >= lf.h.z) break;
                 w = gload(AC + q);
             }
             if (lf.h.x < D.nslots) {
@@ -697,16 +712,17 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
             if (e + nt < D.n_rleaf) nx = load_rec(D.r_leaf, e + nt);
             double acc = 0.0;
             uint4 w = lf.g;
-            for (int q = lf.h.y;; q++) {
-                const double v0 = rhs_value(D, w.x, c, cnm, capq, vsv);
-                const double v1 = rhs_value(D, w.y, c, cnm, capq, vsv);
-                const double v2 = rhs_value(D, w.z, c, cnm, capq, vsv);
-                const double v3 = rhs_value(D, w.w, c, cnm, capq, vsv);
+            for (int q = lf.h.y;;) {
+                const double4 m = gload4(RV + q);
+                const double v0 = rhs_value(w.x, m.x, capq, vsv);
+                const double v1 = rhs_value(w.y, m.y, capq, vsv);
+                const double v2 = rhs_value(w.z, m.z, capq, vsv);
+                const double v3 = rhs_value(w.w, m.w, capq, vsv);
                 acc += v0;
                 acc += v1;
                 acc += v2;
                 acc += v3;
-                if (q >= lf.h.z) break;
+                if (++q >= lf.h.z) break;
                 w = gload(RC + q);
             }
             if (lf.h.x < N) z[lf.h.x] = acc;

@@ -76,12 +76,16 @@ struct TranDev {
     double *vs_brk;           /* [C][NVS] break_time */
     const double *ic_val;     /* [C][NN+1] */
     /* LU program */
-    /* assembly: a leaf is a 32-byte record {dest, begin, end, 0} + its
-     * first group of four contribution codes; [begin, end) are its further
-     * groups (uint4 units of a_code, padded with zero codes).  It sums into
-     * L+U slot dest (< nslots) or partial dest - nslots; combines (slot,
+    /* assembly: a leaf is a 32-byte record {dest, begin, end, 0} + a copy
+     * of its first group of four contribution codes; [begin, end) are all
+     * its groups (uint4 units of a_code, padded with zero codes).  It sums
+     * into L+U slot dest (< nslots) or partial dest - nslots; combines (slot,
      * pbegin, pend) add partials into a slot; likewise for the RHS rows
-     * (dest < N a row, else partial n_apart + dest - N) */
+     * (dest < N a row, else partial n_apart + dest - N).  MOSFET values come
+     * from the circuit's contribution arrays, which hold, at the position of
+     * each MOSFET code in a_code / r_code, the value the BSIM4 kernel wrote
+     * there (mos_pos / rmos_pos: position of stamp s / RHS row r of MOSFET j
+     * at [s * NM + j], -1 where the row or column is ground). */
     const int4 *a_leaf, *a_comb, *r_leaf, *r_comb;
     int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
     const uint32_t *a_code;   /* contributions */
@@ -110,8 +114,11 @@ struct TranDev {
     double *mstate;           /* [NLEV][29][C*NM] */
     double *cstate;           /* [NLEV][2][C*NCAP] */
     double *cache;            /* [38][C*NM] */
-    double *stamp;            /* [16][C*NM] */
-    double *mrhs;             /* [4][C*NM] */
+    int NA, NR;               /* a_code / r_code entries (contribution array lengths) */
+    const int *mos_pos;       /* [16 * NM] */
+    const int *rmos_pos;      /* [4 * NM] */
+    double *contrib;          /* [C][NA] */
+    double *rcontrib;         /* [C][NR] */
     int *noncon;              /* [C*NM] */
     double *out_t;            /* [C][cap_pts] */
     double *out_x;            /* [C][cap_pts][nprobe] */
