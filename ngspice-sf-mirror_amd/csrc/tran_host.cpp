@@ -745,8 +745,11 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.out_x = dp;
     D.done_count = b->done_count;
     D.counters = b->dcounters;
+    /* LDS: lu, z, capacitor and source values, partial sums (doubles), then
+     * the three level tables (ints) */
     G->lds = sizeof(double) * ((size_t)P.nslots + 1 + N + 1 + 2 * (size_t)NCAP + NVS + P.n_apart +
-                              P.n_rpart);
+                              P.n_rpart) +
+             sizeof(int) * ((size_t)G->nlev_f + G->nlev_fw + G->nlev_bw + 3);
     if (G->lds > 150 * 1024) return fail("circuit too large for the per-circuit LDS factorisation");
     G->method = c0.method;
     G->maxord = c0.maxord;

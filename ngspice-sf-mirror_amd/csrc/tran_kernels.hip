@@ -521,17 +521,17 @@ __device__ __forceinline__ double lu_row_terms(double acc, const Rec &r, const u
 template <class F>
 __device__ __forceinline__ void run_levels(const int *lev, const int4 *ent, int nlev, int tid, int nt,
                                            F body)
-{
+{   /* lev: the level table staged in LDS */
     if (nlev <= 0) return;
-    int e0 = gload(lev) + tid, e1 = gload(lev + 1);
+    int e0 = lev[0] + tid, e1 = lev[1];
     Rec nx;
     if (e0 < e1) nx = load_rec(ent, e0);
     for (int L = 0; L < nlev; L++) {
         const int ce0 = e0, ce1 = e1;
         const Rec cur = nx;
         if (L + 1 < nlev) {
-            e0 = gload(lev + L + 1) + tid;
-            e1 = gload(lev + L + 2);
+            e0 = lev[L + 1] + tid;
+            e1 = lev[L + 2];
             if (e0 < e1) nx = load_rec(ent, e0);
         }
         for (int e = ce0; e < ce1; e += nt) body(e == ce0 ? cur : load_rec(ent, e));
@@ -668,6 +668,14 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
      * not kept; the order here is fixed, so results repeat bit for bit.) */
     const bool icz = (mode & M_TRANOP) && !(mode & M_UIC);
     double *part = vsv + D.NVS;
+    /* the level tables, read at every level, staged in LDS (visible after
+     * the barrier that ends the assembly) */
+    int *levf = reinterpret_cast<int *>(part + D.n_apart + D.n_rpart);
+    int *levw = levf + D.nlev_f + 1;
+    int *levb = levw + D.nlev_fw + 1;
+    for (int k = tid; k <= D.nlev_f; k += nt) levf[k] = gload(D.f_lev + k);
+    for (int k = tid; k <= D.nlev_fw; k += nt) levw[k] = gload(D.fw_lev + k);
+    for (int k = tid; k <= D.nlev_bw; k += nt) levb[k] = gload(D.bw_lev + k);
     const uint4 *AC = reinterpret_cast<const uint4 *>(D.a_code);
     const uint4 *RC = reinterpret_cast<const uint4 *>(D.r_code);
     /* the circuit's contribution values, four per group: consecutive leaves
@@ -756,18 +764,18 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     /* LU factorisation, entry-wise Doolittle by dependency levels, then the
      * two substitutions; see run_levels for the load schedule */
     const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
-    run_levels(D.f_lev, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
+    run_levels(levf, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
                [&](const Rec &r) {
                    const double acc = lu_row_terms(lu[r.h.x], r, FT, lu, lu);
                    lu[r.h.x] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
                });
     /* forward substitution (unit L) */
     const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
-    run_levels(D.fw_lev, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
+    run_levels(levw, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
                [&](const Rec &r) { z[r.h.x] = lu_row_terms(z[r.h.x], r, WT, lu, z); });
     /* back substitution */
     const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
-    run_levels(D.bw_lev, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
+    run_levels(levb, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
                [&](const Rec &r) {
                    const double acc = lu_row_terms(z[r.h.x], r, BT, lu, z);
                    const double piv = lu[r.h.y];
