@@ -262,6 +262,34 @@ def bench_mc(args):
     log('passes %d, accepted %d, rejected %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
         'step %.1f ms, wall %.1f ms' % (passes, acc, rej, its, failed, mos_ms, step_ms,
                                         wall * 1e3))
+    single = None
+    if args.streams is None and not args.no_single_stream:
+        # the default overlaps two halves of the batch on two streams, so each
+        # kernel's launch duration includes time shared with the other
+        # half's kernel; one more timed run on one stream gives the BSIM4
+        # kernel's own roofline
+        ngmi.tune('streams', 1)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        rc1, passes1 = b.tran(workloads.ADDER_TSTEP, tstop, stream=stream)
+        torch.cuda.synchronize()
+        wall1 = time.perf_counter() - w0
+        ngmi.tune('streams', 2)
+        if rc1 != 0:
+            raise RuntimeError('single-stream transient failed: %d %s' % (rc1, ngmi.last_error()))
+        acc1 = sum(b.stats(c)['accepted'] for c in range(C))
+        mos1, step1, evals1, _ = b.kernel_times()
+        ach1 = evals1 * b_eval / (mos1 * 1e-3) / 1e9 if mos1 > 0 else None
+        single = {'value': acc1 / wall1, 'ms': wall1 * 1e3, 'bsim4_kernel_ms_total': mos1,
+                  'step_kernel_ms_total': step1,
+                  'roofline': {'bound': 'hbm', 'achieved': ach1, 'peak': HBM_PEAK_GBS,
+                               'unit': 'GB/s', 'frac': ach1 / HBM_PEAK_GBS if ach1 else None,
+                               'traffic': mb * evals1 / passes1 if (mb and passes1) else None,
+                               'kernel': 'k_tran_mos', 'launches': passes1,
+                               'fp64_tflops': (MEASURED_FLOP_PER_EVAL['k_tran_mos'] * evals1 /
+                                               (mos1 * 1e-3) / 1e12) if mos1 > 0 else None}}
+        log('one stream: %.0f tp*c/s, wall %.1f ms, bsim4 %.1f ms (%.0f GB/s), step %.1f ms'
+            % (single['value'], wall1 * 1e3, mos1, ach1 or 0, step1))
     learned = None
     if not args.no_guess:
         learned = learned_guess_run(args, b, probe, stream, acc, its)
@@ -283,7 +311,8 @@ def bench_mc(args):
         'extra': {'accepted': acc, 'rejected': rej, 'newton_iterations': its,
                   'iterations_per_point': its / max(acc, 1), 'passes': passes,
                   'failed_circuits': failed, 'step_kernel_ms_total': step_ms,
-                  'bsim4_kernel_ms_total': mos_ms, 'learned_guess': learned},
+                  'bsim4_kernel_ms_total': mos_ms, 'streams': args.streams or 2,
+                  'one_stream': single, 'learned_guess': learned},
     }
 
 
@@ -467,6 +496,8 @@ def main():
     ap.add_argument('--cpu-procs', type=int, default=min(16, os.cpu_count() or 1),
                     help='host processes for the CPU baseline (one per core)')
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess run')
+    ap.add_argument('--no-single-stream', action='store_true',
+                    help='skip the extra one-stream run of the default two-stream mode')
     ap.add_argument('--eval-waves', type=int, default=None, help='BSIM4 eval kernel occupancy')
     ap.add_argument('--mos-waves', type=int, default=None, help='transient BSIM4 kernel occupancy')
     ap.add_argument('--specialise', type=int, default=None,

@@ -177,6 +177,15 @@ int ngmi_circuit_set_ic(ngmi_circuit *c, int node, double v);
 /* .options reltol abstol vntol chgtol trtol gmin method(1 trap, 2 gear) maxord */
 int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, double vntol,
                              double chgtol, double trtol, double gmin, int method, int maxord);
+/* .options noopiter gminsteps srcsteps gminfactor gshunt itl2 for the
+ * transient's operating point, CKTop (cktop.c:26-116): direct Newton unless
+ * noopiter, then dynamic_gmin and new_gmin (gminsteps = 1,
+ * cktop.c:161-478), then gillespie_src (srcsteps = 1, cktop.c:480-670);
+ * defaults 0, 1, 1, 10, 0, 50 (cktntask.c:94-132).  gminsteps or srcsteps
+ * above 1 (the spice3 variants) return NGMI_E_UNSUPP.  A batch uses the
+ * options of each topology group's first circuit. */
+int ngmi_circuit_set_op_options(ngmi_circuit *c, int noopiter, int gminsteps, int srcsteps,
+                                double gminfactor, double gshunt, int itl2);
 /* Pivot sequence for the static LU: step k eliminates row rows[k] and
  * column cols[k] (1-based MNA unknowns, n = nodes + voltage sources).  This
  * is where KLU's own ordering enters: after SMPpreOrder's klu_analyze

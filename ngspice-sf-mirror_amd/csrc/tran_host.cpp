@@ -73,6 +73,9 @@ struct ngmi_circuit {
     std::vector<char> ic_given;
     double reltol = 1e-3, abstol = 1e-12, vntol = 1e-6, chgtol = 1e-14, trtol = 7.0, gmin = 1e-12;
     int method = 1, maxord = 2;
+    /* CKTop options (cktntask.c:94-132 defaults) */
+    int noopiter = 0, gminsteps = 1, srcsteps = 1, itl2 = 50;
+    double gminfactor = 10.0, gshunt = 0.0;
     /* optional pivot sequence (1-based unknowns); empty = the engine's own */
     std::vector<int> order_rows, order_cols;
 };
@@ -90,7 +93,7 @@ struct TopoGroup {
     int nslots = 0, nterms = 0, nlev_f = 0, nlev_fw = 0, nlev_bw = 0;
     std::vector<char> ic_nodes_host;
     std::vector<int> nbr_host;
-    int method = 1, maxord = 2;
+    int method = 1, maxord = 2, noopiter = 0;
     ~TopoGroup()
     {
         if (set) ngmi_bsim4_set_free(set);
@@ -230,7 +233,7 @@ struct Plan {
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
     std::vector<int> slot_flags, perm, ic_row, nbr;
-    std::vector<int> mos_pos, rmos_pos;
+    std::vector<int> mos_pos, rmos_pos, diag_slot;
     int NA = 0, NR = 0;
 };
 
@@ -497,6 +500,13 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         const int i = rinv[u], j = cinv[u];
         slot_flags[slot[(size_t)i * N + j]] |= 1;
     }
+    /* the diagonal elements the matrix has (device SMPmakeElt calls and the
+     * .ic rows, CKTic): LoadGmin's targets */
+    P.diag_slot.assign(N + 1, -1);
+    for (const Elt &e : elts)
+        if (e.r == e.c) P.diag_slot[e.r] = slot[(size_t)rinv[e.r] * N + cinv[e.c]];
+    for (int u = 1; u <= NN; u++)
+        if (P.ic_nodes[u]) P.diag_slot[u] = slot[(size_t)rinv[u] * N + cinv[u]];
     /* a slot or row with more than a few contributions is split into
      * leaves that write partial sums, plus one combine step (the supply
      * node's diagonal collects one contribution per connected device) */
@@ -679,6 +689,9 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.cap_pts = b->cap_pts;
     D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
     D.trtol = c0.trtol; D.gmin = c0.gmin;
+    D.numGminSteps = c0.gminsteps; D.numSrcSteps = c0.srcsteps; D.dcTrcvMaxIter = c0.itl2;
+    D.gminFactor = c0.gminfactor; D.gshunt = c0.gshunt;
+    G->noopiter = c0.noopiter;
     int *ip;
     double *dp;
     uint32_t *up;
@@ -713,6 +726,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     UP(P.bw_term, bw_term, uint32_t, up);
     UP(P.perm, perm_unknown, int, ip);
     UP(P.mos_pos, mos_pos, int, ip);
+    UP(P.diag_slot, diag_slot, int, ip);
     UP(P.rmos_pos, rmos_pos, int, ip);
     UP(G->probe, probe, int, ip);
     UP(G->nbr_host, nbr, int, ip);
@@ -733,6 +747,11 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.cache = dp;
     D.NA = P.NA;
     D.NR = P.NR;
+    if (alloc(b, (size_t)C * (N + 1), &dp) != NGMI_OK) return fail(nullptr);
+    D.bak_x = dp;
+    if (alloc(b, (size_t)C * (B4X__COUNT * (size_t)NM + 2 * (size_t)NCAP), &dp) != NGMI_OK)
+        return fail(nullptr);
+    D.bak_s = dp;
     if (alloc(b, (size_t)C * P.NA, &dp) != NGMI_OK) return fail(nullptr);
     D.contrib = dp;
     if (alloc(b, (size_t)C * P.NR, &dp) != NGMI_OK) return fail(nullptr);
@@ -847,6 +866,31 @@ int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, doub
     maxord = std::min(std::max(maxord, 1), 6);
     c->reltol = reltol; c->abstol = abstol; c->vntol = vntol; c->chgtol = chgtol;
     c->trtol = trtol; c->gmin = gmin; c->method = method; c->maxord = maxord;
+    return NGMI_OK;
+}
+
+int ngmi_circuit_set_op_options(ngmi_circuit *c, int noopiter, int gminsteps, int srcsteps,
+                                double gminfactor, double gshunt, int itl2)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (gminsteps < 0 || srcsteps < 0 || itl2 < 1 || !(gminfactor > 1.0) || gshunt < 0) {
+        set_error("operating-point option out of range");
+        return NGMI_E_PARMVAL;
+    }
+    if (gminsteps > 1 || srcsteps > 1) {
+        set_error("gminsteps / srcsteps above 1 select the spice3 stepping, which is not built");
+        return NGMI_E_UNSUPP;
+    }
+    if (noopiter && !gminsteps && !srcsteps) {
+        set_error("noopiter needs gmin or source stepping");
+        return NGMI_E_PARMVAL;
+    }
+    c->noopiter = noopiter;
+    c->gminsteps = gminsteps;
+    c->srcsteps = srcsteps;
+    c->gminfactor = gminfactor;
+    c->gshunt = gshunt;
+    c->itl2 = itl2;
     return NGMI_OK;
 }
 
@@ -1030,6 +1074,26 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
             k.phase = PH_OP;
             for (int i = 0; i < 7; i++) k.deltaOld[i] = maxStep;
             k.learned = b->net ? 1 : 0;
+            /* CKTop: direct Newton first, or straight to the stepping when
+             * noopiter is set (cktop.c:39-52) */
+            k.diagGmin = 0.0;
+            k.gminCur = D.gmin;
+            k.srcFact = 1.0;
+            k.opStage = OP_DIRECT;
+            if (G->noopiter) {
+                if (D.numGminSteps >= 1) {
+                    k.opStage = OP_DYN;
+                    k.opFactor = D.gminFactor;
+                    k.opOldGmin = 1e-2;
+                    k.diagGmin = k.opOldGmin / k.opFactor;
+                    k.opGtarget = std::max(k.gminCur, D.gshunt);
+                } else {
+                    k.opStage = OP_SRC_ZERO;
+                    k.srcFact = 0.0;
+                    k.opConvFact = 0.0;
+                    k.opStartGmin = k.gminCur;
+                }
+            }
         }
         NGMI_HIP(hipMemcpyAsync(D.ctl, ctl.data(), sizeof(CktCtl) * G->C, hipMemcpyHostToDevice, s));
         const int64_t cnm = (int64_t)G->C * G->NM, cnc = (int64_t)G->C * G->NCAP;

@@ -80,7 +80,7 @@ __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D
 #pragma unroll
     for (int k = 0; k < 7; k++) ck.ag[k] = K->ag[k];
     ck.xfact = K->delta / K->deltaOld[1];
-    ck.gmin = D.gmin;
+    ck.gmin = K->gminCur;
     ck.reltol = D.reltol;
     ck.abstol = D.abstol;
     const double *xo = D.x + ((int64_t)K->xsel * D.C + c) * (D.N + 1);
@@ -323,7 +323,231 @@ __device__ double terr_one(double q0, double q1, double q2, double q3, double c0
 
 /* ---------------- per-circuit step kernel ---------------- */
 
-enum : int { ACT_NONE = 0, ACT_ACCEPT = 1, ACT_LTE1 = 2, ACT_RETRY = 3, ACT_STOP = 4 };
+enum : int { ACT_NONE = 0, ACT_ACCEPT = 1, ACT_LTE1 = 2, ACT_RETRY = 3, ACT_STOP = 4,
+             ACT_OPMEM = 100 };
+
+/* ---------------- CKTop stepping (cktop.c:26-670) ---------------- */
+
+/* results of op_next: the operating point converged, failed (status set), or
+ * a stage continues after the workgroup does nothing / zeroes / saves /
+ * restores CKTrhsOld and CKTstate0 */
+enum : int { OPA_OK = 0, OPA_FAIL = 1, OPA_NONE = 2, OPA_ZERO = 3, OPA_SAVE = 4, OPA_RESTORE = 5 };
+
+constexpr int OP_FIRSTMODE = M_TRANOP | M_INITJCT;
+constexpr int OP_CONTINUEMODE = M_TRANOP | M_INITFLOAT;
+
+/* start of one NIiter call of a stage */
+__device__ __forceinline__ void op_niiter(CktCtl *K, int stage)
+{
+    K->opStage = stage;
+    K->iterno = 0;
+    K->opIter0 = K->iters;
+}
+
+__device__ int op_src_init(const TranDev &D, CktCtl *K)
+{   /* gillespie_src start (cktop.c:480-503) */
+    K->mode = (K->mode & M_UIC) | OP_FIRSTMODE;
+    K->srcFact = 0;
+    K->opConvFact = 0;
+    K->opStartGmin = K->gminCur;
+    op_niiter(K, OP_SRC_ZERO);
+    return OPA_ZERO;
+}
+
+__device__ int op_src_ramp_init(CktCtl *K)
+{   /* the source ramp's start (cktop.c:559-575) */
+    K->opRaise = 0.001;
+    K->srcFact = K->opConvFact + K->opRaise;
+    op_niiter(K, OP_SRC_STEP);
+    return OPA_SAVE;
+}
+
+__device__ int op_src_end(CktCtl *K, bool ramped)
+{   /* gillespie_src end (cktop.c:645-668) */
+    if (ramped) K->diagGmin = K->gminCur = K->opStartGmin;
+    K->srcFact = 1;
+    if (K->opConvFact != 1) {
+        K->status = 103;
+        return OPA_FAIL;
+    }
+    return OPA_OK;
+}
+
+/* the next step of CKTop after an NIiter call of the current stage returned
+ * rc, in the order of cktop.c; thread 0 */
+__device__ int op_next(const TranDev &D, CktCtl *K, int rc)
+{
+    const int iters = (int)(K->iters - K->opIter0);
+    const int uic = K->mode & M_UIC;
+    const int lo = D.dcTrcvMaxIter / 4, hi = 3 * D.dcTrcvMaxIter / 4;
+    switch (K->opStage) {
+    case OP_DIRECT:
+        if (rc == 0) return OPA_OK;
+        if (D.numGminSteps >= 1) {
+            /* dynamic_gmin (cktop.c:161-196) */
+            K->mode = uic | OP_FIRSTMODE;
+            K->opFactor = D.gminFactor;
+            K->opOldGmin = 1e-2;
+            K->diagGmin = K->opOldGmin / K->opFactor;
+            K->opGtarget = fmax(K->gminCur, D.gshunt);
+            op_niiter(K, OP_DYN);
+            return OPA_ZERO;
+        }
+        if (D.numSrcSteps >= 1) return op_src_init(D, K);
+        K->status = rc;
+        return OPA_FAIL;
+    case OP_DYN:
+    case OP_NEW: {
+        /* the loops of dynamic_gmin (cktop.c:197-270) and new_gmin
+         * (cktop.c:388-458): the former steps CKTdiagGmin, the latter the
+         * devices' CKTgmin */
+        const bool dyn = K->opStage == OP_DYN;
+        double g = dyn ? K->diagGmin : K->gminCur;
+        int mem = OPA_NONE;
+        bool final_ = false;
+        if (rc == 0) {
+            K->mode = uic | OP_CONTINUEMODE;
+            if (g <= K->opGtarget) {
+                final_ = true;
+            } else {
+                mem = OPA_SAVE;
+                double f = K->opFactor;
+                if (iters <= lo) {
+                    f *= sqrt(f);
+                    if (f > D.gminFactor) f = D.gminFactor;
+                }
+                if (iters > hi) f = fmax(sqrt(f), dyn ? 1.00005 : 3.0);
+                K->opOldGmin = g;
+                if (g < f * K->opGtarget) {
+                    f = g / K->opGtarget;
+                    g = K->opGtarget;
+                } else {
+                    g /= f;
+                }
+                K->opFactor = f;
+            }
+        } else {
+            if (K->opFactor < 1.00005) {
+                final_ = true;
+            } else {
+                K->opFactor = sqrt(sqrt(K->opFactor));
+                g = K->opOldGmin / K->opFactor;
+                mem = OPA_RESTORE;
+            }
+        }
+        if (final_) {
+            if (dyn) {
+                K->diagGmin = D.gshunt;
+                op_niiter(K, OP_DYN_FINAL);
+            } else {
+                K->gminCur = fmax(K->opStartGmin, D.gshunt);
+                op_niiter(K, OP_NEW_FINAL);
+            }
+            return OPA_NONE;
+        }
+        if (dyn) K->diagGmin = g;
+        else K->gminCur = g;
+        op_niiter(K, K->opStage);
+        return mem;
+    }
+    case OP_DYN_FINAL:
+        if (rc == 0) return OPA_OK;
+        /* new_gmin (cktop.c:349-386) */
+        K->mode = uic | OP_FIRSTMODE;
+        K->opStartGmin = K->gminCur;
+        K->opFactor = D.gminFactor;
+        K->opOldGmin = 1e-2;
+        K->gminCur = K->opOldGmin / K->opFactor;
+        K->opGtarget = fmax(K->opStartGmin, D.gshunt);
+        op_niiter(K, OP_NEW);
+        return OPA_ZERO;
+    case OP_NEW_FINAL:
+        if (rc == 0) return OPA_OK;
+        if (D.numSrcSteps >= 1) return op_src_init(D, K);
+        K->status = rc;
+        return OPA_FAIL;
+    case OP_SRC_ZERO:
+        if (rc == 0) return op_src_ramp_init(K);
+        /* gmin ladder for the zero-source solution (cktop.c:517-550) */
+        K->diagGmin = (D.gshunt <= 0) ? K->gminCur : D.gshunt;
+        for (int i = 0; i < 10; i++) K->diagGmin *= 10;
+        K->opSub = 0;
+        op_niiter(K, OP_SRC_GMIN);
+        return OPA_NONE;
+    case OP_SRC_GMIN:
+        if (rc != 0) {
+            K->diagGmin = D.gshunt;
+            return op_src_end(K, false);
+        }
+        K->diagGmin /= 10;
+        K->mode = uic | OP_CONTINUEMODE;
+        if (++K->opSub <= 10) {
+            op_niiter(K, OP_SRC_GMIN);
+            return OPA_NONE;
+        }
+        K->diagGmin = D.gshunt;
+        return op_src_ramp_init(K);
+    case OP_SRC_STEP: {
+        /* the source ramp (cktop.c:577-640) */
+        int mem = OPA_NONE;
+        K->mode = uic | OP_CONTINUEMODE;
+        if (rc == 0) {
+            K->opConvFact = K->srcFact;
+            mem = OPA_SAVE;
+            K->srcFact = K->opConvFact + K->opRaise;
+            if (iters <= lo) K->opRaise *= 1.5;
+            if (iters > hi) K->opRaise *= 0.5;
+        } else {
+            if (K->srcFact - K->opConvFact < 1e-8) return op_src_end(K, true);
+            K->opRaise /= 10;
+            if (K->opRaise > 0.01) K->opRaise = 0.01;
+            K->srcFact = K->opConvFact;
+            mem = OPA_RESTORE;
+        }
+        if (K->srcFact > 1) K->srcFact = 1;
+        /* the loop ends: the saved copy is no longer needed */
+        if (!((K->opRaise >= 1e-7) && (K->opConvFact < 1))) return op_src_end(K, true);
+        op_niiter(K, OP_SRC_STEP);
+        return mem;
+    }
+    default:
+        K->status = 1;
+        return OPA_FAIL;
+    }
+}
+
+/* the workgroup's part of a stepping transition: zero, save or restore
+ * CKTrhsOld and CKTstate0 of circuit c */
+__device__ void op_memory(const TranDev &D, int c, const CktCtl *K, int what)
+{
+    if (what == OPA_NONE) return;
+    const int tid = threadIdx.x, nt = blockDim.x, N = D.N;
+    const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
+    double *x = D.x + ((int64_t)K->xsel * D.C + c) * (N + 1);
+    double *bx = D.bak_x + (int64_t)c * (N + 1);
+    double *bs = D.bak_s + (int64_t)c * (B4X__COUNT * D.NM + 2 * D.NCAP);
+    const int l0 = lvl(K->rb, 0);
+    for (int u = tid; u <= N; u += nt) {
+        if (what == OPA_ZERO) x[u] = 0.0;
+        else if (what == OPA_SAVE) bx[u] = x[u];
+        else x[u] = bx[u];
+    }
+    for (int q = tid; q < B4X__COUNT * D.NM; q += nt) {
+        const int k = q / D.NM, j = q - k * D.NM;
+        double *p = D.mstate + ((int64_t)l0 * B4X__COUNT + k) * cnm + (int64_t)c * D.NM + j;
+        if (what == OPA_ZERO) *p = 0.0;
+        else if (what == OPA_SAVE) bs[q] = *p;
+        else *p = bs[q];
+    }
+    for (int q = tid; q < 2 * D.NCAP; q += nt) {
+        const int k = q / D.NCAP, j = q - k * D.NCAP;
+        double *p = D.cstate + ((int64_t)l0 * 2 + k) * cnc + (int64_t)c * D.NCAP + j;
+        double *b = bs + B4X__COUNT * D.NM + q;
+        if (what == OPA_ZERO) *p = 0.0;
+        else if (what == OPA_SAVE) *b = *p;
+        else *p = *b;
+    }
+}
 
 /* block-wide minimum of one value per thread */
 __device__ double block_min(double v, double *red)
@@ -657,7 +881,12 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         capq[k] = ceq;
     }
     for (int k = tid; k < D.NVS; k += nt)
-        vsv[k] = vsrc_value(D.vs_par + ((int64_t)c * D.NVS + k) * VSP, mode, tnow, K->step);
+    {
+        /* vsrcload.c:453-457: the operating point scales sources by
+         * CKTsrcFact */
+        const double v = vsrc_value(D.vs_par + ((int64_t)c * D.NVS + k) * VSP, mode, tnow, K->step);
+        vsv[k] = (mode & M_TRANOP) ? v * K->srcFact : v;
+    }
     __syncthreads();
 
     /* assembly into the permuted L+U slots and RHS rows.  Each leaf sums at
@@ -759,6 +988,16 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
             z[a] = gload(D.ic_val + (int64_t)c * (D.NN + 1) + u);
         }
     }
+    const double dgmin = K->diagGmin;
+    if (dgmin != 0.0) {
+        /* LoadGmin_CSC (klusmp.c:1765-1777): CKTdiagGmin onto every diagonal
+         * element the matrix has, after the load */
+        __syncthreads();
+        for (int u = tid + 1; u <= N; u += nt) {
+            const int sl = gload(D.diag_slot + u);
+            if (sl >= 0) lu[sl] += dgmin;
+        }
+    }
     __syncthreads();
 
     /* LU factorisation, entry-wise Doolittle by dependency levels, then the
@@ -824,11 +1063,17 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         if (rc == -1) K->xsel ^= 1;   /* SWAP(rhs, rhsOld) */
         s_act = ACT_NONE;
         if (rc >= 0) {
-            if (K->phase == PH_OP) {
-                if (rc != 0) {
-                    fail(D, K, rc);
+            int opa = OPA_OK;
+            if (K->phase == PH_OP) opa = op_next(D, K, rc);
+            if (K->phase == PH_OP && opa != OPA_OK) {
+                if (opa == OPA_FAIL) {
+                    fail(D, K, K->status);
                     s_act = ACT_STOP;
                 } else {
+                    s_act = ACT_OPMEM + opa;   /* a stepping stage continues */
+                }
+            } else if (K->phase == PH_OP) {
+                {
                     /* dctran.c:268-330: into the transient */
                     K->phase = PH_TRAN;
                     K->order = 1;
@@ -860,6 +1105,10 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     __syncthreads();
     int act = s_act;
     if (act == ACT_NONE || act == ACT_STOP) return;
+    if (act >= ACT_OPMEM) {
+        op_memory(D, c, K, act - ACT_OPMEM);
+        return;
+    }
     if (act == 10) {
         copy_level(D, c, K->rb, 0, 1);
         __syncthreads();

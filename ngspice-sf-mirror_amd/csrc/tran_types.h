@@ -43,6 +43,20 @@ struct CktCtl {
     int learned, guess, nhist, hpos;
     double ht[4];
     int64_t guessed, fallbacks;
+    /* CKTop's stepping (cktop.c:161-670): CKTdiagGmin, the devices' CKTgmin,
+     * CKTsrcFact, and the scalars of the stage running */
+    double diagGmin, gminCur, srcFact;
+    double opFactor, opOldGmin, opGtarget, opRaise, opConvFact, opStartGmin;
+    int opStage, opSub;
+    int64_t opIter0;
+};
+
+/* CKTop stages: direct Newton, dynamic_gmin and its final solve, new_gmin
+ * and its final solve, gillespie_src's zero-source solve, its gmin ladder and
+ * its source ramp */
+enum : int {
+    OP_DIRECT = 0, OP_DYN = 1, OP_DYN_FINAL = 2, OP_NEW = 3, OP_NEW_FINAL = 4,
+    OP_SRC_ZERO = 5, OP_SRC_GMIN = 6, OP_SRC_STEP = 7
 };
 
 /* contribution codes of the assembly plans */
@@ -63,6 +77,12 @@ struct TranDev {
     int nprobe;
     int64_t cap_pts;  /* waveform capacity per circuit */
     double reltol, abstol, vntol, chgtol, trtol, gmin;
+    /* CKTop options: gminsteps, srcsteps (0 or 1), itl2, gminfactor, gshunt */
+    int numGminSteps, numSrcSteps, dcTrcvMaxIter;
+    double gminFactor, gshunt;
+    const int *diag_slot;     /* [N+1] L+U slot of A(u, u), -1 where A has no such element */
+    double *bak_x;            /* [C][N+1] CKTop's saved CKTrhsOld */
+    double *bak_s;            /* [C][29 NM + 2 NCAP] CKTop's saved CKTstate0 */
     /* topology (shared) */
     const int *mos_nodes;     /* [4][NM] d g s b */
     const int *cap_nodes;     /* [2][NCAP] */

@@ -287,6 +287,7 @@ def _tran_protos(L):
     L.ngmi_circuit_add_vsrc.argtypes = [vp, i, i, d, ctypes.POINTER(d), i]
     L.ngmi_circuit_set_ic.argtypes = [vp, i, d]
     L.ngmi_circuit_set_options.argtypes = [vp, d, d, d, d, d, d, i, i]
+    L.ngmi_circuit_set_op_options.argtypes = [vp, i, i, i, d, d, i]
     ip = ctypes.POINTER(i)
     L.ngmi_circuit_set_pivot_order.argtypes = [vp, i, ip, ip]
     L.ngmi_circuit_analyze.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
@@ -392,6 +393,11 @@ class Circuit:
         check(L.ngmi_circuit_set_options(self.h, o['reltol'], o['abstol'], o['vntol'],
                                          o['chgtol'], o['trtol'], o['gmin'], o['method'],
                                          o['maxord']), 'options')
+        op = dict(noopiter=0, gminsteps=1, srcsteps=1, gminfactor=10.0, gshunt=0.0, itl2=50)
+        op.update({k: v for k, v in o.items() if k in op})
+        check(L.ngmi_circuit_set_op_options(self.h, op['noopiter'], op['gminsteps'],
+                                            op['srcsteps'], op['gminfactor'], op['gshunt'],
+                                            op['itl2']), 'operating-point options')
         mos = desc['mos']
         if mos:
             keys = list(self.models)

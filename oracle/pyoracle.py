@@ -185,6 +185,7 @@ def _tran_protos():
     L.or_tran_free.argtypes = [vp]
     L.or_tran_use_klu.argtypes = [vp, ctypes.c_char_p]
     L.or_comcof.argtypes = [i, i, d, dp, dp]
+    L.or_tran_set_op_options.argtypes = [vp, i, i, i, d, d, i]
     ip = ctypes.POINTER(ctypes.c_int)
     L.or_tran_klu_order.argtypes = [vp, ip, ip, ip]
     L._tran_ready = True
@@ -211,7 +212,8 @@ class Transient:
     circuit: dict with nnodes, models {key: params}, mos [(key, d, g, s, b,
     {inst params})], caps [(p, n, C)], res [(p, n, R)], vsrc [(p, n, dc,
     pulse tuple or None)], ic {node: v}, temp (K), options {reltol, abstol,
-    vntol, chgtol, trtol, gmin, method, maxord, uic}."""
+    vntol, chgtol, trtol, gmin, method, maxord, uic, noopiter, gminsteps,
+    srcsteps, gminfactor, gshunt, itl2}."""
 
     def __init__(self, circuit):
         import numpy as np
@@ -227,6 +229,11 @@ class Transient:
         o.update(c.get('options', {}))
         L.or_tran_set_options(self.h, o['reltol'], o['abstol'], o['vntol'], o['chgtol'],
                               o['trtol'], o['gmin'], o['method'], o['maxord'], o['uic'])
+        op = dict(noopiter=0, gminsteps=1, srcsteps=1, gminfactor=10.0, gshunt=0.0, itl2=50)
+        op.update({k: v for k, v in o.items() if k in op})
+        if L.or_tran_set_op_options(self.h, op['noopiter'], op['gminsteps'], op['srcsteps'],
+                                    op['gminfactor'], op['gshunt'], op['itl2']) != 0:
+            raise ValueError('operating-point stepping options not restated')
         self.inst = []
         for key, dn, gn, sn, bn, ip in c['mos']:
             inst = Instance(self.models[key], ip)

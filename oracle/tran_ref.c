@@ -63,6 +63,13 @@ typedef struct {
     /* options */
     double reltol, abstol, vntol, chgtol, trtol, gmin, temp;
     int maxOrder, method, uic;
+    /* operating point: noopiter, gminsteps, srcsteps, gminfactor, gshunt,
+     * itl2 (cktntask.c:94-132 defaults 0, 1, 1, 10, 0, 50) */
+    int noOpIter, numGminSteps, numSrcSteps, dcTrcvMaxIter;
+    double gminFactor, gshunt;
+    /* CKTdiagGmin, CKTsrcFact; hasdiag[u]: A(u, u) exists */
+    double diagGmin, srcFact;
+    char *hasdiag;
     /* state */
     int nstates;
     double *stbuf;            /* OR_NSTATE x nstates */
@@ -283,7 +290,11 @@ static int ckt_load(ORTran *T)
         AX(v->n, v->br) -= 1.0;
         AX(v->br, v->p) += 1.0;
         AX(v->br, v->n) -= 1.0;
-        T->rhs[v->br] += vsrc_value(T, v, mode);
+        {   /* vsrcload.c:453-457: TRANOP scales by CKTsrcFact */
+            double value = vsrc_value(T, v, mode);
+            if (mode & OR_MODETRANOP) value *= T->srcFact;
+            T->rhs[v->br] += value;
+        }
     }
     /* .ic rows in the transient operating point (cktload.c:146-173) */
     if ((mode & (OR_MODEDCOP | OR_MODETRANOP | OR_MODEDCTRANCURVE)) &&
@@ -333,6 +344,13 @@ static int solve_klu(ORTran *T)
 /* dense LU with partial pivoting on rows/cols 1..n; solution into rhs */
 static int solve(ORTran *T)
 {
+    /* LoadGmin_CSC (klusmp.c:1765-1777): CKTdiagGmin onto every diagonal
+     * element the matrix has, before the (re)factorisation */
+    if (T->diagGmin != 0.0) {
+        int u;
+        for (u = 1; u <= T->n; u++)
+            if (T->hasdiag[u]) AX(u, u) += T->diagGmin;
+    }
     if (T->klu) return solve_klu(T);
     const int n = T->n, ld = n + 1;
     double *a = T->lu;
@@ -425,6 +443,207 @@ static int ni_iter(ORTran *T, int maxIter)
         }
         { double *t = T->rhs; T->rhs = T->rhsOld; T->rhsOld = t; }
     }
+}
+
+/* ---- CKTop (cktop.c:26-116) with its stepping fallbacks ---- */
+
+static void op_zero(ORTran *T)
+{
+    memset(T->rhsOld, 0, sizeof(double) * (T->n + 1));
+    memset(T->ck.states[0], 0, sizeof(double) * T->nstates);
+}
+
+static void op_save(ORTran *T, double *x, double *s)
+{
+    memcpy(x, T->rhsOld, sizeof(double) * (T->n + 1));
+    memcpy(s, T->ck.states[0], sizeof(double) * T->nstates);
+}
+
+static void op_restore(ORTran *T, const double *x, const double *s)
+{
+    memcpy(T->rhsOld, x, sizeof(double) * (T->n + 1));
+    memcpy(T->ck.states[0], s, sizeof(double) * T->nstates);
+}
+
+/* NIiter and the iterations it used (STATnumIter difference) */
+static int op_niiter(ORTran *T, int maxIter, int *iters)
+{
+    const int64_t n0 = T->niter;
+    const int rc = ni_iter(T, maxIter);
+    *iters = (int)(T->niter - n0);
+    return rc;
+}
+
+/* dynamic_gmin (cktop.c:161-287): CKTdiagGmin from 1e-2 down to gmin */
+static int dynamic_gmin(ORTran *T, int firstmode, int continuemode, int iterlim,
+                        double *ox, double *os)
+{
+    ORCkt *ck = &T->ck;
+    double OldGmin, gtarget, factor;
+    int converged, iters;
+    ck->mode = firstmode;
+    op_zero(T);
+    factor = T->gminFactor;
+    OldGmin = 1e-2;
+    T->diagGmin = OldGmin / factor;
+    gtarget = fmax(T->gmin, T->gshunt);
+    for (;;) {
+        converged = op_niiter(T, T->dcTrcvMaxIter, &iters);
+        if (converged == 0) {
+            ck->mode = continuemode;
+            if (T->diagGmin <= gtarget) break;
+            op_save(T, ox, os);
+            if (iters <= (T->dcTrcvMaxIter / 4)) {
+                factor *= sqrt(factor);
+                if (factor > T->gminFactor) factor = T->gminFactor;
+            }
+            if (iters > (3 * T->dcTrcvMaxIter / 4)) factor = fmax(sqrt(factor), 1.00005);
+            OldGmin = T->diagGmin;
+            if (T->diagGmin < factor * gtarget) {
+                factor = T->diagGmin / gtarget;
+                T->diagGmin = gtarget;
+            } else {
+                T->diagGmin /= factor;
+            }
+        } else {
+            if (factor < 1.00005) break;
+            factor = sqrt(sqrt(factor));
+            T->diagGmin = OldGmin / factor;
+            op_restore(T, ox, os);
+        }
+    }
+    T->diagGmin = T->gshunt;
+    return ni_iter(T, iterlim);
+}
+
+/* new_gmin (cktop.c:349-478): the devices' CKTgmin from 1e-2 down */
+static int new_gmin(ORTran *T, int firstmode, int continuemode, int iterlim, double *ox, double *os)
+{
+    ORCkt *ck = &T->ck;
+    double OldGmin, gtarget, factor, startgmin;
+    int converged, iters;
+    ck->mode = firstmode;
+    op_zero(T);
+    startgmin = ck->gmin;
+    factor = T->gminFactor;
+    OldGmin = 1e-2;
+    ck->gmin = OldGmin / factor;
+    gtarget = fmax(startgmin, T->gshunt);
+    for (;;) {
+        converged = op_niiter(T, T->dcTrcvMaxIter, &iters);
+        if (converged == 0) {
+            ck->mode = continuemode;
+            if (ck->gmin <= gtarget) break;
+            op_save(T, ox, os);
+            if (iters <= (T->dcTrcvMaxIter / 4)) {
+                factor *= sqrt(factor);
+                if (factor > T->gminFactor) factor = T->gminFactor;
+            }
+            if (iters > (3 * T->dcTrcvMaxIter / 4)) factor = fmax(sqrt(factor), 3);
+            OldGmin = ck->gmin;
+            if (ck->gmin < factor * gtarget) {
+                factor = ck->gmin / gtarget;
+                ck->gmin = gtarget;
+            } else {
+                ck->gmin /= factor;
+            }
+        } else {
+            if (factor < 1.00005) break;
+            factor = sqrt(sqrt(factor));
+            ck->gmin = OldGmin / factor;
+            op_restore(T, ox, os);
+        }
+    }
+    ck->gmin = fmax(startgmin, T->gshunt);
+    return ni_iter(T, iterlim);
+}
+
+/* gillespie_src (cktop.c:480-670): sources ramped from 0 */
+static int gillespie_src(ORTran *T, int firstmode, int continuemode, double *ox, double *os)
+{
+    ORCkt *ck = &T->ck;
+    int converged, i, iters;
+    double ConvFact = 0;
+    const double gminstart = ck->gmin;
+    ck->mode = firstmode;
+    T->srcFact = 0;
+    op_zero(T);
+    converged = op_niiter(T, T->dcTrcvMaxIter, &iters);
+    if (converged != 0) {
+        T->diagGmin = (T->gshunt <= 0) ? ck->gmin : T->gshunt;
+        for (i = 0; i < 10; i++) T->diagGmin *= 10;
+        for (i = 0; i <= 10; i++) {
+            converged = op_niiter(T, T->dcTrcvMaxIter, &iters);
+            if (converged != 0) {
+                T->diagGmin = T->gshunt;
+                break;
+            }
+            T->diagGmin /= 10;
+            ck->mode = continuemode;
+        }
+        T->diagGmin = T->gshunt;
+    }
+    if (converged == 0) {
+        double raise = 0.001;
+        op_save(T, ox, os);
+        T->srcFact = ConvFact + raise;
+        do {
+            converged = op_niiter(T, T->dcTrcvMaxIter, &iters);
+            ck->mode = continuemode;
+            if (converged == 0) {
+                ConvFact = T->srcFact;
+                op_save(T, ox, os);
+                T->srcFact = ConvFact + raise;
+                if (iters <= (T->dcTrcvMaxIter / 4)) raise *= 1.5;
+                if (iters > (3 * T->dcTrcvMaxIter / 4)) raise *= 0.5;
+            } else {
+                if (T->srcFact - ConvFact < 1e-8) break;
+                raise /= 10;
+                if (raise > 0.01) raise = 0.01;
+                T->srcFact = ConvFact;
+                op_restore(T, ox, os);
+            }
+            if (T->srcFact > 1) T->srcFact = 1;
+        } while ((raise >= 1e-7) && (ConvFact < 1));
+        T->diagGmin = ck->gmin = gminstart;
+    }
+    T->srcFact = 1;
+    return (ConvFact != 1) ? 103 : 0;
+}
+
+static int ckt_op(ORTran *T, int firstmode, int continuemode, int iterlim)
+{
+    int converged = 1;
+    double *ox = (double *)malloc(sizeof(double) * (T->n + 1));
+    double *os = (double *)malloc(sizeof(double) * (T->nstates + 1));
+    T->ck.mode = firstmode;
+    if (!T->noOpIter) converged = ni_iter(T, iterlim);
+    if (converged != 0 && T->numGminSteps >= 1) {
+        converged = dynamic_gmin(T, firstmode, continuemode, iterlim, ox, os);
+        if (converged != 0) converged = new_gmin(T, firstmode, continuemode, iterlim, ox, os);
+    }
+    if (converged != 0 && T->numSrcSteps >= 1)
+        converged = gillespie_src(T, firstmode, continuemode, ox, os);
+    free(ox);
+    free(os);
+    return converged;
+}
+
+/* operating-point options (cktsopt.c): noopiter, gminsteps (0: off, 1:
+ * dynamic then true gmin stepping), srcsteps (0: off, 1: Gillespie source
+ * stepping), gminfactor, gshunt, itl2.  Returns 0, or -1 for stepping
+ * counts above 1 (the spice3 variants are not restated). */
+int or_tran_set_op_options(ORTran *T, int noopiter, int gminsteps, int srcsteps, double gminfactor,
+                           double gshunt, int itl2)
+{
+    if (gminsteps > 1 || srcsteps > 1) return -1;
+    T->noOpIter = noopiter;
+    T->numGminSteps = gminsteps;
+    T->numSrcSteps = srcsteps;
+    T->gminFactor = gminfactor;
+    T->gshunt = gshunt;
+    T->dcTrcvMaxIter = itl2;
+    return 0;
 }
 
 /* nicomcof.c:17-130 */
@@ -552,10 +771,9 @@ int or_tran_run(ORTran *T, double tstep, double tstop, double tmax)
     ck->method = T->method;
     ck->order = 1;
     for (i = 0; i < T->nvs; i++) T->vs[i].break_time = -1.0;
-    /* CKTop: direct Newton from INITJCT */
-    ck->mode = (T->uic ? OR_MODEUIC : 0) | OR_MODETRANOP | OR_MODEINITJCT;
     memset(T->rhsOld, 0, sizeof(double) * (T->n + 1));
-    converged = ni_iter(T, 100);
+    converged = ckt_op(T, (T->uic ? OR_MODEUIC : 0) | OR_MODETRANOP | OR_MODEINITJCT,
+                       (T->uic ? OR_MODEUIC : 0) | OR_MODETRANOP | OR_MODEINITFLOAT, 100);
     if (converged) { T->noncon_dc_fail = 1; return converged; }
     ck->order = 1;
     for (i = 0; i < 7; i++) ck->deltaOld[i] = T->maxStep;
@@ -664,6 +882,8 @@ ORTran *or_tran_new(int nnodes, int nmos, int ncap, int nres, int nvs)
     T->ic = (double *)calloc((size_t)nnodes + 1, sizeof(double));
     T->reltol = 1e-3; T->abstol = 1e-12; T->vntol = 1e-6; T->chgtol = 1e-14;
     T->trtol = 7; T->gmin = 1e-12; T->temp = 300.15; T->maxOrder = 2;
+    T->noOpIter = 0; T->numGminSteps = 1; T->numSrcSteps = 1; T->dcTrcvMaxIter = 50;
+    T->gminFactor = 10; T->gshunt = 0; T->srcFact = 1; T->diagGmin = 0;
     T->method = OR_TRAPEZOIDAL;
     return T;
 }
@@ -749,6 +969,18 @@ int or_tran_setup(ORTran *T)
     T->rhs = (double *)calloc((size_t)T->n + 1, sizeof(double));
     T->rhsOld = (double *)calloc((size_t)T->n + 1, sizeof(double));
     T->ck.gmin = T->gmin; T->ck.reltol = T->reltol; T->ck.abstol = T->abstol;
+    /* which diagonal elements exist (SMPmakeElt calls of the devices and of
+     * the .ic rows, CKTic) */
+    T->hasdiag = (char *)calloc((size_t)T->n + 1, 1);
+    for (k = 0; k < T->nmos; k++) {
+        int r;
+        for (r = 0; r < 4; r++) T->hasdiag[T->mos[k]->node[r]] = 1;
+    }
+    for (k = 0; k < T->ncap; k++) { T->hasdiag[T->cap_p[k]] = 1; T->hasdiag[T->cap_n[k]] = 1; }
+    for (k = 0; k < T->nres; k++) { T->hasdiag[T->res_p[k]] = 1; T->hasdiag[T->res_n[k]] = 1; }
+    for (k = 1; k <= T->nnodes; k++)
+        if (T->ic_given[k]) T->hasdiag[k] = 1;
+    T->hasdiag[0] = 0;
     T->ck.voltTol = T->vntol; T->ck.temp = T->temp;
     for (k = 0; k < T->nmos; k++)
         if (or_bsim4_inst_temp(T->mos[k], T->temp)) return -1;
@@ -843,7 +1075,7 @@ void or_tran_free(ORTran *T)
     if (!T) return;
     free(T->mos); free(T->cap_p); free(T->cap_n); free(T->cap_q); free(T->cap_c);
     free(T->res_p); free(T->res_n); free(T->res_g); free(T->vs); free(T->ic_given);
-    free(T->ic); free(T->stbuf); free(T->A); free(T->lu); free(T->piv); free(T->rhs);
+    free(T->ic); free(T->stbuf); free(T->hasdiag); free(T->A); free(T->lu); free(T->piv); free(T->rhs);
     free(T->rhsOld); free(T->out_t); free(T->out_x);
     if (T->klu) T->kg_free(T->klu);
     free(T->Ap); free(T->Ai); free(T->Ax); free(T->kb);

@@ -160,3 +160,21 @@ def test_uneven_batch_rings():
         assert st['status'] == 0, st
         te, xe = b.waveform(c)
         _compare('ring%d-c%d' % (stages[c], c), (te, xe, st), _oracle(d, tstop), d['nnodes'])
+
+
+def test_operating_point_stepping():
+    """CKTop's fallbacks (cktop.c:161-670) forced with noopiter: dynamic and
+    true gmin stepping (gminsteps 1), and Gillespie source stepping
+    (gminsteps 0, srcsteps 1), on the engine against the oracle; the Newton
+    iteration counts, which include every stepping solve, must be equal."""
+    print('case: operating point by gmin stepping and by source stepping, 11-stage ring .tran 10p '
+          '1n, adder seed 0 .tran 5p 1.5n', flush=True)
+    for tag, desc, tstop in [('ring11', workloads.ring_oscillator(stages=11), 1e-9),
+                             ('adder0', workloads.mc_adder(0), 1.5e-9)]:
+        for how, opt in [('gmin', {'noopiter': 1}), ('src', {'noopiter': 1, 'gminsteps': 0})]:
+            d = dict(desc)
+            d['options'] = dict(opt)
+            (eng,), info, _ = _engine([d], tstop)
+            ora = _oracle(d, tstop)
+            assert eng[2]['iterations'] == ora[2]['iterations'], (eng[2], ora[2])
+            _compare('%s-%s' % (tag, how), eng, ora, d['nnodes'])

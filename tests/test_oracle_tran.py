@@ -151,3 +151,24 @@ def test_gear_rc_transient():
     a = (0.1 - (1 - np.exp(-0.1))) / 0.1
     ref = np.where(t <= 0.1, (t - (1 - np.exp(-t))) / 0.1, 1 - (1 - a) * np.exp(-(t - 0.1)))
     assert np.abs(x[:, 2] - ref).max() < 2e-3
+
+
+@pytest.mark.parametrize('how', ['gmin', 'src'])
+def test_operating_point_stepping_reaches_the_direct_solution(how):
+    """CKTop's stepping restatement (dynamic_gmin + new_gmin, or
+    gillespie_src) forced with noopiter converges to the operating point the
+    direct Newton finds, within the Newton tolerance, and the transient that
+    follows accepts the same timepoints."""
+    desc = workloads.ring_oscillator(stages=11)
+    ref = po.Transient(dict(desc))
+    rc, t0, x0 = ref.run(desc['tstep'], 0.5e-9)
+    assert rc == 0
+    d = dict(desc)
+    d['options'] = {'noopiter': 1} if how == 'gmin' else {'noopiter': 1, 'gminsteps': 0}
+    T = po.Transient(d)
+    rc, t, x = T.run(desc['tstep'], 0.5e-9)
+    assert rc == 0
+    assert T.stats()['iterations'] > ref.stats()['iterations']
+    assert len(t) == len(t0) and np.allclose(t, t0, rtol=1e-9)
+    assert np.abs(x[0] - x0[0]).max() < 1e-6 * 10
+    assert np.abs(x - x0).max() < 1e-6
