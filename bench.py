@@ -34,10 +34,10 @@ FP64_PEAK_TFLOPS = 78.6
 # counters read for a device copy of known size (FETCH_SIZE x 2.00, WRITE_SIZE
 # x 1.00), divided by the evaluations the engine counted.  None where not
 # measured for the selected variant.
-MEASURED_HBM_BYTES_PER_EVAL = {('k_tran_mos', 2): 1823, ('k_bsim4_eval', 2): 1749}
+MEASURED_HBM_BYTES_PER_EVAL = {('k_tran_mos', 2): 1805, ('k_bsim4_eval', 2): 1598}
 # FP64 FLOP per evaluation from SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes
 # (FMA counted twice), same runs
-MEASURED_FLOP_PER_EVAL = {'k_tran_mos': 6630, 'k_bsim4_eval': 6583}
+MEASURED_FLOP_PER_EVAL = {'k_tran_mos': 6614, 'k_bsim4_eval': 6566}
 
 
 def log(*a):
@@ -256,7 +256,9 @@ def bench_mc(args):
     b_eval = 8 * (icols + 4 + 4 + 6 + 21 + 38 + 38 + 20) + 4 + 4
     achieved = evals * b_eval / (mos_ms * 1e-3) / 1e9 if mos_ms > 0 else None
     mb = MEASURED_HBM_BYTES_PER_EVAL.get(('k_tran_mos', args.mos_waves or 2))
-    traffic = mb * evals / passes if (mb and passes) else None
+    # with two streams every pass launches the BSIM4 kernel once per half
+    mos_launches = passes * (args.streams or 2)
+    traffic = mb * evals / mos_launches if (mb and mos_launches) else None
     fp64_tflops = (MEASURED_FLOP_PER_EVAL['k_tran_mos'] * evals / (mos_ms * 1e-3) / 1e12
                    if mos_ms > 0 else None)
     log('passes %d, accepted %d, rejected %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
@@ -306,7 +308,7 @@ def bench_mc(args):
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': traffic,
                      'kernel': 'k_tran_mos', 'kernel_ms_total': mos_ms,
-                     'launches': passes, 'bytes_per_eval': b_eval, 'evals': evals,
+                     'launches': mos_launches, 'bytes_per_eval': b_eval, 'evals': evals,
                      'fp64_tflops': fp64_tflops, 'fp64_peak_tflops': FP64_PEAK_TFLOPS},
         'extra': {'accepted': acc, 'rejected': rej, 'newton_iterations': its,
                   'iterations_per_point': its / max(acc, 1), 'passes': passes,
