@@ -506,6 +506,8 @@ def main():
                     help='0: BSIM4 kernels read the selector word per instance')
     ap.add_argument('--streams', type=int, default=None,
                     help='2: the plain transient alternates circuit halves on two streams')
+    ap.add_argument('--order-slack', type=int, default=None,
+                    help='pivot order: degree slack traded for a shorter elimination tree')
     ap.add_argument('--train-circuits', type=int, default=32)
     ap.add_argument('--train-steps', type=int, default=1500)
     args = ap.parse_args()
@@ -538,6 +540,8 @@ def main():
         ngmi.tune('streams', args.streams)
     if args.specialise is not None:
         ngmi.tune('specialise', args.specialise)
+    if args.order_slack is not None:
+        ngmi.tune('order_slack', args.order_slack)
     if args.workload == 'mc':
         res = bench_mc(args)
     elif args.workload == 'copy':

@@ -58,7 +58,11 @@ const char *ngmi_last_error(void);
  * without the learned guess alternates the two halves of its circuits on two
  * streams so one half's step kernel overlaps the other half's BSIM4 kernel;
  * "specialise" (0 or 1, default 1): compile the default card's selectors as
- * constants when every instance of a set shares them.
+ * constants when every instance of a set shares them;
+ * "order_slack" (0..4, default 0): the engine's own pivot order may pick a
+ * node up to this much above the minimum degree when that shortens the
+ * elimination tree (fewer factor and solve levels, possibly more fill);
+ * it applies to batches built afterwards.
  * Process-wide; defaults are the measured best on MI355X. */
 int ngmi_tune(const char *knob, int value);
 

@@ -10,7 +10,7 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {2, 2, 2, 1};
+static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 0};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
 
@@ -38,11 +38,12 @@ int ngmi_tune(const char *knob, int value)
     else if (!std::strcmp(knob, "mos_waves")) k = TUNE_MOS_WAVES;
     else if (!std::strcmp(knob, "streams")) k = TUNE_STREAMS;
     else if (!std::strcmp(knob, "specialise")) k = TUNE_SPECIALISE;
+    else if (!std::strcmp(knob, "order_slack")) k = TUNE_ORDER_SLACK;
     if (k < 0) {
         set_error("unknown tuning knob '%s'", knob);
         return NGMI_E_BADPARM;
     }
-    const int vmin = (k == TUNE_SPECIALISE) ? 0 : 1;
+    const int vmin = (k == TUNE_SPECIALISE || k == TUNE_ORDER_SLACK) ? 0 : 1;
     const int vmax = (k == TUNE_STREAMS) ? 2 : (k == TUNE_SPECIALISE) ? 1 : 4;
     if (value < vmin || value > vmax) {
         set_error("knob '%s' takes %d..%d", knob, vmin, vmax);

@@ -16,7 +16,14 @@ namespace ngmi {
 void set_error(const char *fmt, ...);
 
 /* launch-shape knobs (ngmi_tune); defaults are the measured best */
-enum { TUNE_EVAL_WAVES = 0, TUNE_MOS_WAVES = 1, TUNE_STREAMS = 2, TUNE_SPECIALISE = 3, TUNE__COUNT };
+enum {
+    TUNE_EVAL_WAVES = 0,
+    TUNE_MOS_WAVES = 1,
+    TUNE_STREAMS = 2,
+    TUNE_SPECIALISE = 3,
+    TUNE_ORDER_SLACK = 4,
+    TUNE__COUNT
+};
 int tune_get(int knob);
 
 #define NGMI_HIP(call)                                                          \

@@ -50,6 +50,10 @@ using namespace ngmi;
  * would need more than ASM_MAXPART partial sums */
 constexpr int ASM_LEAF = 16;
 constexpr int ASM_MAXPART = 48;
+/* LU and substitution program records: PROG_INTS ints, of which the last
+ * 4 * PROG_INLINE are the inline term groups */
+constexpr int PROG_INLINE = 3;
+constexpr int PROG_INTS = 4 + 4 * PROG_INLINE;
 
 struct ngmi_circuit {
     int nnodes = 0;
@@ -184,20 +188,34 @@ struct Elt {
 
 uint32_t mk(uint32_t type, uint32_t idx, bool neg) { return (type << CT_TYPE_SHIFT) | (neg ? CT_NEG : 0u) | idx; }
 
-/* greedy minimum degree on an undirected graph (adjacency bitsets) */
-std::vector<int> min_degree(int n, std::vector<std::vector<uint64_t>> adj)
+/* greedy minimum degree on an undirected graph (adjacency bitsets), steered
+ * towards a short elimination tree.  Each node carries the level at which
+ * it can be eliminated: one more than that of its latest eliminated
+ * neighbour.  Among the nodes whose degree is within `slack` of the
+ * minimum, the one of lowest level goes next (then lowest degree, then
+ * lowest index).  The level-scheduled factor and solves take one
+ * barrier-separated step per level, so a chain of nodes (a ring, a carry
+ * chain) is better eliminated from many points at once than end to end,
+ * which plain minimum degree does at no lower fill. */
+std::vector<int> min_degree(int n, std::vector<std::vector<uint64_t>> adj, int slack)
 {
     const int W = (n + 63) / 64;
     std::vector<char> gone(n, 0);
-    std::vector<int> order;
+    std::vector<int> order, lev(n, 0), deg(n, 0);
     order.reserve(n);
     for (int step = 0; step < n; step++) {
-        int best = -1, bestdeg = 1 << 30;
+        int md = 1 << 30;
         for (int v = 0; v < n; v++) {
             if (gone[v]) continue;
             int d = 0;
             for (int w = 0; w < W; w++) d += __builtin_popcountll(adj[v][w]);
-            if (d < bestdeg) { bestdeg = d; best = v; }
+            deg[v] = d;
+            md = std::min(md, d);
+        }
+        int best = -1;
+        for (int v = 0; v < n; v++) {
+            if (gone[v] || deg[v] > md + slack) continue;
+            if (best < 0 || lev[v] < lev[best] || (lev[v] == lev[best] && deg[v] < deg[best])) best = v;
         }
         const int v = best;
         gone[v] = 1;
@@ -212,6 +230,7 @@ std::vector<int> min_degree(int n, std::vector<std::vector<uint64_t>> adj)
             }
         }
         for (int a : nb) {
+            lev[a] = std::max(lev[a], lev[v] + 1);
             adj[a][v >> 6] &= ~(1ull << (v & 63));
             for (int b2 : nb)
                 if (b2 != a) adj[a][b2 >> 6] |= 1ull << (b2 & 63);
@@ -232,7 +251,7 @@ struct Plan {
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
-    std::vector<int> slot_flags, perm, ic_row, nbr;
+    std::vector<int> perm, ic_row, nbr;
     std::vector<int> mos_pos, rmos_pos, diag_slot;
     int NA = 0, NR = 0;
 };
@@ -326,7 +345,8 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         for (int a = 0; a < N; a++)
             for (int c = 0; c < N; c++)
                 if (a != c && (Bp[a][c] || Bp[c][a])) adj[a][c >> 6] |= 1ull << (c & 63);
-        const std::vector<int> order = min_degree(N, adj);   /* B index eliminated q-th */
+        /* B index eliminated q-th */
+        const std::vector<int> order = min_degree(N, adj, tune_get(TUNE_ORDER_SLACK));
         for (int q = 0; q < N; q++) {
             rowp[q] = prow[order[q] + 1];
             colq[q] = order[q] + 1;
@@ -398,18 +418,21 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     if (nslots + 1 > 65535 || N + 1 > 65535) return fail("circuit too large for 16-bit LU indices");
     /* term lists: two 16-bit indices per word, padded to whole groups of
      * four with a pair whose product is exactly zero (lu[nslots] and z[N]
-     * hold 0).  The first group sits in the entry's own 32-byte record
-     * {x, y, begin, end, group 0}; [begin, end) are the further groups, in
-     * uint4 units of dst. */
+     * hold 0).  An entry is a 64-byte record {target | g << 16, y, begin,
+     * end} + three groups: its first g (1..3) groups inline, the rest at
+     * [begin, end) in uint4 units of dst. */
     auto put_terms = [](std::vector<uint32_t> &dst, const std::vector<int2> &t, uint32_t pad,
-                        std::vector<int> &rec) {
+                        std::vector<int> &rec, int target, int y) {
         std::vector<uint32_t> w;
         for (const int2 &x : t) w.push_back((uint32_t)x.x | ((uint32_t)x.y << 16));
-        while (w.size() < 4 || w.size() % 4) w.push_back(pad);
+        const int g = std::max(1, std::min(PROG_INLINE, (int)(w.size() + 3) / 4));
+        while ((int)w.size() < 4 * g || w.size() % 4) w.push_back(pad);
+        rec.push_back(target | (g << 16));
+        rec.push_back(y);
         rec.push_back((int)(dst.size() / 4));
-        dst.insert(dst.end(), w.begin() + 4, w.end());
+        if ((int)w.size() > 4 * PROG_INLINE) dst.insert(dst.end(), w.begin() + 4 * PROG_INLINE, w.end());
         rec.push_back((int)(dst.size() / 4));
-        for (int k = 0; k < 4; k++) rec.push_back((int)w[k]);
+        for (int k = 0; k < 4 * PROG_INLINE; k++) rec.push_back(k < (int)w.size() ? (int)w[k] : (int)pad);
     };
     const uint32_t pad_lu = (uint32_t)nslots | ((uint32_t)nslots << 16);
     const uint32_t pad_z = (uint32_t)nslots | ((uint32_t)N << 16);
@@ -420,16 +443,14 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         std::vector<std::vector<int>> by(nlev);
         for (int k = 0; k < (int)ents.size(); k++) by[ents[k].lev].push_back(k);
         for (int L = 0; L < nlev; L++) {
-            f_lev[L] = (int)(f_ent.size() / 8);
+            f_lev[L] = (int)(f_ent.size() / PROG_INTS);
             for (int k : by[L]) {
                 const Ent &e = ents[k];
-                f_ent.push_back(e.t);
-                f_ent.push_back(e.piv);
-                put_terms(f_term, e.terms, pad_lu, f_ent);
+                put_terms(f_term, e.terms, pad_lu, f_ent, e.t, e.piv);
                 nterms += (int)e.terms.size();
             }
         }
-        f_lev[nlev] = (int)(f_ent.size() / 8);
+        f_lev[nlev] = (int)(f_ent.size() / PROG_INTS);
     }
     P.nlev_f = nlev;
     P.nterms = nterms;
@@ -449,17 +470,15 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         std::vector<std::vector<int>> by(nl);
         for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
         for (int L = 0; L < nl; L++) {
-            fw_lev.push_back((int)(fw_ent.size() / 8));
+            fw_lev.push_back((int)(fw_ent.size() / PROG_INTS));
             for (int a : by[L]) {
                 std::vector<int2> t;
                 for (int k = 0; k < a; k++)
                     if (S[a][k]) t.push_back(make_int2(slot[(size_t)a * N + k], k));
-                fw_ent.push_back(a);
-                fw_ent.push_back(-1);
-                put_terms(fw_term, t, pad_z, fw_ent);
+                put_terms(fw_term, t, pad_z, fw_ent, a, -1);
             }
         }
-        fw_lev.push_back((int)(fw_ent.size() / 8));
+        fw_lev.push_back((int)(fw_ent.size() / PROG_INTS));
         P.nlev_fw = nl;
     }
     {
@@ -474,17 +493,15 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         std::vector<std::vector<int>> by(nl);
         for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
         for (int L = 0; L < nl; L++) {
-            bw_lev.push_back((int)(bw_ent.size() / 8));
+            bw_lev.push_back((int)(bw_ent.size() / PROG_INTS));
             for (int a : by[L]) {
                 std::vector<int2> t;
                 for (int j = a + 1; j < N; j++)
                     if (S[a][j]) t.push_back(make_int2(slot[(size_t)a * N + j], j));
-                bw_ent.push_back(a);
-                bw_ent.push_back(slot[(size_t)a * N + a]);
-                put_terms(bw_term, t, pad_z, bw_ent);
+                put_terms(bw_term, t, pad_z, bw_ent, a, slot[(size_t)a * N + a]);
             }
         }
-        bw_lev.push_back((int)(bw_ent.size() / 8));
+        bw_lev.push_back((int)(bw_ent.size() / PROG_INTS));
         P.nlev_bw = nl;
     }
 
@@ -510,40 +527,58 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     /* a slot or row with more than a few contributions is split into
      * leaves that write partial sums, plus one combine step (the supply
      * node's diagonal collects one contribution per connected device) */
-    auto split = [](const std::vector<uint32_t> &codes, int dest, int partial_dest0, int &npart,
-                    std::vector<uint32_t> &code, std::vector<int4> &leaf, std::vector<int4> &comb) {
-        /* contributions in whole groups of four, padded with zero codes;
-         * a leaf is a 32-byte record {dest, begin, end, 0, copy of group 0}
-         * with [begin, end) all its groups in uint4 units of code */
+    struct Leaf {
+        int dest, flag;
+        std::vector<uint32_t> w;   /* whole groups of four codes */
+    };
+    auto split = [](const std::vector<uint32_t> &codes, int dest, int flag, int partial_dest0,
+                    int &npart, std::vector<Leaf> &leaves, std::vector<int4> &comb) {
+        /* contributions in whole groups of four, padded with zero codes */
         const int n = (int)codes.size();
-        auto put = [&](int d, int q0, int q1) {
-            std::vector<uint32_t> w(codes.begin() + q0, codes.begin() + q1);
-            while (w.size() < 4 || w.size() % 4) w.push_back(CT_ZERO << CT_TYPE_SHIFT);
-            const int b = (int)(code.size() / 4);
-            code.insert(code.end(), w.begin(), w.end());
-            leaf.push_back(make_int4(d, b, (int)(code.size() / 4), 0));
-            leaf.push_back(make_int4((int)w[0], (int)w[1], (int)w[2], (int)w[3]));
+        auto put = [&](int d, int f, int q0, int q1) {
+            Leaf lf{d, f, std::vector<uint32_t>(codes.begin() + q0, codes.begin() + q1)};
+            while (lf.w.size() < 4 || lf.w.size() % 4) lf.w.push_back(CT_ZERO << CT_TYPE_SHIFT);
+            leaves.push_back(std::move(lf));
         };
         if (n <= ASM_LEAF) {
-            put(dest, 0, n);
+            put(dest, flag, 0, n);
             return;
         }
         const int chunk = (std::max(ASM_LEAF, (n + ASM_MAXPART - 1) / ASM_MAXPART) + 3) / 4 * 4;
         const int p0 = npart;
-        for (int q = 0; q < n; q += chunk) put(partial_dest0 + npart++, q, std::min(n, q + chunk));
-        comb.push_back(make_int4(dest, p0, npart, 0));
+        for (int q = 0; q < n; q += chunk) put(partial_dest0 + npart++, 0, q, std::min(n, q + chunk));
+        comb.push_back(make_int4(dest, p0, npart, flag));
+    };
+    /* group e of the code (and contribution) array is leaf e's first group,
+     * so a lane finds a leaf's first values without reading its record
+     * first; the further groups follow all first groups.  A leaf is a
+     * 32-byte record {dest, begin, end, flag} + a copy of its first group,
+     * with [begin, end) its further groups in uint4 units of code. */
+    auto layout = [](const std::vector<Leaf> &leaves, std::vector<uint32_t> &code,
+                     std::vector<int4> &leaf) {
+        for (const Leaf &lf : leaves) code.insert(code.end(), lf.w.begin(), lf.w.begin() + 4);
+        for (const Leaf &lf : leaves) {
+            const int b = (int)(code.size() / 4);
+            code.insert(code.end(), lf.w.begin() + 4, lf.w.end());
+            leaf.push_back(make_int4(lf.dest, b, (int)(code.size() / 4), lf.flag));
+            leaf.push_back(make_int4((int)lf.w[0], (int)lf.w[1], (int)lf.w[2], (int)lf.w[3]));
+        }
     };
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
+    std::vector<Leaf> a_leaves, r_leaves;
     int n_apart = 0, n_rpart = 0;
-    for (int s = 0; s < nslots; s++) split(per_slot[s], s, nslots, n_apart, a_code, a_leaf, a_comb);
+    for (int s = 0; s < nslots; s++)
+        split(per_slot[s], s, slot_flags[s] & 1, nslots, n_apart, a_leaves, a_comb);
     std::vector<int> perm(2 * N), ic_row;
     for (int a = 0; a < N; a++) {
         perm[a] = colq[a];                    /* unknown solved in column a */
         perm[N + a] = rowp[a];                /* original row assembled in row a */
-        split(rhs_codes[rowp[a]], a, N, n_rpart, r_code, r_leaf, r_comb);
+        split(rhs_codes[rowp[a]], a, 0, N, n_rpart, r_leaves, r_comb);
         if (perm[N + a] <= NN && P.ic_nodes[perm[N + a]]) ic_row.push_back(a);
     }
+    layout(a_leaves, a_code, a_leaf);
+    layout(r_leaves, r_code, r_leaf);
     P.n_aleaf = (int)a_leaf.size() / 2; P.n_acomb = (int)a_comb.size();
     P.n_rleaf = (int)r_leaf.size() / 2; P.n_rcomb = (int)r_comb.size(); P.n_icrow = (int)ic_row.size();
     P.n_apart = n_apart; P.n_rpart = n_rpart;
@@ -594,7 +629,7 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     P.a_code = std::move(a_code); P.r_code = std::move(r_code);
     P.a_leaf = std::move(a_leaf); P.a_comb = std::move(a_comb);
     P.r_leaf = std::move(r_leaf); P.r_comb = std::move(r_comb);
-    P.slot_flags = std::move(slot_flags); P.perm = std::move(perm); P.ic_row = std::move(ic_row);
+    P.perm = std::move(perm); P.ic_row = std::move(ic_row);
     return NGMI_OK;
 }
 
@@ -708,7 +743,6 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     UP(vs_par, vs_par, double, dp);
     UP(ic_val, ic_val, double, dp);
     UP(P.a_code, a_code, uint32_t, up);
-    UP(P.slot_flags, slot_flags, int, ip);
     UP(P.r_code, r_code, uint32_t, up);
     UP(P.a_leaf, a_leaf, int4, i4);
     UP(P.a_comb, a_comb, int4, i4);

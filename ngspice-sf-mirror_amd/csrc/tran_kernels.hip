@@ -719,47 +719,118 @@ __device__ __forceinline__ double sub_terms(double acc, uint4 w, const double *x
     return acc;
 }
 
-/* acc minus the record's term groups: the inline first group, then the
- * further groups [h.z, h.w) with the next one's load in flight */
-__device__ __forceinline__ double lu_row_terms(double acc, const Rec &r, const uint4 *T,
+/* a 64-byte LU or substitution record: {target | g << 16, y, begin, end},
+ * then three term groups of which the first g are the entry's */
+struct Prog {
+    int4 h;
+    uint4 g0, g1, g2;
+};
+
+__device__ __forceinline__ Prog load_prog(const int4 *base, int e)
+{
+    Prog r;
+    r.h = gload(base + 4 * e);
+    r.g0 = gload(reinterpret_cast<const uint4 *>(base + 4 * e + 1));
+    r.g1 = gload(reinterpret_cast<const uint4 *>(base + 4 * e + 2));
+    r.g2 = gload(reinterpret_cast<const uint4 *>(base + 4 * e + 3));
+    return r;
+}
+
+/* acc minus the record's term groups, in order: the inline ones, then the
+ * further groups [h.z, h.w), loaded up to four at a time before use */
+__device__ __forceinline__ double lu_row_terms(double acc, const Prog &r, const uint4 *T,
                                                const double *x, const double *y)
 {
-    acc = sub_terms(acc, r.g, x, y);
-    int q = r.h.z;
-    if (q < r.h.w) {
-        uint4 w = gload(T + q);
-        while (++q < r.h.w) {
-            const uint4 wn = gload(T + q);
-            acc = sub_terms(acc, w, x, y);
-            w = wn;
-        }
-        acc = sub_terms(acc, w, x, y);
+    const int g = r.h.x >> 16;
+    acc = sub_terms(acc, r.g0, x, y);
+    if (g > 1) acc = sub_terms(acc, r.g1, x, y);
+    if (g > 2) acc = sub_terms(acc, r.g2, x, y);
+    for (int q = r.h.z; q < r.h.w; q += 4) {
+        uint4 w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (q + k < r.h.w) w[k] = gload(T + q + k);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (q + k < r.h.w) acc = sub_terms(acc, w[k], x, y);
     }
     return acc;
 }
 
 /* Level-scheduled program: entries of level L (lev[L] .. lev[L+1]) are
- * independent; a barrier separates levels.  Each lane's first record of the
- * next level is loaded before the current level runs, so its latency hides
- * behind the current level's work and barrier. */
+ * independent; a barrier separates levels.  Lane t runs the entries
+ * lev[0] + t + k nt, whatever their level, and keeps the records of its
+ * next two entries in flight.  Where levels are narrow (a chain of a few
+ * entries each) those records belong to levels far ahead, so their loads
+ * overlap many levels' work. */
 template <class F>
 __device__ __forceinline__ void run_levels(const int *lev, const int4 *ent, int nlev, int tid, int nt,
                                            F body)
 {   /* lev: the level table staged in LDS */
     if (nlev <= 0) return;
-    int e0 = lev[0] + tid, e1 = lev[1];
-    Rec nx;
-    if (e0 < e1) nx = load_rec(ent, e0);
+    const int total = lev[nlev];
+    int e = lev[0] + tid;
+    Prog p0, p1;
+    if (e < total) p0 = load_prog(ent, e);
+    if (e + nt < total) p1 = load_prog(ent, e + nt);
     for (int L = 0; L < nlev; L++) {
-        const int ce0 = e0, ce1 = e1;
-        const Rec cur = nx;
-        if (L + 1 < nlev) {
-            e0 = lev[L + 1] + tid;
-            e1 = lev[L + 2];
-            if (e0 < e1) nx = load_rec(ent, e0);
+        const int end = lev[L + 1];
+        while (e < end) {
+            const Prog cur = p0;
+            p0 = p1;
+            if (e + 2 * nt < total) p1 = load_prog(ent, e + 2 * nt);
+            body(cur);
+            e += nt;
         }
-        for (int e = ce0; e < ce1; e += nt) body(e == ce0 ? cur : load_rec(ent, e));
         __syncthreads();
+    }
+}
+
+/* Assembly leaves: lane t sums leaves t + k nt.  Leaf e's first values are
+ * group e of the contribution array, so the next U leaves' records and
+ * first values load together, while the current U leaves are summed.  The
+ * second group of each current leaf that has one is requested before the
+ * next leaves' loads, so waiting for it does not wait for them (loads
+ * complete in order). */
+template <int U, class F>
+__device__ __forceinline__ void run_leaves(const int4 *leaf, const double4 *val, const uint4 *code,
+                                           int n, int tid, int nt, F body)
+{
+    Rec rb[U];
+    double4 vb[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const int e = tid + k * nt;
+        if (e < n) {
+            rb[k] = load_rec(leaf, e);
+            vb[k] = gload4(val + e);
+        }
+    }
+    for (int e0 = tid; e0 < n; e0 += U * nt) {
+        Rec rc[U];
+        double4 vc[U];
+        uint4 xw[U];
+        double4 xv[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            rc[k] = rb[k];
+            vc[k] = vb[k];
+            if (e0 + k * nt < n && rc[k].h.y < rc[k].h.z) {
+                xw[k] = gload(code + rc[k].h.y);
+                xv[k] = gload4(val + rc[k].h.y);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int e = e0 + (U + k) * nt;
+            if (e < n) {
+                rb[k] = load_rec(leaf, e);
+                vb[k] = gload4(val + e);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++)
+            if (e0 + k * nt < n) body(rc[k], vc[k], xw[k], xv[k]);
     }
 }
 
@@ -802,6 +873,8 @@ __device__ __forceinline__ double rhs_value(uint32_t code, double mv, const doub
  * dozen entries per level, and a one-wave workgroup makes each level's
  * barrier nearly free while many circuits share a compute unit */
 constexpr int STEP_THREADS = 64;
+/* assembly leaves whose loads are issued together, per lane */
+constexpr int ASM_UNROLL = 4;
 
 /* circuit c of the topology D, one workgroup */
 __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
@@ -911,67 +984,62 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
      * read consecutive groups, so a wavefront's loads are contiguous */
     const double4 *AV = reinterpret_cast<const double4 *>(D.contrib + (int64_t)c * D.NA);
     const double4 *RV = reinterpret_cast<const double4 *>(D.rcontrib + (int64_t)c * D.NR);
-    /* each lane walks its leaves with the next leaf's record in flight */
-    {
-        Rec nx;
-        if (tid < D.n_aleaf) nx = load_rec(D.a_leaf, tid);
-        for (int e = tid; e < D.n_aleaf; e += nt) {
-            const Rec lf = nx;
-            if (e + nt < D.n_aleaf) nx = load_rec(D.a_leaf, e + nt);
-            double acc = 0.0;
-            uint4 w = lf.g;
-            for (int q = lf.h.y;;) {
-                const double4 m = gload4(AV + q);
-                const double v0 = mat_value(D, w.x, m.x, c, capg, icz);
-                const double v1 = mat_value(D, w.y, m.y, c, capg, icz);
-                const double v2 = mat_value(D, w.z, m.z, c, capg, icz);
-                const double v3 = mat_value(D, w.w, m.w, c, capg, icz);
-                acc += v0;
-                acc += v1;
-                acc += v2;
-                acc += v3;
-                if (++q >= lf.h.z) break;
-                w = gload(AC + q);
-            }
-            if (lf.h.x < D.nslots) {
-                if ((gload(D.slot_flags + lf.h.x) & 1) && icz) acc = 1.0;
-                lu[lf.h.x] = acc;
-            } else {
-                part[lf.h.x - D.nslots] = acc;
-            }
+    /* the sum of one leaf: its first group (codes from the record, values
+     * already loaded), then its further groups in order */
+    run_leaves<ASM_UNROLL>(D.a_leaf, AV, AC, D.n_aleaf, tid, nt,
+                           [&](const Rec &lf, const double4 &m0, const uint4 &w1, const double4 &m1) {
+        double acc = 0.0;
+        acc += mat_value(D, lf.g.x, m0.x, c, capg, icz);
+        acc += mat_value(D, lf.g.y, m0.y, c, capg, icz);
+        acc += mat_value(D, lf.g.z, m0.z, c, capg, icz);
+        acc += mat_value(D, lf.g.w, m0.w, c, capg, icz);
+        if (lf.h.y < lf.h.z) {
+            acc += mat_value(D, w1.x, m1.x, c, capg, icz);
+            acc += mat_value(D, w1.y, m1.y, c, capg, icz);
+            acc += mat_value(D, w1.z, m1.z, c, capg, icz);
+            acc += mat_value(D, w1.w, m1.w, c, capg, icz);
         }
-    }
-    {
-        Rec nx;
-        if (tid < D.n_rleaf) nx = load_rec(D.r_leaf, tid);
-        for (int e = tid; e < D.n_rleaf; e += nt) {
-            const Rec lf = nx;
-            if (e + nt < D.n_rleaf) nx = load_rec(D.r_leaf, e + nt);
-            double acc = 0.0;
-            uint4 w = lf.g;
-            for (int q = lf.h.y;;) {
-                const double4 m = gload4(RV + q);
-                const double v0 = rhs_value(w.x, m.x, capq, vsv);
-                const double v1 = rhs_value(w.y, m.y, capq, vsv);
-                const double v2 = rhs_value(w.z, m.z, capq, vsv);
-                const double v3 = rhs_value(w.w, m.w, capq, vsv);
-                acc += v0;
-                acc += v1;
-                acc += v2;
-                acc += v3;
-                if (++q >= lf.h.z) break;
-                w = gload(RC + q);
-            }
-            if (lf.h.x < N) z[lf.h.x] = acc;
-            else part[D.n_apart + lf.h.x - N] = acc;
+        for (int q = lf.h.y + 1; q < lf.h.z; q++) {
+            const uint4 w = gload(AC + q);
+            const double4 m = gload4(AV + q);
+            acc += mat_value(D, w.x, m.x, c, capg, icz);
+            acc += mat_value(D, w.y, m.y, c, capg, icz);
+            acc += mat_value(D, w.z, m.z, c, capg, icz);
+            acc += mat_value(D, w.w, m.w, c, capg, icz);
         }
-    }
+        if (lf.h.x < D.nslots) lu[lf.h.x] = ((lf.h.w & 1) && icz) ? 1.0 : acc;
+        else part[lf.h.x - D.nslots] = acc;
+    });
+    run_leaves<ASM_UNROLL>(D.r_leaf, RV, RC, D.n_rleaf, tid, nt,
+                           [&](const Rec &lf, const double4 &m0, const uint4 &w1, const double4 &m1) {
+        double acc = 0.0;
+        acc += rhs_value(lf.g.x, m0.x, capq, vsv);
+        acc += rhs_value(lf.g.y, m0.y, capq, vsv);
+        acc += rhs_value(lf.g.z, m0.z, capq, vsv);
+        acc += rhs_value(lf.g.w, m0.w, capq, vsv);
+        if (lf.h.y < lf.h.z) {
+            acc += rhs_value(w1.x, m1.x, capq, vsv);
+            acc += rhs_value(w1.y, m1.y, capq, vsv);
+            acc += rhs_value(w1.z, m1.z, capq, vsv);
+            acc += rhs_value(w1.w, m1.w, capq, vsv);
+        }
+        for (int q = lf.h.y + 1; q < lf.h.z; q++) {
+            const uint4 w = gload(RC + q);
+            const double4 m = gload4(RV + q);
+            acc += rhs_value(w.x, m.x, capq, vsv);
+            acc += rhs_value(w.y, m.y, capq, vsv);
+            acc += rhs_value(w.z, m.z, capq, vsv);
+            acc += rhs_value(w.w, m.w, capq, vsv);
+        }
+        if (lf.h.x < N) z[lf.h.x] = acc;
+        else part[D.n_apart + lf.h.x - N] = acc;
+    });
     __syncthreads();
     for (int e = tid; e < D.n_acomb; e += nt) {
         const int4 cb = gload(D.a_comb + e);
         double acc = 0.0;
         for (int p = cb.y; p < cb.z; p++) acc += part[p];
-        if ((gload(D.slot_flags + cb.x) & 1) && icz) acc = 1.0;
+        if ((cb.w & 1) && icz) acc = 1.0;
         lu[cb.x] = acc;
     }
     for (int e = tid; e < D.n_rcomb; e += nt) {
@@ -1004,22 +1072,27 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
      * two substitutions; see run_levels for the load schedule */
     const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
     run_levels(levf, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
-               [&](const Rec &r) {
-                   const double acc = lu_row_terms(lu[r.h.x], r, FT, lu, lu);
-                   lu[r.h.x] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
+               [&](const Prog &r) {
+                   const int t = r.h.x & 0xffff;
+                   const double acc = lu_row_terms(lu[t], r, FT, lu, lu);
+                   lu[t] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
                });
     /* forward substitution (unit L) */
     const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
     run_levels(levw, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
-               [&](const Rec &r) { z[r.h.x] = lu_row_terms(z[r.h.x], r, WT, lu, z); });
+               [&](const Prog &r) {
+                   const int a = r.h.x & 0xffff;
+                   z[a] = lu_row_terms(z[a], r, WT, lu, z);
+               });
     /* back substitution */
     const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
     run_levels(levb, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
-               [&](const Rec &r) {
-                   const double acc = lu_row_terms(z[r.h.x], r, BT, lu, z);
+               [&](const Prog &r) {
+                   const int a = r.h.x & 0xffff;
+                   const double acc = lu_row_terms(z[a], r, BT, lu, z);
                    const double piv = lu[r.h.y];
                    if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
-                   z[r.h.x] = acc / piv;
+                   z[a] = acc / piv;
                });
     /* new iterate; NIconvTest (niconv.c:21-90) and the device votes */
     for (int a = tid; a < N; a += nt) xn[D.perm_unknown[a]] = z[a];

@@ -96,12 +96,14 @@ struct TranDev {
     double *vs_brk;           /* [C][NVS] break_time */
     const double *ic_val;     /* [C][NN+1] */
     /* LU program */
-    /* assembly: a leaf is a 32-byte record {dest, begin, end, 0} + a copy
-     * of its first group of four contribution codes; [begin, end) are all
-     * its groups (uint4 units of a_code, padded with zero codes).  It sums
+    /* assembly: leaf e is a 32-byte record {dest, begin, end, flag} + a
+     * copy of its first group of four contribution codes, which is group e
+     * of a_code; [begin, end) are its further groups (uint4 units of
+     * a_code, after all first groups; padded with zero codes).  It sums
      * into L+U slot dest (< nslots) or partial dest - nslots; combines (slot,
-     * pbegin, pend) add partials into a slot; likewise for the RHS rows
-     * (dest < N a row, else partial n_apart + dest - N).  MOSFET values come
+     * pbegin, pend, flag) add partials into a slot; likewise for the RHS
+     * rows (dest < N a row, else partial n_apart + dest - N).  flag bit 0:
+     * the slot is an .ic diagonal.  MOSFET values come
      * from the circuit's contribution arrays, which hold, at the position of
      * each MOSFET code in a_code / r_code, the value the BSIM4 kernel wrote
      * there (mos_pos / rmos_pos: position of stamp s / RHS row r of MOSFET j
@@ -109,21 +111,21 @@ struct TranDev {
     const int4 *a_leaf, *a_comb, *r_leaf, *r_comb;
     int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
     const uint32_t *a_code;   /* contributions */
-    const int *slot_flags;    /* [nslots] bit0: .ic diagonal */
     const uint32_t *r_code;
     const int *ic_row;        /* permuted rows of .ic nodes */
     int n_icrow;
     /* LU program.  Terms are 16-bit index pairs packed in one word (low |
      * high << 16), grouped four to a uint4 and padded with pairs whose
      * product is exactly zero (LDS lu[nslots] = z[N] = 0).  An entry is a
-     * 32-byte record: four ints, then its first term group; the ints' term
+     * 64-byte record: four ints {target | g << 16, y, begin, end}, then
+     * three term groups of which the first g (1..3) are its own; the term
      * range [begin, end) holds the further groups, in uint4 units. */
     const int *f_lev;         /* [nlev_f+1] -> entries */
-    const int *f_ent;         /* [ne][8]: target, pivot (-1 for U), term begin, term end, group 0 */
+    const int *f_ent;         /* [ne][16]: y = pivot slot (-1 for U) */
     const uint32_t *f_term;   /* (a, b) slot pairs: lu[t] -= lu[a] lu[b] */
-    const int *fw_lev, *fw_ent;   /* forward solve: [ne][8] row, -1, term begin, end, group 0 */
+    const int *fw_lev, *fw_ent;   /* forward solve: [ne][16] target row, y = -1 */
     const uint32_t *fw_term;      /* (L slot, z index) */
-    const int *bw_lev, *bw_ent;   /* backward solve: [ne][8] row, diag slot, term begin, end, group 0 */
+    const int *bw_lev, *bw_ent;   /* backward solve: [ne][16] target row, y = diag slot */
     const uint32_t *bw_term;      /* (U slot, z index) */
     const int *perm_unknown;  /* [2N]: [a] unknown (1-based) solved in permuted column a;
                                  [N + a] original row assembled into permuted row a */
