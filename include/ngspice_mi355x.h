@@ -59,7 +59,7 @@ const char *ngmi_last_error(void);
  * streams so one half's step kernel overlaps the other half's BSIM4 kernel;
  * "specialise" (0 or 1, default 1): compile the default card's selectors as
  * constants when every instance of a set shares them;
- * "order_slack" (0..4, default 0): the engine's own pivot order may pick a
+ * "order_slack" (0..4, default 1): the engine's own pivot order may pick a
  * node up to this much above the minimum degree when that shortens the
  * elimination tree (fewer factor and solve levels, possibly more fill);
  * it applies to batches built afterwards.

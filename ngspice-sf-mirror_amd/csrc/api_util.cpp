@@ -10,7 +10,7 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 0};
+static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 1};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
 

@@ -85,6 +85,19 @@ def test_small_ring_oscillator():
     _compare('ring11', eng, ora, desc['nnodes'])
 
 
+def test_ring101_full_length():
+    """The reference's own CPU-sized case at full length: 101-stage ring,
+    .ic v(1)=0.51, .tran 10p 200n (about 45 oscillation periods, 2e4
+    accepted points)."""
+    desc = workloads.ring_oscillator()
+    tstop = desc['tstop']
+    print('case: 101-stage ring, .tran 10p 200n', flush=True)
+    (eng,), info, passes = _engine([desc], tstop)
+    print('lu', info, 'passes', passes, flush=True)
+    ora = _oracle(desc, tstop)
+    _compare('ring101', eng, ora, desc['nnodes'])
+
+
 def test_adder_batch():
     descs = [workloads.mc_adder(s) for s in (0, 1, 2)]
     tstop = 2.5e-9
