@@ -261,6 +261,49 @@ int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int
 /* waveform of circuit c (CKTdump rows, cktdump.c:33-44): t[n], x[n][nprobe] */
 int ngmi_batch_waveform(const ngmi_batch *b, int c, double *t, double *x, int64_t maxpts,
                         int64_t *npts);
+/* Circuit c's waveform as a raw file in the layout of ngspice's batch
+ * output (outitf.c:881-1110: header, "Variables:", then "Values:" rows of
+ * 15-digit values, or "Binary:" rows of native doubles when binary != 0).
+ * names[k] names recorded unknown k (a node name, or "<source>#branch" for
+ * a source current, written v(...) / i(...) as guess_type does,
+ * outitf.c:928-990); NULL names use the unknowns' indices. */
+int ngmi_batch_write_raw(const ngmi_batch *b, int c, const char *path, const char *title,
+                         const char *const *names, int binary);
+
+/* ------------------------------------------------------------------ */
+/* SPICE decks                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct ngmi_netlist ngmi_netlist;
+
+/* Parses a deck (the reference's input language, inp_readall /
+ * inp_subcktexpand / INPpas1-2, for the subset a BSIM4 transient uses:
+ * M C R V X elements, .subckt, .model nmos|pmos level=54, .param with
+ * expressions, .tran, .options, .temp, .ic, .global).  Unknowns are
+ * numbered as ngspice numbers them: nodes by first appearance in the
+ * flattened deck (CKTnewNode, cktnewn.c), then source branches in
+ * VSRCsetup order, the reverse of deck order (vsrcset.c:41-44,
+ * cktcrte.c:62-64).  seed drives agauss/gauss/aunif/unif.  NULL on error
+ * (ngmi_last_error names the line). */
+ngmi_netlist *ngmi_netlist_parse(const char *text, uint64_t seed);
+void ngmi_netlist_free(ngmi_netlist *nl);
+const char *ngmi_netlist_title(const ngmi_netlist *nl);
+int ngmi_netlist_size(const ngmi_netlist *nl, int *n_nodes, int *n_unknowns, int *n_elements);
+/* name of unknown k (0 = ground "0"; nodes; then "<source>#branch") */
+const char *ngmi_netlist_name(const ngmi_netlist *nl, int k);
+/* unknown number of a node or "<source>#branch" name, -1 if absent */
+int ngmi_netlist_find(const ngmi_netlist *nl, const char *name);
+/* .tran tstep tstop [tstart [tmax]] [uic] */
+int ngmi_netlist_tran(const ngmi_netlist *nl, double *tstep, double *tstop, double *tstart,
+                      double *tmax, int *uic);
+/* flattened element k in deck order: kind ('m' 'c' 'r' 'v'), its terminal
+ * unknowns (up to 4) and value (C, R, or the source's DC value); returns
+ * the terminal count, -1 when k is out of range.  Element names follow
+ * ngspice's flattening (translate_inst_name, subckt.c:1130-1143). */
+int ngmi_netlist_element(const ngmi_netlist *nl, int k, char *kind, int *nodes, double *value);
+const char *ngmi_netlist_element_name(const ngmi_netlist *nl, int k);
+/* a new circuit of the deck (free with ngmi_circuit_free); its models
+ * belong to the netlist, which must outlive the batches built from it */
+ngmi_circuit *ngmi_netlist_circuit(const ngmi_netlist *nl);
 
 /* ------------------------------------------------------------------ */
 /* Newton initial-guess network (no counterpart in the reference)      */

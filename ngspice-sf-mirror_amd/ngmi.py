@@ -305,6 +305,22 @@ def _tran_protos(L):
                                               ctypes.POINTER(i), i, i, ctypes.POINTER(cp),
                                               ctypes.POINTER(d)]
     L.ngmi_batch_groups.argtypes = [vp, ctypes.POINTER(i)]
+    L.ngmi_batch_write_raw.argtypes = [vp, i, cp, cp, ctypes.POINTER(cp), i]
+    L.ngmi_netlist_parse.argtypes = [cp, ctypes.c_uint64]
+    L.ngmi_netlist_parse.restype = vp
+    L.ngmi_netlist_free.argtypes = [vp]
+    L.ngmi_netlist_title.argtypes = [vp]
+    L.ngmi_netlist_title.restype = cp
+    L.ngmi_netlist_size.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.ngmi_netlist_name.argtypes = [vp, i]
+    L.ngmi_netlist_name.restype = cp
+    L.ngmi_netlist_find.argtypes = [vp, cp]
+    L.ngmi_netlist_tran.argtypes = [vp] + [ctypes.POINTER(d)] * 4 + [ctypes.POINTER(i)]
+    L.ngmi_netlist_element.argtypes = [vp, i, cp, ctypes.POINTER(i), ctypes.POINTER(d)]
+    L.ngmi_netlist_element_name.argtypes = [vp, i]
+    L.ngmi_netlist_element_name.restype = cp
+    L.ngmi_netlist_circuit.argtypes = [vp]
+    L.ngmi_netlist_circuit.restype = vp
     L.ngmi_batch_circuit_info.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i),
                                           ctypes.POINTER(i), ctypes.POINTER(i)]
     L.ngmi_batch_set_profile.argtypes = [vp, i]
@@ -574,8 +590,94 @@ class Batch:
                                         ctypes.byref(got)), 'waveform')
         return t[:got.value], x[:got.value]
 
+    def write_raw(self, c, path, title='', names=None, binary=False):
+        """Circuit c's waveform as an ngspice batch-mode raw file; names[k]
+        names recorded unknown k (node name or '<source>#branch')."""
+        arr = None
+        if names is not None:
+            arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+        check(lib().ngmi_batch_write_raw(self.h, c, str(path).encode(), title.encode(), arr,
+                                         1 if binary else 0), 'write_raw')
+
     def __del__(self):
         try:
             lib().ngmi_batch_free(self.h)
+        except Exception:
+            pass
+
+
+class Netlist:
+    """A SPICE deck parsed by the engine's front end (ngmi_netlist_parse):
+    MNA numbering as ngspice gives the same deck."""
+
+    def __init__(self, text, seed=0):
+        L = _tran_protos(lib())
+        self.h = L.ngmi_netlist_parse(text.encode(), seed)
+        if not self.h:
+            raise EngineError(1, 'netlist: ' + last_error())
+
+    @property
+    def title(self):
+        return lib().ngmi_netlist_title(self.h).decode()
+
+    def size(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().ngmi_netlist_size(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+              'netlist size')
+        return dict(nodes=a.value, unknowns=b.value, elements=c.value)
+
+    def names(self):
+        """unknown names, index 0 = ground"""
+        n = self.size()['unknowns']
+        return [lib().ngmi_netlist_name(self.h, k).decode() for k in range(n + 1)]
+
+    def find(self, name):
+        return lib().ngmi_netlist_find(self.h, name.encode())
+
+    def tran(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        u = ctypes.c_int()
+        check(lib().ngmi_netlist_tran(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(u)),
+              'netlist tran')
+        return dict(tstep=v[0].value, tstop=v[1].value, tstart=v[2].value, tmax=v[3].value,
+                    uic=u.value)
+
+    def elements(self):
+        """flattened elements: (name, kind, terminal unknowns, value)"""
+        out = []
+        for k in range(self.size()['elements']):
+            kind = ctypes.create_string_buffer(1)
+            nodes = (ctypes.c_int * 4)()
+            val = ctypes.c_double()
+            n = lib().ngmi_netlist_element(self.h, k, kind, nodes, ctypes.byref(val))
+            out.append((lib().ngmi_netlist_element_name(self.h, k).decode(), kind.raw.decode(),
+                        list(nodes[:n]), val.value))
+        return out
+
+    def circuit(self):
+        return NetlistCircuit(self)
+
+    def __del__(self):
+        try:
+            lib().ngmi_netlist_free(self.h)
+        except Exception:
+            pass
+
+
+class NetlistCircuit:
+    """A circuit built from a Netlist (usable in Batch like Circuit)."""
+
+    def __init__(self, netlist):
+        self.netlist = netlist
+        self.h = lib().ngmi_netlist_circuit(netlist.h)
+        if not self.h:
+            raise EngineError(1, 'netlist circuit: ' + last_error())
+
+    def analyze(self):
+        return Circuit.analyze(self)
+
+    def __del__(self):
+        try:
+            lib().ngmi_circuit_free(self.h)
         except Exception:
             pass

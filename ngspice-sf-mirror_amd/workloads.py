@@ -202,3 +202,39 @@ def mc_adder(seed, bits=ADDER_BITS, tstep=ADDER_TSTEP, tstop=ADDER_TSTOP):
     return {'nnodes': nn, 'models': models, 'mos': mos, 'caps': [], 'res': [], 'vsrc': vsrc,
             'ic': {}, 'tstep': tstep, 'tstop': tstop, 'names': names,
             'bits0': bits0, 'bits1': bits1}
+
+
+def deck(desc, title='engine workload'):
+    """A SPICE deck of a circuit description (node k named "k"; source j
+    named "v<j>", written in reverse order so that the deck's branch
+    numbering, the reverse of deck order, puts source j at unknown
+    nnodes + 1 + j as the description does)."""
+    out = [title]
+    for key, card in desc['models'].items():
+        typ = 'nmos' if card.get('nmos') else 'pmos'
+        ps = ' '.join('%s=%r' % (k, float(v)) for k, v in card.items() if k not in ('nmos', 'pmos'))
+        out.append('.model %s %s level=54 %s' % (key, typ, ps))
+    for k, (key, d, g, s, b, ip) in enumerate(desc['mos']):
+        ps = ' '.join('%s=%r' % (n, float(v)) for n, v in ip.items())
+        out.append('m%d %d %d %d %d %s %s' % (k, d, g, s, b, key, ps))
+    for k, (p, n, c) in enumerate(desc.get('caps', [])):
+        out.append('c%d %d %d %r' % (k, p, n, float(c)))
+    for k, (p, n, r) in enumerate(desc.get('res', [])):
+        out.append('r%d %d %d %r' % (k, p, n, float(r)))
+    for k in reversed(range(len(desc['vsrc']))):
+        p, n, dc, pulse = desc['vsrc'][k]
+        line = 'v%d %d %d dc %r' % (k, p, n, float(dc))
+        if pulse is not None:
+            line += ' pulse(%s)' % ' '.join(repr(float(x)) for x in pulse)
+        out.append(line)
+    ics = ' '.join('v(%d)=%r' % (k, float(v)) for k, v in desc.get('ic', {}).items())
+    if ics:
+        out.append('.ic ' + ics)
+    o = desc.get('options', {})
+    if o:
+        out.append('.options ' + ' '.join('%s=%s' % (k, ('gear' if v == 2 else 'trap')
+                                                      if k == 'method' else repr(v))
+                                          for k, v in o.items()))
+    out.append('.tran %r %r' % (float(desc['tstep']), float(desc['tstop'])))
+    out.append('.end')
+    return '\n'.join(out) + '\n'
