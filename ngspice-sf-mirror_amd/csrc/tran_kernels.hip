@@ -549,23 +549,17 @@ __device__ void op_memory(const TranDev &D, int c, const CktCtl *K, int what)
     }
 }
 
-/* block-wide minimum of one value per thread */
-__device__ double block_min(double v, double *red)
+/* minimum of one value per lane over the step kernel's workgroup, which
+ * is one wavefront: register exchanges, no LDS (min is exact in any order) */
+__device__ double wave_min(double v)
 {
-    const int tid = threadIdx.x;
-    red[tid] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = fmin(red[tid], red[tid + s]);
-        __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v = fmin(v, __shfl_xor(v, s));
+    return v;
 }
 
 /* LTE over every charge state of the circuit (CAPtrunc + BSIM4trunc) */
-__device__ double circuit_terr(const TranDev &D, const CktCtl *K, int c, double *red)
+__device__ double circuit_terr(const TranDev &D, const CktCtl *K, int c)
 {
     const int rb = K->rb, order = K->order;
     const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
@@ -591,7 +585,7 @@ __device__ double circuit_terr(const TranDev &D, const CktCtl *K, int c, double 
         const double c0 = Q[lvl(rb, 0) * lv + cnm], c1 = Q[lvl(rb, 1) * lv + cnm];
         m = fmin(m, terr_one(q0, q1, q2, q3, c0, c1, K, D));
     }
-    return block_min(m, red);
+    return wave_min(m);
 }
 
 /* copy state level `from` to level `to` for every state of the circuit */
@@ -873,6 +867,7 @@ __device__ __forceinline__ double rhs_value(uint32_t code, double mv, const doub
  * dozen entries per level, and a one-wave workgroup makes each level's
  * barrier nearly free while many circuits share a compute unit */
 constexpr int STEP_THREADS = 64;
+static_assert(STEP_THREADS == 64, "the step kernel's reductions assume one wavefront");
 /* assembly leaves whose loads are issued together, per lane */
 constexpr int ASM_UNROLL = 4;
 
@@ -880,7 +875,6 @@ constexpr int ASM_UNROLL = 4;
 __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
 {
     extern __shared__ double sm[];
-    __shared__ double red[STEP_THREADS];
     __shared__ int s_flag[4];
     __shared__ int s_act;
 
@@ -1211,7 +1205,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         __syncthreads();
         act = s_act;
         if (act == ACT_LTE1) {
-            const double t1 = circuit_terr(D, K, c, red);
+            const double t1 = circuit_terr(D, K, c);
             if (tid == 0) {
                 const double nd = fmin(2 * K->delta, t1);
 
@@ -1229,7 +1223,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
             __syncthreads();
             act = s_act;
             if (act == 30) {
-                const double t2 = circuit_terr(D, K, c, red);
+                const double t2 = circuit_terr(D, K, c);
                 if (tid == 0) {
                     const double nd = fmin(2 * K->delta, t2);
                     if (nd <= 1.05 * K->delta) K->order = 1;
