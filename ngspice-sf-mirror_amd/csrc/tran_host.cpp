@@ -117,7 +117,6 @@ struct ngmi_batch {
     int n_mos_blk = 0, n_step_blk = 0;
     size_t lds = 0;
     int *done_count = nullptr;
-    unsigned long long *dcounters = nullptr;
     /* kernel timing of the last run (events on the launch stream) */
     bool profile = false;
     double mos_ms = 0, step_ms = 0, guess_ms = 0;
@@ -251,7 +250,7 @@ struct Plan {
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
-    std::vector<int> perm, ic_row, nbr;
+    std::vector<int> perm, unk_pos, ic_row, nbr;
     std::vector<int> mos_pos, rmos_pos, diag_slot;
     int NA = 0, NR = 0;
 };
@@ -629,6 +628,8 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     P.a_code = std::move(a_code); P.r_code = std::move(r_code);
     P.a_leaf = std::move(a_leaf); P.a_comb = std::move(a_comb);
     P.r_leaf = std::move(r_leaf); P.r_comb = std::move(r_comb);
+    P.unk_pos.assign(N + 1, N);
+    for (int a = 0; a < N; a++) P.unk_pos[perm[a]] = a;
     P.perm = std::move(perm); P.ic_row = std::move(ic_row);
     return NGMI_OK;
 }
@@ -759,6 +760,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     UP(P.bw_ent, bw_ent, int, ip);
     UP(P.bw_term, bw_term, uint32_t, up);
     UP(P.perm, perm_unknown, int, ip);
+    UP(P.unk_pos, unk_pos, int, ip);
     UP(P.mos_pos, mos_pos, int, ip);
     UP(P.diag_slot, diag_slot, int, ip);
     UP(P.rmos_pos, rmos_pos, int, ip);
@@ -797,7 +799,6 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     if (alloc(b, (size_t)C * b->cap_pts * D.nprobe, &dp) != NGMI_OK) return fail(nullptr);
     D.out_x = dp;
     D.done_count = b->done_count;
-    D.counters = b->dcounters;
     /* LDS: lu, z, capacitor and source values, partial sums (doubles), then
      * the three level tables (ints) */
     G->lds = sizeof(double) * ((size_t)P.nslots + 1 + N + 1 + 2 * (size_t)NCAP + NVS + P.n_apart +
@@ -1006,7 +1007,7 @@ ngmi_batch *ngmi_batch_new(ngmi_circuit *const *ckts, int C, const int *probe, i
         b->where[c] = make_int2(g, (int)members[g].size());
         members[g].push_back(c);
     }
-    if (alloc(b, 1, &b->done_count) != NGMI_OK || alloc(b, 2, &b->dcounters) != NGMI_OK)
+    if (alloc(b, 1, &b->done_count) != NGMI_OK)
         return bail(nullptr);
     std::vector<TranDev> Ds;
     std::vector<B4Tab> tabs;
@@ -1141,7 +1142,6 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         NGMI_HIP(hipStreamSynchronize(s));
     }
     NGMI_HIP(hipMemsetAsync(b->done_count, 0, sizeof(int), s));
-    NGMI_HIP(hipMemsetAsync(b->dcounters, 0, 2 * sizeof(unsigned long long), s));
     const TopoGroup &G0 = *b->groups[0];
     const bool one = b->groups.size() == 1;
     const B4Tab *tab0 = set_table(G0.set);
@@ -1245,7 +1245,16 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         (void)hipEventDestroy(join);
         (void)hipStreamDestroy(s2);
     }
-    NGMI_HIP(hipMemcpy(b->counters, b->dcounters, sizeof(b->counters), hipMemcpyDeviceToHost));
+    /* evaluations and passes from the circuits' own counters */
+    b->counters[0] = b->counters[1] = 0;
+    for (TopoGroup *G : b->groups) {
+        std::vector<CktCtl> k(G->C);
+        NGMI_HIP(hipMemcpy(k.data(), G->D.ctl, sizeof(CktCtl) * G->C, hipMemcpyDeviceToHost));
+        for (const CktCtl &x : k) {
+            b->counters[0] += (unsigned long long)x.passes * G->NM;
+            b->counters[1] += (unsigned long long)x.passes;
+        }
+    }
     if (passes_out) *passes_out = pass;
     if (rc != NGMI_OK) return rc;
     return done >= C ? NGMI_OK : NGMI_E_ITERLIM;

@@ -870,6 +870,8 @@ constexpr int STEP_THREADS = 64;
 static_assert(STEP_THREADS == 64, "the step kernel's reductions assume one wavefront");
 /* assembly leaves whose loads are issued together, per lane */
 constexpr int ASM_UNROLL = 4;
+/* unknowns (and device votes) a lane loads together in the convergence test */
+constexpr int CONV_CHUNK = 8;
 
 /* circuit c of the topology D, one workgroup */
 __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
@@ -899,8 +901,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         z[N] = 0.0;
     }
     if (tid == 0) {
-        atomicAdd(D.counters, (unsigned long long)D.NM);
-        atomicAdd(D.counters + 1, 1ull);
+        K->passes++;   /* this circuit's Newton passes (its BSIM4 evaluations: passes x NM) */
         if (K->guess == 1) K->guess = 2;   /* this iteration starts from the learned guess */
     }
 
@@ -1089,17 +1090,41 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                    z[a] = acc / piv;
                });
     /* new iterate; NIconvTest (niconv.c:21-90) and the device votes */
-    for (int a = tid; a < N; a += nt) xn[D.perm_unknown[a]] = z[a];
+    /* unknown u is z[unk_pos[u]]; the loads of a lane's unknowns (and of
+     * its device votes) are independent, so each chunk issues them all
+     * before it waits */
     if (tid == 0) xn[0] = 0.0;
-    int vote = 0;
-    for (int j = tid; j < D.NM; j += nt) vote |= D.noncon[(int64_t)c * D.NM + j];
     int conv = 0;
-    for (int a = tid; a < N; a += nt) {
-        const int u = D.perm_unknown[a];
-        const double nw = z[a], old = xo[u];
-        if (isnan(nw)) { conv = 1; continue; }
-        const double tol = D.reltol * fmax(fabs(old), fabs(nw)) + (u <= D.NN ? D.vntol : D.abstol);
-        if (fabs(nw - old) > tol) conv = 1;
+    for (int u0 = tid + 1; u0 <= N; u0 += CONV_CHUNK * nt) {
+        int pos[CONV_CHUNK];
+        double old[CONV_CHUNK];
+#pragma unroll
+        for (int k = 0; k < CONV_CHUNK; k++) {
+            const int u = u0 + k * nt;
+            if (u <= N) {
+                pos[k] = gload(D.unk_pos + u);
+                old[k] = xo[u];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CONV_CHUNK; k++) {
+            const int u = u0 + k * nt;
+            if (u > N) continue;
+            const double nw = z[pos[k]];
+            xn[u] = nw;
+            if (isnan(nw)) { conv = 1; continue; }
+            const double tol = D.reltol * fmax(fabs(old[k]), fabs(nw)) + (u <= D.NN ? D.vntol : D.abstol);
+            if (fabs(nw - old[k]) > tol) conv = 1;
+        }
+    }
+    int vote = 0;
+    const int *nc = D.noncon + (int64_t)c * D.NM;
+    for (int j0 = tid; j0 < D.NM; j0 += CONV_CHUNK * nt) {
+        int v[CONV_CHUNK];
+#pragma unroll
+        for (int k = 0; k < CONV_CHUNK; k++) v[k] = (j0 + k * nt < D.NM) ? gload(nc + j0 + k * nt) : 0;
+#pragma unroll
+        for (int k = 0; k < CONV_CHUNK; k++) vote |= v[k];
     }
     if (vote) s_flag[1] = 1;
     if (conv) s_flag[2] = 1;

@@ -36,6 +36,7 @@ struct CktCtl {
     int phase, status, firsttime, iterno;
     int nbreaks, rb, xsel, pad;
     int64_t npts, accepted, rejected, iters;
+    int64_t passes;   /* Newton passes this circuit took part in */
     /* learned initial guess: learned = enabled for this circuit; guess = 0
      * none, 1 features requested for the step being started, 2 guess applied
      * (first Newton iteration runs from it); history ring of the last 4
@@ -127,6 +128,7 @@ struct TranDev {
     const uint32_t *fw_term;      /* (L slot, z index) */
     const int *bw_lev, *bw_ent;   /* backward solve: [ne][16] target row, y = diag slot */
     const uint32_t *bw_term;      /* (U slot, z index) */
+    const int *unk_pos;       /* [N+1]: permuted column of unknown u (1-based; [0] = N) */
     const int *perm_unknown;  /* [2N]: [a] unknown (1-based) solved in permuted column a;
                                  [N + a] original row assembled into permuted row a */
     const int *probe;         /* [nprobe] unknowns written at each accepted point */
@@ -147,7 +149,6 @@ struct TranDev {
     int *done_count;          /* circuits finished or failed */
     double *hist;             /* [4][C][N+1] accepted node vectors (learned guess) */
     const int *nbr;           /* [NN+1][8] neighbour nodes (0 = none) */
-    unsigned long long *counters;   /* [0] MOSFET evaluations, [1] circuit Newton passes */
 };
 
 }  // namespace ngmi
