@@ -250,6 +250,7 @@ struct Plan {
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
+    int n_aleaf1 = 0, n_rleaf1 = 0;
     std::vector<int> perm, unk_pos, ic_row, nbr;
     std::vector<int> mos_pos, rmos_pos, diag_slot;
     int NA = 0, NR = 0;
@@ -418,13 +419,13 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     /* term lists: two 16-bit indices per word, padded to whole groups of
      * four with a pair whose product is exactly zero (lu[nslots] and z[N]
      * hold 0).  An entry is a 64-byte record {target | g << 16, y, begin,
-     * end} + three groups: its first g (1..3) groups inline, the rest at
+     * end} + three groups: its first g (0..3) groups inline, the rest at
      * [begin, end) in uint4 units of dst. */
     auto put_terms = [](std::vector<uint32_t> &dst, const std::vector<int2> &t, uint32_t pad,
                         std::vector<int> &rec, int target, int y) {
         std::vector<uint32_t> w;
         for (const int2 &x : t) w.push_back((uint32_t)x.x | ((uint32_t)x.y << 16));
-        const int g = std::max(1, std::min(PROG_INLINE, (int)(w.size() + 3) / 4));
+        const int g = std::min(PROG_INLINE, (int)(w.size() + 3) / 4);
         while ((int)w.size() < 4 * g || w.size() % 4) w.push_back(pad);
         rec.push_back(target | (g << 16));
         rec.push_back(y);
@@ -435,23 +436,28 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     };
     const uint32_t pad_lu = (uint32_t)nslots | ((uint32_t)nslots << 16);
     const uint32_t pad_z = (uint32_t)nslots | ((uint32_t)N << 16);
-    std::vector<int> f_lev(nlev + 1, 0), f_ent;
+    /* entries that change nothing (a U or diagonal entry with no terms: its
+     * assembled value is final) are left out, and levels left empty with
+     * them are dropped */
+    std::vector<int> f_lev, f_ent;
     std::vector<uint32_t> f_term;
     int nterms = 0;
     {
         std::vector<std::vector<int>> by(nlev);
-        for (int k = 0; k < (int)ents.size(); k++) by[ents[k].lev].push_back(k);
+        for (int k = 0; k < (int)ents.size(); k++)
+            if (!ents[k].terms.empty() || ents[k].piv >= 0) by[ents[k].lev].push_back(k);
         for (int L = 0; L < nlev; L++) {
-            f_lev[L] = (int)(f_ent.size() / PROG_INTS);
+            if (by[L].empty()) continue;
+            f_lev.push_back((int)(f_ent.size() / PROG_INTS));
             for (int k : by[L]) {
                 const Ent &e = ents[k];
                 put_terms(f_term, e.terms, pad_lu, f_ent, e.t, e.piv);
                 nterms += (int)e.terms.size();
             }
         }
-        f_lev[nlev] = (int)(f_ent.size() / PROG_INTS);
+        P.nlev_f = (int)f_lev.size();
+        f_lev.push_back((int)(f_ent.size() / PROG_INTS));
     }
-    P.nlev_f = nlev;
     P.nterms = nterms;
 
     /* ---- forward (unit L) and backward programs */
@@ -466,9 +472,15 @@ int analyze(const ngmi_circuit &c0, Plan &P)
             lv[a] = L;
             nl = std::max(nl, L + 1);
         }
+        /* rows of L without terms leave z unchanged: left out */
         std::vector<std::vector<int>> by(nl);
-        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
+        for (int a = 0; a < N; a++) {
+            bool any = false;
+            for (int k = 0; k < a && !any; k++) any = S[a][k];
+            if (any) by[lv[a]].push_back(a);
+        }
         for (int L = 0; L < nl; L++) {
+            if (by[L].empty()) continue;
             fw_lev.push_back((int)(fw_ent.size() / PROG_INTS));
             for (int a : by[L]) {
                 std::vector<int2> t;
@@ -477,8 +489,8 @@ int analyze(const ngmi_circuit &c0, Plan &P)
                 put_terms(fw_term, t, pad_z, fw_ent, a, -1);
             }
         }
+        P.nlev_fw = (int)fw_lev.size();
         fw_lev.push_back((int)(fw_ent.size() / PROG_INTS));
-        P.nlev_fw = nl;
     }
     {
         std::vector<int> lv(N, 0);
@@ -552,9 +564,14 @@ int analyze(const ngmi_circuit &c0, Plan &P)
      * so a lane finds a leaf's first values without reading its record
      * first; the further groups follow all first groups.  A leaf is a
      * 32-byte record {dest, begin, end, flag} + a copy of its first group,
-     * with [begin, end) its further groups in uint4 units of code. */
-    auto layout = [](const std::vector<Leaf> &leaves, std::vector<uint32_t> &code,
+     * with [begin, end) its further groups in uint4 units of code.  The
+     * single-group leaves come first (returns their count). */
+    auto layout = [](std::vector<Leaf> leaves, std::vector<uint32_t> &code,
                      std::vector<int4> &leaf) {
+        std::stable_partition(leaves.begin(), leaves.end(),
+                              [](const Leaf &lf) { return lf.w.size() == 4; });
+        int singles = 0;
+        for (const Leaf &lf : leaves) singles += lf.w.size() == 4;
         for (const Leaf &lf : leaves) code.insert(code.end(), lf.w.begin(), lf.w.begin() + 4);
         for (const Leaf &lf : leaves) {
             const int b = (int)(code.size() / 4);
@@ -562,6 +579,7 @@ int analyze(const ngmi_circuit &c0, Plan &P)
             leaf.push_back(make_int4(lf.dest, b, (int)(code.size() / 4), lf.flag));
             leaf.push_back(make_int4((int)lf.w[0], (int)lf.w[1], (int)lf.w[2], (int)lf.w[3]));
         }
+        return singles;
     };
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
@@ -576,8 +594,8 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         split(rhs_codes[rowp[a]], a, 0, N, n_rpart, r_leaves, r_comb);
         if (perm[N + a] <= NN && P.ic_nodes[perm[N + a]]) ic_row.push_back(a);
     }
-    layout(a_leaves, a_code, a_leaf);
-    layout(r_leaves, r_code, r_leaf);
+    P.n_aleaf1 = layout(std::move(a_leaves), a_code, a_leaf);
+    P.n_rleaf1 = layout(std::move(r_leaves), r_code, r_leaf);
     P.n_aleaf = (int)a_leaf.size() / 2; P.n_acomb = (int)a_comb.size();
     P.n_rleaf = (int)r_leaf.size() / 2; P.n_rcomb = (int)r_comb.size(); P.n_icrow = (int)ic_row.size();
     P.n_apart = n_apart; P.n_rpart = n_rpart;
@@ -721,6 +739,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.nslots = P.nslots; D.nlev_f = G->nlev_f; D.nlev_fw = G->nlev_fw; D.nlev_bw = G->nlev_bw;
     D.n_aleaf = P.n_aleaf; D.n_acomb = P.n_acomb; D.n_rleaf = P.n_rleaf; D.n_rcomb = P.n_rcomb;
     D.n_apart = P.n_apart; D.n_rpart = P.n_rpart; D.n_icrow = P.n_icrow;
+    D.n_aleaf1 = P.n_aleaf1; D.n_rleaf1 = P.n_rleaf1;
     D.nprobe = (int)G->probe.size();
     D.cap_pts = b->cap_pts;
     D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
@@ -790,6 +809,28 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.bak_s = dp;
     if (alloc(b, (size_t)C * P.NA, &dp) != NGMI_OK) return fail(nullptr);
     D.contrib = dp;
+    {
+        /* the values the BSIM4 kernel never writes: resistor conductances
+         * and source +-1 entries, signed, and 0 at the padding codes; they
+         * stay for the whole run */
+        const size_t chunk = 256;
+        std::vector<double> row;
+        for (size_t c0 = 0; c0 < (size_t)C; c0 += chunk) {
+            const size_t c1 = std::min<size_t>(C, c0 + chunk);
+            row.assign((c1 - c0) * P.NA, 0.0);
+            for (size_t c = c0; c < c1; c++)
+                for (int p = 0; p < P.NA; p++) {
+                    const uint32_t code = P.a_code[p], type = code >> CT_TYPE_SHIFT;
+                    double v = 0.0;
+                    if (type == CT_RES) v = res_g[c * NRES + (code & CT_INDEX_MASK)];
+                    else if (type == CT_ONE) v = 1.0;
+                    else continue;
+                    row[(c - c0) * P.NA + p] = (code & CT_NEG) ? -v : v;
+                }
+            NGMI_HIP(hipMemcpy(dp + c0 * P.NA, row.data(), row.size() * sizeof(double),
+                               hipMemcpyHostToDevice));
+        }
+    }
     if (alloc(b, (size_t)C * P.NR, &dp) != NGMI_OK) return fail(nullptr);
     D.rcontrib = dp;
     if (alloc(b, (size_t)cnm, &ip) != NGMI_OK) return fail(nullptr);

@@ -714,7 +714,7 @@ __device__ __forceinline__ double sub_terms(double acc, uint4 w, const double *x
 }
 
 /* a 64-byte LU or substitution record: {target | g << 16, y, begin, end},
- * then three term groups of which the first g are the entry's */
+ * then three term groups of which the first g (0..3) are the entry's */
 struct Prog {
     int4 h;
     uint4 g0, g1, g2;
@@ -736,7 +736,7 @@ __device__ __forceinline__ double lu_row_terms(double acc, const Prog &r, const 
                                                const double *x, const double *y)
 {
     const int g = r.h.x >> 16;
-    acc = sub_terms(acc, r.g0, x, y);
+    if (g > 0) acc = sub_terms(acc, r.g0, x, y);
     if (g > 1) acc = sub_terms(acc, r.g1, x, y);
     if (g > 2) acc = sub_terms(acc, r.g2, x, y);
     for (int q = r.h.z; q < r.h.w; q += 4) {
@@ -763,16 +763,18 @@ __device__ __forceinline__ void run_levels(const int *lev, const int4 *ent, int 
 {   /* lev: the level table staged in LDS */
     if (nlev <= 0) return;
     const int total = lev[nlev];
+    if (total <= lev[0]) return;
     int e = lev[0] + tid;
-    Prog p0, p1;
-    if (e < total) p0 = load_prog(ent, e);
-    if (e + nt < total) p1 = load_prog(ent, e + nt);
+    /* unconditional prefetches (clamped to the last entry), so that the
+     * compiler's wait counts stay exact */
+    Prog p0 = load_prog(ent, min(e, total - 1));
+    Prog p1 = load_prog(ent, min(e + nt, total - 1));
     for (int L = 0; L < nlev; L++) {
         const int end = lev[L + 1];
         while (e < end) {
             const Prog cur = p0;
             p0 = p1;
-            if (e + 2 * nt < total) p1 = load_prog(ent, e + 2 * nt);
+            p1 = load_prog(ent, min(e + 2 * nt, total - 1));
             body(cur);
             e += nt;
         }
@@ -780,27 +782,31 @@ __device__ __forceinline__ void run_levels(const int *lev, const int4 *ent, int 
     }
 }
 
-/* Assembly leaves: lane t sums leaves t + k nt.  Leaf e's first values are
- * group e of the contribution array, so the next U leaves' records and
- * first values load together, while the current U leaves are summed.  The
- * second group of each current leaf that has one is requested before the
- * next leaves' loads, so waiting for it does not wait for them (loads
- * complete in order). */
-template <int U, class F>
+/* Assembly leaves e_begin .. n-1: lane t sums leaves e_begin + t + k nt.
+ * Leaf e's first values are group e of the contribution array, so the next
+ * U leaves' records and first values load together, while the current U
+ * leaves are summed.  The single-group leaves come first and run with
+ * MULTI = false: nothing else is loaded, so each wait is for loads issued
+ * one batch earlier.  With MULTI, the second group of each current leaf
+ * that has one is requested before the next leaves' loads, so waiting for
+ * it does not wait for them (loads complete in order). */
+template <int U, bool MULTI, class F>
 __device__ __forceinline__ void run_leaves(const int4 *leaf, const double4 *val, const uint4 *code,
-                                           int n, int tid, int nt, F body)
+                                           int e_begin, int n, int tid, int nt, F body)
 {
+    if (n <= e_begin) return;
+    /* prefetch loads are unconditional (past-the-end indices clamped to the
+     * last leaf): the compiler can then count exactly how many loads are
+     * outstanding and wait only for the ones a leaf needs */
     Rec rb[U];
     double4 vb[U];
 #pragma unroll
     for (int k = 0; k < U; k++) {
-        const int e = tid + k * nt;
-        if (e < n) {
-            rb[k] = load_rec(leaf, e);
-            vb[k] = gload4(val + e);
-        }
+        const int e = min(e_begin + tid + k * nt, n - 1);
+        rb[k] = load_rec(leaf, e);
+        vb[k] = gload4(val + e);
     }
-    for (int e0 = tid; e0 < n; e0 += U * nt) {
+    for (int e0 = e_begin + tid; e0 < n; e0 += U * nt) {
         Rec rc[U];
         double4 vc[U];
         uint4 xw[U];
@@ -809,18 +815,16 @@ __device__ __forceinline__ void run_leaves(const int4 *leaf, const double4 *val,
         for (int k = 0; k < U; k++) {
             rc[k] = rb[k];
             vc[k] = vb[k];
-            if (e0 + k * nt < n && rc[k].h.y < rc[k].h.z) {
+            if (MULTI && e0 + k * nt < n && rc[k].h.y < rc[k].h.z) {
                 xw[k] = gload(code + rc[k].h.y);
                 xv[k] = gload4(val + rc[k].h.y);
             }
         }
 #pragma unroll
         for (int k = 0; k < U; k++) {
-            const int e = e0 + (U + k) * nt;
-            if (e < n) {
-                rb[k] = load_rec(leaf, e);
-                vb[k] = gload4(val + e);
-            }
+            const int e = min(e0 + (U + k) * nt, n - 1);
+            rb[k] = load_rec(leaf, e);
+            vb[k] = gload4(val + e);
         }
 #pragma unroll
         for (int k = 0; k < U; k++)
@@ -828,22 +832,18 @@ __device__ __forceinline__ void run_leaves(const int4 *leaf, const double4 *val,
     }
 }
 
-/* value of one matrix contribution code (MOS stamp = mv, the value the
- * BSIM4 kernel left at this code's position; capacitor companion
- * conductance, resistor conductance, source +-1, padding 0) */
-__device__ __forceinline__ double mat_value(const TranDev &D, uint32_t code, double mv, int c,
-                                            const double *capg, bool icz)
+/* value of one matrix contribution code: mv, the value at the code's
+ * position in the contribution array (the MOS stamp the BSIM4 kernel wrote
+ * there, or the signed resistor conductance, source +-1 or padding 0 put
+ * there when the batch was built), except a capacitor companion
+ * conductance, which the step kernel computes each pass.  No global load
+ * here: one in a branch would make every use wait for all outstanding
+ * loads, the prefetched leaves' included. */
+__device__ __forceinline__ double mat_value(uint32_t code, double mv, const double *capg, bool icz)
 {
     if ((code & CT_ICZ) && icz) return 0.0;
-    const uint32_t idx = code & CT_INDEX_MASK;
-    double v;
-    switch (code >> CT_TYPE_SHIFT) {
-    case CT_MOS: v = mv; break;
-    case CT_CAP: v = capg[idx]; break;
-    case CT_RES: v = gload(D.res_g + (int64_t)c * D.NRES + idx); break;
-    case CT_ONE: v = 1.0; break;
-    default: v = 0.0; break;
-    }
+    if ((code >> CT_TYPE_SHIFT) != CT_CAP) return mv;
+    const double v = capg[code & CT_INDEX_MASK];
     return (code & CT_NEG) ? -v : v;
 }
 
@@ -981,32 +981,44 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     const double4 *RV = reinterpret_cast<const double4 *>(D.rcontrib + (int64_t)c * D.NR);
     /* the sum of one leaf: its first group (codes from the record, values
      * already loaded), then its further groups in order */
-    run_leaves<ASM_UNROLL>(D.a_leaf, AV, AC, D.n_aleaf, tid, nt,
-                           [&](const Rec &lf, const double4 &m0, const uint4 &w1, const double4 &m1) {
+    auto a_leaf_sum = [&](const Rec &lf, const double4 &m0, const uint4 &w1, const double4 &m1) {
         double acc = 0.0;
-        acc += mat_value(D, lf.g.x, m0.x, c, capg, icz);
-        acc += mat_value(D, lf.g.y, m0.y, c, capg, icz);
-        acc += mat_value(D, lf.g.z, m0.z, c, capg, icz);
-        acc += mat_value(D, lf.g.w, m0.w, c, capg, icz);
+        acc += mat_value(lf.g.x, m0.x, capg, icz);
+        acc += mat_value(lf.g.y, m0.y, capg, icz);
+        acc += mat_value(lf.g.z, m0.z, capg, icz);
+        acc += mat_value(lf.g.w, m0.w, capg, icz);
         if (lf.h.y < lf.h.z) {
-            acc += mat_value(D, w1.x, m1.x, c, capg, icz);
-            acc += mat_value(D, w1.y, m1.y, c, capg, icz);
-            acc += mat_value(D, w1.z, m1.z, c, capg, icz);
-            acc += mat_value(D, w1.w, m1.w, c, capg, icz);
+            acc += mat_value(w1.x, m1.x, capg, icz);
+            acc += mat_value(w1.y, m1.y, capg, icz);
+            acc += mat_value(w1.z, m1.z, capg, icz);
+            acc += mat_value(w1.w, m1.w, capg, icz);
         }
         for (int q = lf.h.y + 1; q < lf.h.z; q++) {
             const uint4 w = gload(AC + q);
             const double4 m = gload4(AV + q);
-            acc += mat_value(D, w.x, m.x, c, capg, icz);
-            acc += mat_value(D, w.y, m.y, c, capg, icz);
-            acc += mat_value(D, w.z, m.z, c, capg, icz);
-            acc += mat_value(D, w.w, m.w, c, capg, icz);
+            acc += mat_value(w.x, m.x, capg, icz);
+            acc += mat_value(w.y, m.y, capg, icz);
+            acc += mat_value(w.z, m.z, capg, icz);
+            acc += mat_value(w.w, m.w, capg, icz);
         }
         if (lf.h.x < D.nslots) lu[lf.h.x] = ((lf.h.w & 1) && icz) ? 1.0 : acc;
         else part[lf.h.x - D.nslots] = acc;
+    };
+    /* single-group leaves: no further groups, and no code that could load
+     * them (a load anywhere in the body makes the compiler wait for all
+     * outstanding loads after it, the prefetched leaves' included) */
+    run_leaves<ASM_UNROLL, false>(D.a_leaf, AV, AC, 0, D.n_aleaf1, tid, nt,
+                                  [&](const Rec &lf, const double4 &m0, const uint4 &, const double4 &) {
+        double acc = 0.0;
+        acc += mat_value(lf.g.x, m0.x, capg, icz);
+        acc += mat_value(lf.g.y, m0.y, capg, icz);
+        acc += mat_value(lf.g.z, m0.z, capg, icz);
+        acc += mat_value(lf.g.w, m0.w, capg, icz);
+        if (lf.h.x < D.nslots) lu[lf.h.x] = ((lf.h.w & 1) && icz) ? 1.0 : acc;
+        else part[lf.h.x - D.nslots] = acc;
     });
-    run_leaves<ASM_UNROLL>(D.r_leaf, RV, RC, D.n_rleaf, tid, nt,
-                           [&](const Rec &lf, const double4 &m0, const uint4 &w1, const double4 &m1) {
+    run_leaves<ASM_UNROLL, true>(D.a_leaf, AV, AC, D.n_aleaf1, D.n_aleaf, tid, nt, a_leaf_sum);
+    auto r_leaf_sum = [&](const Rec &lf, const double4 &m0, const uint4 &w1, const double4 &m1) {
         double acc = 0.0;
         acc += rhs_value(lf.g.x, m0.x, capq, vsv);
         acc += rhs_value(lf.g.y, m0.y, capq, vsv);
@@ -1028,7 +1040,18 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         }
         if (lf.h.x < N) z[lf.h.x] = acc;
         else part[D.n_apart + lf.h.x - N] = acc;
+    };
+    run_leaves<ASM_UNROLL, false>(D.r_leaf, RV, RC, 0, D.n_rleaf1, tid, nt,
+                                  [&](const Rec &lf, const double4 &m0, const uint4 &, const double4 &) {
+        double acc = 0.0;
+        acc += rhs_value(lf.g.x, m0.x, capq, vsv);
+        acc += rhs_value(lf.g.y, m0.y, capq, vsv);
+        acc += rhs_value(lf.g.z, m0.z, capq, vsv);
+        acc += rhs_value(lf.g.w, m0.w, capq, vsv);
+        if (lf.h.x < N) z[lf.h.x] = acc;
+        else part[D.n_apart + lf.h.x - N] = acc;
     });
+    run_leaves<ASM_UNROLL, true>(D.r_leaf, RV, RC, D.n_rleaf1, D.n_rleaf, tid, nt, r_leaf_sum);
     __syncthreads();
     for (int e = tid; e < D.n_acomb; e += nt) {
         const int4 cb = gload(D.a_comb + e);

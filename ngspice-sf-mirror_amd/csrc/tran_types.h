@@ -111,6 +111,7 @@ struct TranDev {
      * at [s * NM + j], -1 where the row or column is ground). */
     const int4 *a_leaf, *a_comb, *r_leaf, *r_comb;
     int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
+    int n_aleaf1, n_rleaf1;   /* leading leaves with a single group */
     const uint32_t *a_code;   /* contributions */
     const uint32_t *r_code;
     const int *ic_row;        /* permuted rows of .ic nodes */
@@ -119,7 +120,7 @@ struct TranDev {
      * high << 16), grouped four to a uint4 and padded with pairs whose
      * product is exactly zero (LDS lu[nslots] = z[N] = 0).  An entry is a
      * 64-byte record: four ints {target | g << 16, y, begin, end}, then
-     * three term groups of which the first g (1..3) are its own; the term
+     * three term groups of which the first g (0..3) are its own; the term
      * range [begin, end) holds the further groups, in uint4 units. */
     const int *f_lev;         /* [nlev_f+1] -> entries */
     const int *f_ent;         /* [ne][16]: y = pivot slot (-1 for U) */
