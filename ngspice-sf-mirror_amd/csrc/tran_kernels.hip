@@ -730,8 +730,13 @@ __device__ __forceinline__ Prog load_prog(const int4 *base, int e)
     return r;
 }
 
+/* further term groups loaded per batch by the one lane that owns a long
+ * row (a supply node's row has ~45 groups on the adder) */
+constexpr int TERM_BATCH = 8;
+
 /* acc minus the record's term groups, in order: the inline ones, then the
- * further groups [h.z, h.w), loaded up to four at a time before use */
+ * further groups [h.z, h.w), TERM_BATCH at a time, loaded unconditionally
+ * (clamped to the last group) so that the wait counts stay exact */
 __device__ __forceinline__ double lu_row_terms(double acc, const Prog &r, const uint4 *T,
                                                const double *x, const double *y)
 {
@@ -739,13 +744,12 @@ __device__ __forceinline__ double lu_row_terms(double acc, const Prog &r, const 
     if (g > 0) acc = sub_terms(acc, r.g0, x, y);
     if (g > 1) acc = sub_terms(acc, r.g1, x, y);
     if (g > 2) acc = sub_terms(acc, r.g2, x, y);
-    for (int q = r.h.z; q < r.h.w; q += 4) {
-        uint4 w[4];
+    for (int q = r.h.z; q < r.h.w; q += TERM_BATCH) {
+        uint4 w[TERM_BATCH];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (q + k < r.h.w) w[k] = gload(T + q + k);
+        for (int k = 0; k < TERM_BATCH; k++) w[k] = gload(T + min(q + k, r.h.w - 1));
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < TERM_BATCH; k++)
             if (q + k < r.h.w) acc = sub_terms(acc, w[k], x, y);
     }
     return acc;
