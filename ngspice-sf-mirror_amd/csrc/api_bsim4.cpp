@@ -239,8 +239,10 @@ int ngmi_bsim4_set_finalize(ngmi_bsim4_set *s)
             double *dst = &itab[(size_t)s->code[f] * n];
             for (size_t k = 0; k < n; k++) dst[k] = s->rows[k * F + f];
         } else {
-            double *dst = &mtab[(size_t)(~s->code[f]) * nm];
-            for (int m = 0; m < nm; m++) dst[m] = s->rows[firstOf[m] * F + f];
+            /* model-major: one card's per-model values are contiguous, so
+             * the few cards a workgroup reads stay in its L1 */
+            for (int m = 0; m < nm; m++)
+                mtab[(size_t)m * s->mcols + (~s->code[f])] = s->rows[firstOf[m] * F + f];
         }
     }
     /* one selector word for the whole set lets the kernels compile the
@@ -266,6 +268,7 @@ int ngmi_bsim4_set_finalize(ngmi_bsim4_set *s)
     s->tab.model_of = s->d_model_of;
     s->tab.n = (int64_t)n;
     s->tab.nm = nm;
+    s->tab.mcols = s->mcols;
     s->tab.code = s->d_code;
     s->final_ = true;
     return NGMI_OK;

@@ -39,7 +39,8 @@ int tune_get(int knob);
 /*
  * Device view of a BSIM4 evaluation table.  Each field is either a
  * per-instance column (itab[col * n + i]) or, when every instance of a
- * model card has the same value, a per-model column (mtab[col * nm + m]).
+ * model card has the same value, a per-model column (mtab[m * mcols +
+ * col], model-major so that one card's values share cache lines).
  * code[f] = col for a per-instance field and ~col for a per-model one; the
  * code array lives in device memory and is read with a scalar load next to
  * each use (a kernel-argument copy would pin ~190 SGPRs for the whole
@@ -50,7 +51,8 @@ struct B4Tab {
     const double *mtab;
     const int *model_of;
     int64_t n;       /* instances (row stride of itab) */
-    int nm;          /* models (row stride of mtab) */
+    int nm;          /* models */
+    int mcols;       /* per-model columns (row stride of mtab) */
     const int *code;
     int flags;       /* the selector word when every instance shares it, else -1 */
 };

@@ -39,7 +39,7 @@ struct TranPrm {
     __device__ __forceinline__ double operator()(int f) const
     {
         const int c = t.code[f];
-        return c >= 0 ? t.itab[(int64_t)c * t.n + i] : t.mtab[(int64_t)(~c) * t.nm + mi];
+        return c >= 0 ? t.itab[(int64_t)c * t.n + i] : t.mtab[(int64_t)mi * t.mcols + (~c)];
     }
 };
 
