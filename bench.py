@@ -34,10 +34,10 @@ FP64_PEAK_TFLOPS = 78.6
 # counters read for a device copy of known size (FETCH_SIZE x 2.00, WRITE_SIZE
 # x 1.00), divided by the evaluations the engine counted.  None where not
 # measured for the selected variant.
-MEASURED_HBM_BYTES_PER_EVAL = {('k_tran_mos', 2): 1805, ('k_bsim4_eval', 2): 1598}
+MEASURED_HBM_BYTES_PER_EVAL = {('k_tran_mos', 2): 1764, ('k_bsim4_eval', 2): 1595}
 # FP64 FLOP per evaluation from SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes
 # (FMA counted twice), same runs
-MEASURED_FLOP_PER_EVAL = {'k_tran_mos': 6614, 'k_bsim4_eval': 6566}
+MEASURED_FLOP_PER_EVAL = {'k_tran_mos': 6617, 'k_bsim4_eval': 6568}
 
 
 def log(*a):
