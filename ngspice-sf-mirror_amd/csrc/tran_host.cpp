@@ -22,6 +22,7 @@
  * circuit's Newton iteration.
  */
 #include <algorithm>
+#include <tuple>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -250,7 +251,8 @@ struct Plan {
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
-    int n_aleaf1 = 0, n_rleaf1 = 0;
+    int n_aleaf1 = 0, n_rleaf1 = 0, n_aplain = 0, n_rplain = 0;
+    std::vector<int> a_dest, r_dest;
     std::vector<int> perm, unk_pos, ic_row, nbr;
     std::vector<int> mos_pos, rmos_pos, diag_slot;
     int NA = 0, NR = 0;
@@ -564,14 +566,33 @@ int analyze(const ngmi_circuit &c0, Plan &P)
      * so a lane finds a leaf's first values without reading its record
      * first; the further groups follow all first groups.  A leaf is a
      * 32-byte record {dest, begin, end, flag} + a copy of its first group,
-     * with [begin, end) its further groups in uint4 units of code.  The
-     * single-group leaves come first (returns their count). */
+     * with [begin, end) its further groups in uint4 units of code.  Leaves
+     * come in three runs: plain ones (a single group whose values are all
+     * read from the contribution array as they are: no capacitor or source
+     * code, no .ic handling), whose destinations also go to dest so the
+     * step kernel reads 4 bytes per leaf instead of the record; other
+     * single-group leaves; multi-group leaves.  Returns the run lengths
+     * (plain, plain + single). */
     auto layout = [](std::vector<Leaf> leaves, std::vector<uint32_t> &code,
-                     std::vector<int4> &leaf) {
-        std::stable_partition(leaves.begin(), leaves.end(),
-                              [](const Leaf &lf) { return lf.w.size() == 4; });
-        int singles = 0;
-        for (const Leaf &lf : leaves) singles += lf.w.size() == 4;
+                     std::vector<int4> &leaf, std::vector<int> &dest, bool rhs) {
+        auto kind = [rhs](const Leaf &lf) {
+            if (lf.w.size() != 4) return 2;
+            if (lf.flag) return 1;
+            for (int k = 0; k < 4; k++) {
+                const uint32_t t = lf.w[k] >> CT_TYPE_SHIFT;
+                if (t == CT_CAP || (rhs && t == CT_VSRC) || (!rhs && (lf.w[k] & CT_ICZ))) return 1;
+            }
+            return 0;
+        };
+        std::stable_sort(leaves.begin(), leaves.end(),
+                         [&](const Leaf &a, const Leaf &b) { return kind(a) < kind(b); });
+        int plain = 0, singles = 0;
+        for (const Leaf &lf : leaves) {
+            plain += kind(lf) == 0;
+            singles += kind(lf) <= 1;
+        }
+        for (int e = 0; e < plain; e++) dest.push_back(leaves[e].dest);
+        if (dest.empty()) dest.push_back(0);
         for (const Leaf &lf : leaves) code.insert(code.end(), lf.w.begin(), lf.w.begin() + 4);
         for (const Leaf &lf : leaves) {
             const int b = (int)(code.size() / 4);
@@ -579,7 +600,7 @@ int analyze(const ngmi_circuit &c0, Plan &P)
             leaf.push_back(make_int4(lf.dest, b, (int)(code.size() / 4), lf.flag));
             leaf.push_back(make_int4((int)lf.w[0], (int)lf.w[1], (int)lf.w[2], (int)lf.w[3]));
         }
-        return singles;
+        return std::make_pair(plain, singles);
     };
     std::vector<uint32_t> a_code, r_code;
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
@@ -594,8 +615,8 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         split(rhs_codes[rowp[a]], a, 0, N, n_rpart, r_leaves, r_comb);
         if (perm[N + a] <= NN && P.ic_nodes[perm[N + a]]) ic_row.push_back(a);
     }
-    P.n_aleaf1 = layout(std::move(a_leaves), a_code, a_leaf);
-    P.n_rleaf1 = layout(std::move(r_leaves), r_code, r_leaf);
+    std::tie(P.n_aplain, P.n_aleaf1) = layout(std::move(a_leaves), a_code, a_leaf, P.a_dest, false);
+    std::tie(P.n_rplain, P.n_rleaf1) = layout(std::move(r_leaves), r_code, r_leaf, P.r_dest, true);
     P.n_aleaf = (int)a_leaf.size() / 2; P.n_acomb = (int)a_comb.size();
     P.n_rleaf = (int)r_leaf.size() / 2; P.n_rcomb = (int)r_comb.size(); P.n_icrow = (int)ic_row.size();
     P.n_apart = n_apart; P.n_rpart = n_rpart;
@@ -740,6 +761,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.n_aleaf = P.n_aleaf; D.n_acomb = P.n_acomb; D.n_rleaf = P.n_rleaf; D.n_rcomb = P.n_rcomb;
     D.n_apart = P.n_apart; D.n_rpart = P.n_rpart; D.n_icrow = P.n_icrow;
     D.n_aleaf1 = P.n_aleaf1; D.n_rleaf1 = P.n_rleaf1;
+    D.n_aplain = P.n_aplain; D.n_rplain = P.n_rplain;
     D.nprobe = (int)G->probe.size();
     D.cap_pts = b->cap_pts;
     D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
@@ -765,6 +787,8 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     UP(P.a_code, a_code, uint32_t, up);
     UP(P.r_code, r_code, uint32_t, up);
     UP(P.a_leaf, a_leaf, int4, i4);
+    UP(P.a_dest, a_dest, int, ip);
+    UP(P.r_dest, r_dest, int, ip);
     UP(P.a_comb, a_comb, int4, i4);
     UP(P.r_leaf, r_leaf, int4, i4);
     UP(P.r_comb, r_comb, int4, i4);

@@ -212,7 +212,7 @@ __device__ double vsrc_value(const double *vp, int mode, double tnow, double ste
     return P.V2 + (P.V1 - P.V2) * (time - (P.TR + P.PW)) / P.TF;
 }
 
-__device__ int vsrc_accept(const TranDev &D, int c, CktCtl *K)
+__device__ __forceinline__ int vsrc_accept(const TranDev &D, int c, CktCtl *K)
 {   /* vsrcacct.c:20-160 (PULSE corners) */
     if (!(K->mode & (M_TRAN | M_TRANOP))) return 0;
     for (int k = 0; k < D.NVS; k++) {
@@ -344,7 +344,7 @@ __device__ __forceinline__ void op_niiter(CktCtl *K, int stage)
     K->opIter0 = K->iters;
 }
 
-__device__ int op_src_init(const TranDev &D, CktCtl *K)
+__device__ __forceinline__ int op_src_init(const TranDev &D, CktCtl *K)
 {   /* gillespie_src start (cktop.c:480-503) */
     K->mode = (K->mode & M_UIC) | OP_FIRSTMODE;
     K->srcFact = 0;
@@ -375,7 +375,7 @@ __device__ int op_src_end(CktCtl *K, bool ramped)
 
 /* the next step of CKTop after an NIiter call of the current stage returned
  * rc, in the order of cktop.c; thread 0 */
-__device__ int op_next(const TranDev &D, CktCtl *K, int rc)
+__device__ __forceinline__ int op_next(const TranDev &D, CktCtl *K, int rc)
 {
     const int iters = (int)(K->iters - K->opIter0);
     const int uic = K->mode & M_UIC;
@@ -518,7 +518,7 @@ __device__ int op_next(const TranDev &D, CktCtl *K, int rc)
 
 /* the workgroup's part of a stepping transition: zero, save or restore
  * CKTrhsOld and CKTstate0 of circuit c */
-__device__ void op_memory(const TranDev &D, int c, const CktCtl *K, int what)
+__device__ __forceinline__ void op_memory(const TranDev &D, int c, const CktCtl *K, int what)
 {
     if (what == OPA_NONE) return;
     const int tid = threadIdx.x, nt = blockDim.x, N = D.N;
@@ -559,7 +559,7 @@ __device__ double wave_min(double v)
 }
 
 /* LTE over every charge state of the circuit (CAPtrunc + BSIM4trunc) */
-__device__ double circuit_terr(const TranDev &D, const CktCtl *K, int c)
+__device__ __forceinline__ double circuit_terr(const TranDev &D, const CktCtl *K, int c)
 {
     const int rb = K->rb, order = K->order;
     const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
@@ -589,7 +589,7 @@ __device__ double circuit_terr(const TranDev &D, const CktCtl *K, int c)
 }
 
 /* copy state level `from` to level `to` for every state of the circuit */
-__device__ void copy_level(const TranDev &D, int c, int rb, int from, int to)
+__device__ __forceinline__ void copy_level(const TranDev &D, int c, int rb, int from, int to)
 {
     const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
     const int pf = lvl(rb, from), pt = lvl(rb, to);
@@ -606,7 +606,7 @@ __device__ void copy_level(const TranDev &D, int c, int rb, int from, int to)
     }
 }
 
-__device__ void fail(const TranDev &D, CktCtl *K, int code)
+__device__ __forceinline__ void fail(const TranDev &D, CktCtl *K, int code)
 {
     K->phase = PH_FAIL;
     K->status = code;
@@ -786,6 +786,55 @@ __device__ __forceinline__ void run_levels(const int *lev, const int4 *ent, int 
     }
 }
 
+/* Plain assembly leaves 0 .. n-1 (one group, values used as they are):
+ * lane t sums leaves t + k nt from their destination (4 bytes) and their
+ * four values, U leaves per batch with the next batch's loads in flight;
+ * past-the-end loads are clamped to the last leaf so that the wait counts
+ * stay exact. */
+template <int U, class F>
+__device__ __forceinline__ void run_plain(const int *dest, const double4 *val, int n, int tid, int nt,
+                                          F body)
+{
+    if (n <= 0) return;
+    /* two half-batches of U/2 leaves each, held in scalar registers */
+    constexpr int H = U / 2;
+    int da[H], db[H];
+    double4 va[H], vb[H];
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        const int e = min(tid + k * nt, n - 1), f = min(tid + (H + k) * nt, n - 1);
+        da[k] = gload(dest + e);
+        va[k] = gload4(val + e);
+        db[k] = gload(dest + f);
+        vb[k] = gload4(val + f);
+    }
+    for (int e0 = tid; e0 < n; e0 += U * nt) {
+        int dc[H], dd[H];
+        double4 vc[H], vd[H];
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            dc[k] = da[k];
+            vc[k] = va[k];
+            dd[k] = db[k];
+            vd[k] = vb[k];
+        }
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            const int e = min(e0 + (U + k) * nt, n - 1), f = min(e0 + (U + H + k) * nt, n - 1);
+            da[k] = gload(dest + e);
+            va[k] = gload4(val + e);
+            db[k] = gload(dest + f);
+            vb[k] = gload4(val + f);
+        }
+#pragma unroll
+        for (int k = 0; k < H; k++)
+            if (e0 + k * nt < n) body(dc[k], vc[k]);
+#pragma unroll
+        for (int k = 0; k < H; k++)
+            if (e0 + (H + k) * nt < n) body(dd[k], vd[k]);
+    }
+}
+
 /* Assembly leaves e_begin .. n-1: lane t sums leaves e_begin + t + k nt.
  * Leaf e's first values are group e of the contribution array, so the next
  * U leaves' records and first values load together, while the current U
@@ -874,6 +923,7 @@ constexpr int STEP_THREADS = 64;
 static_assert(STEP_THREADS == 64, "the step kernel's reductions assume one wavefront");
 /* assembly leaves whose loads are issued together, per lane */
 constexpr int ASM_UNROLL = 4;
+constexpr int PLAIN_UNROLL = 8;
 /* unknowns (and device votes) a lane loads together in the convergence test */
 constexpr int CONV_CHUNK = 8;
 
@@ -1011,7 +1061,16 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     /* single-group leaves: no further groups, and no code that could load
      * them (a load anywhere in the body makes the compiler wait for all
      * outstanding loads after it, the prefetched leaves' included) */
-    run_leaves<ASM_UNROLL, false>(D.a_leaf, AV, AC, 0, D.n_aleaf1, tid, nt,
+    run_plain<PLAIN_UNROLL>(D.a_dest, AV, D.n_aplain, tid, nt, [&](int d, double4 m) {
+        double acc = 0.0;
+        acc += m.x;
+        acc += m.y;
+        acc += m.z;
+        acc += m.w;
+        if (d < D.nslots) lu[d] = acc;
+        else part[d - D.nslots] = acc;
+    });
+    run_leaves<ASM_UNROLL, false>(D.a_leaf, AV, AC, D.n_aplain, D.n_aleaf1, tid, nt,
                                   [&](const Rec &lf, const double4 &m0, const uint4 &, const double4 &) {
         double acc = 0.0;
         acc += mat_value(lf.g.x, m0.x, capg, icz);
@@ -1045,7 +1104,16 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         if (lf.h.x < N) z[lf.h.x] = acc;
         else part[D.n_apart + lf.h.x - N] = acc;
     };
-    run_leaves<ASM_UNROLL, false>(D.r_leaf, RV, RC, 0, D.n_rleaf1, tid, nt,
+    run_plain<PLAIN_UNROLL>(D.r_dest, RV, D.n_rplain, tid, nt, [&](int d, double4 m) {
+        double acc = 0.0;
+        acc += m.x;
+        acc += m.y;
+        acc += m.z;
+        acc += m.w;
+        if (d < N) z[d] = acc;
+        else part[D.n_apart + d - N] = acc;
+    });
+    run_leaves<ASM_UNROLL, false>(D.r_leaf, RV, RC, D.n_rplain, D.n_rleaf1, tid, nt,
                                   [&](const Rec &lf, const double4 &m0, const uint4 &, const double4 &) {
         double acc = 0.0;
         acc += rhs_value(lf.g.x, m0.x, capq, vsv);

@@ -112,6 +112,8 @@ struct TranDev {
     const int4 *a_leaf, *a_comb, *r_leaf, *r_comb;
     int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
     int n_aleaf1, n_rleaf1;   /* leading leaves with a single group */
+    int n_aplain, n_rplain;   /* of those, the leading plain ones: values used as they are */
+    const int *a_dest, *r_dest;   /* destinations of the plain leaves */
     const uint32_t *a_code;   /* contributions */
     const uint32_t *r_code;
     const int *ic_row;        /* permuted rows of .ic nodes */
