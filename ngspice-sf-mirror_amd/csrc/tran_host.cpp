@@ -55,6 +55,12 @@ constexpr int ASM_MAXPART = 48;
  * 4 * PROG_INLINE are the inline term groups */
 constexpr int PROG_INLINE = 3;
 constexpr int PROG_INTS = 4 + 4 * PROG_INLINE;
+/* record kinds (bits 20-21 of the first int): a plain entry, a partial sum
+ * of one chunk of a long entry's terms (into a partial slot), and the
+ * combine that adds a long entry's partial sums */
+constexpr int PROG_PLAIN = 0, PROG_PARTIAL = 1, PROG_COMBINE = 2;
+/* entries with more terms run as partial sums over several lanes */
+constexpr int SPLIT_TERMS = 24;
 
 struct ngmi_circuit {
     int nnodes = 0;
@@ -252,6 +258,7 @@ struct Plan {
     std::vector<int4> a_leaf, a_comb, r_leaf, r_comb;
     int n_aleaf = 0, n_acomb = 0, n_rleaf = 0, n_rcomb = 0, n_apart = 0, n_rpart = 0, n_icrow = 0;
     int n_aleaf1 = 0, n_rleaf1 = 0, n_aplain = 0, n_rplain = 0;
+    int n_ppart = 0;   /* partial slots the programs use (two halves) */
     std::vector<int> a_dest, r_dest;
     std::vector<int> perm, unk_pos, ic_row, nbr;
     std::vector<int> mos_pos, rmos_pos, diag_slot;
@@ -389,7 +396,10 @@ int analyze(const ngmi_circuit &c0, Plan &P)
 
     /* ---- factor program: entries ordered by m = min(i, j), diagonal first */
     std::vector<int> lev(nslots, 0);
-    struct Ent { int t, piv, lev; std::vector<int2> terms; };
+    /* an entry with more than SPLIT_TERMS terms runs as partial sums of up
+     * to 4 * PROG_INLINE terms on several lanes at level plev, added into
+     * the entry (and divided by its pivot) one level later */
+    struct Ent { int t, piv, lev, plev; std::vector<int2> terms; };
     std::vector<Ent> ents;
     ents.reserve(nslots);
     for (int m = 0; m < N; m++) {
@@ -409,6 +419,8 @@ int analyze(const ngmi_circuit &c0, Plan &P)
                     L = std::max(L, std::max(lev[a], lev[c2]) + 1);
                 }
             }
+            e.plev = L;
+            if ((int)e.terms.size() > SPLIT_TERMS) L = std::max(L + 1, 1);
             if (e.piv >= 0) L = std::max(L, lev[e.piv] + 1);
             lev[e.t] = L;
             e.lev = L;
@@ -424,99 +436,144 @@ int analyze(const ngmi_circuit &c0, Plan &P)
      * end} + three groups: its first g (0..3) groups inline, the rest at
      * [begin, end) in uint4 units of dst. */
     auto put_terms = [](std::vector<uint32_t> &dst, const std::vector<int2> &t, uint32_t pad,
-                        std::vector<int> &rec, int target, int y) {
+                        std::vector<int> &rec, int target, int y, int kind = PROG_PLAIN) {
         std::vector<uint32_t> w;
         for (const int2 &x : t) w.push_back((uint32_t)x.x | ((uint32_t)x.y << 16));
         const int g = std::min(PROG_INLINE, (int)(w.size() + 3) / 4);
         while ((int)w.size() < 4 * g || w.size() % 4) w.push_back(pad);
-        rec.push_back(target | (g << 16));
+        rec.push_back(target | (g << 16) | (kind << 20));
         rec.push_back(y);
         rec.push_back((int)(dst.size() / 4));
         if ((int)w.size() > 4 * PROG_INLINE) dst.insert(dst.end(), w.begin() + 4 * PROG_INLINE, w.end());
         rec.push_back((int)(dst.size() / 4));
         for (int k = 0; k < 4 * PROG_INLINE; k++) rec.push_back(k < (int)w.size() ? (int)w[k] : (int)pad);
     };
+    /* a combine record: target, y, the range of partial slots to add */
+    auto put_combine = [](std::vector<int> &rec, int target, int y, int p0, int p1, uint32_t pad) {
+        rec.push_back(target | (PROG_COMBINE << 20));
+        rec.push_back(y);
+        rec.push_back(p0);
+        rec.push_back(p1);
+        for (int k = 0; k < 4 * PROG_INLINE; k++) rec.push_back((int)pad);
+    };
+    /* one record to emit: entry idx whole (chunk -1), its partial sum of
+     * terms [chunk * CH, chunk * CH + CH) (chunk >= 0), or its combine (-2) */
+    struct Item { int idx, chunk; };
+    const int CH = 4 * PROG_INLINE;
+    int part_half = 0;   /* partial slots per level parity, over all programs */
+    auto count_parts = [&](const std::vector<std::vector<Item>> &by) {
+        for (const auto &items : by) {
+            int n = 0;
+            for (const Item &it : items) n += it.chunk >= 0;
+            part_half = std::max(part_half, n);
+        }
+    };
     const uint32_t pad_lu = (uint32_t)nslots | ((uint32_t)nslots << 16);
     const uint32_t pad_z = (uint32_t)nslots | ((uint32_t)N << 16);
     /* entries that change nothing (a U or diagonal entry with no terms: its
      * assembled value is final) are left out, and levels left empty with
      * them are dropped */
-    std::vector<int> f_lev, f_ent;
-    std::vector<uint32_t> f_term;
+    /* the three programs as entries {target, y, level, partial level,
+     * terms}: the factor's (no-op entries left out), the forward rows with
+     * terms, every backward row */
+    struct PEnt { int t, y, lev, plev; std::vector<int2> terms; };
+    std::vector<PEnt> fe, we, be;
     int nterms = 0;
-    {
-        std::vector<std::vector<int>> by(nlev);
-        for (int k = 0; k < (int)ents.size(); k++)
-            if (!ents[k].terms.empty() || ents[k].piv >= 0) by[ents[k].lev].push_back(k);
-        for (int L = 0; L < nlev; L++) {
-            if (by[L].empty()) continue;
-            f_lev.push_back((int)(f_ent.size() / PROG_INTS));
-            for (int k : by[L]) {
-                const Ent &e = ents[k];
-                put_terms(f_term, e.terms, pad_lu, f_ent, e.t, e.piv);
-                nterms += (int)e.terms.size();
-            }
-        }
-        P.nlev_f = (int)f_lev.size();
-        f_lev.push_back((int)(f_ent.size() / PROG_INTS));
+    for (const Ent &e : ents) {
+        if (e.terms.empty() && e.piv < 0) continue;
+        fe.push_back({e.t, e.piv, e.lev, e.plev, e.terms});
+        nterms += (int)e.terms.size();
     }
     P.nterms = nterms;
-
-    /* ---- forward (unit L) and backward programs */
-    std::vector<int> fw_lev, fw_ent, bw_lev, bw_ent;
-    std::vector<uint32_t> fw_term, bw_term;
     {
         std::vector<int> lv(N, 0);
-        int nl = 0;
         for (int a = 0; a < N; a++) {
+            PEnt r{a, -1, 0, 0, {}};
             int L = 0;
-            for (int k = 0; k < a; k++) if (S[a][k]) L = std::max(L, lv[k] + 1);
-            lv[a] = L;
-            nl = std::max(nl, L + 1);
+            for (int k = 0; k < a; k++)
+                if (S[a][k]) {
+                    r.terms.push_back(make_int2(slot[(size_t)a * N + k], k));
+                    L = std::max(L, lv[k] + 1);
+                }
+            r.plev = L;
+            if ((int)r.terms.size() > SPLIT_TERMS) L++;
+            lv[a] = r.lev = L;
+            if (!r.terms.empty()) we.push_back(std::move(r));   /* rows without terms change nothing */
         }
-        /* rows of L without terms leave z unchanged: left out */
-        std::vector<std::vector<int>> by(nl);
-        for (int a = 0; a < N; a++) {
-            bool any = false;
-            for (int k = 0; k < a && !any; k++) any = S[a][k];
-            if (any) by[lv[a]].push_back(a);
-        }
-        for (int L = 0; L < nl; L++) {
-            if (by[L].empty()) continue;
-            fw_lev.push_back((int)(fw_ent.size() / PROG_INTS));
-            for (int a : by[L]) {
-                std::vector<int2> t;
-                for (int k = 0; k < a; k++)
-                    if (S[a][k]) t.push_back(make_int2(slot[(size_t)a * N + k], k));
-                put_terms(fw_term, t, pad_z, fw_ent, a, -1);
-            }
-        }
-        P.nlev_fw = (int)fw_lev.size();
-        fw_lev.push_back((int)(fw_ent.size() / PROG_INTS));
     }
     {
         std::vector<int> lv(N, 0);
-        int nl = 0;
         for (int a = N - 1; a >= 0; a--) {
+            PEnt r{a, slot[(size_t)a * N + a], 0, 0, {}};
             int L = 0;
-            for (int j = a + 1; j < N; j++) if (S[a][j]) L = std::max(L, lv[j] + 1);
-            lv[a] = L;
-            nl = std::max(nl, L + 1);
+            for (int j = a + 1; j < N; j++)
+                if (S[a][j]) {
+                    r.terms.push_back(make_int2(slot[(size_t)a * N + j], j));
+                    L = std::max(L, lv[j] + 1);
+                }
+            r.plev = L;
+            if ((int)r.terms.size() > SPLIT_TERMS) L++;
+            lv[a] = r.lev = L;
+            be.push_back(std::move(r));
         }
-        std::vector<std::vector<int>> by(nl);
-        for (int a = 0; a < N; a++) by[lv[a]].push_back(a);
-        for (int L = 0; L < nl; L++) {
-            bw_lev.push_back((int)(bw_ent.size() / PROG_INTS));
-            for (int a : by[L]) {
-                std::vector<int2> t;
-                for (int j = a + 1; j < N; j++)
-                    if (S[a][j]) t.push_back(make_int2(slot[(size_t)a * N + j], j));
-                put_terms(bw_term, t, pad_z, bw_ent, a, slot[(size_t)a * N + a]);
+        std::reverse(be.begin(), be.end());   /* ascending rows within a level */
+    }
+    auto items = [&](const std::vector<PEnt> &ps) {
+        int nl = 0;
+        for (const PEnt &e : ps) nl = std::max(nl, e.lev + 1);
+        std::vector<std::vector<Item>> by(nl);
+        for (int k = 0; k < (int)ps.size(); k++) {
+            const PEnt &e = ps[k];
+            if ((int)e.terms.size() > SPLIT_TERMS) {
+                for (int q = 0; q * CH < (int)e.terms.size(); q++) by[e.plev].push_back({k, q});
+                by[e.lev].push_back({k, -2});
+            } else {
+                by[e.lev].push_back({k, -1});
             }
         }
-        bw_lev.push_back((int)(bw_ent.size() / PROG_INTS));
-        P.nlev_bw = nl;
-    }
+        return by;
+    };
+    const auto fby = items(fe), wby = items(we), bby = items(be);
+    count_parts(fby);
+    count_parts(wby);
+    count_parts(bby);
+    /* partial slots: executed level i uses half (i & 1), so the partials a
+     * combine reads (written one level earlier) differ from those being
+     * written beside it */
+    auto emit = [&](const std::vector<PEnt> &ps, const std::vector<std::vector<Item>> &by, uint32_t pad,
+                    std::vector<int> &lev_out, std::vector<int> &ent, std::vector<uint32_t> &term) {
+        std::vector<int2> prange(ps.size(), make_int2(0, 0));
+        for (const auto &its : by) {
+            if (its.empty()) continue;
+            const int base = ((int)lev_out.size() & 1) * part_half;
+            int used = 0;
+            lev_out.push_back((int)(ent.size() / PROG_INTS));
+            for (const Item &it : its) {
+                const PEnt &e = ps[it.idx];
+                if (it.chunk == -1) {
+                    put_terms(term, e.terms, pad, ent, e.t, e.y);
+                } else if (it.chunk >= 0) {
+                    const int q0 = it.chunk * CH, q1 = std::min((int)e.terms.size(), q0 + CH);
+                    const std::vector<int2> part(e.terms.begin() + q0, e.terms.begin() + q1);
+                    const int ps_slot = base + used++;
+                    if (it.chunk == 0) prange[it.idx].x = ps_slot;
+                    prange[it.idx].y = ps_slot + 1;
+                    put_terms(term, part, pad, ent, ps_slot, -1, PROG_PARTIAL);
+                } else {
+                    put_combine(ent, e.t, e.y, prange[it.idx].x, prange[it.idx].y, pad);
+                }
+            }
+        }
+        const int n = (int)lev_out.size();
+        lev_out.push_back((int)(ent.size() / PROG_INTS));
+        return n;
+    };
+    std::vector<int> f_lev, f_ent, fw_lev, fw_ent, bw_lev, bw_ent;
+    std::vector<uint32_t> f_term, fw_term, bw_term;
+    P.nlev_f = emit(fe, fby, pad_lu, f_lev, f_ent, f_term);
+    P.nlev_fw = emit(we, wby, pad_z, fw_lev, fw_ent, fw_term);
+    P.nlev_bw = emit(be, bby, pad_z, bw_lev, bw_ent, bw_term);
+    P.n_ppart = 2 * part_half;
 
     /* ---- assembly plans in permuted coordinates */
     std::vector<std::vector<uint32_t>> per_slot(nslots);
@@ -762,6 +819,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.n_apart = P.n_apart; D.n_rpart = P.n_rpart; D.n_icrow = P.n_icrow;
     D.n_aleaf1 = P.n_aleaf1; D.n_rleaf1 = P.n_rleaf1;
     D.n_aplain = P.n_aplain; D.n_rplain = P.n_rplain;
+    D.n_part = std::max(P.n_apart + P.n_rpart, P.n_ppart);
     D.nprobe = (int)G->probe.size();
     D.cap_pts = b->cap_pts;
     D.reltol = c0.reltol; D.abstol = c0.abstol; D.vntol = c0.vntol; D.chgtol = c0.chgtol;
@@ -866,8 +924,8 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.done_count = b->done_count;
     /* LDS: lu, z, capacitor and source values, partial sums (doubles), then
      * the three level tables (ints) */
-    G->lds = sizeof(double) * ((size_t)P.nslots + 1 + N + 1 + 2 * (size_t)NCAP + NVS + P.n_apart +
-                              P.n_rpart) +
+    G->lds = sizeof(double) * ((size_t)P.nslots + 1 + N + 1 + 2 * (size_t)NCAP + NVS +
+                              std::max(P.n_apart + P.n_rpart, P.n_ppart)) +
              sizeof(int) * ((size_t)G->nlev_f + G->nlev_fw + G->nlev_bw + 3);
     if (G->lds > 150 * 1024) return fail("circuit too large for the per-circuit LDS factorisation");
     G->method = c0.method;

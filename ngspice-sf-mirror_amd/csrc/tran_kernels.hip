@@ -740,7 +740,7 @@ constexpr int TERM_BATCH = 8;
 __device__ __forceinline__ double lu_row_terms(double acc, const Prog &r, const uint4 *T,
                                                const double *x, const double *y)
 {
-    const int g = r.h.x >> 16;
+    const int g = (r.h.x >> 16) & 3;
     if (g > 0) acc = sub_terms(acc, r.g0, x, y);
     if (g > 1) acc = sub_terms(acc, r.g1, x, y);
     if (g > 2) acc = sub_terms(acc, r.g2, x, y);
@@ -752,6 +752,16 @@ __device__ __forceinline__ double lu_row_terms(double acc, const Prog &r, const 
         for (int k = 0; k < TERM_BATCH; k++)
             if (q + k < r.h.w) acc = sub_terms(acc, w[k], x, y);
     }
+    return acc;
+}
+
+/* record kinds (bits 20-21 of the header's first int) */
+constexpr int PROG_PARTIAL = 1, PROG_COMBINE = 2;
+
+/* acc plus the partial sums [h.z, h.w) of a combine record, in order */
+__device__ __forceinline__ double add_parts(double acc, const Prog &r, const double *part)
+{
+    for (int p = r.h.z; p < r.h.w; p++) acc += part[p];
     return acc;
 }
 
@@ -1021,7 +1031,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     double *part = vsv + D.NVS;
     /* the level tables, read at every level, staged in LDS (visible after
      * the barrier that ends the assembly) */
-    int *levf = reinterpret_cast<int *>(part + D.n_apart + D.n_rpart);
+    int *levf = reinterpret_cast<int *>(part + D.n_part);
     int *levw = levf + D.nlev_f + 1;
     int *levb = levw + D.nlev_fw + 1;
     for (int k = tid; k <= D.nlev_f; k += nt) levf[k] = gload(D.f_lev + k);
@@ -1163,23 +1173,35 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
     run_levels(levf, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
                [&](const Prog &r) {
-                   const int t = r.h.x & 0xffff;
-                   const double acc = lu_row_terms(lu[t], r, FT, lu, lu);
+                   const int t = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                   if (kind == PROG_PARTIAL) {
+                       part[t] = lu_row_terms(0.0, r, FT, lu, lu);
+                       return;
+                   }
+                   const double acc = (kind == PROG_COMBINE) ? add_parts(lu[t], r, part)
+                                                             : lu_row_terms(lu[t], r, FT, lu, lu);
                    lu[t] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
                });
     /* forward substitution (unit L) */
     const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
     run_levels(levw, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
                [&](const Prog &r) {
-                   const int a = r.h.x & 0xffff;
-                   z[a] = lu_row_terms(z[a], r, WT, lu, z);
+                   const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                   if (kind == PROG_PARTIAL) part[a] = lu_row_terms(0.0, r, WT, lu, z);
+                   else if (kind == PROG_COMBINE) z[a] = add_parts(z[a], r, part);
+                   else z[a] = lu_row_terms(z[a], r, WT, lu, z);
                });
     /* back substitution */
     const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
     run_levels(levb, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
                [&](const Prog &r) {
-                   const int a = r.h.x & 0xffff;
-                   const double acc = lu_row_terms(z[a], r, BT, lu, z);
+                   const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                   if (kind == PROG_PARTIAL) {
+                       part[a] = lu_row_terms(0.0, r, BT, lu, z);
+                       return;
+                   }
+                   const double acc = (kind == PROG_COMBINE) ? add_parts(z[a], r, part)
+                                                             : lu_row_terms(z[a], r, BT, lu, z);
                    const double piv = lu[r.h.y];
                    if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
                    z[a] = acc / piv;

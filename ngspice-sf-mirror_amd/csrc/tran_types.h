@@ -113,6 +113,7 @@ struct TranDev {
     int n_aleaf, n_acomb, n_rleaf, n_rcomb, n_apart, n_rpart;
     int n_aleaf1, n_rleaf1;   /* leading leaves with a single group */
     int n_aplain, n_rplain;   /* of those, the leading plain ones: values used as they are */
+    int n_part;               /* LDS partial sums: the assembly's, or the programs' two halves */
     const int *a_dest, *r_dest;   /* destinations of the plain leaves */
     const uint32_t *a_code;   /* contributions */
     const uint32_t *r_code;
@@ -123,7 +124,11 @@ struct TranDev {
      * product is exactly zero (LDS lu[nslots] = z[N] = 0).  An entry is a
      * 64-byte record: four ints {target | g << 16, y, begin, end}, then
      * three term groups of which the first g (0..3) are its own; the term
-     * range [begin, end) holds the further groups, in uint4 units. */
+     * range [begin, end) holds the further groups, in uint4 units.  Bits
+     * 20-21 of the first int give the kind: 0 an entry, 1 a partial sum of
+     * some of a long entry's terms into partial slot target, 2 the combine
+     * that adds partial slots [begin, end) into target (then divides by the
+     * pivot y when y >= 0). */
     const int *f_lev;         /* [nlev_f+1] -> entries */
     const int *f_ent;         /* [ne][16]: y = pivot slot (-1 for U) */
     const uint32_t *f_term;   /* (a, b) slot pairs: lu[t] -= lu[a] lu[b] */
