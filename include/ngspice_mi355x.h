@@ -184,10 +184,11 @@ int ngmi_circuit_set_options(ngmi_circuit *c, double reltol, double abstol, doub
 /* .options noopiter gminsteps srcsteps gminfactor gshunt itl2 for the
  * transient's operating point, CKTop (cktop.c:26-116): direct Newton unless
  * noopiter, then dynamic_gmin and new_gmin (gminsteps = 1,
- * cktop.c:161-478), then gillespie_src (srcsteps = 1, cktop.c:480-670);
- * defaults 0, 1, 1, 10, 0, 50 (cktntask.c:94-132).  gminsteps or srcsteps
- * above 1 (the spice3 variants) return NGMI_E_UNSUPP.  A batch uses the
- * options of each topology group's first circuit. */
+ * cktop.c:161-478) or spice3_gmin in gminsteps steps (gminsteps > 1,
+ * cktop.c:289-347), then gillespie_src (srcsteps = 1, cktop.c:480-670) or
+ * spice3_src in srcsteps steps (srcsteps > 1, cktop.c:672-720); defaults
+ * 0, 1, 1, 10, 0, 50 (cktntask.c:94-132).  A batch uses the options of
+ * each topology group's first circuit. */
 int ngmi_circuit_set_op_options(ngmi_circuit *c, int noopiter, int gminsteps, int srcsteps,
                                 double gminfactor, double gshunt, int itl2);
 /* Pivot sequence for the static LU: step k eliminates row rows[k] and

@@ -1035,10 +1035,6 @@ int ngmi_circuit_set_op_options(ngmi_circuit *c, int noopiter, int gminsteps, in
         set_error("operating-point option out of range");
         return NGMI_E_PARMVAL;
     }
-    if (gminsteps > 1 || srcsteps > 1) {
-        set_error("gminsteps / srcsteps above 1 select the spice3 stepping, which is not built");
-        return NGMI_E_UNSUPP;
-    }
     if (noopiter && !gminsteps && !srcsteps) {
         set_error("noopiter needs gmin or source stepping");
         return NGMI_E_PARMVAL;
@@ -1239,12 +1235,22 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
             k.srcFact = 1.0;
             k.opStage = OP_DIRECT;
             if (G->noopiter) {
-                if (D.numGminSteps >= 1) {
+                if (D.numGminSteps == 1) {
                     k.opStage = OP_DYN;
                     k.opFactor = D.gminFactor;
                     k.opOldGmin = 1e-2;
                     k.diagGmin = k.opOldGmin / k.opFactor;
                     k.opGtarget = std::max(k.gminCur, D.gshunt);
+                } else if (D.numGminSteps > 1) {
+                    /* spice3_gmin (cktop.c:297-302) */
+                    k.opStage = OP_S3_GMIN;
+                    k.diagGmin = (D.gshunt == 0) ? k.gminCur : D.gshunt;
+                    for (int i = 0; i < D.numGminSteps; i++) k.diagGmin *= D.gminFactor;
+                    k.opSub = 0;
+                } else if (D.numSrcSteps > 1) {
+                    k.opStage = OP_S3_SRC;   /* spice3_src, CKTsrcFact = 0 / steps */
+                    k.srcFact = 0.0;
+                    k.opSub = 0;
                 } else {
                     k.opStage = OP_SRC_ZERO;
                     k.srcFact = 0.0;

@@ -354,6 +354,35 @@ __device__ __forceinline__ int op_src_init(const TranDev &D, CktCtl *K)
     return OPA_ZERO;
 }
 
+__device__ __forceinline__ int op_s3gmin_init(const TranDev &D, CktCtl *K)
+{   /* spice3_gmin start (cktop.c:289-303) */
+    K->mode = (K->mode & M_UIC) | OP_FIRSTMODE;
+    K->diagGmin = (D.gshunt == 0) ? K->gminCur : D.gshunt;
+    for (int i = 0; i < D.numGminSteps; i++) K->diagGmin *= D.gminFactor;
+    K->opSub = 0;
+    op_niiter(K, OP_S3_GMIN);
+    return OPA_NONE;
+}
+
+__device__ __forceinline__ int op_s3src_init(const TranDev &D, CktCtl *K)
+{   /* spice3_src start (cktop.c:672-686) */
+    K->mode = (K->mode & M_UIC) | OP_FIRSTMODE;
+    K->opSub = 0;
+    K->srcFact = 0.0;
+    op_niiter(K, OP_S3_SRC);
+    return OPA_NONE;
+}
+
+/* source stepping after the gmin stepping failed or was not asked for
+ * (cktop.c:78-86) */
+__device__ __forceinline__ int op_src_any(const TranDev &D, CktCtl *K, int rc)
+{
+    if (D.numSrcSteps == 1) return op_src_init(D, K);
+    if (D.numSrcSteps > 1) return op_s3src_init(D, K);
+    K->status = rc;
+    return OPA_FAIL;
+}
+
 __device__ int op_src_ramp_init(CktCtl *K)
 {   /* the source ramp's start (cktop.c:559-575) */
     K->opRaise = 0.001;
@@ -383,7 +412,8 @@ __device__ __forceinline__ int op_next(const TranDev &D, CktCtl *K, int rc)
     switch (K->opStage) {
     case OP_DIRECT:
         if (rc == 0) return OPA_OK;
-        if (D.numGminSteps >= 1) {
+        if (D.numGminSteps > 1) return op_s3gmin_init(D, K);
+        if (D.numGminSteps == 1) {
             /* dynamic_gmin (cktop.c:161-196) */
             K->mode = uic | OP_FIRSTMODE;
             K->opFactor = D.gminFactor;
@@ -393,9 +423,7 @@ __device__ __forceinline__ int op_next(const TranDev &D, CktCtl *K, int rc)
             op_niiter(K, OP_DYN);
             return OPA_ZERO;
         }
-        if (D.numSrcSteps >= 1) return op_src_init(D, K);
-        K->status = rc;
-        return OPA_FAIL;
+        return op_src_any(D, K, rc);
     case OP_DYN:
     case OP_NEW: {
         /* the loops of dynamic_gmin (cktop.c:197-270) and new_gmin
@@ -462,10 +490,37 @@ __device__ __forceinline__ int op_next(const TranDev &D, CktCtl *K, int rc)
         op_niiter(K, OP_NEW);
         return OPA_ZERO;
     case OP_NEW_FINAL:
+    case OP_S3_GMIN_FINAL:
         if (rc == 0) return OPA_OK;
-        if (D.numSrcSteps >= 1) return op_src_init(D, K);
-        K->status = rc;
-        return OPA_FAIL;
+        return op_src_any(D, K, rc);
+    case OP_S3_GMIN:
+        /* the spice3_gmin loop (cktop.c:304-324) */
+        if (rc == 0) {
+            K->diagGmin /= D.gminFactor;
+            K->mode = uic | OP_CONTINUEMODE;
+            if (++K->opSub <= D.numGminSteps) {
+                op_niiter(K, OP_S3_GMIN);
+                return OPA_NONE;
+            }
+        }
+        K->diagGmin = D.gshunt;
+        op_niiter(K, OP_S3_GMIN_FINAL);
+        return OPA_NONE;
+    case OP_S3_SRC:
+        /* the spice3_src loop (cktop.c:688-712) */
+        K->mode = uic | OP_CONTINUEMODE;
+        if (rc != 0) {
+            K->srcFact = 1.0;
+            K->status = rc;
+            return OPA_FAIL;
+        }
+        if (++K->opSub <= D.numSrcSteps) {
+            K->srcFact = ((double)K->opSub) / ((double)D.numSrcSteps);
+            op_niiter(K, OP_S3_SRC);
+            return OPA_NONE;
+        }
+        K->srcFact = 1.0;
+        return OPA_OK;
     case OP_SRC_ZERO:
         if (rc == 0) return op_src_ramp_init(K);
         /* gmin ladder for the zero-source solution (cktop.c:517-550) */

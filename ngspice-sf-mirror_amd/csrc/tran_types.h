@@ -57,7 +57,8 @@ struct CktCtl {
  * its source ramp */
 enum : int {
     OP_DIRECT = 0, OP_DYN = 1, OP_DYN_FINAL = 2, OP_NEW = 3, OP_NEW_FINAL = 4,
-    OP_SRC_ZERO = 5, OP_SRC_GMIN = 6, OP_SRC_STEP = 7
+    OP_SRC_ZERO = 5, OP_SRC_GMIN = 6, OP_SRC_STEP = 7,
+    OP_S3_GMIN = 8, OP_S3_GMIN_FINAL = 9, OP_S3_SRC = 10
 };
 
 /* contribution codes of the assembly plans */

@@ -611,6 +611,47 @@ static int gillespie_src(ORTran *T, int firstmode, int continuemode, double *ox,
     return (ConvFact != 1) ? 103 : 0;
 }
 
+/* spice3_gmin (cktop.c:289-347): CKTdiagGmin from gmin * gminFactor^steps
+ * down by gminFactor per converged step, then a plain NIiter */
+static int spice3_gmin(ORTran *T, int firstmode, int continuemode, int iterlim)
+{
+    ORCkt *ck = &T->ck;
+    int converged, i;
+    ck->mode = firstmode;
+    T->diagGmin = (T->gshunt == 0) ? ck->gmin : T->gshunt;
+    for (i = 0; i < T->numGminSteps; i++) T->diagGmin *= T->gminFactor;
+    for (i = 0; i <= T->numGminSteps; i++) {
+        converged = ni_iter(T, T->dcTrcvMaxIter);
+        if (converged != 0) {
+            T->diagGmin = T->gshunt;
+            break;
+        }
+        T->diagGmin /= T->gminFactor;
+        ck->mode = continuemode;
+    }
+    T->diagGmin = T->gshunt;
+    return ni_iter(T, iterlim);
+}
+
+/* spice3_src (cktop.c:672-720): CKTsrcFact = i / steps for i = 0..steps */
+static int spice3_src(ORTran *T, int firstmode, int continuemode)
+{
+    ORCkt *ck = &T->ck;
+    int converged, i;
+    ck->mode = firstmode;
+    for (i = 0; i <= T->numSrcSteps; i++) {
+        T->srcFact = ((double)i) / ((double)T->numSrcSteps);
+        converged = ni_iter(T, T->dcTrcvMaxIter);
+        ck->mode = continuemode;
+        if (converged != 0) {
+            T->srcFact = 1;
+            return converged;
+        }
+    }
+    T->srcFact = 1;
+    return 0;
+}
+
 static int ckt_op(ORTran *T, int firstmode, int continuemode, int iterlim)
 {
     int converged = 1;
@@ -618,25 +659,31 @@ static int ckt_op(ORTran *T, int firstmode, int continuemode, int iterlim)
     double *os = (double *)malloc(sizeof(double) * (T->nstates + 1));
     T->ck.mode = firstmode;
     if (!T->noOpIter) converged = ni_iter(T, iterlim);
+    /* cktop.c:54-91 (the frontend's dyngmin variable unset) */
     if (converged != 0 && T->numGminSteps >= 1) {
-        converged = dynamic_gmin(T, firstmode, continuemode, iterlim, ox, os);
-        if (converged != 0) converged = new_gmin(T, firstmode, continuemode, iterlim, ox, os);
+        if (T->numGminSteps == 1) {
+            converged = dynamic_gmin(T, firstmode, continuemode, iterlim, ox, os);
+            if (converged != 0) converged = new_gmin(T, firstmode, continuemode, iterlim, ox, os);
+        } else {
+            converged = spice3_gmin(T, firstmode, continuemode, iterlim);
+        }
     }
-    if (converged != 0 && T->numSrcSteps >= 1)
-        converged = gillespie_src(T, firstmode, continuemode, ox, os);
+    if (converged != 0 && T->numSrcSteps >= 1) {
+        if (T->numSrcSteps == 1) converged = gillespie_src(T, firstmode, continuemode, ox, os);
+        else converged = spice3_src(T, firstmode, continuemode);
+    }
     free(ox);
     free(os);
     return converged;
 }
 
 /* operating-point options (cktsopt.c): noopiter, gminsteps (0: off, 1:
- * dynamic then true gmin stepping), srcsteps (0: off, 1: Gillespie source
- * stepping), gminfactor, gshunt, itl2.  Returns 0, or -1 for stepping
- * counts above 1 (the spice3 variants are not restated). */
+ * dynamic then true gmin stepping, above 1: spice3 gmin stepping in that
+ * many steps), srcsteps (0: off, 1: Gillespie source stepping, above 1:
+ * spice3 source stepping), gminfactor, gshunt, itl2.  Returns 0. */
 int or_tran_set_op_options(ORTran *T, int noopiter, int gminsteps, int srcsteps, double gminfactor,
                            double gshunt, int itl2)
 {
-    if (gminsteps > 1 || srcsteps > 1) return -1;
     T->noOpIter = noopiter;
     T->numGminSteps = gminsteps;
     T->numSrcSteps = srcsteps;

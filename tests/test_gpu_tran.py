@@ -176,15 +176,18 @@ def test_uneven_batch_rings():
 
 
 def test_operating_point_stepping():
-    """CKTop's fallbacks (cktop.c:161-670) forced with noopiter: dynamic and
-    true gmin stepping (gminsteps 1), and Gillespie source stepping
-    (gminsteps 0, srcsteps 1), on the engine against the oracle; the Newton
-    iteration counts, which include every stepping solve, must be equal."""
-    print('case: operating point by gmin stepping and by source stepping, 11-stage ring .tran 10p '
-          '1n, adder seed 0 .tran 5p 1.5n', flush=True)
+    """CKTop's fallbacks (cktop.c:161-720) forced with noopiter: dynamic and
+    true gmin stepping (gminsteps 1), Gillespie source stepping (gminsteps
+    0, srcsteps 1), and the spice3 variants in 5 steps each, on the engine
+    against the oracle; the Newton iteration counts, which include every
+    stepping solve, must be equal."""
+    print('case: operating point by gmin and source stepping (default and spice3), 11-stage ring '
+          '.tran 10p 1n, adder seed 0 .tran 5p 1.5n', flush=True)
     for tag, desc, tstop in [('ring11', workloads.ring_oscillator(stages=11), 1e-9),
                              ('adder0', workloads.mc_adder(0), 1.5e-9)]:
-        for how, opt in [('gmin', {'noopiter': 1}), ('src', {'noopiter': 1, 'gminsteps': 0})]:
+        for how, opt in [('gmin', {'noopiter': 1}), ('src', {'noopiter': 1, 'gminsteps': 0}),
+                         ('s3gmin', {'noopiter': 1, 'gminsteps': 5}),
+                         ('s3src', {'noopiter': 1, 'gminsteps': 0, 'srcsteps': 5})]:
             d = dict(desc)
             d['options'] = dict(opt)
             (eng,), info, _ = _engine([d], tstop)

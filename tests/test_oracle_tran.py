@@ -153,18 +153,27 @@ def test_gear_rc_transient():
     assert np.abs(x[:, 2] - ref).max() < 2e-3
 
 
-@pytest.mark.parametrize('how', ['gmin', 'src'])
+OP_STEPPING = {
+    'gmin': {'noopiter': 1},
+    'src': {'noopiter': 1, 'gminsteps': 0},
+    's3gmin': {'noopiter': 1, 'gminsteps': 5},
+    's3src': {'noopiter': 1, 'gminsteps': 0, 'srcsteps': 5},
+}
+
+
+@pytest.mark.parametrize('how', list(OP_STEPPING))
 def test_operating_point_stepping_reaches_the_direct_solution(how):
-    """CKTop's stepping restatement (dynamic_gmin + new_gmin, or
-    gillespie_src) forced with noopiter converges to the operating point the
-    direct Newton finds, within the Newton tolerance, and the transient that
-    follows accepts the same timepoints."""
+    """CKTop's stepping restatement (dynamic_gmin + new_gmin, gillespie_src,
+    spice3_gmin in 5 steps, spice3_src in 5 steps) forced with noopiter
+    converges to the operating point the direct Newton finds, within the
+    Newton tolerance, and the transient that follows accepts the same
+    timepoints."""
     desc = workloads.ring_oscillator(stages=11)
     ref = po.Transient(dict(desc))
     rc, t0, x0 = ref.run(desc['tstep'], 0.5e-9)
     assert rc == 0
     d = dict(desc)
-    d['options'] = {'noopiter': 1} if how == 'gmin' else {'noopiter': 1, 'gminsteps': 0}
+    d['options'] = dict(OP_STEPPING[how])
     T = po.Transient(d)
     rc, t, x = T.run(desc['tstep'], 0.5e-9)
     assert rc == 0
