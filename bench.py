@@ -10,6 +10,9 @@ Workloads
          backward-Euler transient load each plus deterministic assembly of
          their 16 Jacobian entries into a random 2**21-row CSC matrix.  A step
          is one evaluation pass + one assembly pass.
+  rings: the uneven batch of 2048 ring oscillators (11..301 stages).
+  mesh : the 1024 x 1024 power-grid mesh's Newton matrix (~1.1M unknowns);
+         a step is one refactor + one solve by the whole-GPU multifrontal LU.
 
 Inputs are resident in HBM before the timed region.  Kernel durations come
 from HIP events recorded on the stream the kernels are launched on.
@@ -38,6 +41,8 @@ MEASURED_HBM_BYTES_PER_EVAL = {('k_tran_mos', 2): 1764, ('k_bsim4_eval', 2): 159
 # FP64 FLOP per evaluation from SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes
 # (FMA counted twice), same runs
 MEASURED_FLOP_PER_EVAL = {'k_tran_mos': 6617, 'k_bsim4_eval': 6568}
+# inverter loads per 1024 x 1024 mesh, scaled with the mesh area
+MESH_LOADS_PER_SIDE2 = 16384
 
 
 def log(*a):
@@ -372,6 +377,95 @@ def bench_rings(args):
     }
 
 
+def bench_mesh(args):
+    """One large system: the power-grid mesh's Newton matrix (1024 x 1024
+    RC mesh, 256 VDD pads, 16384 inverter loads; ~1.1M unknowns) refactored
+    and solved by the whole-GPU multifrontal LU.  A step is one refactor
+    (klu_refactor) plus one solve (klu_solve) of new values on the fixed
+    pattern, as one Newton iteration of the mesh needs."""
+    import torch
+    import ngmi
+    import workloads
+
+    side = args.mesh_side
+    loads = MESH_LOADS_PER_SIDE2 * side * side // (1024 * 1024)
+    t0 = time.time()
+    n, Ap, Ai, Ax = workloads.power_grid_jacobian(side, loads)
+    b = workloads.rng(5, side).uniform(-1, 1, n)
+    t1 = time.time()
+    lu = ngmi.SparseLU(n, Ap, Ai)
+    info = lu.info()
+    log('mesh %dx%d, %d loads: n %d, nnz %d, matrix %.1fs, analyze %.1fs, %s'
+        % (side, side, loads, n, len(Ai), t1 - t0, time.time() - t1, info))
+    dA = torch.from_numpy(Ax).cuda()
+    db0 = torch.from_numpy(b).cuda()
+    db = db0.clone()
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(args.warmup):
+        lu.factor(dA.data_ptr(), stream)
+        db.copy_(db0)
+        lu.solve(db.data_ptr(), stream)
+    torch.cuda.synchronize()
+    tf, ts = ngmi.Timer(), ngmi.Timer()
+    f_ms = s_ms = 0.0
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        tf.start(stream)
+        lu.factor(dA.data_ptr(), stream)
+        tf.stop(stream)
+        db.copy_(db0)
+        ts.start(stream)
+        lu.solve(db.data_ptr(), stream)
+        ts.stop(stream)
+        torch.cuda.synchronize()
+        f_ms += tf.ms()
+        s_ms += ts.ms()
+    wall = (time.perf_counter() - w0) / args.steps
+    x = db.cpu().numpy()
+    import scipy.sparse as sp
+    A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+    res = np.abs(A @ x - b).max() / (abs(A).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
+    f_ms /= args.steps
+    s_ms /= args.steps
+    tflops = info['flops'] / (f_ms * 1e-3) / 1e12
+    log('mesh: refactor %.3f ms (%.2f TFLOP/s FP64), solve %.3f ms, wall %.3f ms per step, '
+        'scaled residual %.2e' % (f_ms, tflops, s_ms, wall * 1e3, res))
+    return {
+        'metric': 'power-grid mesh refactor + solve per second',
+        'value': 1.0 / wall, 'unit': 'refactor+solve/s', 'ms_per_step': wall * 1e3, 'dtype': 'f64',
+        'config': {'workload': 'power-grid Newton matrix: %dx%d RC mesh (0.1 ohm, 10 fF), %d VDD pads, '
+                               '%d inverter loads; refactor + solve' % (side, side, side * side // 4096,
+                                                                        loads),
+                   'unknowns': n, 'nonzeros': int(len(Ai)), 'fronts': info['fronts'],
+                   'max_front': info['max_front'], 'lu_nonzeros': info['lu_nonzeros']},
+        'roofline': {'bound': 'mfma', 'achieved': tflops, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': tflops / FP64_PEAK_TFLOPS, 'traffic': None,
+                     'kernel': 'refactor (every k_sp_* launch of one ngmi_splu_factor)',
+                     'flops_per_refactor': info['flops'], 'refactor_ms': f_ms},
+        'extra': {'solve_ms': s_ms, 'refactor_ms': f_ms, 'scaled_residual': float(res),
+                  'front_bytes': info['front_doubles'] * 8},
+    }
+
+
+def cpu_baseline_mesh(side):
+    """The reference KLU (oracle/_ref) on the same matrix: klu_factor once,
+    then one timed klu_refactor and the solve, on one host core."""
+    sys.path.insert(0, os.path.join(REPO, 'oracle'))
+    import pyoracle as po
+    import workloads
+    loads = MESH_LOADS_PER_SIDE2 * side * side // (1024 * 1024)
+    n, Ap, Ai, Ax = workloads.power_grid_jacobian(side, loads)
+    b = workloads.rng(5, side).uniform(-1, 1, n)
+    got = po.klu_csc(n, Ap, Ai, Ax, b, refactors=1)
+    if got is None:
+        return None
+    _, tf, tr = got
+    return {'value': 1.0 / tr, 'unit': 'refactor+solve/s (refactor only)', 'cores': 1, 'kind': 'reference',
+            'sample': 'the reference KLU (klu_factor once, then one timed klu_refactor) on the same '
+                      '%dx%d mesh matrix; klu_factor took %.1f s' % (side, side, tf)}
+
+
 def learned_guess_run(args, b, probe, stream, plain_acc, plain_its):
     """Train the guess network on engine waveforms of held-out seeds, then
     run the same batch with the learned guess; returns its figures and the
@@ -486,7 +580,8 @@ def main():
                     help='timed steps (mc: whole .tran runs of the batch; default 1)')
     ap.add_argument('--warmup', type=int, default=None,
                     help='untimed steps before (mc default 1)')
-    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings'])
+    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings', 'mesh'])
+    ap.add_argument('--mesh-side', type=int, default=1024, help='mesh: nodes per side')
     ap.add_argument('--copy-bytes', type=int, default=1 << 30)
     ap.add_argument('--n', type=int, default=1 << 20)
     ap.add_argument('--circuits', type=int, default=None,
@@ -527,6 +622,9 @@ def main():
     if args.workload == 'mc' and not args.no_cpu_baseline:
         cpu = cpu_baseline_mc(args.tstop, procs=args.cpu_procs)
         log('cpu baseline: %s' % cpu)
+    if args.workload == 'mesh' and not args.no_cpu_baseline:
+        cpu = cpu_baseline_mesh(args.mesh_side)
+        log('cpu baseline: %s' % cpu)
     import torch
     if not torch.cuda.is_available():
         print('bench.py needs a GPU', file=sys.stderr)
@@ -550,6 +648,8 @@ def main():
     elif args.workload == 'rings':
         res = bench_rings(args)
         cpu = None
+    elif args.workload == 'mesh':
+        res = bench_mesh(args)
     else:
         res = bench_eval(args)
         cpu = None

@@ -149,6 +149,41 @@ void ngmi_gather_free(ngmi_gather *g);
 int ngmi_gather_run(const ngmi_gather *g, const double *src, double *Ax, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* One large sparse system on the whole GPU (multifrontal LU)          */
+/* ------------------------------------------------------------------ */
+/* For a single circuit too large for the per-circuit LDS programs (the
+ * 1024 x 1024 power-grid mesh, ~1.1M unknowns).  n x n, CSC with 0-based
+ * rows ascending per column, as ngmi_csc_from_elements writes it.
+ *
+ * ngmi_splu_analyze replaces klu_analyze (klu_analyze.c:257-488, called from
+ * SMPpreOrder, klusmp.c:1255-1285) and fixes the pivot sequence there, where
+ * KLU's klu_factor (klu_factor.c:15-260) picks pivots numerically: a column
+ * with no structural diagonal (a voltage-source branch) is paired with a
+ * node it is incident to and their rows exchanged; the order is nested
+ * dissection.  Host-only: it touches no GPU.  NGMI_E_SINGULAR when an
+ * unknown without a diagonal has no node to pair with. */
+typedef struct ngmi_splu ngmi_splu;
+int ngmi_splu_analyze(int n, const int64_t *Ap, const int *Ai, ngmi_splu **out);
+void ngmi_splu_free(ngmi_splu *s);
+/* sizes: fronts, tree heights, largest front, front storage in doubles,
+ * nonzeros of L + U (dense fronts), factorization flops */
+int ngmi_splu_info(const ngmi_splu *s, int64_t *fronts, int64_t *heights, int64_t *max_front,
+                   int64_t *front_doubles, int64_t *lu_nonzeros, double *flops);
+/* front f (postorder): pivot count, boundary count, parent (-1 at a root),
+ * and its ns + nb unknowns (pivots, then boundary) */
+int ngmi_splu_front(const ngmi_splu *s, int f, int *ns, int *nb, int *parent, int *idx);
+/* row r of the factored matrix is row rowsrc[r] of A (n entries) */
+int ngmi_splu_rowsrc(const ngmi_splu *s, int *rowsrc);
+/* klu_refactor (klu_refactor.c:18-482, via SMPluFac, klusmp.c:573-624):
+ * numeric factorization of the device values Ax (nnz doubles, the pattern's
+ * order).  The first call allocates the device fronts.  Synchronizes stream;
+ * NGMI_E_SINGULAR on a zero or non-finite pivot. */
+int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream);
+/* klu_solve (klu_solve.c:14-400, via SMPsolve, klusmp.c:950-1011): b (n
+ * doubles, device) is overwritten with the solution.  Asynchronous. */
+int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Circuits and batched transient analysis                             */
 /* ------------------------------------------------------------------ */
 typedef struct ngmi_circuit ngmi_circuit;

@@ -250,6 +250,69 @@ class Gather:
             pass
 
 
+class SparseLU:
+    """One large sparse system factored by the whole GPU (ngmi_splu_*): the
+    klu_analyze / klu_refactor / klu_solve trio for circuits beyond the
+    per-circuit programs.  Ap (n+1) and Ai are a CSC pattern with ascending
+    rows; analysis runs on the host at construction, factor() and solve()
+    take device pointers (e.g. torch tensors' data_ptr())."""
+
+    def __init__(self, n, Ap, Ai):
+        L = lib()
+        vp, i64 = ctypes.c_void_p, ctypes.c_int64
+        L.ngmi_splu_analyze.argtypes = [ctypes.c_int, vp, vp, ctypes.POINTER(vp)]
+        L.ngmi_splu_free.argtypes = [vp]
+        L.ngmi_splu_info.argtypes = [vp] + [ctypes.POINTER(i64)] * 5 + [ctypes.POINTER(ctypes.c_double)]
+        L.ngmi_splu_front.argtypes = [vp, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3 + [vp]
+        L.ngmi_splu_rowsrc.argtypes = [vp, vp]
+        L.ngmi_splu_factor.argtypes = [vp, vp, vp]
+        L.ngmi_splu_solve.argtypes = [vp, vp, vp]
+        self.n = int(n)
+        Ap = np.ascontiguousarray(Ap, dtype=np.int64)
+        Ai = np.ascontiguousarray(Ai, dtype=np.int32)
+        h = vp()
+        check(L.ngmi_splu_analyze(self.n, Ap.ctypes.data, Ai.ctypes.data, ctypes.byref(h)), 'splu_analyze')
+        self.h = h.value
+
+    def info(self):
+        v = [ctypes.c_int64() for _ in range(5)]
+        fl = ctypes.c_double()
+        check(lib().ngmi_splu_info(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(fl)), 'splu_info')
+        keys = ('fronts', 'heights', 'max_front', 'front_doubles', 'lu_nonzeros')
+        out = {k: x.value for k, x in zip(keys, v)}
+        out['flops'] = fl.value
+        return out
+
+    def front(self, f):
+        """(pivot count, boundary count, parent, unknowns) of front f"""
+        ns, nb, par = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L = lib()
+        check(L.ngmi_splu_front(self.h, f, ctypes.byref(ns), ctypes.byref(nb), ctypes.byref(par), None),
+              'splu_front')
+        idx = np.zeros(ns.value + nb.value, dtype=np.int32)
+        check(L.ngmi_splu_front(self.h, f, ctypes.byref(ns), ctypes.byref(nb), ctypes.byref(par),
+                                idx.ctypes.data), 'splu_front')
+        return ns.value, nb.value, par.value, idx
+
+    def rowsrc(self):
+        out = np.zeros(self.n, dtype=np.int32)
+        check(lib().ngmi_splu_rowsrc(self.h, out.ctypes.data), 'splu_rowsrc')
+        return out
+
+    def factor(self, Ax_ptr, stream=0):
+        check(lib().ngmi_splu_factor(self.h, Ax_ptr, stream), 'splu_factor')
+
+    def solve(self, b_ptr, stream=0):
+        check(lib().ngmi_splu_solve(self.h, b_ptr, stream), 'splu_solve')
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().ngmi_splu_free(self.h)
+        except Exception:
+            pass
+
+
 class Timer:
     """HIP events recorded on the stream the timed kernels run on."""
 

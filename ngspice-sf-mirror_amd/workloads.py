@@ -238,3 +238,85 @@ def deck(desc, title='engine workload'):
     out.append('.tran %r %r' % (float(desc['tstep']), float(desc['tstop'])))
     out.append('.end')
     return '\n'.join(out) + '\n'
+
+
+MESH_SIDE = 1024
+MESH_LOADS = 16384
+MESH_PAD_EVERY = 64
+MESH_G = 10.0           # 0.1 ohm segments
+MESH_C = 10e-15         # per node
+MESH_LOAD_C = 1e-15     # inverter output load
+
+
+def power_grid_jacobian(side=MESH_SIDE, loads=MESH_LOADS, pad_every=MESH_PAD_EVERY, seed=0, h=10e-12,
+                        gmin=1e-12):
+    """The Newton matrix of the power-grid mesh at one trapezoidal step h:
+    a side x side RC mesh (0.1 ohm segments, 10 fF per node to ground),
+    VDD pads (voltage sources to ground) every pad_every nodes in both
+    directions, and `loads` CMOS inverters at seeded random mesh nodes
+    (Philox stream (4, seed)), each PMOS drawing from its mesh node, its
+    input driven by its own clock source, 1 fF on its output.  MOSFET
+    small-signal values are seeded draws in the ranges a 45 nm inverter
+    takes (gds 1e-6..1e-4 S, gm 0..2e-4 S, gate capacitances 0.05..0.2
+    fF), stamped as BSIM4load stamps a drain/source row (b4ld.c:5056-5389);
+    capacitors enter as (2/h) C, gmin on every node diagonal
+    (LoadGmin_CSC).
+
+    Unknowns (0-based, ground removed): mesh node (i, j) at i*side + j,
+    inverter k's input at side^2 + 2k and output at side^2 + 2k + 1, then
+    the pad branches, then the clock-source branches.  Returns (n, Ap, Ai,
+    Ax) in CSC with ascending rows, Ap int64, Ai int32."""
+    import scipy.sparse as sp
+    g = rng(4, seed)
+    nm = side * side
+    ag0 = 2.0 / h
+    pads = [(a * pad_every) * side + b * pad_every for a in range(side // pad_every)
+            for b in range(side // pad_every)]
+    npad = len(pads)
+    n = nm + 2 * loads + npad + loads
+    rows, cols, vals = [], [], []
+
+    def add(r, c, v):
+        rows.append(np.asarray(r, dtype=np.int64))
+        cols.append(np.asarray(c, dtype=np.int64))
+        vals.append(np.broadcast_to(np.asarray(v, dtype=np.float64), np.shape(r)).copy())
+
+    def two_terminal(a, b, gv):
+        add(a, a, gv), add(b, b, gv), add(a, b, -gv), add(b, a, -gv)
+
+    ij = np.arange(nm).reshape(side, side)
+    two_terminal(ij[:, :-1].ravel(), ij[:, 1:].ravel(), MESH_G)
+    two_terminal(ij[:-1, :].ravel(), ij[1:, :].ravel(), MESH_G)
+    node_diag = np.arange(nm + 2 * loads)
+    add(node_diag, node_diag, gmin)
+    add(np.arange(nm), np.arange(nm), ag0 * MESH_C)
+    pads = np.array(pads, dtype=np.int64)
+    br = nm + 2 * loads + np.arange(npad)
+    add(pads, br, 1.0), add(br, pads, 1.0)
+    k = np.arange(loads)
+    tap = g.integers(0, nm, size=loads)
+    vin = nm + 2 * k
+    out = vin + 1
+    cb = nm + 2 * loads + npad + k
+    add(vin, cb, 1.0), add(cb, vin, 1.0)
+    add(out, out, ag0 * MESH_LOAD_C)
+    gds_n, gds_p = g.uniform(1e-6, 1e-4, loads), g.uniform(1e-6, 1e-4, loads)
+    gm_n, gm_p = g.uniform(0, 2e-4, loads), g.uniform(0, 2e-4, loads)
+    cgd_n, cgd_p = g.uniform(0.05e-15, 0.2e-15, loads), g.uniform(0.05e-15, 0.2e-15, loads)
+    cgs_n, cgs_p = g.uniform(0.05e-15, 0.2e-15, loads), g.uniform(0.05e-15, 0.2e-15, loads)
+    # NMOS d=out g=in s=b=ground: only the drain row remains
+    add(out, out, gds_n), add(out, vin, gm_n)
+    # PMOS d=out g=in s=b=tap (gmbs cancels with the bulk on the source)
+    add(out, out, gds_p), add(out, vin, gm_p), add(out, tap, -(gds_p + gm_p))
+    add(tap, out, -gds_p), add(tap, vin, -gm_p), add(tap, tap, gds_p + gm_p)
+    # gate capacitances as (2/h) C two-terminal stamps
+    two_terminal(vin, out, ag0 * (cgd_n + cgd_p))
+    add(vin, vin, ag0 * cgs_n)
+    two_terminal(vin, tap, ag0 * cgs_p)
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    v = np.concatenate(vals)
+    A = sp.csc_matrix((v, (r, c)), shape=(n, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    return n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64)

@@ -309,3 +309,46 @@ class Transient:
             _tran_protos().or_tran_free(self.h)
         except Exception:
             pass
+
+
+def klu_csc(n, Ap, Ai, Ax, b, pivrel=1e-3, refactors=0):
+    """A x = b by the reference KLU: klu_analyze + klu_factor + klu_solve
+    (through klu_glue.c), then `refactors` timed klu_refactor calls on the
+    same values.  Returns (x, factor seconds, seconds per refactor), or
+    None when oracle/_ref is not built."""
+    import time
+    import numpy as np
+    K = klu_ref()
+    if K is None:
+        return None
+    vp = ctypes.c_void_p
+    K.kg_new.restype = vp
+    K.kg_new.argtypes = [ctypes.c_int, vp, vp]
+    K.kg_factor.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double]
+    K.kg_solve.argtypes = [vp, vp]
+    K.kg_free.argtypes = [vp]
+    Ap32 = np.ascontiguousarray(Ap, dtype=np.int32)
+    Ai32 = np.ascontiguousarray(Ai, dtype=np.int32)
+    Axc = np.ascontiguousarray(Ax, dtype=np.float64)
+    h = K.kg_new(int(n), Ap32.ctypes.data, Ai32.ctypes.data)
+    if not h:
+        raise RuntimeError('klu_analyze failed')
+    try:
+        t = time.perf_counter()
+        rc = K.kg_factor(h, Axc.ctypes.data, 1, pivrel)
+        tf = time.perf_counter() - t
+        if rc != 0:
+            raise RuntimeError('klu_factor returned %d' % rc)
+        tr = 0.0
+        if refactors:
+            t = time.perf_counter()
+            for _ in range(refactors):
+                if K.kg_factor(h, Axc.ctypes.data, 0, pivrel) != 0:
+                    raise RuntimeError('klu_refactor failed')
+            tr = (time.perf_counter() - t) / refactors
+        x = np.array(b, dtype=np.float64, copy=True)
+        if K.kg_solve(h, x.ctypes.data) != 0:
+            raise RuntimeError('klu_solve failed')
+        return x, tf, tr
+    finally:
+        K.kg_free(h)
