@@ -38,6 +38,15 @@ constexpr int WG = 256;
 constexpr int NB = SPLU_NB;
 constexpr int TL = SPLU_TILE;
 
+/* a wave-uniform lane's double, through two scalar readlanes */
+__device__ __forceinline__ double rdlane(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __global__ void __launch_bounds__(WG) k_sp_zero(double *__restrict__ F, int64_t n, int *__restrict__ flag)
 {
     const int64_t stride = (int64_t)gridDim.x * WG;
@@ -89,48 +98,73 @@ __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict_
     if (blockIdx.y > 0 && (int64_t)blockIdx.y * WG >= 2 * (int64_t)m) return;
     double *Fr = F + d.off[f];
     __shared__ double D[NB][NB + 1];
+    __shared__ double dinv[NB];
     const int tid = threadIdx.x;
-    for (int e = tid; e < NB * NB; e += WG) {
-        const int i = e % NB, j = e / NB;
-        D[i][j] = (i < w && j < w) ? Fr[(int64_t)(kb + j) * nf + kb + i] : (i == j ? 1.0 : 0.0);
+    if (tid < 64) {
+        /* one wave: lane j holds column j; step k scales lane k's column by
+         * the pivot's reciprocal and every later column subtracts the
+         * multipliers, read from lane k, times its own row-k entry */
+        const int j = tid;
+        double c[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+            c[i] = (j < w && i < w) ? Fr[(int64_t)(kb + j) * nf + kb + i] : (i == j ? 1.0 : 0.0);
+        bool bad = false;
+        double diag = 1.0;
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const double piv = rdlane(c[k], k);
+            if (j == k) diag = piv;
+            bad |= !(fabs(piv) > 0.0) || !isfinite(piv);
+            const double inv = 1.0 / piv;
+#pragma unroll
+            for (int i = 0; i < NB; i++)
+                if (i > k && j == k) c[i] *= inv;
+            const double ukj = c[k];
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+                if (i > k) {
+                    const double l = rdlane(c[i], k);
+                    if (j > k) c[i] -= l * ukj;
+                }
+            }
+        }
+        if (j < NB) {
+#pragma unroll
+            for (int i = 0; i < NB; i++) D[i][j] = c[i];
+            dinv[j] = 1.0 / diag;
+        }
+        if (blockIdx.y == 0 && j == 0 && bad) flag[0] = 1;
+        if (blockIdx.y == 0 && j < w) {
+            double *df = dfac + (int64_t)blockIdx.x * NB * NB + (int64_t)j * w;
+#pragma unroll
+            for (int i = 0; i < NB; i++)
+                if (i < w) df[i] = c[i];
+        }
     }
     __syncthreads();
-    for (int k = 0; k < w - 1; k++) {
-        if (tid > k && tid < w) D[tid][k] /= D[k][k];
-        __syncthreads();
-        const int r = w - k - 1;
-        for (int e = tid; e < r * r; e += WG) {
-            const int i = k + 1 + e % r, j = k + 1 + e / r;
-            D[i][j] -= D[i][k] * D[k][j];
-        }
-        __syncthreads();
-    }
-    if (blockIdx.y == 0) {
-        if (tid == 0) {
-            bool bad = false;
-            for (int k = 0; k < w; k++) bad |= !(fabs(D[k][k]) > 0.0) || !isfinite(D[k][k]);
-            if (bad) flag[0] = 1;
-        }
-        double *df = dfac + (int64_t)blockIdx.x * NB * NB;
-        for (int e = tid; e < w * w; e += WG) df[e] = D[e % w][e / w];
-    }
     const int t = blockIdx.y * WG + tid;
     if (t < m) {
         const int i = t0 + t;
         double x[NB];
 #pragma unroll
         for (int k = 0; k < NB; k++) x[k] = k < w ? Fr[(int64_t)(kb + k) * nf + i] : 0.0;
-        /* column sweep; pinning each step's results keeps its products in
-         * the step, so the 528 LDS operands are never live at once */
+        /* column sweep.  Each step reads its row of U from LDS in one
+         * batch (the memory clobber keeps the batch in the step) and pins
+         * its results, so neither the 528 operands nor the products pile
+         * up in registers across steps. */
 #pragma unroll
         for (int k = 0; k < NB; k++) {
-            const double xk = x[k] / D[k][k];
+            asm volatile("" ::: "memory");
+            double u[NB];
+#pragma unroll
+            for (int q = k + 1; q < NB; q++) u[q] = D[k][q];
+            const double xk = x[k] * dinv[k];
             x[k] = xk;
 #pragma unroll
-            for (int q = k + 1; q < NB; q++) {
-                x[q] -= xk * D[k][q];
-                asm volatile("" : "+v"(x[q]));
-            }
+            for (int q = k + 1; q < NB; q++) x[q] -= xk * u[q];
+#pragma unroll
+            for (int q = k + 1; q < NB; q++) asm volatile("" : "+v"(x[q]));
         }
 #pragma unroll
         for (int k = 0; k < NB; k++)
@@ -143,12 +177,15 @@ __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict_
         for (int k = 0; k < NB; k++) y[k] = k < w ? col[k] : 0.0;
 #pragma unroll
         for (int k = 0; k < NB - 1; k++) {
+            asm volatile("" ::: "memory");
+            double l[NB];
+#pragma unroll
+            for (int q = k + 1; q < NB; q++) l[q] = D[q][k];
             const double yk = y[k];
 #pragma unroll
-            for (int q = k + 1; q < NB; q++) {
-                y[q] -= D[q][k] * yk;
-                asm volatile("" : "+v"(y[q]));
-            }
+            for (int q = k + 1; q < NB; q++) y[q] -= l[q] * yk;
+#pragma unroll
+            for (int q = k + 1; q < NB; q++) asm volatile("" : "+v"(y[q]));
         }
 #pragma unroll
         for (int k = 0; k < NB; k++)
@@ -217,25 +254,31 @@ __global__ void __launch_bounds__(WG) k_sp_perm(const int *__restrict__ rowsrc, 
     if (r < n) out[r] = b[rowsrc[r]];
 }
 
-/* forward substitution for the fronts of one height: r = P b on the
- * pivots, children's update vectors added, L11 and L21 applied */
-__global__ void __launch_bounds__(WG) k_sp_fwd(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
-                                               const double *__restrict__ bp, double *__restrict__ upd,
-                                               double *__restrict__ y)
+constexpr int SWG = 512;   /* triangle kernels: more rows in flight per CU */
+
+/* forward substitution, pivot block, for the fronts of one height: r = P b
+ * on the pivots plus the children's update vectors, then L11 by blocks of
+ * 32 (one wave solves the triangle, all threads apply it to the later
+ * pivot rows).  The boundary part of r goes to upd; k_sp_fwd_off then
+ * subtracts L21 y from it. */
+__global__ void __launch_bounds__(SWG) k_sp_fwd(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
+                                                const double *__restrict__ bp, double *__restrict__ upd,
+                                                double *__restrict__ y)
 {
     extern __shared__ double r[];
+    __shared__ double cy[NB];
     const int f = lev[blockIdx.x];
     const int nsf = d.ns[f], nbf = d.nb[f], nf = nsf + nbf;
     const double *Fr = F + d.off[f];
     const int *id = d.idx + d.idx_off[f];
     const int tid = threadIdx.x;
-    for (int k = tid; k < nf; k += WG) r[k] = k < nsf ? bp[id[k]] : 0.0;
+    for (int k = tid; k < nf; k += SWG) r[k] = k < nsf ? bp[id[k]] : 0.0;
     __syncthreads();
     for (int q = d.kid_ptr[f]; q < d.kid_ptr[f + 1]; q++) {
         const int c = d.kids[q];
         const int *rl = d.rel + d.idx_off[c] + d.ns[c];
         const double *u = upd + d.upd_off[c];
-        for (int k = tid; k < d.nb[c]; k += WG) r[rl[k]] += u[k];
+        for (int k = tid; k < d.nb[c]; k += SWG) r[rl[k]] += u[k];
         __syncthreads();
     }
     for (int kb = 0; kb < nsf; kb += NB) {
@@ -245,45 +288,97 @@ __global__ void __launch_bounds__(WG) k_sp_fwd(SpDev d, const int *__restrict__ 
             double lv[NB];
 #pragma unroll
             for (int k = 0; k < NB; k++)
-                lv[k] = (k < w && l > k && l < w) ? Fr[(int64_t)(kb + k) * nf + kb + l] : 0.0;
+                lv[k] = Fr[(int64_t)(kb + min(k, w - 1)) * nf + kb + min(l, w - 1)];
             double v = l < w ? r[kb + l] : 0.0;
 #pragma unroll
             for (int k = 0; k < NB; k++) {
-                const double yk = __shfl(v, k);
-                v -= lv[k] * yk;
+                const double yk = rdlane(v, k);
+                if (k < w && l > k && l < w) v -= lv[k] * yk;
             }
             if (l < w) r[kb + l] = v;
+            if (l < NB) cy[l] = l < w ? v : 0.0;
         }
         __syncthreads();
-        for (int i = kb + w + tid; i < nf; i += WG) {
+        for (int i = kb + w + tid; i < nsf; i += SWG) {
             double s = 0.0;
-            for (int k = 0; k < w; k++) s += Fr[(int64_t)(kb + k) * nf + i] * r[kb + k];
+#pragma unroll
+            for (int k = 0; k < NB; k++) s += Fr[(int64_t)(kb + min(k, w - 1)) * nf + i] * cy[k];
             r[i] -= s;
         }
         __syncthreads();
     }
-    for (int k = tid; k < nsf; k += WG) y[id[k]] = r[k];
+    for (int k = tid; k < nsf; k += SWG) y[id[k]] = r[k];
     double *uo = upd + d.upd_off[f];
-    for (int k = tid; k < nbf; k += WG) uo[k] = r[nsf + k];
+    for (int k = tid; k < nbf; k += SWG) uo[k] = r[nsf + k];
 }
 
-/* backward substitution for the fronts of one height, ancestors solved */
-__global__ void __launch_bounds__(WG) k_sp_bwd(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
-                                               const double *__restrict__ y, double *__restrict__ x)
+/* upd[f] -= L21 y_f: one boundary row per thread, blockIdx.y chunks of
+ * WG rows, the pivots' y staged in LDS */
+__global__ void __launch_bounds__(WG) k_sp_fwd_off(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
+                                                   const double *__restrict__ y, double *__restrict__ upd)
 {
-    extern __shared__ double r[];
+    __shared__ double yv[WG];
     const int f = lev[blockIdx.x];
     const int nsf = d.ns[f], nbf = d.nb[f], nf = nsf + nbf;
+    if ((int)blockIdx.y * WG >= nbf) return;
     const double *Fr = F + d.off[f];
     const int *id = d.idx + d.idx_off[f];
     const int tid = threadIdx.x;
-    for (int k = tid; k < nf; k += WG) r[k] = k < nsf ? y[id[k]] : x[id[k]];
-    __syncthreads();
-    for (int i = tid; i < nsf; i += WG) {
-        double s = 0.0;
-        for (int j = 0; j < nbf; j++) s += Fr[(int64_t)(nsf + j) * nf + i] * r[nsf + j];
-        r[i] -= s;
+    const int k = blockIdx.y * WG + tid;
+    const int kr = min(k, nbf - 1);
+    double s = 0.0;
+    for (int j0 = 0; j0 < nsf; j0 += WG) {
+        __syncthreads();
+        yv[tid] = j0 + tid < nsf ? y[id[j0 + tid]] : 0.0;
+        __syncthreads();
+        const int jn = min(WG, nsf - j0);
+        const double *col = Fr + (int64_t)j0 * nf + nsf + kr;
+#pragma unroll 8
+        for (int j = 0; j < jn; j++) s += col[(int64_t)j * nf] * yv[j];
     }
+    if (k < nbf) upd[d.upd_off[f] + k] -= s;
+}
+
+/* y_f -= U12 x_boundary for the fronts of one height (ancestors solved):
+ * one pivot row per thread, the boundary's x staged in LDS */
+__global__ void __launch_bounds__(WG) k_sp_bwd_off(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
+                                                   double *__restrict__ y, const double *__restrict__ x)
+{
+    __shared__ double xv[WG];
+    const int f = lev[blockIdx.x];
+    const int nsf = d.ns[f], nbf = d.nb[f], nf = nsf + nbf;
+    if ((int)blockIdx.y * WG >= nsf || nbf == 0) return;
+    const double *Fr = F + d.off[f];
+    const int *id = d.idx + d.idx_off[f];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.y * WG + tid;
+    const int ir = min(i, nsf - 1);
+    double s = 0.0;
+    for (int j0 = 0; j0 < nbf; j0 += WG) {
+        __syncthreads();
+        xv[tid] = j0 + tid < nbf ? x[id[nsf + j0 + tid]] : 0.0;
+        __syncthreads();
+        const int jn = min(WG, nbf - j0);
+        const double *col = Fr + (int64_t)(nsf + j0) * nf + ir;
+#pragma unroll 8
+        for (int j = 0; j < jn; j++) s += col[(int64_t)j * nf] * xv[j];
+    }
+    if (i < nsf) y[id[i]] -= s;
+}
+
+/* backward substitution, pivot block: U11 x = y by blocks of 32 from the
+ * last, one wave per triangle */
+__global__ void __launch_bounds__(SWG) k_sp_bwd(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
+                                                const double *__restrict__ y, double *__restrict__ x)
+{
+    extern __shared__ double r[];
+    __shared__ double cx[NB];
+    const int f = lev[blockIdx.x];
+    const int nsf = d.ns[f], nf = nsf + d.nb[f];
+    const double *Fr = F + d.off[f];
+    const int *id = d.idx + d.idx_off[f];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < nsf; k += SWG) r[k] = y[id[k]];
     __syncthreads();
     for (int kb = ((nsf - 1) / NB) * NB; kb >= 0; kb -= NB) {
         const int w = min(NB, nsf - kb);
@@ -292,27 +387,29 @@ __global__ void __launch_bounds__(WG) k_sp_bwd(SpDev d, const int *__restrict__ 
             double uv[NB];
 #pragma unroll
             for (int k = 0; k < NB; k++)
-                uv[k] = (k < w && l <= k) ? Fr[(int64_t)(kb + k) * nf + kb + l] : 1.0;
+                uv[k] = Fr[(int64_t)(kb + min(k, w - 1)) * nf + kb + min(l, w - 1)];
             double v = l < w ? r[kb + l] : 0.0;
 #pragma unroll
             for (int k = NB - 1; k >= 0; k--) {
                 if (k < w) {
                     if (l == k) v /= uv[k];
-                    const double xk = __shfl(v, k);
+                    const double xk = rdlane(v, k);
                     if (l < k) v -= uv[k] * xk;
                 }
             }
             if (l < w) r[kb + l] = v;
+            if (l < NB) cx[l] = l < w ? v : 0.0;
         }
         __syncthreads();
-        for (int i = tid; i < kb; i += WG) {
+        for (int i = tid; i < kb; i += SWG) {
             double s = 0.0;
-            for (int k = 0; k < w; k++) s += Fr[(int64_t)(kb + k) * nf + i] * r[kb + k];
+#pragma unroll
+            for (int k = 0; k < NB; k++) s += Fr[(int64_t)(kb + min(k, w - 1)) * nf + i] * cx[k];
             r[i] -= s;
         }
         __syncthreads();
     }
-    for (int k = tid; k < nsf; k += WG) x[id[k]] = r[k];
+    for (int k = tid; k < nsf; k += SWG) x[id[k]] = r[k];
 }
 
 enum { L_EXTADD, L_PANEL, L_UPDATE };
@@ -343,6 +440,7 @@ struct ngmi_splu {
     double *F = nullptr, *dfac = nullptr, *upd = nullptr, *vb = nullptr, *vy = nullptr, *vx = nullptr;
     std::vector<Step> plan;
     std::vector<int> lev_lds;   /* dynamic LDS bytes of the solve launches by height */
+    std::vector<int> lev_ns, lev_nb;   /* largest pivot and boundary counts by height */
 };
 
 static int splu_device(ngmi_splu *s)
@@ -380,6 +478,8 @@ static int splu_device(ngmi_splu *s)
     /* launch plan: for each height, the extend-add slots, then the panels */
     s->plan.clear();
     s->lev_lds.assign(S.H + 1, 0);
+    s->lev_ns.assign(S.H + 1, 0);
+    s->lev_nb.assign(S.H + 1, 0);
     for (int h = 0; h <= S.H; h++) {
         for (int sl = 0; sl < S.max_slots; sl++) {
             const int b = S.ea_ptr[(size_t)h * S.max_slots + sl], e = S.ea_ptr[(size_t)h * S.max_slots + sl + 1];
@@ -395,6 +495,9 @@ static int splu_device(ngmi_splu *s)
             maxnf = std::max(maxnf, S.ns[S.lev[q]] + S.nb[S.lev[q]]);
         }
         s->lev_lds[h] = maxnf * (int)sizeof(double);
+        s->lev_ns[h] = maxns;
+        s->lev_nb[h] = 0;
+        for (int q = S.lev_ptr[h]; q < S.lev_ptr[h + 1]; q++) s->lev_nb[h] = std::max(s->lev_nb[h], S.nb[S.lev[q]]);
         const unsigned nl = (unsigned)(S.lev_ptr[h + 1] - S.lev_ptr[h]);
         for (int p = 0; p * NB < maxns; p++) {
             int64_t chunks = 1, tiles = 1;
@@ -526,13 +629,19 @@ int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream)
     hipLaunchKernelGGL(k_sp_perm, dim3((S.n + WG - 1) / WG), dim3(WG), 0, st, s->d_rowsrc, b, s->vb, S.n);
     for (int h = 0; h <= S.H; h++) {
         const unsigned nl = (unsigned)(S.lev_ptr[h + 1] - S.lev_ptr[h]);
-        hipLaunchKernelGGL(k_sp_fwd, dim3(nl), dim3(WG), s->lev_lds[h], st, s->d, s->d_lev + S.lev_ptr[h], s->F,
-                           s->vb, s->upd, s->vy);
+        const int *lev = s->d_lev + S.lev_ptr[h];
+        hipLaunchKernelGGL(k_sp_fwd, dim3(nl), dim3(SWG), s->lev_lds[h], st, s->d, lev, s->F, s->vb, s->upd, s->vy);
+        if (s->lev_nb[h] > 0)
+            hipLaunchKernelGGL(k_sp_fwd_off, dim3(nl, (s->lev_nb[h] + WG - 1) / WG), dim3(WG), 0, st, s->d, lev,
+                               s->F, s->vy, s->upd);
     }
     for (int h = S.H; h >= 0; h--) {
         const unsigned nl = (unsigned)(S.lev_ptr[h + 1] - S.lev_ptr[h]);
-        hipLaunchKernelGGL(k_sp_bwd, dim3(nl), dim3(WG), s->lev_lds[h], st, s->d, s->d_lev + S.lev_ptr[h], s->F,
-                           s->vy, s->vx);
+        const int *lev = s->d_lev + S.lev_ptr[h];
+        if (s->lev_nb[h] > 0)
+            hipLaunchKernelGGL(k_sp_bwd_off, dim3(nl, (s->lev_ns[h] + WG - 1) / WG), dim3(WG), 0, st, s->d, lev,
+                               s->F, s->vy, s->vx);
+        hipLaunchKernelGGL(k_sp_bwd, dim3(nl), dim3(SWG), s->lev_lds[h], st, s->d, lev, s->F, s->vy, s->vx);
     }
     NGMI_HIP(hipGetLastError());
     NGMI_HIP(hipMemcpyAsync(b, s->vx, (size_t)S.n * sizeof(double), hipMemcpyDeviceToDevice, st));
