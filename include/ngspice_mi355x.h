@@ -290,6 +290,18 @@ int ngmi_batch_neighbours(const ngmi_batch *b, int *out);
  * guess kernels' total milliseconds of the last profiled run */
 int ngmi_batch_guess_stats(const ngmi_batch *b, int c, int64_t *guessed, int64_t *fallbacks,
                            double *guess_ms);
+/* Live data stream, the batched counterpart of SendData / OUTpData
+ * (sharedspice.c:2280, outitf.c:633): while ngmi_batch_tran runs, after
+ * every 32 Newton passes and once at its end, cb receives each watched
+ * circuit's points recorded since its last call, on the calling thread:
+ * circuit, index of the first point, count, t[count] and x[count][nprobe].
+ * Its return value is ignored, as SendData's is.  circuits = NULL watches
+ * all; cb = NULL turns the stream off.  Points past max_points are not
+ * recorded and are not sent. */
+typedef int (*ngmi_data_cb)(int circuit, int64_t first, int64_t count, const double *t, const double *x,
+                            int nprobe, void *user);
+int ngmi_batch_set_data_callback(ngmi_batch *b, ngmi_data_cb cb, const int *circuits, int ncircuits,
+                                 void *user);
 /* per circuit: status (0 finished, -1 running, else the error number),
  * recorded points, accepted / rejected timepoints, Newton iterations */
 int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int64_t *accepted,

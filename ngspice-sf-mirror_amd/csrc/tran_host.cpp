@@ -133,6 +133,11 @@ struct ngmi_batch {
     int *grow = nullptr, *gcount = nullptr;
     double2 *gbase = nullptr;   /* per row: linear extrapolation, activity */
     unsigned long long counters[2] = {0, 0};
+    /* live data stream: watched circuits and the points already handed out */
+    ngmi_data_cb data_cb = nullptr;
+    void *data_user = nullptr;
+    std::vector<int> watch;
+    std::vector<int64_t> sent;
     ~ngmi_batch()
     {
         for (void *p : allocs) (void)hipFree(p);
@@ -1195,6 +1200,33 @@ int ngmi_batch_circuit_info(const ngmi_batch *b, int c, int *group, int *n_unkno
 
 /* TRANinit (traninit.c:17-40) + DCtran start (dctran.c:117-200) per circuit,
  * then passes until every circuit is done or max_passes is reached. */
+/* hand every watched circuit's points recorded since the last call to the
+ * data callback (the device is idle: called after the pass loop's sync) */
+static int send_data(ngmi_batch *b)
+{
+    std::vector<double> t, x;
+    for (size_t w = 0; w < b->watch.size(); w++) {
+        const int c = b->watch[w];
+        CktCtl k;
+        NGMI_HIP(hipMemcpy(&k, b->ctl_of(c), sizeof(k), hipMemcpyDeviceToHost));
+        const int64_t have = std::min<int64_t>(k.npts, b->cap_pts);
+        const int64_t first = b->sent[w];
+        if (have <= first) continue;
+        const TranDev &D = b->group_of(c).D;
+        const size_t lc = (size_t)b->where[c].y;
+        const int64_t cnt = have - first;
+        t.resize((size_t)cnt);
+        x.resize((size_t)cnt * D.nprobe);
+        NGMI_HIP(hipMemcpy(t.data(), D.out_t + lc * b->cap_pts + first, cnt * sizeof(double),
+                           hipMemcpyDeviceToHost));
+        NGMI_HIP(hipMemcpy(x.data(), D.out_x + (lc * b->cap_pts + first) * D.nprobe,
+                           cnt * D.nprobe * sizeof(double), hipMemcpyDeviceToHost));
+        b->sent[w] = have;
+        (void)b->data_cb(c, first, cnt, t.data(), x.data(), D.nprobe, b->data_user);
+    }
+    return NGMI_OK;
+}
+
 int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int64_t max_passes,
                     void *stream, int64_t *passes_out)
 {
@@ -1271,6 +1303,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         NGMI_HIP(hipStreamSynchronize(s));
     }
     NGMI_HIP(hipMemsetAsync(b->done_count, 0, sizeof(int), s));
+    std::fill(b->sent.begin(), b->sent.end(), 0);
     const TopoGroup &G0 = *b->groups[0];
     const bool one = b->groups.size() == 1;
     const B4Tab *tab0 = set_table(G0.set);
@@ -1364,6 +1397,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
                 }
             }
         }
+        if (rc == NGMI_OK && b->data_cb) rc = send_data(b);
         if (done >= C) break;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
@@ -1387,6 +1421,32 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     if (passes_out) *passes_out = pass;
     if (rc != NGMI_OK) return rc;
     return done >= C ? NGMI_OK : NGMI_E_ITERLIM;
+}
+
+int ngmi_batch_set_data_callback(ngmi_batch *b, ngmi_data_cb cb, const int *circuits, int ncircuits,
+                                 void *user)
+{
+    if (!b) return NGMI_E_PANIC;
+    b->data_cb = cb;
+    b->data_user = user;
+    b->watch.clear();
+    if (cb) {
+        if (!circuits || ncircuits <= 0) {
+            for (int c = 0; c < b->C; c++) b->watch.push_back(c);
+        } else {
+            for (int i = 0; i < ncircuits; i++) {
+                if (circuits[i] < 0 || circuits[i] >= b->C) {
+                    set_error("data callback: no circuit %d", circuits[i]);
+                    b->data_cb = nullptr;
+                    b->watch.clear();
+                    return NGMI_E_PARMVAL;
+                }
+                b->watch.push_back(circuits[i]);
+            }
+        }
+    }
+    b->sent.assign(b->watch.size(), 0);
+    return NGMI_OK;
 }
 
 int ngmi_batch_set_guess(ngmi_batch *b, ngmi_mlp *net)

@@ -640,6 +640,33 @@ class Batch:
         return dict(status=st.value, npts=a[0].value, accepted=a[1].value, rejected=a[2].value,
                     iterations=a[3].value)
 
+    _DATA_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                ctypes.c_int, ctypes.c_void_p)
+
+    def set_data_callback(self, fn, circuits=None):
+        """Live data stream (ngmi_batch_set_data_callback): fn(circuit,
+        first, t, x) receives each watched circuit's newly recorded points
+        (numpy copies, x as [count, nprobe]) during tran(); None turns it
+        off."""
+        L = lib()
+        L.ngmi_batch_set_data_callback.argtypes = [ctypes.c_void_p, self._DATA_CB, ctypes.c_void_p,
+                                                   ctypes.c_int, ctypes.c_void_p]
+        if fn is None:
+            self._data_cb = None
+            check(L.ngmi_batch_set_data_callback(self.h, self._DATA_CB(), None, 0, None), 'data callback')
+            return
+
+        def tramp(c, first, count, t, x, nprobe, user):
+            fn(c, first, np.ctypeslib.as_array(t, (count,)).copy(),
+               np.ctypeslib.as_array(x, (count, nprobe)).copy())
+            return 0
+
+        self._data_cb = self._DATA_CB(tramp)
+        ids = None if circuits is None else np.ascontiguousarray(circuits, dtype=np.int32)
+        check(L.ngmi_batch_set_data_callback(self.h, self._data_cb, None if ids is None else ids.ctypes.data,
+                                             0 if ids is None else len(ids), None), 'data callback')
+
     def waveform(self, c, nprobe=None):
         if nprobe is None:
             nprobe = self.circuit_info(c)['nprobe']

@@ -106,3 +106,27 @@ def test_raw_file_holds_the_waveform(tmp_path):
     data = np.frombuffer(blob[at:], dtype=np.float64).reshape(len(t), len(names) + 1)
     assert np.array_equal(data[:, 0], t) and np.array_equal(data[:, 1:], x)
     print('raw files: %d points x %d variables, ASCII and binary' % data.shape, flush=True)
+
+
+def test_live_data_stream_matches_the_waveform():
+    """ngmi_batch_set_data_callback (the SendData / OUTpData stream): the
+    points handed out during the run, concatenated, are the recorded
+    waveform bit for bit, in order, for each watched circuit."""
+    descs = [workloads.ring_oscillator(stages=11, tstop=2e-9),
+             workloads.ring_oscillator(stages=11, cload=1.5e-15, tstop=2e-9)]
+    print('case: live data stream, two 11-stage rings, .tran 10p 2n, watching circuit 1', flush=True)
+    b = ngmi.Batch([ngmi.Circuit(d) for d in descs], max_points=200000)
+    got = []
+    b.set_data_callback(lambda c, first, t, x: got.append((c, first, t, x)), circuits=[1])
+    rc, passes = b.tran(10e-12, 2e-9)
+    assert rc == 0, (rc, ngmi.last_error())
+    assert len(got) > 3 and all(g[0] == 1 for g in got)
+    firsts = [g[1] for g in got]
+    assert firsts[0] == 0
+    assert all(firsts[i + 1] == firsts[i] + len(got[i][2]) for i in range(len(got) - 1))
+    t = np.concatenate([g[2] for g in got])
+    x = np.concatenate([g[3] for g in got])
+    tw, xw = b.waveform(1)
+    print('%d calls, %d points streamed, %d recorded' % (len(got), len(t), len(tw)), flush=True)
+    assert np.array_equal(t, tw) and np.array_equal(x, xw)
+    b.set_data_callback(None)
