@@ -13,6 +13,8 @@ Workloads
   rings: the uneven batch of 2048 ring oscillators (11..301 stages).
   mesh : the 1024 x 1024 power-grid mesh's Newton matrix (~1.1M unknowns);
          a step is one refactor + one solve by the whole-GPU multifrontal LU.
+  c3   : the rings' transient with the mesh's refactor + solve running
+         alongside on a second stream (configuration 3 together).
 
 Inputs are resident in HBM before the timed region.  Kernel durations come
 from HIP events recorded on the stream the kernels are launched on.
@@ -448,6 +450,72 @@ def bench_mesh(args):
     }
 
 
+def bench_c3(args):
+    """Configuration 3 together: the uneven batch of 2048 rings runs its
+    transient on one stream while a host thread refactors and solves the
+    1024 x 1024 power-grid matrix on a second stream, over and over, for as
+    long as the rings run.  Reports the rings' rate with the mesh alongside,
+    and the mesh's refactor+solve rate in the same window."""
+    import threading
+    import torch
+    import ngmi
+    import workloads
+
+    res = bench_rings(args)   # alone first: its warm-up and timed runs
+    alone = res['value']
+    descs = workloads.uneven_rings(count=args.circuits, tstop=args.tstop)
+    b = ngmi.Batch([ngmi.Circuit(d) for d in descs], probe=[1], max_points=args.max_points)
+    C = len(descs)
+    del descs
+    side = args.mesh_side
+    loads = MESH_LOADS_PER_SIDE2 * side * side // (1024 * 1024)
+    n, Ap, Ai, Ax = workloads.power_grid_jacobian(side, loads)
+    lu = ngmi.SparseLU(n, Ap, Ai)
+    dA = torch.from_numpy(Ax).cuda()
+    db0 = torch.from_numpy(workloads.rng(5, side).uniform(-1, 1, n)).cuda()
+    db = db0.clone()
+    ms = torch.cuda.Stream()
+    lu.factor(dA.data_ptr(), ms.cuda_stream)
+    torch.cuda.synchronize()
+    stop = threading.Event()
+    count = [0]
+
+    def mesh_loop():
+        with torch.cuda.stream(ms):
+            while not stop.is_set():
+                lu.factor(dA.data_ptr(), ms.cuda_stream)
+                db.copy_(db0)
+                lu.solve(db.data_ptr(), ms.cuda_stream)
+                ms.synchronize()
+                count[0] += 1
+
+    rs = torch.cuda.Stream()
+    th = threading.Thread(target=mesh_loop)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    th.start()
+    rc, passes = b.tran(10e-12, args.tstop, stream=rs.cuda_stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    stop.set()
+    th.join()
+    if rc != 0:
+        raise RuntimeError('transient failed: %d %s' % (rc, ngmi.last_error()))
+    st = [b.stats(c) for c in range(C)]
+    acc = sum(s_['accepted'] for s_ in st)
+    log('c3: rings %.0f tp*c/s with the mesh alongside (%.0f alone); mesh %d refactor+solve in %.2f s '
+        '= %.1f/s alongside' % (acc / wall, alone, count[0], wall, count[0] / wall))
+    res['metric'] = 'uneven-batch transient timepoints x circuits/s with the power-grid LU alongside'
+    res['value'] = acc / wall
+    res['ms_per_step'] = wall * 1e3
+    res['config']['workload'] += ('; alongside: refactor + solve of the %dx%d power-grid matrix '
+                                  '(%d unknowns) on a second stream' % (side, side, n))
+    res['extra'].update({'rings_alone': alone, 'mesh_refactor_solve_per_s_alongside': count[0] / wall,
+                         'mesh_refactor_solves': count[0], 'accepted_alongside': acc,
+                         'failed_alongside': sum(1 for s_ in st if s_['status'])})
+    return res
+
+
 def cpu_baseline_mesh(side):
     """The reference KLU (oracle/_ref) on the same matrix: klu_factor once,
     then one timed klu_refactor and the solve, on one host core."""
@@ -580,7 +648,7 @@ def main():
                     help='timed steps (mc: whole .tran runs of the batch; default 1)')
     ap.add_argument('--warmup', type=int, default=None,
                     help='untimed steps before (mc default 1)')
-    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings', 'mesh'])
+    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings', 'mesh', 'c3'])
     ap.add_argument('--mesh-side', type=int, default=1024, help='mesh: nodes per side')
     ap.add_argument('--copy-bytes', type=int, default=1 << 30)
     ap.add_argument('--n', type=int, default=1 << 20)
@@ -607,13 +675,13 @@ def main():
     ap.add_argument('--train-steps', type=int, default=1500)
     args = ap.parse_args()
     if args.circuits is None:
-        args.circuits = 2048 if args.workload == 'rings' else 4096
+        args.circuits = 2048 if args.workload in ('rings', 'c3') else 4096
     if args.tstop is None:
-        args.tstop = 10e-9 if args.workload == 'rings' else 20e-9
+        args.tstop = 10e-9 if args.workload in ('rings', 'c3') else 20e-9
     if args.steps is None:
-        args.steps = 1 if args.workload in ('mc', 'rings') else 20
+        args.steps = 1 if args.workload in ('mc', 'rings', 'c3') else 20
     if args.warmup is None:
-        args.warmup = 1 if args.workload in ('mc', 'rings') else 3
+        args.warmup = 1 if args.workload in ('mc', 'rings', 'c3') else 3
     if args.gpus != 1:
         print('bench.py runs on one GPU only (the engine has no multi-GPU path); '
               'got --gpus %d' % args.gpus, file=sys.stderr)
@@ -650,6 +718,9 @@ def main():
         cpu = None
     elif args.workload == 'mesh':
         res = bench_mesh(args)
+    elif args.workload == 'c3':
+        res = bench_c3(args)
+        cpu = None
     else:
         res = bench_eval(args)
         cpu = None
