@@ -84,7 +84,11 @@ struct Nested {
         order.reserve(verts.size());
         bfs(verts[0], st, order);
         if (order.size() < verts.size()) {
-            /* disconnected: every component becomes its own subtree */
+            /* disconnected: every large component becomes its own subtree;
+             * small ones are packed together into leaves of up to LEAF
+             * vertices (block-diagonal dense fronts), which keeps the
+             * number of children per front, and so of extend-add
+             * launches, small */
             std::vector<std::vector<int>> comps;
             comps.push_back(order);
             for (int v : verts)
@@ -93,7 +97,19 @@ struct Nested {
                     bfs(v, st, c);
                     comps.push_back(std::move(c));
                 }
-            for (auto &c : comps) split(std::move(c), par);
+            std::vector<int> bin;
+            for (auto &c : comps) {
+                if ((int)c.size() > SPLU_LEAF) {
+                    split(std::move(c), par);
+                    continue;
+                }
+                if ((int)(bin.size() + c.size()) > SPLU_LEAF) {
+                    add(std::move(bin), par);
+                    bin.clear();
+                }
+                bin.insert(bin.end(), c.begin(), c.end());
+            }
+            if (!bin.empty()) add(std::move(bin), par);
             return;
         }
         const int s = order.back();
@@ -155,21 +171,11 @@ int splu_analyze(int n, const int64_t *Ap, const int *Ai, SpluSym &S)
     S.n = n;
     S.nnz = nnz;
 
-    /* structural diagonal and transposed pattern */
+    /* structural diagonal */
     std::vector<char> diag(n, 0);
-    std::vector<int64_t> rp(n + 1, 0);
     for (int c = 0; c < n; c++)
-        for (int64_t e = Ap[c]; e < Ap[c + 1]; e++) {
-            rp[Ai[e] + 1]++;
+        for (int64_t e = Ap[c]; e < Ap[c + 1]; e++)
             if (Ai[e] == c) diag[c] = 1;
-        }
-    for (int i = 0; i < n; i++) rp[i + 1] += rp[i];
-    std::vector<int> rc(nnz);
-    {
-        std::vector<int64_t> w(rp.begin(), rp.end() - 1);
-        for (int c = 0; c < n; c++)
-            for (int64_t e = Ap[c]; e < Ap[c + 1]; e++) rc[w[Ai[e]]++] = c;
-    }
     auto has = [&](int r, int c) {
         for (int64_t e = Ap[c]; e < Ap[c + 1]; e++)
             if (Ai[e] == r) return true;
