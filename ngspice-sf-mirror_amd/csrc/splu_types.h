@@ -13,7 +13,8 @@ namespace ngmi {
 constexpr int SPLU_LEAF = 64;          /* nested dissection stops at this many vertices */
 constexpr int SPLU_NB = 32;            /* panel width of the front factorization */
 constexpr int SPLU_TILE = 64;          /* trailing-update tile edge */
-constexpr int SPLU_MAX_FRONT = 8192;   /* a front's right-hand side fits 64 KB of LDS */
+constexpr int SPLU_MAX_FRONT = 8000;   /* a front's right-hand side and the solve kernels'
+                                          static LDS fit 64 KB */
 
 struct SpluSym {
     int n = 0;

@@ -1205,10 +1205,16 @@ int ngmi_batch_circuit_info(const ngmi_batch *b, int c, int *group, int *n_unkno
 static int send_data(ngmi_batch *b)
 {
     std::vector<double> t, x;
+    std::vector<std::vector<CktCtl>> ctl(b->groups.size());   /* one copy per group */
     for (size_t w = 0; w < b->watch.size(); w++) {
         const int c = b->watch[w];
-        CktCtl k;
-        NGMI_HIP(hipMemcpy(&k, b->ctl_of(c), sizeof(k), hipMemcpyDeviceToHost));
+        std::vector<CktCtl> &gk = ctl[b->where[c].x];
+        if (gk.empty()) {
+            const TopoGroup &G = *b->groups[b->where[c].x];
+            gk.resize(G.C);
+            NGMI_HIP(hipMemcpy(gk.data(), G.D.ctl, sizeof(CktCtl) * G.C, hipMemcpyDeviceToHost));
+        }
+        const CktCtl &k = gk[b->where[c].y];
         const int64_t have = std::min<int64_t>(k.npts, b->cap_pts);
         const int64_t first = b->sent[w];
         if (have <= first) continue;
