@@ -130,3 +130,39 @@ def test_unknown_without_a_pivot_is_rejected():
     with pytest.raises(ngmi.EngineError):
         ngmi.SparseLU(2, np.array([0, 2, 2]), np.array([0, 1]))
     assert 'no node to pair with' in ngmi.last_error()
+
+
+def test_bad_patterns_are_rejected():
+    with pytest.raises(ngmi.EngineError) as e:
+        ngmi.SparseLU(2, np.array([0, 1, 2]), np.array([0, 5]))
+    assert e.value.code == 11 and 'out of range' in ngmi.last_error()
+    with pytest.raises(ngmi.EngineError) as e:
+        ngmi.SparseLU(2, np.array([0, 2, 1]), np.array([0, 1]))
+    assert e.value.code == 11 and 'decrease' in ngmi.last_error()
+
+
+@pytest.mark.parametrize('seed', [0, 1])
+def test_small_dense_and_source_systems(seed):
+    """a 5x5 dense matrix (one leaf front) and a 3-node circuit with two
+    voltage sources (two row exchanges), against numpy's dense solve"""
+    g = np.random.default_rng(seed)
+    A = g.uniform(-1, 1, (5, 5)) + 5 * np.eye(5)
+    S = sp.csc_matrix(A)
+    x = frontal_solve(ngmi.SparseLU(5, S.indptr, S.indices), 5, S.indptr, S.indices, S.data,
+                      b := g.uniform(-1, 1, 5))
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-12, atol=1e-12)
+    # nodes 0..2, branches 3 (source on node 0) and 4 (source from node 2 to node 1)
+    M = np.zeros((5, 5))
+    for a, c, gv in ((0, 1, 1e-3), (1, 2, 2e-3)):
+        M[a, a] += gv; M[c, c] += gv; M[a, c] -= gv; M[c, a] -= gv
+    M[2, 2] += 1e-4
+    M[0, 3] = M[3, 0] = 1.0
+    M[2, 4] = M[4, 2] = 1.0
+    M[1, 4] = M[4, 1] = -1.0
+    S = sp.csc_matrix(M)
+    lu = ngmi.SparseLU(5, S.indptr, S.indices)
+    # each branch pairs with the first node in its column: 3 with 0, 4 with 1
+    assert list(np.flatnonzero(lu.rowsrc() != np.arange(5))) == [0, 1, 3, 4]
+    b = g.uniform(-1, 1, 5)
+    assert np.allclose(frontal_solve(lu, 5, S.indptr, S.indices, S.data, b), np.linalg.solve(M, b),
+                       rtol=1e-10, atol=1e-12)
