@@ -80,3 +80,53 @@ def test_zero_pivot_is_reported():
     with pytest.raises(ngmi.EngineError) as e:
         s.factor(dA.data_ptr())
     assert e.value.code == ngmi.E_SINGULAR
+
+
+def test_concurrent_with_a_transient_batch():
+    """The large-system LU refactoring and solving on its own stream and host
+    thread while a ring batch runs its transient on another (how the
+    configuration-3 bench runs them): both give bit for bit what they give
+    alone."""
+    import threading
+    print('case: 128x128 power grid LU x6 beside four rings .tran 10p 1n', flush=True)
+    n, Ap, Ai, Ax = workloads.power_grid_jacobian(128, 256, 32, seed=7)
+    b = workloads.rng(5, 7).uniform(-1, 1, n)
+    s = ngmi.SparseLU(n, Ap, Ai)
+    x_alone = _gpu(s, Ax, b)
+    descs = [workloads.ring_oscillator(stages=k, tstop=1e-9) for k in (11, 13, 21, 31)]
+
+    def rings():
+        bt = ngmi.Batch([ngmi.Circuit(d) for d in descs], max_points=200000)
+        return bt
+
+    b1 = rings()
+    rc, _ = b1.tran(10e-12, 1e-9)
+    assert rc == 0
+    alone = [b1.waveform(c) for c in range(len(descs))]
+    b2 = rings()
+    dA = torch.from_numpy(Ax).cuda()
+    db0 = torch.from_numpy(b).cuda()
+    ms = torch.cuda.Stream()
+    outs = []
+
+    def mesh_loop():
+        with torch.cuda.stream(ms):
+            for _ in range(6):
+                db = db0.clone()
+                s.factor(dA.data_ptr(), ms.cuda_stream)
+                s.solve(db.data_ptr(), ms.cuda_stream)
+                ms.synchronize()
+                outs.append(db.cpu().numpy())
+
+    rs = torch.cuda.Stream()
+    th = threading.Thread(target=mesh_loop)
+    th.start()
+    rc, _ = b2.tran(10e-12, 1e-9, stream=rs.cuda_stream)
+    th.join()
+    torch.cuda.synchronize()
+    assert rc == 0, ngmi.last_error()
+    assert len(outs) == 6 and all(np.array_equal(o, x_alone) for o in outs)
+    for c in range(len(descs)):
+        t, x = b2.waveform(c)
+        assert np.array_equal(t, alone[c][0]) and np.array_equal(x, alone[c][1])
+    print('6 concurrent refactor+solves and 4 ring waveforms identical to the runs alone', flush=True)
