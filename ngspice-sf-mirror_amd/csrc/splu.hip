@@ -312,58 +312,81 @@ __global__ void __launch_bounds__(SWG) k_sp_fwd(SpDev d, const int *__restrict__
     for (int k = tid; k < nbf; k += SWG) uo[k] = r[nsf + k];
 }
 
-/* upd[f] -= L21 y_f: one boundary row per thread, blockIdx.y chunks of
- * WG rows, the pivots' y staged in LDS */
+constexpr int OFF_ROWS = 64;   /* rows per workgroup of the off-diagonal products */
+
+/* upd[f] -= L21 y_f: 64 boundary rows per workgroup (blockIdx.y), the four
+ * waves taking interleaved 64-column slices of each 256-column chunk of y
+ * (staged in LDS); the four partial sums are added in wave order, so the
+ * result does not depend on timing */
 __global__ void __launch_bounds__(WG) k_sp_fwd_off(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
                                                    const double *__restrict__ y, double *__restrict__ upd)
 {
     __shared__ double yv[WG];
+    __shared__ double part[WG / 64][OFF_ROWS];
     const int f = lev[blockIdx.x];
     const int nsf = d.ns[f], nbf = d.nb[f], nf = nsf + nbf;
-    if ((int)blockIdx.y * WG >= nbf) return;
+    if ((int)blockIdx.y * OFF_ROWS >= nbf) return;
     const double *Fr = F + d.off[f];
     const int *id = d.idx + d.idx_off[f];
-    const int tid = threadIdx.x;
-    const int k = blockIdx.y * WG + tid;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int k = blockIdx.y * OFF_ROWS + lane;
     const int kr = min(k, nbf - 1);
     double s = 0.0;
     for (int j0 = 0; j0 < nsf; j0 += WG) {
         __syncthreads();
         yv[tid] = j0 + tid < nsf ? y[id[j0 + tid]] : 0.0;
         __syncthreads();
-        const int jn = min(WG, nsf - j0);
-        const double *col = Fr + (int64_t)j0 * nf + nsf + kr;
+        const int js = j0 + wv * 64;
+        const int jn = min(64, nsf - js);
+        const double *col = Fr + (int64_t)js * nf + nsf + kr;
 #pragma unroll 8
-        for (int j = 0; j < jn; j++) s += col[(int64_t)j * nf] * yv[j];
+        for (int j = 0; j < jn; j++) s += col[(int64_t)j * nf] * yv[wv * 64 + j];
     }
-    if (k < nbf) upd[d.upd_off[f] + k] -= s;
+    part[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && k < nbf) {
+        double t = part[0][lane];
+#pragma unroll
+        for (int q = 1; q < WG / 64; q++) t += part[q][lane];
+        upd[d.upd_off[f] + k] -= t;
+    }
 }
 
 /* y_f -= U12 x_boundary for the fronts of one height (ancestors solved):
- * one pivot row per thread, the boundary's x staged in LDS */
+ * 64 pivot rows per workgroup, the boundary's x staged in LDS and split
+ * over the four waves as above */
 __global__ void __launch_bounds__(WG) k_sp_bwd_off(SpDev d, const int *__restrict__ lev, const double *__restrict__ F,
                                                    double *__restrict__ y, const double *__restrict__ x)
 {
     __shared__ double xv[WG];
+    __shared__ double part[WG / 64][OFF_ROWS];
     const int f = lev[blockIdx.x];
     const int nsf = d.ns[f], nbf = d.nb[f], nf = nsf + nbf;
-    if ((int)blockIdx.y * WG >= nsf || nbf == 0) return;
+    if ((int)blockIdx.y * OFF_ROWS >= nsf || nbf == 0) return;
     const double *Fr = F + d.off[f];
     const int *id = d.idx + d.idx_off[f];
-    const int tid = threadIdx.x;
-    const int i = blockIdx.y * WG + tid;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int i = blockIdx.y * OFF_ROWS + lane;
     const int ir = min(i, nsf - 1);
     double s = 0.0;
     for (int j0 = 0; j0 < nbf; j0 += WG) {
         __syncthreads();
         xv[tid] = j0 + tid < nbf ? x[id[nsf + j0 + tid]] : 0.0;
         __syncthreads();
-        const int jn = min(WG, nbf - j0);
-        const double *col = Fr + (int64_t)(nsf + j0) * nf + ir;
+        const int js = j0 + wv * 64;
+        const int jn = min(64, nbf - js);
+        const double *col = Fr + (int64_t)(nsf + js) * nf + ir;
 #pragma unroll 8
-        for (int j = 0; j < jn; j++) s += col[(int64_t)j * nf] * xv[j];
+        for (int j = 0; j < jn; j++) s += col[(int64_t)j * nf] * xv[wv * 64 + j];
     }
-    if (i < nsf) y[id[i]] -= s;
+    part[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && i < nsf) {
+        double t = part[0][lane];
+#pragma unroll
+        for (int q = 1; q < WG / 64; q++) t += part[q][lane];
+        y[id[i]] -= t;
+    }
 }
 
 /* backward substitution, pivot block: U11 x = y by blocks of 32 from the
@@ -632,14 +655,14 @@ int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream)
         const int *lev = s->d_lev + S.lev_ptr[h];
         hipLaunchKernelGGL(k_sp_fwd, dim3(nl), dim3(SWG), s->lev_lds[h], st, s->d, lev, s->F, s->vb, s->upd, s->vy);
         if (s->lev_nb[h] > 0)
-            hipLaunchKernelGGL(k_sp_fwd_off, dim3(nl, (s->lev_nb[h] + WG - 1) / WG), dim3(WG), 0, st, s->d, lev,
+            hipLaunchKernelGGL(k_sp_fwd_off, dim3(nl, (s->lev_nb[h] + OFF_ROWS - 1) / OFF_ROWS), dim3(WG), 0, st, s->d, lev,
                                s->F, s->vy, s->upd);
     }
     for (int h = S.H; h >= 0; h--) {
         const unsigned nl = (unsigned)(S.lev_ptr[h + 1] - S.lev_ptr[h]);
         const int *lev = s->d_lev + S.lev_ptr[h];
         if (s->lev_nb[h] > 0)
-            hipLaunchKernelGGL(k_sp_bwd_off, dim3(nl, (s->lev_ns[h] + WG - 1) / WG), dim3(WG), 0, st, s->d, lev,
+            hipLaunchKernelGGL(k_sp_bwd_off, dim3(nl, (s->lev_ns[h] + OFF_ROWS - 1) / OFF_ROWS), dim3(WG), 0, st, s->d, lev,
                                s->F, s->vy, s->vx);
         hipLaunchKernelGGL(k_sp_bwd, dim3(nl), dim3(SWG), s->lev_lds[h], st, s->d, lev, s->F, s->vy, s->vx);
     }
