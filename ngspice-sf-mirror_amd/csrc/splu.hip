@@ -15,7 +15,7 @@
  *   k_sp_panel    per 32-column panel: the diagonal block factored in LDS
  *                 (Doolittle, no pivoting), then L21 rows and U12 columns by
  *                 triangular solves, one row or column per thread;
- *   k_sp_update   the trailing block minus L21 * U12, 64 x 64 tiles of 4 x 4
+ *   k_sp_update   the trailing block minus L21 * U12, 32 x 32 tiles of 2 x 2
  *                 per thread with both operands staged in LDS.
  * Solve: rows permuted (k_sp_perm), forward substitution leaves first with
  * per-front update vectors passed to the parent (k_sp_fwd), backward from
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict_
     }
 }
 
-/* F[t0:, t0:] -= L21 * U12 for panel p, 64 x 64 tile blockIdx.y; block 0
+/* F[t0:, t0:] -= L21 * U12 for panel p, TL x TL tile blockIdx.y; block 0
  * also puts the factored diagonal block back from dfac. */
 __global__ void __launch_bounds__(WG) k_sp_update(SpDev d, const int *__restrict__ lev, int p, double *__restrict__ F,
                                                   const double *__restrict__ dfac)
@@ -224,23 +224,24 @@ __global__ void __launch_bounds__(WG) k_sp_update(SpDev d, const int *__restrict
     }
     __syncthreads();
     const int tx = tid % 16, ty = tid / 16;
-    double acc[4][4] = {};
+    constexpr int RB = TL / 16;   /* accumulators per thread and dimension */
+    double acc[RB][RB] = {};
 #pragma unroll 8
     for (int k = 0; k < NB; k++) {
-        double a[4], b[4];
+        double a[RB], b[RB];
 #pragma unroll
-        for (int q = 0; q < 4; q++) a[q] = As[k][tx + 16 * q], b[q] = Bs[k][ty + 16 * q];
+        for (int q = 0; q < RB; q++) a[q] = As[k][tx + 16 * q], b[q] = Bs[k][ty + 16 * q];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = 0; q < RB; q++)
 #pragma unroll
-            for (int s = 0; s < 4; s++) acc[q][s] = fma(a[q], b[s], acc[q][s]);
+            for (int s = 0; s < RB; s++) acc[q][s] = fma(a[q], b[s], acc[q][s]);
     }
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
+    for (int s = 0; s < RB; s++) {
         const int c = c0 + ty + 16 * s;
         if (c >= nf) continue;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < RB; q++) {
             const int r = r0 + tx + 16 * q;
             if (r < nf) Fr[(int64_t)c * nf + r] -= acc[q][s];
         }

@@ -12,7 +12,7 @@ namespace ngmi {
 
 constexpr int SPLU_LEAF = 64;          /* nested dissection stops at this many vertices */
 constexpr int SPLU_NB = 32;            /* panel width of the front factorization */
-constexpr int SPLU_TILE = 64;          /* trailing-update tile edge */
+constexpr int SPLU_TILE = 32;          /* trailing-update tile edge (16 x 16 threads, 2 x 2 each) */
 constexpr int SPLU_MAX_FRONT = 8000;   /* a front's right-hand side and the solve kernels'
                                           static LDS fit 64 KB */
 
