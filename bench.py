@@ -295,13 +295,31 @@ def bench_mc(args):
                                'unit': 'GB/s', 'frac': ach1 / HBM_PEAK_GBS if ach1 else None,
                                'traffic': mb * evals1 / passes1 if (mb and passes1) else None,
                                'kernel': 'k_tran_mos', 'launches': passes1,
+                               'kernel_ms_total': mos1, 'bytes_per_eval': b_eval, 'evals': evals1,
                                'fp64_tflops': (MEASURED_FLOP_PER_EVAL['k_tran_mos'] * evals1 /
-                                               (mos1 * 1e-3) / 1e12) if mos1 > 0 else None}}
+                                               (mos1 * 1e-3) / 1e12) if mos1 > 0 else None,
+                               'fp64_peak_tflops': FP64_PEAK_TFLOPS}}
         log('one stream: %.0f tp*c/s, wall %.1f ms, bsim4 %.1f ms (%.0f GB/s), step %.1f ms'
             % (single['value'], wall1 * 1e3, mos1, ach1 or 0, step1))
     learned = None
     if not args.no_guess:
         learned = learned_guess_run(args, b, probe, stream, acc, its)
+    overlapped = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                  'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': traffic,
+                  'kernel': 'k_tran_mos', 'kernel_ms_total': mos_ms,
+                  'launches': mos_launches, 'bytes_per_eval': b_eval, 'evals': evals,
+                  'fp64_tflops': fp64_tflops, 'fp64_peak_tflops': FP64_PEAK_TFLOPS}
+    # The headline roofline is priced on launches that do not share the chip:
+    # with two streams a half's BSIM4 launch runs beside the other half's step
+    # kernel, so its duration overstates the kernel's own time (the summed
+    # durations exceed the wall time).  The one-stream run's launches run alone.
+    if single is not None:
+        roof = dict(single['roofline'])
+        roof['timed_on'] = 'the one-stream run (no overlap; kernel time per .tran %.0f ms <= its wall %.0f ms)' \
+            % (single['bsim4_kernel_ms_total'], single['ms'])
+    else:
+        roof = dict(overlapped)
+        roof['timed_on'] = 'this run (%d stream(s))' % (args.streams or 2)
     return {
         'metric': 'MC transient timepoints x circuits/s at reltol 1e-3 (plain vs learned NR guess)',
         'value': acc / wall,
@@ -312,12 +330,8 @@ def bench_mc(args):
                                '.tran 5p %gn, reltol 1e-3, plain predictor' % (C, tstop * 1e9),
                    'circuits': C, 'unknowns': info['unknowns'],
                    'lu_nonzeros': info['lu_nonzeros']},
-        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': traffic,
-                     'kernel': 'k_tran_mos', 'kernel_ms_total': mos_ms,
-                     'launches': mos_launches, 'bytes_per_eval': b_eval, 'evals': evals,
-                     'fp64_tflops': fp64_tflops, 'fp64_peak_tflops': FP64_PEAK_TFLOPS},
-        'extra': {'accepted': acc, 'rejected': rej, 'newton_iterations': its,
+        'roofline': roof,
+        'extra': {'accepted': acc, 'two_stream_roofline': overlapped, 'rejected': rej, 'newton_iterations': its,
                   'iterations_per_point': its / max(acc, 1), 'passes': passes,
                   'failed_circuits': failed, 'step_kernel_ms_total': step_ms,
                   'bsim4_kernel_ms_total': mos_ms, 'streams': args.streams or 2,
@@ -495,8 +509,9 @@ def bench_c3(args):
     w0 = time.perf_counter()
     th.start()
     rc, passes = b.tran(10e-12, args.tstop, stream=rs.cuda_stream)
-    torch.cuda.synchronize()
+    rs.synchronize()   # the rings' stream only: the mesh iteration in flight is not waited for
     wall = time.perf_counter() - w0
+    mesh_done = count[0]   # mesh iterations finished inside the rings' window
     stop.set()
     th.join()
     if rc != 0:
@@ -504,14 +519,14 @@ def bench_c3(args):
     st = [b.stats(c) for c in range(C)]
     acc = sum(s_['accepted'] for s_ in st)
     log('c3: rings %.0f tp*c/s with the mesh alongside (%.0f alone); mesh %d refactor+solve in %.2f s '
-        '= %.1f/s alongside' % (acc / wall, alone, count[0], wall, count[0] / wall))
+        '= %.1f/s alongside' % (acc / wall, alone, mesh_done, wall, mesh_done / wall))
     res['metric'] = 'uneven-batch transient timepoints x circuits/s with the power-grid LU alongside'
     res['value'] = acc / wall
     res['ms_per_step'] = wall * 1e3
     res['config']['workload'] += ('; alongside: refactor + solve of the %dx%d power-grid matrix '
                                   '(%d unknowns) on a second stream' % (side, side, n))
-    res['extra'].update({'rings_alone': alone, 'mesh_refactor_solve_per_s_alongside': count[0] / wall,
-                         'mesh_refactor_solves': count[0], 'accepted_alongside': acc,
+    res['extra'].update({'rings_alone': alone, 'mesh_refactor_solve_per_s_alongside': mesh_done / wall,
+                         'mesh_refactor_solves': mesh_done, 'accepted_alongside': acc,
                          'failed_alongside': sum(1 for s_ in st if s_['status'])})
     return res
 
@@ -641,6 +656,74 @@ def waveform_error(b, probe, ora):
             'engine_points': int(len(te)), 'same_timepoints': same}
 
 
+def host_cores():
+    """Cores this process may run on (sched_getaffinity), os.cpu_count(), and
+    the count the CPU baseline uses: the affinity count, capped by
+    OMP_NUM_THREADS where the host sets it (its CPU share per GPU)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else (os.cpu_count() or 1)
+    omp = os.environ.get('OMP_NUM_THREADS')
+    used = aff
+    if omp and omp.isdigit() and int(omp) > 0:
+        used = min(aff, int(omp))
+    return {'affinity_cores': aff, 'cpu_count': os.cpu_count(), 'omp_num_threads': omp, 'used': used}
+
+
+C0_TSTOP = 200e-9
+C0_CPU_SAMPLE_TSTOP = 20e-9
+
+
+def cpu_baseline_c0():
+    """The oracle's DCtran restatement (with the reference KLU when built) on
+    the 101-stage ring at one thread, over the first 20 ns of its .tran 10p
+    200n (a bounded sample: about 2000 of its 20000 accepted points)."""
+    sys.path.insert(0, os.path.join(REPO, 'oracle'))
+    import pyoracle as po
+    import workloads
+    d = workloads.ring_oscillator()
+    T = po.Transient(d)
+    klu = T.use_reference_klu()
+    t0 = time.perf_counter()
+    rc, t, _ = T.run(d['tstep'], C0_CPU_SAMPLE_TSTOP)
+    secs = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    return {'value': len(t) / secs, 'unit': 'timepoints/s', 'cores': 1, 'kind': 'port',
+            'sample': '101-stage ring, first %g ns of .tran 10p 200n: %d accepted points in %.2f s, '
+                      'oracle DCtran restatement with %s' % (C0_CPU_SAMPLE_TSTOP * 1e9, len(t), secs,
+                                                             'the reference KLU' if klu else 'dense LU')}
+
+
+def bench_c0(cpu):
+    """The reference's CPU-sized case on the GPU: one 101-stage ring,
+    .ic v(1)=0.51, .tran 10p 200n, reltol 1e-3, trapezoidal; one circuit, so
+    every Newton pass is launch-bound.  Timed once after one warm-up run."""
+    import torch
+    import ngmi
+    import workloads
+    d = workloads.ring_oscillator(tstop=C0_TSTOP)
+    b = ngmi.Batch([ngmi.Circuit(d)], probe=[1], max_points=32768)
+    stream = torch.cuda.current_stream().cuda_stream
+    b.tran(d['tstep'], C0_TSTOP, stream=stream)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    rc, passes = b.tran(d['tstep'], C0_TSTOP, stream=stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    if rc != 0:
+        raise RuntimeError('101-stage ring transient failed: %d %s' % (rc, ngmi.last_error()))
+    st = b.stats(0)
+    out = {'workload': '1 circuit: 101-stage ring (202 BSIM4), .tran 10p 200n, reltol 1e-3',
+           'value': st['accepted'] / wall, 'unit': 'timepoints/s', 'ms': wall * 1e3,
+           'accepted': st['accepted'], 'newton_iterations': st['iterations'], 'passes': passes,
+           'us_per_pass': wall * 1e6 / max(passes, 1), 'cpu_baseline': cpu}
+    if cpu and cpu.get('value'):
+        out['gpu_over_cpu'] = out['value'] / cpu['value']
+    log('c0: %.0f timepoints/s on the GPU (%d points, %d passes, %.1f us per pass)%s'
+        % (out['value'], st['accepted'], passes, out['us_per_pass'],
+           '; CPU %.0f/s at 1 thread' % cpu['value'] if cpu and cpu.get('value') else ''))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -658,9 +741,12 @@ def main():
                     help='mc default 20 ns, rings default 10 ns')
     ap.add_argument('--max-points', type=int, default=8192)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-procs', type=int, default=min(16, os.cpu_count() or 1),
-                    help='host processes for the CPU baseline (one per core)')
+    ap.add_argument('--cpu-procs', type=int, default=None,
+                    help='host processes for the CPU baseline (one per core); default: the cores '
+                         'this process may run on, capped by OMP_NUM_THREADS where that is set')
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess run')
+    ap.add_argument('--no-c0', action='store_true',
+                    help='mc: skip the single-circuit 101-stage ring run reported in extra.c0')
     ap.add_argument('--no-single-stream', action='store_true',
                     help='skip the extra one-stream run of the default two-stream mode')
     ap.add_argument('--eval-waves', type=int, default=None, help='BSIM4 eval kernel occupancy')
@@ -682,14 +768,22 @@ def main():
         args.steps = 1 if args.workload in ('mc', 'rings', 'c3') else 20
     if args.warmup is None:
         args.warmup = 1 if args.workload in ('mc', 'rings', 'c3') else 3
+    host = host_cores()
+    if args.cpu_procs is None:
+        args.cpu_procs = host['used']
     if args.gpus != 1:
         print('bench.py runs on one GPU only (the engine has no multi-GPU path); '
               'got --gpus %d' % args.gpus, file=sys.stderr)
         sys.exit(2)
     cpu = None
+    c0_cpu = None
     if args.workload == 'mc' and not args.no_cpu_baseline:
         cpu = cpu_baseline_mc(args.tstop, procs=args.cpu_procs)
+        cpu['host'] = host
         log('cpu baseline: %s' % cpu)
+        if not args.no_c0:
+            c0_cpu = cpu_baseline_c0()
+            log('cpu baseline, 101-stage ring: %s' % c0_cpu)
     if args.workload == 'mesh' and not args.no_cpu_baseline:
         cpu = cpu_baseline_mesh(args.mesh_side)
         log('cpu baseline: %s' % cpu)
@@ -710,6 +804,8 @@ def main():
         ngmi.tune('order_slack', args.order_slack)
     if args.workload == 'mc':
         res = bench_mc(args)
+        if not args.no_c0:
+            res['extra']['c0'] = bench_c0(c0_cpu)
     elif args.workload == 'copy':
         res = bench_copy(args)
         cpu = None

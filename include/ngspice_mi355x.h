@@ -180,7 +180,9 @@ int ngmi_splu_rowsrc(const ngmi_splu *s, int *rowsrc);
  * NGMI_E_SINGULAR on a zero or non-finite pivot. */
 int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream);
 /* klu_solve (klu_solve.c:14-400, via SMPsolve, klusmp.c:950-1011): b (n
- * doubles, device) is overwritten with the solution.  Asynchronous. */
+ * doubles, device) is overwritten with the solution.  Asynchronous.
+ * NGMI_E_SINGULAR when the last ngmi_splu_factor did not succeed (or none
+ * ran), where klu_solve would run on the failed factors. */
 int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream);
 
 /* ------------------------------------------------------------------ */
@@ -372,6 +374,18 @@ int ngmi_mlp_forward(ngmi_mlp *n, const float *X, float *y, int M, void *stream)
  * before the step goes to *loss_host when given (synchronises) */
 int ngmi_mlp_train_step(ngmi_mlp *n, const float *X, const float *target, int M, float lr,
                         float *loss_host, void *stream);
+
+/* The guess network's feature rows, computed by the same device function
+ * the batched transient's features kernel uses, for M caller-supplied
+ * windows (device arrays): row i from the last four accepted times
+ * t[4i..4i+3] and values v[4i..4i+3] of a node (newest first), the step
+ * h[i], neighbour values nbv[8i..8i+7] present where bit k of nbmask[i] is
+ * set.  F[16i..16i+15] receives the row and act[i] the node's activity;
+ * rows too quiet for the network get zeros and act[i] <= 0.  For training
+ * pipelines that build rows on the host and check them against the
+ * engine's (no counterpart in the reference).  Asynchronous on stream. */
+int ngmi_guess_features(int M, const double *t, const double *h, const double *v, const double *nbv,
+                        const unsigned *nbmask, float *F, double *act, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device memory helpers for callers without a HIP binding             */

@@ -455,7 +455,8 @@ int upload(T **dst, const std::vector<T> &v)
 
 struct ngmi_splu {
     SpluSym S;
-    bool ready = false;
+    bool ready = false;      /* device buffers allocated and plan built */
+    bool factored = false;   /* the last ngmi_splu_factor succeeded */
     SpDev d{};
     int *d_ns = nullptr, *d_nb = nullptr, *d_parent = nullptr, *d_idx = nullptr, *d_rel = nullptr;
     int *d_kid_ptr = nullptr, *d_kids = nullptr, *d_lev = nullptr, *d_ea = nullptr, *d_rowsrc = nullptr;
@@ -467,9 +468,35 @@ struct ngmi_splu {
     std::vector<int> lev_ns, lev_nb;   /* largest pivot and boundary counts by height */
 };
 
+/* frees every device buffer and nulls its pointer, so a later
+ * splu_device call starts from nothing */
+static void splu_release(ngmi_splu *s)
+{
+    void **bufs[] = {(void **)&s->d_ns, (void **)&s->d_nb, (void **)&s->d_parent, (void **)&s->d_idx,
+                     (void **)&s->d_rel, (void **)&s->d_kid_ptr, (void **)&s->d_kids, (void **)&s->d_lev,
+                     (void **)&s->d_ea, (void **)&s->d_rowsrc, (void **)&s->d_flag, (void **)&s->d_off,
+                     (void **)&s->d_idx_off, (void **)&s->d_upd_off, (void **)&s->d_ent, (void **)&s->F,
+                     (void **)&s->dfac, (void **)&s->upd, (void **)&s->vb, (void **)&s->vy, (void **)&s->vx};
+    for (void **b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    s->ready = false;
+    s->factored = false;
+}
+
+static int splu_device_alloc(ngmi_splu *s);
+
 static int splu_device(ngmi_splu *s)
 {
     if (s->ready) return NGMI_OK;
+    const int rc = splu_device_alloc(s);
+    if (rc != NGMI_OK) splu_release(s);
+    return rc;
+}
+
+static int splu_device_alloc(ngmi_splu *s)
+{
     const SpluSym &S = s->S;
     int rc;
 #define UP(dst, v) \
@@ -565,11 +592,7 @@ int ngmi_splu_analyze(int n, const int64_t *Ap, const int *Ai, ngmi_splu **out)
 void ngmi_splu_free(ngmi_splu *s)
 {
     if (!s) return;
-    void *bufs[] = {s->d_ns, s->d_nb, s->d_parent, s->d_idx, s->d_rel, s->d_kid_ptr, s->d_kids, s->d_lev,
-                    s->d_ea, s->d_rowsrc, s->d_flag, s->d_off, s->d_idx_off, s->d_upd_off, s->d_ent,
-                    s->F, s->dfac, s->upd, s->vb, s->vy, s->vx};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
+    splu_release(s);
     delete s;
 }
 
@@ -612,6 +635,7 @@ int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream)
     if (!s || !Ax) return NGMI_E_PANIC;
     int rc = splu_device(s);
     if (rc != NGMI_OK) return rc;
+    s->factored = false;
     const SpluSym &S = s->S;
     hipStream_t st = (hipStream_t)stream;
     const unsigned zb = (unsigned)std::min<int64_t>(8192, (S.store + WG - 1) / WG + 1);
@@ -638,15 +662,16 @@ int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream)
         set_error("splu_factor: zero or non-finite pivot");
         return NGMI_E_SINGULAR;
     }
+    s->factored = true;
     return NGMI_OK;
 }
 
 int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream)
 {
     if (!s || !b) return NGMI_E_PANIC;
-    if (!s->ready) {
-        set_error("splu_solve: no factorization");
-        return NGMI_E_PANIC;
+    if (!s->ready || !s->factored) {
+        set_error("splu_solve: no successful factorization");
+        return NGMI_E_SINGULAR;
     }
     const SpluSym &S = s->S;
     hipStream_t st = (hipStream_t)stream;

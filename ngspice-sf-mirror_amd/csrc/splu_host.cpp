@@ -156,17 +156,34 @@ int splu_analyze(int n, const int64_t *Ap, const int *Ai, SpluSym &S)
         set_error("splu_analyze: empty system");
         return NGMI_E_PARMVAL;
     }
+    /* the pattern checks of klu_analyze_given.c:55-95: Ap[0] == 0, column
+     * pointers non-decreasing, rows in range, no row twice in a column */
+    if (Ap[0] != 0) {
+        set_error("splu_analyze: Ap[0] is %lld, not 0", (long long)Ap[0]);
+        return NGMI_E_PARMVAL;
+    }
     const int64_t nnz = Ap[n];
     for (int c = 0; c < n; c++)
         if (Ap[c + 1] < Ap[c]) {
             set_error("splu_analyze: column pointers decrease at %d", c);
             return NGMI_E_PARMVAL;
         }
-    for (int64_t e = 0; e < nnz; e++)
-        if (Ai[e] < 0 || Ai[e] >= n) {
-            set_error("splu_analyze: row index %d out of range at %lld", Ai[e], (long long)e);
-            return NGMI_E_PARMVAL;
-        }
+    {
+        std::vector<int> seen(n, -1);
+        for (int c = 0; c < n; c++)
+            for (int64_t e = Ap[c]; e < Ap[c + 1]; e++) {
+                const int r = Ai[e];
+                if (r < 0 || r >= n) {
+                    set_error("splu_analyze: row index %d out of range at %lld", r, (long long)e);
+                    return NGMI_E_PARMVAL;
+                }
+                if (seen[r] == c) {
+                    set_error("splu_analyze: row %d appears twice in column %d", r, c);
+                    return NGMI_E_PARMVAL;
+                }
+                seen[r] = c;
+            }
+    }
     S = SpluSym();
     S.n = n;
     S.nnz = nnz;

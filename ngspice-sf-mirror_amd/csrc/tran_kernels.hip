@@ -1514,6 +1514,55 @@ __device__ __forceinline__ double clamp_ratio(double r)
     return r != r ? 0.0 : fmin(fmax(r, 0.0), GUESS_RATIO_MAX);
 }
 
+/* One feature row of the guess network for a node whose last four accepted
+ * values v0..v3 sit at t0 > t1 > t2 > t3, for the step t0 -> t0 + h;
+ * nbv[k] is neighbour k's value at t0 where bit k of nbmask is set.  Writes
+ * the linear extrapolation and the activity; returns false (f untouched)
+ * when the node is too quiet for the network.  The training rows are built
+ * by the same formula on the host. */
+__device__ __forceinline__ bool guess_feature_row(double v0, double v1, double v2, double v3, double t0,
+                                                  double t1, double t2, double t3, double h,
+                                                  const double *nbv, unsigned nbmask, float *f,
+                                                  double *lin_out, double *act_out)
+{
+    const double r0 = clamp_ratio(h / (t0 - t1));
+    const double r1 = clamp_ratio((t0 - t1) / (t1 - t2));
+    const double r2 = clamp_ratio((t1 - t2) / (t2 - t3));
+    const double d0 = v0 - v1, d1 = v1 - v2, d2 = v2 - v3;
+    const double act = fabs(d0) + fabs(d1) + fabs(d2);
+    *lin_out = v0 + r0 * d0;
+    *act_out = act;
+    if (!(act >= GUESS_MIN_ACTIVITY)) return false;
+    f[0] = (float)v0;
+    f[1] = (float)(d0 / act);
+    f[2] = (float)(d1 / act);
+    f[3] = (float)(d2 / act);
+    f[4] = (float)r0;
+    f[5] = (float)r1;
+    f[6] = (float)r2;
+    for (int k = 0; k < 8; k++) f[7 + k] = (nbmask >> k) & 1u ? (float)(nbv[k] - v0) : 0.0f;
+    f[15] = (float)(log10(act) / 5.0);
+    return true;
+}
+
+/* the same rows for caller-supplied windows: row i from t[4i..4i+3],
+ * v[4i..4i+3], h[i], nbv[8i..8i+7], nbmask[i]; quiet rows get zeros and a
+ * negative activity in act[i] */
+__global__ void __launch_bounds__(256)
+k_guess_feature_rows(int M, const double *t, const double *h, const double *v, const double *nbv,
+                     const unsigned *nbmask, float *F, double *act)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    float f[16];
+    double lin, a;
+    const bool on = guess_feature_row(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3], t[4 * i],
+                                      t[4 * i + 1], t[4 * i + 2], t[4 * i + 3], h[i], nbv + 8 * i, nbmask[i],
+                                      f, &lin, &a);
+    for (int k = 0; k < 16; k++) F[(int64_t)i * 16 + k] = on ? f[k] : 0.0f;
+    act[i] = on ? a : -a;
+}
+
 /*
  * Features of node u of circuit c for the step the circuit is starting
  * (t_n -> t_n + h), from the last four accepted points v0..v3 at t0 > t1 >
@@ -1547,31 +1596,23 @@ k_guess_features(TranDev D, float *F, int *rowmap, double2 *base, int *count)
     const double v1 = h0p[u + ((s1 - s0) * stride)];
     const double v2 = h0p[u + ((s2 - s0) * stride)];
     const double v3 = h0p[u + ((s3 - s0) * stride)];
-    const double t0 = K->ht[s0], t1 = K->ht[s1], t2 = K->ht[s2], t3 = K->ht[s3];
-    const double r0 = clamp_ratio(K->delta / (t0 - t1));
-    const double r1 = clamp_ratio((t0 - t1) / (t1 - t2));
-    const double r2 = clamp_ratio((t1 - t2) / (t2 - t3));
-    const double d0 = v0 - v1, d1 = v1 - v2, d2 = v2 - v3;
-    const double act = fabs(d0) + fabs(d1) + fabs(d2);
-    const double lin = v0 + r0 * d0;
-    if (!(act >= GUESS_MIN_ACTIVITY)) {
+    double nbv[8];
+    unsigned nbmask = 0;
+    for (int k = 0; k < 8; k++) {
+        const int nb = D.nbr[u * 8 + k];
+        nbv[k] = nb >= 0 ? h0p[nb] : 0.0;
+        nbmask |= nb >= 0 ? 1u << k : 0u;
+    }
+    float f[16];
+    double lin, act;
+    if (!guess_feature_row(v0, v1, v2, v3, K->ht[s0], K->ht[s1], K->ht[s2], K->ht[s3], K->delta, nbv,
+                           nbmask, f, &lin, &act)) {
         xo[u] = lin;
         return;
     }
     const int row = atomicAdd(count, 1);
-    float *f = F + (int64_t)row * 16;
-    f[0] = (float)v0;
-    f[1] = (float)(d0 / act);
-    f[2] = (float)(d1 / act);
-    f[3] = (float)(d2 / act);
-    f[4] = (float)r0;
-    f[5] = (float)r1;
-    f[6] = (float)r2;
-    for (int k = 0; k < 8; k++) {
-        const int nb = D.nbr[u * 8 + k];
-        f[7 + k] = nb >= 0 ? (float)(h0p[nb] - v0) : 0.0f;
-    }
-    f[15] = (float)(log10(act) / 5.0);
+    float *fr = F + (int64_t)row * 16;
+    for (int k = 0; k < 16; k++) fr[k] = f[k];
     rowmap[row] = c * (D.N + 1) + u;
     base[row] = make_double2(lin, act);
 }
@@ -1632,6 +1673,20 @@ int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, cons
     const int blocks = std::max(1, std::min((maxrows + 255) / 256, 1024));
     hipLaunchKernelGGL(k_guess_apply, dim3((unsigned)blocks), dim3(256), 0, s, D, y,
                        rowmap, base, count);
+    NGMI_HIP(hipGetLastError());
+    return NGMI_OK;
+}
+
+extern "C" int ngmi_guess_features(int M, const double *t, const double *h, const double *v, const double *nbv,
+                                   const unsigned *nbmask, float *F, double *act, void *stream)
+{
+    if (M < 0 || (M > 0 && (!t || !h || !v || !nbv || !nbmask || !F || !act))) {
+        set_error("guess_features: bad arguments");
+        return NGMI_E_PARMVAL;
+    }
+    if (M == 0) return NGMI_OK;
+    hipLaunchKernelGGL(k_guess_feature_rows, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, M, t, h, v, nbv, nbmask, F, act);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }

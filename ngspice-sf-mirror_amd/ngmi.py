@@ -456,6 +456,26 @@ class GuessNet:
             pass
 
 
+def guess_features(t4, h, v4, nbv, nbmask, stream=0):
+    """The engine's guess feature rows (the device function the in-loop
+    features kernel uses) for caller-supplied windows, through
+    ngmi_guess_features: t4[M,4] and v4[M,4] the last four accepted times
+    and values (newest first), h[M] the step, nbv[M,8] neighbour values,
+    nbmask[M] their presence bits.  Device pointers (torch tensors' data_ptr)
+    in, (F [M,16] float32, act [M] float64) device tensors out."""
+    import torch
+    M = int(h.numel())
+    F = torch.empty((M, 16), dtype=torch.float32, device=h.device)
+    act = torch.empty(M, dtype=torch.float64, device=h.device)
+    L = lib()
+    vp = ctypes.c_void_p
+    L.ngmi_guess_features.argtypes = [ctypes.c_int] + [vp] * 8
+    check(L.ngmi_guess_features(M, t4.data_ptr(), h.data_ptr(), v4.data_ptr(), nbv.data_ptr(),
+                                nbmask.data_ptr(), F.data_ptr(), act.data_ptr(), stream),
+          'guess features')
+    return F, act
+
+
 class Circuit:
     """A netlist for the engine from a circuit description dict (nnodes,
     models, mos, caps, res, vsrc, ic, options); see workloads.py."""

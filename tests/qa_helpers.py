@@ -15,6 +15,8 @@ import pyoracle as po  # noqa: E402
 MODETRAN = 0x1
 MODEDCTRANCURVE = 0x40
 MODEINITFLOAT = 0x100
+# integration coefficient used to read the capacitances out of the stamps
+AG0_C = 1e15
 
 
 def qa_match(ref, sim, clip, ndigit=6, reltol=1e-6):
@@ -58,8 +60,11 @@ def ac_admittance(model, inst_params, temp_c, bias, omega):
     inst.load(MODEDCTRANCURVE | MODEINITFLOAT, T, rhs_old, states, reltol=1e-5, abstol=1e-15)
     stamp0, _, _ = inst.load(MODETRAN | MODEINITFLOAT, T, rhs_old, states,
                              ag=(0.0,) * 7, reltol=1e-5, abstol=1e-15)
+    # The stamps are G + ag0 C.  ag0 = 1 would leave C (~1e-15 F) at the
+    # rounding level of G (~1e-4 S) in every entry holding both, so C is
+    # taken with ag0 large enough that ag0 C dominates, and divided back out.
     stamp1, _, _ = inst.load(MODETRAN | MODEINITFLOAT, T, rhs_old, states,
-                             ag=(1.0,) + (0.0,) * 6, reltol=1e-5, abstol=1e-15)
+                             ag=(AG0_C,) + (0.0,) * 6, reltol=1e-5, abstol=1e-15)
     G = np.zeros((nn, nn))
     C = np.zeros((nn, nn))
     for s, (r, c) in enumerate(po.slot_roles()):
@@ -67,7 +72,7 @@ def ac_admittance(model, inst_params, temp_c, bias, omega):
         if nr == 0 or nc == 0:
             continue
         G[nr, nc] += stamp0[s]
-        C[nr, nc] += stamp1[s] - stamp0[s]
+        C[nr, nc] += (stamp1[s] - stamp0[s]) / AG0_C
     Y = G + 1j * omega * C
     ext = [1, 2, 3, 4]
     intn = list(range(5, nn))

@@ -20,6 +20,11 @@ import pytest
 
 from engine_path import ngmi, workloads
 import guess
+import os
+import sys
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), '..', 'oracle'))
+import pyoracle as po  # noqa: E402
 
 torch = pytest.importorskip('torch')
 pytestmark = pytest.mark.gpu
@@ -38,8 +43,11 @@ def run(seeds, net=None, probe=None):
     return b, descs
 
 
-def test_learned_guess_end_to_end():
-    print('case: learned guess, train seeds 4096..4103, eval seeds 0..3', flush=True)
+@pytest.fixture(scope='module')
+def trained():
+    """the guess network trained on engine waveforms of held-out seeds
+    4096..4103 (.tran 5p 2.5n); returns (net, training rows, loss, baseline)"""
+    print('case: guess network training, seeds 4096..4103', flush=True)
     train_seeds = list(range(4096, 4104))
     b, descs = run(train_seeds)
     nn = descs[0]['nnodes']
@@ -68,10 +76,19 @@ def test_learned_guess_end_to_end():
     loss = net.train_step(dF.data_ptr(), dT.data_ptr(), min(len(F), 65536), 0.0)
     print('training rows %d, target mean square %.4e, trained loss %.4e' % (len(F), base_loss, loss),
           flush=True)
+    return net, len(F), loss, base_loss
+
+
+def test_learned_guess_end_to_end(trained):
+    print('case: learned guess, train seeds 4096..4103, eval seeds 0..3', flush=True)
+    net, _, loss, base_loss = trained
     assert loss < 0.5 * base_loss
+    nn = workloads.mc_adder(0)['nnodes']
 
     eval_seeds = [0, 1, 2, 3]
     bp, dp = run(eval_seeds)
+    nbr = bp.neighbours(nn)
+    n = bp.info()['unknowns']
     Fs, Ts, As = [], [], []
     for c in range(len(eval_seeds)):
         t, x = bp.waveform(c, n)
@@ -115,3 +132,93 @@ def test_learned_guess_end_to_end():
     print('Newton iterations per point: plain %.3f, learned %.3f'
           % (its_plain / len(eval_seeds), its_learned / len(eval_seeds)), flush=True)
     assert its_learned <= 1.10 * its_plain
+
+
+def test_features_match_the_engine_bit_for_bit():
+    """guess.features (the host side that builds training rows) and the
+    engine's features kernel (through ngmi_guess_features, the device
+    function the in-loop kernel calls) give the same float32 rows, bit for
+    bit, on the windows of real adder waveforms; the quiet-node rule
+    (activity below 10 uV) selects the same rows."""
+    print('case: guess feature rows, host vs engine, adder seeds 0..1 .tran 5p 2.5n', flush=True)
+    b, descs = run([0, 1])
+    nn = descs[0]['nnodes']
+    nbr = b.neighbours(nn).reshape(-1, 8)
+    n = b.info()['unknowns']
+    checked = 0
+    for c in range(2):
+        t, x = b.waveform(c, n)
+        F_host, T_host, A_host = guess.features(t, x, nbr.reshape(-1), nn, with_scale=True)
+        P = len(t)
+        steps = np.arange(3, P - 1)
+        V = x[:, :nn]
+        S = len(steps)
+        # every (step, node) window, step-major then node, the order guess.features keeps
+        t4 = np.stack([t[steps], t[steps - 1], t[steps - 2], t[steps - 3]], axis=1)     # [S, 4]
+        t4 = np.repeat(t4[:, None, :], nn, axis=1).reshape(-1, 4)
+        h = np.repeat((t[steps + 1] - t[steps])[:, None], nn, axis=1).reshape(-1)
+        v4 = np.stack([V[steps], V[steps - 1], V[steps - 2], V[steps - 3]], axis=2).reshape(-1, 4)
+        nb = nbr[1:nn + 1]                                                                # [nn, 8]
+        mask = ((nb >= 1).astype(np.uint32) << np.arange(8, dtype=np.uint32)).sum(axis=1).astype(np.uint32)
+        nbv = np.where(nb[None, :, :] >= 1, V[steps][:, np.clip(nb - 1, 0, nn - 1)], 0.0).reshape(-1, 8)
+        dev = {k: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+               for k, a in (('t4', t4), ('h', h), ('v4', v4), ('nbv', nbv),
+                            ('mask', np.tile(mask, S).astype(np.int32)))}
+        F_dev, act_dev = ngmi.guess_features(dev['t4'], dev['h'], dev['v4'], dev['nbv'], dev['mask'])
+        torch.cuda.synchronize()
+        F_dev = F_dev.cpu().numpy()
+        act_dev = act_dev.cpu().numpy()
+        keep = act_dev > 0
+        assert keep.sum() == len(F_host), (keep.sum(), len(F_host))
+        F_eng = F_dev[keep]
+        same = F_eng.view(np.uint32) == F_host.view(np.uint32)
+        print('circuit %d: %d windows, %d active rows, columns not bit-equal: %s'
+              % (c, len(h), len(F_host), sorted(set(np.nonzero(~same)[1].tolist()))), flush=True)
+        assert same.all(), np.argwhere(~same)[:5]
+        assert np.array_equal(act_dev[keep], A_host)
+        checked += len(F_host)
+    assert checked > 1000
+
+
+@pytest.mark.parametrize('seed', [0, 1])
+def test_learned_guess_full_length_against_oracle(trained, seed):
+    """A learned run of one adder at full length (.tran 5p 20n) against the
+    oracle.  The learned run's Newton iterates start elsewhere, so its
+    converged points differ from the oracle's within the Newton tolerance
+    (reltol 1e-3 of the value + vntol 1e-6 V, niconv.c:50-60), and its LTE
+    control may place timepoints differently; the bound is that tolerance
+    at every oracle timepoint (the learned waveform interpolated there), and
+    the error is printed per probed output (the 16 sums and the carry)."""
+    net = trained[0]
+    d = workloads.mc_adder(seed)
+    tstop = workloads.ADDER_TSTOP
+    names = d['names']
+    outs = ['s%d' % i for i in range(workloads.ADDER_BITS)] + ['co%d' % (workloads.ADDER_BITS - 1)]
+    probe = [names[k] for k in outs]
+    print('case: learned guess, adder seed %d, .tran 5p %gn, against the oracle' % (seed, tstop * 1e9),
+          flush=True)
+    b = ngmi.Batch([ngmi.Circuit(d)], probe=probe, max_points=8192)
+    b.set_guess(net)
+    rc, _ = b.tran(workloads.ADDER_TSTEP, tstop)
+    assert rc == 0, ngmi.last_error()
+    st, g = b.stats(0), b.guess_stats(0)
+    assert st['status'] == 0 and g['guessed'] > 0
+    te, xe = b.waveform(0, len(probe))
+    T = po.Transient(d)
+    T.use_reference_klu()
+    rc, to, xo = T.run(d['tstep'], tstop)
+    assert rc == 0
+    same = len(te) == len(to) and bool(np.allclose(te, to, rtol=1e-9, atol=0))
+    worst = 0.0
+    per = []
+    for k, u in enumerate(probe):
+        ref = xo[:, u]
+        got = np.interp(to, te, xe[:, k])
+        err = np.abs(got - ref)
+        tol = 1e-3 * np.abs(ref) + 1e-6
+        per.append('%s %.1e' % (outs[k], err.max()))
+        worst = max(worst, float((err / tol).max()))
+    print('learned: %d points (oracle %d, same timepoints: %s), %d steps from the guess, %d fallbacks; '
+          'max |v - v_oracle| per output: %s; worst error / Newton tolerance %.2e'
+          % (len(te), len(to), same, g['guessed'], g['fallbacks'], ', '.join(per), worst), flush=True)
+    assert worst <= 1.0

@@ -77,9 +77,24 @@ def test_zero_pivot_is_reported():
     Ax = np.array([0.0, 1.0, 1.0, 1.0])
     s = ngmi.SparseLU(2, Ap, Ai)
     dA = torch.from_numpy(Ax).cuda()
+    db = torch.ones(2, dtype=torch.float64, device='cuda')
+    # no factorization yet: the solve refuses
+    with pytest.raises(ngmi.EngineError) as e:
+        s.solve(db.data_ptr())
+    assert e.value.code == ngmi.E_SINGULAR
     with pytest.raises(ngmi.EngineError) as e:
         s.factor(dA.data_ptr())
     assert e.value.code == ngmi.E_SINGULAR
+    # after a failed factorization the solve refuses too, instead of running on inf/NaN fronts
+    with pytest.raises(ngmi.EngineError) as e:
+        s.solve(db.data_ptr())
+    assert e.value.code == ngmi.E_SINGULAR
+    # a good matrix afterwards factors and solves
+    dA.copy_(torch.tensor([2.0, 1.0, 1.0, 1.0], dtype=torch.float64))
+    s.factor(dA.data_ptr())
+    s.solve(db.data_ptr())
+    torch.cuda.synchronize()
+    assert np.allclose(db.cpu().numpy(), np.linalg.solve(np.array([[2.0, 1.0], [1.0, 1.0]]), np.ones(2)))
 
 
 def test_concurrent_with_a_transient_batch():

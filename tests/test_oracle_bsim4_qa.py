@@ -10,8 +10,11 @@ listed here explicitly:
     convergence scatter of the reference run at the 1e-4 level (errors of both
     signs at isolated low-Vd points, clean elsewhere);
   * NMOS long-channel (L >= 0.1 um) high-Vg points with igcMod=1 and
-    rgateMod=1, and the gate capacitance through the rgate node: the mismatch
-    scales with gate leakage times gate resistance (1e-5..5e-5).
+    rgateMod=1, and the NMOS AC sets with the same selectors: the mismatch
+    scales with gate leakage times gate resistance, and for AC it is removed
+    by moving the internal gate node within the reference's own Newton
+    convergence window (test_ac_igc_rgate_offset_is_a_convergence_window).
+Every other AC set meets the 1e-6 rule at every point.
 """
 import json
 import math
@@ -81,7 +84,50 @@ def test_ac(pol, name):
                 if not qa_match(r[ci + 1], v, clip=1e-20):
                     fails += 1
                     worst = max(worst, abs(v - r[ci + 1]) / abs(r[ci + 1]))
-    if (pol, name) in {('pmos', 'acVd'), ('nmos', 'acFreq_igcmod')}:
-        assert fails == 0, (fails, worst)
+    if pol == 'nmos' and m0.get('igcmod', 0) == 1 and m0.get('rgatemod', 0) == 1:
+        # gate tunnelling through the gate resistance: the golden's operating
+        # point carries the reference's Newton convergence error at the
+        # internal gate node (see test_ac_igc_rgate_offset_is_a_convergence_window)
+        assert worst < 3e-5, (fails, worst)
     else:
-        assert worst < 1e-4, (fails, worst)
+        assert fails == 0, (fails, worst)
+
+
+def test_ac_igc_rgate_offset_is_a_convergence_window():
+    """The NMOS AC sets with igcMod=1 and rgateMod=1 miss the 1e-6 rule by
+    1.4e-5 in c(g,d), at every frequency alike, so the difference sits in
+    the operating point, not in the C-V.  Moving the internal gate node by
+    at most one of the reference's own Newton convergence windows
+    (reltol |V| + vntol with the QA deck's reltol 1e-5 and the default vntol
+    1e-6, niconv.c:50-60; 13 uV at Vg = 1.2 V) brings c(g,d) within 1e-6 at
+    every point of the set: the reference's .op stopped inside that window,
+    while the oracle solves the point to machine precision."""
+    t = _test('nmos', 'acFreq', 'ac_tests')
+    T = t['temps'][0]
+    model = model_for(t, 'nmos', T)
+    ci = t['columns'][1:].index('c(g,d)')
+    window = 1e-5 * abs(t['bias']['g']) + 1e-6
+    orig = po.Instance.qa_dc
+    best = None
+    for k in range(-20, 21):
+        dv = window * k / 20
+
+        def shifted(self, temp, vd, vg, vs, vb, gmin=1e-12, want_nodes=False, dv=dv):
+            cur, v = orig(self, temp, vd, vg, vs, vb, gmin, want_nodes=True)
+            v = list(v)
+            v[5] += dv   # node 5: the internal gate node of the rgateMod=1 instance
+            return cur, v
+        po.Instance.qa_dc = shifted
+        try:
+            worst = 0.0
+            for r in t['rows']:
+                om = 2 * math.pi * r[0]
+                Y = ac_admittance(model, t['instance'], T, dict(t['bias']), om)
+                v = ac_quantity(Y, 'c(g,d)', om)
+                worst = max(worst, abs(v - r[ci + 1]) / abs(r[ci + 1]))
+        finally:
+            po.Instance.qa_dc = orig
+        if best is None or worst < best[1]:
+            best = (dv, worst)
+    print('c(g,d): best shift %.2e V (window %.2e V) leaves %.1e' % (best[0], window, best[1]))
+    assert abs(best[0]) <= window and best[1] < 1e-6

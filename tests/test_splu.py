@@ -139,6 +139,13 @@ def test_bad_patterns_are_rejected():
     with pytest.raises(ngmi.EngineError) as e:
         ngmi.SparseLU(2, np.array([0, 2, 1]), np.array([0, 1]))
     assert e.value.code == 11 and 'decrease' in ngmi.last_error()
+    # klu_analyze_given.c:55-95 also rejects Ap[0] != 0 and a row twice in a column
+    with pytest.raises(ngmi.EngineError) as e:
+        ngmi.SparseLU(2, np.array([1, 2, 3]), np.array([0, 0, 1]))
+    assert e.value.code == 11 and 'Ap[0]' in ngmi.last_error()
+    with pytest.raises(ngmi.EngineError) as e:
+        ngmi.SparseLU(2, np.array([0, 2, 3]), np.array([0, 0, 1]))
+    assert e.value.code == 11 and 'twice' in ngmi.last_error()
 
 
 @pytest.mark.parametrize('seed', [0, 1])
