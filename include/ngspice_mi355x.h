@@ -186,6 +186,52 @@ int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream);
 int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* One circuit in host memory: the cores a DEVload / SMP binding calls */
+/* ------------------------------------------------------------------ */
+/* For a reference-shaped binding (DEVload, devdefs.h:57; SMP*,
+ * smpdefs.h:61-86) of a single circuit whose CKTcircuit vectors and KLU
+ * matrix live in host memory.
+ *
+ * ngmi_devload_new is the setup half of BSIM4setup + BSIM4bindCSC
+ * (b4set.c:2318 state offsets, b4bindCSC.c:12-111 matrix pointers): set
+ * holds the circuit's BSIM4 instances (finalized); instance i has terminal
+ * unknowns nodes[4i..4i+3] (d g s b, 0 = ground, at most n_unknowns), its
+ * 29 states at state_off[i] of each state vector (num_states long), and
+ * its 16 Jacobian entries (row-major over d g s b) at positions
+ * ax_pos[16i..16i+15] of the caller's matrix array (-1: not stamped, a
+ * ground row or column).  Device memory is allocated here. */
+typedef struct ngmi_devload ngmi_devload;
+ngmi_devload *ngmi_devload_new(const ngmi_bsim4_set *set, int n_unknowns, int num_states, const int *nodes,
+                               const int *state_off, const int64_t *ax_pos);
+void ngmi_devload_free(ngmi_devload *d);
+/* BSIM4load (b4ld.c:69-5399) for those instances: reads rhsOld[0..n] and
+ * states[0..2] (host arrays, CKTrhsOld and CKTstate0..2), writes back
+ * state0 (and state1, which INITTRAN copies), adds each instance's stamps
+ * into Ax at its positions and its RHS values into rhs[node], instance by
+ * instance in set order, and adds the device convergence votes to
+ * *noncon (CKTnoncon, b4ld.c:4072-4110).  Synchronous. */
+int ngmi_devload_run(ngmi_devload *d, const ngmi_ckt_scalars *ck, const double *rhsOld, double *const states[3],
+                     double *Ax, double *rhs, int *noncon);
+
+/* SMPpreOrder (klusmp.c:1255-1285): symbolic analysis of an n x n CSC
+ * pattern (0-based rows, ascending per column) by the whole-GPU sparse LU
+ * (ngmi_splu_analyze); host-only. */
+typedef struct ngmi_smp ngmi_smp;
+int ngmi_smp_new(int n, const int64_t *Ap, const int *Ai, ngmi_smp **out);
+void ngmi_smp_free(ngmi_smp *m);
+/* SMPluFac / SMPreorder (klusmp.c:573-624, 752-806): LoadGmin_CSC
+ * (klusmp.c:1765-1777, gmin onto every existing diagonal entry) on a device
+ * copy of Ax (host, nnz values; the caller's array is not changed), then
+ * the numeric factorization on the pivot sequence fixed at analysis.
+ * NGMI_E_SINGULAR on a zero or non-finite pivot, as KLU_SINGULAR maps to
+ * E_SINGULAR.  Synchronous. */
+int ngmi_smp_factor(ngmi_smp *m, const double *Ax, double gmin);
+/* SMPsolve (klusmp.c:950-1011): rhs[1..n] (host, 1-based, rhs[0] the
+ * ground slot) solved in place; rhs[0] is set to 0.  NGMI_E_SINGULAR when
+ * the last factorization failed.  Synchronous. */
+int ngmi_smp_solve(ngmi_smp *m, double *rhs);
+
+/* ------------------------------------------------------------------ */
 /* Circuits and batched transient analysis                             */
 /* ------------------------------------------------------------------ */
 typedef struct ngmi_circuit ngmi_circuit;

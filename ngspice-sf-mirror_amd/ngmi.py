@@ -456,6 +456,81 @@ class GuessNet:
             pass
 
 
+class DevLoad:
+    """BSIM4load for one circuit whose vectors live in host memory
+    (ngmi_devload_*): the core a reference-shaped DEVload binding calls.
+    set: a finalized InstanceSet; nodes [n,4] (d g s b, 0 = ground);
+    state_off [n]; ax_pos [n,16] positions of each instance's Jacobian
+    entries in the caller's matrix array (-1 none)."""
+
+    def __init__(self, set_, n_unknowns, num_states, nodes, state_off, ax_pos):
+        L = lib()
+        vp = ctypes.c_void_p
+        L.ngmi_devload_new.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+        L.ngmi_devload_new.restype = vp
+        L.ngmi_devload_free.argtypes = [vp]
+        L.ngmi_devload_run.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        self.set = set_
+        self._nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        self._soff = np.ascontiguousarray(state_off, dtype=np.int32)
+        self._pos = np.ascontiguousarray(ax_pos, dtype=np.int64)
+        self.h = L.ngmi_devload_new(set_.h, n_unknowns, num_states, self._nodes.ctypes.data,
+                                    self._soff.ctypes.data, self._pos.ctypes.data)
+        if not self.h:
+            raise EngineError(1, 'devload_new')
+
+    @staticmethod
+    def run_fn():
+        """address of ngmi_devload_run, for callers that take a C function"""
+        return ctypes.cast(lib().ngmi_devload_run, ctypes.c_void_p).value
+
+    def __del__(self):
+        try:
+            lib().ngmi_devload_free(self.h)
+        except Exception:
+            pass
+
+
+class SMP:
+    """SMPpreOrder / SMPluFac / SMPsolve for one circuit's host-memory CSC
+    matrix on the GPU (ngmi_smp_*)."""
+
+    def __init__(self, n, Ap, Ai):
+        L = lib()
+        vp = ctypes.c_void_p
+        L.ngmi_smp_new.argtypes = [ctypes.c_int, vp, vp, ctypes.POINTER(vp)]
+        L.ngmi_smp_free.argtypes = [vp]
+        L.ngmi_smp_factor.argtypes = [vp, vp, ctypes.c_double]
+        L.ngmi_smp_solve.argtypes = [vp, vp]
+        Ap = np.ascontiguousarray(Ap, dtype=np.int64)
+        Ai = np.ascontiguousarray(Ai, dtype=np.int32)
+        h = vp()
+        check(L.ngmi_smp_new(int(n), Ap.ctypes.data, Ai.ctypes.data, ctypes.byref(h)), 'smp_new')
+        self.h = h.value
+
+    @staticmethod
+    def factor_fn():
+        return ctypes.cast(lib().ngmi_smp_factor, ctypes.c_void_p).value
+
+    @staticmethod
+    def solve_fn():
+        return ctypes.cast(lib().ngmi_smp_solve, ctypes.c_void_p).value
+
+    def factor(self, Ax, gmin=0.0):
+        Ax = np.ascontiguousarray(Ax, dtype=np.float64)
+        check(lib().ngmi_smp_factor(self.h, Ax.ctypes.data, gmin), 'smp_factor')
+
+    def solve(self, rhs):
+        """rhs: float64 array [n+1], 1-based, solved in place"""
+        check(lib().ngmi_smp_solve(self.h, rhs.ctypes.data), 'smp_solve')
+
+    def __del__(self):
+        try:
+            lib().ngmi_smp_free(self.h)
+        except Exception:
+            pass
+
+
 def guess_features(t4, h, v4, nbv, nbmask, stream=0):
     """The engine's guess feature rows (the device function the in-loop
     features kernel uses) for caller-supplied windows, through

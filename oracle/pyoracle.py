@@ -185,6 +185,10 @@ def _tran_protos():
     L.or_tran_free.argtypes = [vp]
     L.or_tran_use_klu.argtypes = [vp, ctypes.c_char_p]
     L.or_comcof.argtypes = [i, i, d, dp, dp]
+    L.or_tran_csc.argtypes = [vp, vp, vp]
+    L.or_tran_csc.restype = ctypes.c_int64
+    L.or_tran_mos_info.argtypes = [vp, i, ctypes.POINTER(i), vp]
+    L.or_tran_use_dropin.argtypes = [vp, vp, vp, vp, vp, vp]
     L.or_tran_set_op_options.argtypes = [vp, i, i, i, d, d, i]
     ip = ctypes.POINTER(ctypes.c_int)
     L.or_tran_klu_order.argtypes = [vp, ip, ip, ip]
@@ -266,6 +270,36 @@ class Transient:
             return False
         self.solver = 'klu'
         return True
+
+    def csc(self):
+        """the circuit matrix's structural CSC pattern (Ap int64, Ai int32)"""
+        L = _tran_protos()
+        np = self.np
+        nnz = L.or_tran_csc(self.h, None, None)
+        Ap = np.zeros(L.or_tran_n(self.h) + 1, dtype=np.int64)
+        Ai = np.zeros(max(nnz, 1), dtype=np.int32)
+        L.or_tran_csc(self.h, Ap.ctypes.data, Ai.ctypes.data)
+        return Ap, Ai[:nnz]
+
+    def mos_info(self, k):
+        """(state offset, [d, g, s, b]) of MOSFET k"""
+        st = ctypes.c_int()
+        nodes = (ctypes.c_int * 4)()
+        if _tran_protos().or_tran_mos_info(self.h, k, ctypes.byref(st), nodes) != 0:
+            raise IndexError(k)
+        return st.value, list(nodes)
+
+    @property
+    def n(self):
+        return _tran_protos().or_tran_n(self.h)
+
+    def use_dropin(self, run_fn, dl_ctx, factor_fn, solve_fn, smp_ctx):
+        """Replace the MOSFET loads and the linear solve by caller functions
+        (addresses of C functions with the signatures or_tran_use_dropin
+        documents) behind the same DCtran / NIiter."""
+        if _tran_protos().or_tran_use_dropin(self.h, run_fn, dl_ctx, factor_fn, solve_fn, smp_ctx) != 0:
+            raise RuntimeError('drop-in setup failed')
+        self.solver = 'drop-in'
 
     def run(self, tstep, tstop, tmax=0.0):
         """Returns (rc, t[npts], x[npts, n+1]) with x[:, 0] = ground."""

@@ -97,7 +97,25 @@ typedef struct {
     int (*kg_solve)(void *, double *);
     void (*kg_free)(void *);
     int (*kg_order)(void *, int *, int *, int *);
+    /* optional: a drop-in BSIM4 load and linear solver behind the same
+     * DCtran / NIiter (the engine's per-circuit cores, through function
+     * pointers the caller supplies; the oracle links nothing of it) */
+    void *dl_ctx;
+    int (*dl_run)(void *, const void *, const double *, double *const *, double *, double *, int *);
+    void *smp_ctx;
+    int (*smp_factor)(void *, const double *, double);
+    int (*smp_solve)(void *, double *);
 } ORTran;
+
+/* the scalar block the drop-in load receives (the layout of the engine's
+ * ngmi_ckt_scalars: CKTmode, order, method, ag[7], delta / deltaOld[1],
+ * gmin, reltol, abstol) */
+typedef struct {
+    int mode, order, method;
+    double ag[7];
+    double xfact;
+    double gmin, reltol, abstol;
+} ORDropInScalars;
 
 /* equality.c:22-60 */
 static int almost_equal_ulps(double A, double B, int maxUlps)
@@ -229,8 +247,21 @@ static int ckt_load(ORTran *T)
     memset(T->A, 0, sizeof(double) * (size_t)(n + 1) * (n + 1));
     memset(T->rhs, 0, sizeof(double) * (size_t)(n + 1));
     ck->rhsOld = T->rhsOld;
+    if (T->dl_run) {
+        /* DEVload through the drop-in: same place in CKTload, same vectors */
+        ORDropInScalars sc;
+        double *st[3];
+        int nc = 0, i;
+        sc.mode = mode; sc.order = ck->order; sc.method = ck->method;
+        for (i = 0; i < 7; i++) sc.ag[i] = ck->ag[i];
+        sc.xfact = ck->deltaOld[1] != 0.0 ? ck->delta / ck->deltaOld[1] : 0.0;
+        sc.gmin = ck->gmin; sc.reltol = ck->reltol; sc.abstol = ck->abstol;
+        st[0] = ck->states[0]; st[1] = ck->states[1]; st[2] = ck->states[2];
+        if (T->dl_run(T->dl_ctx, &sc, T->rhsOld, st, T->A, T->rhs, &nc)) return -1;
+        ck->noncon += nc;
+    }
     /* b4ld.c:69-5433 */
-    for (k = T->nmos - 1; k >= 0; k--) {   /* instances load newest first */
+    for (k = T->dl_run ? -1 : T->nmos - 1; k >= 0; k--) {   /* instances load newest first */
         ORInst *h = T->mos[k];
         if (or_bsim4_load(h, ck)) return -1;
         for (s = 0; s < P__COUNT; s++) {
@@ -344,6 +375,19 @@ static int solve_klu(ORTran *T)
 /* dense LU with partial pivoting on rows/cols 1..n; solution into rhs */
 static int solve(ORTran *T)
 {
+    if (T->smp_factor) {
+        /* SMPluFac / SMPsolve through the drop-in: it adds CKTdiagGmin to
+         * the diagonal itself (LoadGmin_CSC) */
+        const int n = T->n;
+        int c, p, rc;
+        for (c = 0; c < n; c++)
+            for (p = T->Ap[c]; p < T->Ap[c + 1]; p++) T->Ax[p] = AX(T->Ai[p] + 1, c + 1);
+        rc = T->smp_factor(T->smp_ctx, T->Ax, T->diagGmin);
+        if (rc) return rc;
+        if (T->smp_solve(T->smp_ctx, T->rhs)) return 1;
+        T->rhs[0] = 0.0;
+        return 0;
+    }
     /* LoadGmin_CSC (klusmp.c:1765-1777): CKTdiagGmin onto every diagonal
      * element the matrix has, before the (re)factorisation */
     if (T->diagGmin != 0.0) {
@@ -1036,18 +1080,13 @@ int or_tran_setup(ORTran *T)
 
 /* switch the linear solve to the reference's KLU in the library at path
  * (oracle/_ref/libklu_ref.so); call after or_tran_setup.  Returns 0. */
-int or_tran_use_klu(ORTran *T, const char *path)
+/* the structural CSC pattern (0-based) of the circuit's matrix: every
+ * element the devices' SMPmakeElt calls create */
+static int build_csc(ORTran *T)
 {
-    void *dl = dlopen(path, RTLD_NOW | RTLD_LOCAL);
     int n = T->n, r, c, k, s, nz = 0;
     char *S;
-    if (!dl) return -1;
-    T->kg_new = (void *(*)(int, const int *, const int *))dlsym(dl, "kg_new");
-    T->kg_factor = (int (*)(void *, double *, int, double))dlsym(dl, "kg_factor");
-    T->kg_solve = (int (*)(void *, double *))dlsym(dl, "kg_solve");
-    T->kg_free = (void (*)(void *))dlsym(dl, "kg_free");
-    T->kg_order = (int (*)(void *, int *, int *, int *))dlsym(dl, "kg_order");
-    if (!T->kg_new || !T->kg_factor || !T->kg_solve || !T->kg_free) return -2;
+    if (T->Ap) return 0;
     S = (char *)calloc((size_t)(n + 1) * (n + 1), 1);
 #define MARK(a, b) do { if ((a) && (b)) S[(size_t)(a) * (n + 1) + (b)] = 1; } while (0)
     for (k = 0; k < T->nmos; k++)
@@ -1085,8 +1124,63 @@ int or_tran_use_klu(ORTran *T, const char *path)
     }
     T->Ap[n] = nz;
     free(S);
-    T->klu = T->kg_new(n, T->Ap, T->Ai);
+    return 0;
+}
+
+int or_tran_use_klu(ORTran *T, const char *path)
+{
+    void *dl = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!dl) return -1;
+    T->kg_new = (void *(*)(int, const int *, const int *))dlsym(dl, "kg_new");
+    T->kg_factor = (int (*)(void *, double *, int, double))dlsym(dl, "kg_factor");
+    T->kg_solve = (int (*)(void *, double *))dlsym(dl, "kg_solve");
+    T->kg_free = (void (*)(void *))dlsym(dl, "kg_free");
+    T->kg_order = (int (*)(void *, int *, int *, int *))dlsym(dl, "kg_order");
+    if (!T->kg_new || !T->kg_factor || !T->kg_solve || !T->kg_free) return -2;
+    if (build_csc(T)) return -3;
+    T->klu = T->kg_new(T->n, T->Ap, T->Ai);
     return T->klu ? 0 : -3;
+}
+
+/* The CSC pattern for a caller's own solver: Ap[n+1] and Ai[nnz] (0-based)
+ * into the given arrays when non-NULL; returns nnz.  Call after
+ * or_tran_setup. */
+int64_t or_tran_csc(ORTran *T, int64_t *Ap, int *Ai)
+{
+    int k;
+    if (build_csc(T)) return -1;
+    if (Ap) for (k = 0; k <= T->n; k++) Ap[k] = T->Ap[k];
+    if (Ai) for (k = 0; k < T->Ap[T->n]; k++) Ai[k] = T->Ai[k];
+    return T->Ap[T->n];
+}
+
+/* State offset and terminals (d g s b) of MOSFET k, in insertion order. */
+int or_tran_mos_info(const ORTran *T, int k, int *state, int *nodes)
+{
+    if (k < 0 || k >= T->nmos) return -1;
+    if (state) *state = T->mos[k]->state;
+    if (nodes) { nodes[0] = T->mos[k]->node[N_d]; nodes[1] = T->mos[k]->node[N_g];
+                 nodes[2] = T->mos[k]->node[N_s]; nodes[3] = T->mos[k]->node[N_b]; }
+    return 0;
+}
+
+/* Drop-in BSIM4 load and linear solver (NULL functions restore the
+ * oracle's own): run(ctx, scalars, rhsOld, states[3], A, rhs, &noncon)
+ * replaces the MOSFET loads of CKTload, adding into the dense matrix A
+ * ((n+1) x (n+1), row-major, entry (r, c) at r (n+1) + c) and rhs;
+ * factor(ctx, Ax, gmin) and solve(ctx, rhs) replace the linear solve, Ax in
+ * the order of or_tran_csc's pattern, rhs 1-based.  Call after
+ * or_tran_setup. */
+int or_tran_use_dropin(ORTran *T, void *run, void *dl_ctx, void *factor, void *solve_fn, void *smp_ctx)
+{
+    T->dl_run = (int (*)(void *, const void *, const double *, double *const *, double *, double *, int *))run;
+    T->dl_ctx = dl_ctx;
+    T->smp_factor = (int (*)(void *, const double *, double))factor;
+    T->smp_solve = (int (*)(void *, double *))solve_fn;
+    T->smp_ctx = smp_ctx;
+    if (T->smp_factor && build_csc(T)) return -1;
+    if (T->smp_factor && !T->Ax) T->Ax = (double *)calloc((size_t)T->Ap[T->n] + 1, sizeof(double));
+    return 0;
 }
 
 /* The reference KLU's pivot sequence from its last klu_factor, as 1-based

@@ -17,7 +17,7 @@ def _declared_functions():
     for h in glob.glob(os.path.join(REPO, 'include', '*.h')):
         src = open(h).read()
         src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
-        for m in re.finditer(r'^[A-Za-z_][\w \*]*?\b(ngmi_\w+)\s*\(', src, flags=re.M):
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(ngmi_\w+|ngSpice_\w+|ngGet_\w+)\s*\(", src, flags=re.M):
             names.append(m.group(1))
     return sorted(set(names))
 
