@@ -13,8 +13,10 @@ Workloads
   rings: the uneven batch of 2048 ring oscillators (11..301 stages).
   mesh : the 1024 x 1024 power-grid mesh's Newton matrix (~1.1M unknowns);
          a step is one refactor + one solve by the whole-GPU multifrontal LU.
-  c3   : the rings' transient with the mesh's refactor + solve running
-         alongside on a second stream (configuration 3 together).
+  mesh_tran: the mesh with its 16384 clocked inverters as a transient
+         (ngmi_large_*), timed over a window of its .tran 10p 10n.
+  c3   : the rings' transient with the mesh transient running alongside on
+         a second stream (configuration 3 together).
 
 Inputs are resident in HBM before the timed region.  Kernel durations come
 from HIP events recorded on the stream the kernels are launched on.
@@ -466,10 +468,12 @@ def bench_mesh(args):
 
 def bench_c3(args):
     """Configuration 3 together: the uneven batch of 2048 rings runs its
-    transient on one stream while a host thread refactors and solves the
-    1024 x 1024 power-grid matrix on a second stream, over and over, for as
-    long as the rings run.  Reports the rings' rate with the mesh alongside,
-    and the mesh's refactor+solve rate in the same window."""
+    transient on one stream while a host thread runs the 1024 x 1024
+    power-grid mesh's transient (windows of --mesh-window simulated
+    seconds, back to back, each from its operating point) on a second
+    stream for as long as the rings run.  Reports the rings' rate with the
+    mesh alongside and the mesh's accepted timepoints per second in the
+    same window."""
     import threading
     import torch
     import ngmi
@@ -482,26 +486,24 @@ def bench_c3(args):
     C = len(descs)
     del descs
     side = args.mesh_side
-    loads = MESH_LOADS_PER_SIDE2 * side * side // (1024 * 1024)
-    n, Ap, Ai, Ax = workloads.power_grid_jacobian(side, loads)
-    lu = ngmi.SparseLU(n, Ap, Ai)
-    dA = torch.from_numpy(Ax).cuda()
-    db0 = torch.from_numpy(workloads.rng(5, side).uniform(-1, 1, n)).cuda()
-    db = db0.clone()
+    d = workloads.power_grid(side=side, tstop=10e-9)
+    lg = ngmi.Large(ngmi.Circuit(d), probe=[1], max_points=1000000)
+    n = lg.info()['unknowns']
+    del d
     ms = torch.cuda.Stream()
-    lu.factor(dA.data_ptr(), ms.cuda_stream)
+    win = args.mesh_window
+    if lg.tran(10e-12, win, 10e-12, stream=ms.cuda_stream) != 0:   # warm-up
+        raise RuntimeError('mesh transient failed: %s' % ngmi.last_error())
     torch.cuda.synchronize()
     stop = threading.Event()
-    count = [0]
+    mesh_pts = [0, 0]   # accepted points, windows finished
 
     def mesh_loop():
-        with torch.cuda.stream(ms):
-            while not stop.is_set():
-                lu.factor(dA.data_ptr(), ms.cuda_stream)
-                db.copy_(db0)
-                lu.solve(db.data_ptr(), ms.cuda_stream)
-                ms.synchronize()
-                count[0] += 1
+        while not stop.is_set():
+            if lg.tran(10e-12, win, 10e-12, stream=ms.cuda_stream) != 0:
+                break
+            mesh_pts[0] += lg.stats()['accepted']
+            mesh_pts[1] += 1
 
     rs = torch.cuda.Stream()
     th = threading.Thread(target=mesh_loop)
@@ -509,26 +511,89 @@ def bench_c3(args):
     w0 = time.perf_counter()
     th.start()
     rc, passes = b.tran(10e-12, args.tstop, stream=rs.cuda_stream)
-    rs.synchronize()   # the rings' stream only: the mesh iteration in flight is not waited for
+    rs.synchronize()   # the rings' stream only: the mesh window in flight is not waited for
     wall = time.perf_counter() - w0
-    mesh_done = count[0]   # mesh iterations finished inside the rings' window
+    mesh_done = list(mesh_pts)   # mesh windows finished inside the rings' window
     stop.set()
     th.join()
     if rc != 0:
         raise RuntimeError('transient failed: %d %s' % (rc, ngmi.last_error()))
     st = [b.stats(c) for c in range(C)]
     acc = sum(s_['accepted'] for s_ in st)
-    log('c3: rings %.0f tp*c/s with the mesh alongside (%.0f alone); mesh %d refactor+solve in %.2f s '
-        '= %.1f/s alongside' % (acc / wall, alone, mesh_done, wall, mesh_done / wall))
-    res['metric'] = 'uneven-batch transient timepoints x circuits/s with the power-grid LU alongside'
+    log('c3: rings %.0f tp*c/s with the mesh transient alongside (%.0f alone); mesh %d accepted points in %d '
+        'windows in %.2f s = %.1f timepoints/s alongside' % (acc / wall, alone, mesh_done[0], mesh_done[1], wall,
+                                                            mesh_done[0] / wall))
+    res['metric'] = 'uneven-batch transient timepoints x circuits/s with the power-grid mesh transient alongside'
     res['value'] = acc / wall
     res['ms_per_step'] = wall * 1e3
-    res['config']['workload'] += ('; alongside: refactor + solve of the %dx%d power-grid matrix '
-                                  '(%d unknowns) on a second stream' % (side, side, n))
-    res['extra'].update({'rings_alone': alone, 'mesh_refactor_solve_per_s_alongside': mesh_done / wall,
-                         'mesh_refactor_solves': mesh_done, 'accepted_alongside': acc,
-                         'failed_alongside': sum(1 for s_ in st if s_['status'])})
+    res['config']['workload'] += ('; alongside: the %dx%d power-grid mesh transient (%d unknowns, 16384 clocked '
+                                  'inverters, windows of %g ps) on a second stream' % (side, side, n, win * 1e12))
+    res['extra'].update({'rings_alone': alone, 'mesh_timepoints_per_s_alongside': mesh_done[0] / wall,
+                         'mesh_accepted_points': mesh_done[0], 'mesh_windows': mesh_done[1],
+                         'accepted_alongside': acc})
     return res
+
+
+MESH_TRAN_WINDOW = 3e-12   # the first 3 ps of configuration 3's .tran 10p 10n
+
+
+def bench_mesh_tran(args):
+    """Configuration 3's mesh as a transient: the 1024 x 1024 power grid
+    with 16384 clocked inverter loads (1.1M unknowns), .tran 10p 10n with
+    TMAX 10 ps, run over a window (its first --mesh-window seconds of
+    simulated time: the 16384 clock phases put a PULSE corner every ~15 fs,
+    so the full 10 ns is about 6.5e5 accepted points).  A step is the whole
+    window; value = accepted timepoints per second (one circuit)."""
+    import torch
+    import ngmi
+    import workloads
+
+    side = args.mesh_side
+    t0 = time.time()
+    d = workloads.power_grid(side=side, tstop=10e-9)
+    nn = d['nnodes']
+    probe = [1, 1 + (side // 2) * side + side // 2, nn]
+    lg = ngmi.Large(ngmi.Circuit(d), probe=probe, max_points=1000000)
+    info = lg.info()
+    del d
+    log('mesh transient setup %.1fs: %s' % (time.time() - t0, info))
+    stream = torch.cuda.current_stream().cuda_stream
+    win = args.mesh_window
+    rc = lg.tran(10e-12, win, 10e-12, stream=stream)     # warm-up (code objects, fronts)
+    if rc != 0:
+        raise RuntimeError('mesh transient failed: %d %s' % (rc, ngmi.last_error()))
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    rc = lg.tran(10e-12, win, 10e-12, stream=stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    if rc != 0:
+        raise RuntimeError('mesh transient failed: %d %s' % (rc, ngmi.last_error()))
+    st = lg.stats()
+    tm = lg.times()
+    kcl, per = lg.kcl()
+    its = st['iterations']
+    refactor_ms = tm['factor_ms'] / max(its, 1)
+    tflops = info['flops'] / (refactor_ms * 1e-3) / 1e12 if refactor_ms > 0 else None
+    log('mesh transient: %d accepted, %d rejected, %d iterations in %.2f s (%.1f ms/iteration: load %.2f, '
+        'refactor %.2f, solve %.2f, conv %.2f); KCL %.2e'
+        % (st['accepted'], st['rejected'], its, wall, wall * 1e3 / max(its, 1), tm['load_ms'] / max(its, 1),
+           refactor_ms, tm['solve_ms'] / max(its, 1), tm['conv_ms'] / max(its, 1), kcl))
+    return {
+        'metric': 'power-grid mesh transient timepoints/s (windowed)',
+        'value': st['accepted'] / wall, 'unit': 'timepoints/s', 'ms_per_step': wall * 1e3, 'dtype': 'f64',
+        'config': {'workload': 'power-grid mesh %dx%d, 16384 clocked inverter loads (%d unknowns), .tran 10p '
+                               '10n (TMAX 10p) over its first %g ps' % (side, side, info['unknowns'], win * 1e12),
+                   'unknowns': info['unknowns'], 'lu_nonzeros': info['lu_nonzeros']},
+        'roofline': {'bound': 'mfma', 'achieved': tflops, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': tflops / FP64_PEAK_TFLOPS if tflops else None, 'traffic': None,
+                     'kernel': 'refactor (every k_sp_* launch of one ngmi_splu_factor) inside the transient',
+                     'flops_per_refactor': info['flops'], 'refactor_ms': refactor_ms},
+        'extra': {'accepted': st['accepted'], 'rejected': st['rejected'], 'newton_iterations': its,
+                  'ms_per_iteration': wall * 1e3 / max(its, 1),
+                  'phase_ms_per_iteration': {k: v / max(its, 1) for k, v in tm.items()},
+                  'kcl_scaled_residual_final_point': kcl, 'windowed': True},
+    }
 
 
 def cpu_baseline_mesh(side):
@@ -731,7 +796,9 @@ def main():
                     help='timed steps (mc: whole .tran runs of the batch; default 1)')
     ap.add_argument('--warmup', type=int, default=None,
                     help='untimed steps before (mc default 1)')
-    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings', 'mesh', 'c3'])
+    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings', 'mesh', 'mesh_tran', 'c3'])
+    ap.add_argument('--mesh-window', type=float, default=MESH_TRAN_WINDOW,
+                    help='mesh_tran / c3: simulated seconds of the mesh transient timed')
     ap.add_argument('--mesh-side', type=int, default=1024, help='mesh: nodes per side')
     ap.add_argument('--copy-bytes', type=int, default=1 << 30)
     ap.add_argument('--n', type=int, default=1 << 20)
@@ -814,6 +881,9 @@ def main():
         cpu = None
     elif args.workload == 'mesh':
         res = bench_mesh(args)
+    elif args.workload == 'mesh_tran':
+        res = bench_mesh_tran(args)
+        cpu = None
     elif args.workload == 'c3':
         res = bench_c3(args)
         cpu = None

@@ -62,9 +62,13 @@ const char *ngmi_last_error(void);
  * "order_slack" (0..4, default 1): the engine's own pivot order may pick a
  * node up to this much above the minimum degree when that shortens the
  * elimination tree (fewer factor and solve levels, possibly more fill);
- * it applies to batches built afterwards.
+ * it applies to batches built afterwards;
+ * "graph" (0 or 1, default 1): ngmi_batch_tran replays chunks of 32 Newton
+ * passes as one captured HIP graph instead of launching each kernel.
  * Process-wide; defaults are the measured best on MI355X. */
 int ngmi_tune(const char *knob, int value);
+/* current value of a knob, or minus NGMI_E_BADPARM for an unknown name */
+int ngmi_tune_get(const char *knob);
 
 /* ------------------------------------------------------------------ */
 /* BSIM4 model cards                                                   */
@@ -184,6 +188,43 @@ int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream);
  * NGMI_E_SINGULAR when the last ngmi_splu_factor did not succeed (or none
  * ran), where klu_solve would run on the failed factors. */
 int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* One large circuit on the whole GPU: transient analysis              */
+/* ------------------------------------------------------------------ */
+/* For one circuit too large for the batched per-circuit programs (the
+ * 1024 x 1024 power-grid mesh with its inverter loads, ~1.1M unknowns).
+ * ngmi_large_new takes a circuit built with the ngmi_circuit_* calls (its
+ * models must outlive the object): CSC pattern and symbolic LU
+ * (ngmi_splu_analyze, the klu_analyze of SMPpreOrder), the signed assembly
+ * plans of CKTload (cktload.c:31-180) and the BSIM4 tables; probe lists
+ * the 1-based unknowns recorded at each accepted point (NULL: all),
+ * max_points bounds the record.  NULL on error (ngmi_last_error).
+ * .ic values and Gear orders above 2 are not supported. */
+typedef struct ngmi_large ngmi_large;
+ngmi_large *ngmi_large_new(const struct ngmi_circuit *c, const int *probe, int nprobe, int64_t max_points);
+void ngmi_large_free(ngmi_large *L);
+/* unknowns, matrix nonzeros, L+U nonzeros and refactor flops */
+int ngmi_large_info(const ngmi_large *L, int64_t *n_unknowns, int64_t *nnz, int64_t *lu_nonzeros, double *flops);
+/* .tran tstep tstop [tmax]: CKTop's direct Newton (cktop.c:26-116 without
+ * the stepping fallbacks), then DCtran (dctran.c:66-971) with NIiter
+ * (niiter.c:28-367) driven from the host, the loads, LU (ngmi_splu_*),
+ * NIconvTest and CKTterr on the device of the calling thread, on stream.
+ * Returns when the run ends; NGMI_E_TIMESTEP, NGMI_E_ITERLIM (operating
+ * point) or NGMI_E_SINGULAR (operating point) on failure. */
+int ngmi_large_tran(ngmi_large *L, double tstep, double tstop, double tmax, void *stream);
+/* recorded points, accepted / rejected timepoints, Newton iterations */
+int ngmi_large_stats(const ngmi_large *L, int64_t *npts, int64_t *accepted, int64_t *rejected, int64_t *iterations);
+/* the record: t[npts], x[npts][nprobe] */
+int ngmi_large_waveform(const ngmi_large *L, double *t, double *x, int64_t maxpts, int64_t *npts);
+/* KCL at the last accepted solution x: the circuit equations F(x) =
+ * A(x) x - b(x) assembled there (one more load in MODETRAN | INITFLOAT);
+ * *scaled = max over rows of |F| / (sum |A x| + |b|), per_kind[0..1] =
+ * max |F| over node rows (amperes) and branch rows (volts).  After a run. */
+int ngmi_large_kcl(ngmi_large *L, double *scaled, double *per_kind);
+/* host wall milliseconds of the last run by phase: loads and assembly,
+ * refactor, solve, convergence test */
+int ngmi_large_times(const ngmi_large *L, double *load_ms, double *factor_ms, double *solve_ms, double *other_ms);
 
 /* ------------------------------------------------------------------ */
 /* One circuit in host memory: the cores a DEVload / SMP binding calls */
@@ -365,6 +406,20 @@ int ngmi_batch_waveform(const ngmi_batch *b, int c, double *t, double *x, int64_
  * outitf.c:928-990); NULL names use the unknowns' indices. */
 int ngmi_batch_write_raw(const ngmi_batch *b, int c, const char *path, const char *title,
                          const char *const *names, int binary);
+
+/* Autotuner of the launch knobs (no counterpart in the reference): times
+ * trial_passes Newton passes of the batched transient of ckts from t = 0
+ * under each candidate value of order_slack, mos_waves, streams and graph
+ * (see ngmi_tune), one knob at a time from the current settings, keeps the
+ * fastest (a change must win by 2 %), leaves the process-wide knobs set to
+ * the choice, appends it to the text file cache_path (NULL: none) under the
+ * batch's shape, and writes the timings to report.  Builds and frees its
+ * own batches. */
+int ngmi_autotune_batch(ngmi_circuit *const *ckts, int C, double tstep, double tstop, int64_t trial_passes,
+                        const char *cache_path, char *report, int report_len);
+/* applies the newest cached choice for the shape of ckts;
+ * NGMI_E_NOTFOUND when the cache has none */
+int ngmi_autotune_load(const char *cache_path, ngmi_circuit *const *ckts, int C);
 
 /* ------------------------------------------------------------------ */
 /* SPICE decks                                                         */

@@ -10,7 +10,7 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 1};
+static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 1, 1};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
 
@@ -30,6 +30,28 @@ extern "C" {
 
 const char *ngmi_last_error(void) { return g_err; }
 
+static int knob_index(const char *knob)
+{
+    if (!knob) return -1;
+    if (!std::strcmp(knob, "eval_waves")) return TUNE_EVAL_WAVES;
+    if (!std::strcmp(knob, "mos_waves")) return TUNE_MOS_WAVES;
+    if (!std::strcmp(knob, "streams")) return TUNE_STREAMS;
+    if (!std::strcmp(knob, "specialise")) return TUNE_SPECIALISE;
+    if (!std::strcmp(knob, "order_slack")) return TUNE_ORDER_SLACK;
+    if (!std::strcmp(knob, "graph")) return TUNE_GRAPH;
+    return -1;
+}
+
+int ngmi_tune_get(const char *knob)
+{
+    const int k = knob_index(knob);
+    if (k < 0) {
+        set_error("unknown tuning knob '%s'", knob ? knob : "(null)");
+        return -NGMI_E_BADPARM;
+    }
+    return tune_get(k);
+}
+
 int ngmi_tune(const char *knob, int value)
 {
     if (!knob) return NGMI_E_PANIC;
@@ -39,12 +61,13 @@ int ngmi_tune(const char *knob, int value)
     else if (!std::strcmp(knob, "streams")) k = TUNE_STREAMS;
     else if (!std::strcmp(knob, "specialise")) k = TUNE_SPECIALISE;
     else if (!std::strcmp(knob, "order_slack")) k = TUNE_ORDER_SLACK;
+    else if (!std::strcmp(knob, "graph")) k = TUNE_GRAPH;
     if (k < 0) {
         set_error("unknown tuning knob '%s'", knob);
         return NGMI_E_BADPARM;
     }
-    const int vmin = (k == TUNE_SPECIALISE || k == TUNE_ORDER_SLACK) ? 0 : 1;
-    const int vmax = (k == TUNE_STREAMS) ? 2 : (k == TUNE_SPECIALISE) ? 1 : 4;
+    const int vmin = (k == TUNE_SPECIALISE || k == TUNE_ORDER_SLACK || k == TUNE_GRAPH) ? 0 : 1;
+    const int vmax = (k == TUNE_STREAMS) ? 2 : (k == TUNE_SPECIALISE || k == TUNE_GRAPH) ? 1 : 4;
     if (value < vmin || value > vmax) {
         set_error("knob '%s' takes %d..%d", knob, vmin, vmax);
         return NGMI_E_PARMVAL;

@@ -22,6 +22,7 @@ enum {
     TUNE_STREAMS = 2,
     TUNE_SPECIALISE = 3,
     TUNE_ORDER_SLACK = 4,
+    TUNE_GRAPH = 5,
     TUNE__COUNT
 };
 int tune_get(int knob);

@@ -36,6 +36,7 @@ int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, int c_begin, int c_en
                      const TranDev *Ds, const int2 *mos_blk, int n_mos_blk, const int2 *step_blk,
                      int n_step_blk, size_t lds_bytes, hipStream_t s, hipEvent_t *ev);
 int set_tran_lds_limit(size_t bytes);
+int launch_zero_int(int *p, hipStream_t s);
 int launch_guess_features(const TranDev &D, float *F, int *rowmap, double2 *base, int *count,
                           hipStream_t s);
 int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double2 *base,
@@ -62,34 +63,7 @@ constexpr int PROG_PLAIN = 0, PROG_PARTIAL = 1, PROG_COMBINE = 2;
 /* entries with more terms run as partial sums over several lanes */
 constexpr int SPLIT_TERMS = 24;
 
-struct ngmi_circuit {
-    int nnodes = 0;
-    struct Mos {
-        ngmi_bsim4_model *model;
-        int node[4];
-        std::vector<std::string> names;
-        std::vector<double> values;
-    };
-    std::vector<Mos> mos;
-    std::vector<std::array<double, 3>> caps;   /* p n C */
-    std::vector<std::array<double, 3>> res;    /* p n R */
-    struct Vs {
-        int p, n;
-        double dc;
-        int npulse;
-        double pulse[7];
-    };
-    std::vector<Vs> vs;
-    std::vector<double> ic;
-    std::vector<char> ic_given;
-    double reltol = 1e-3, abstol = 1e-12, vntol = 1e-6, chgtol = 1e-14, trtol = 7.0, gmin = 1e-12;
-    int method = 1, maxord = 2;
-    /* CKTop options (cktntask.c:94-132 defaults) */
-    int noopiter = 0, gminsteps = 1, srcsteps = 1, itl2 = 50;
-    double gminfactor = 10.0, gshunt = 0.0;
-    /* optional pivot sequence (1-based unknowns); empty = the engine's own */
-    std::vector<int> order_rows, order_cols;
-};
+#include "circuit_types.h"
 
 /* circuits of one topology inside a batch: shared symbolic analysis and
  * BSIM4 table, per-circuit device arrays */
@@ -1346,14 +1320,96 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     }
     b->mos_ms = b->step_ms = b->guess_ms = 0;
     int rc = NGMI_OK;
+    /* Whole chunks of check_every passes are captured once into a HIP graph
+     * and replayed: every pass launches the same kernels with the same
+     * arguments (all per-circuit state lives in device memory), so a chunk
+     * is one graph launch instead of 2-5 launches per pass.  The first
+     * chunk runs eagerly (it makes the lazy allocations of the guess
+     * network); profiling, which times each launch with events, runs
+     * eagerly throughout. */
+    const bool use_graph = tune_get(TUNE_GRAPH) && !b->profile;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr;
+    auto launch_chunk = [&](hipStream_t sa, int count, bool capturing) -> int {
+        int r = NGMI_OK;
+        if (split && capturing) {
+            NGMI_HIP(hipEventRecord(fork, sa));
+            NGMI_HIP(hipStreamWaitEvent(s2, fork, 0));
+        }
+        for (int q = 0; q < count && r == NGMI_OK; q++) {
+            if (b->net) {
+                const int maxrows = G0.C * G0.N;
+                r = launch_zero_int(b->gcount, sa);
+                if (r == NGMI_OK) r = launch_guess_features(G0.D, b->gF, b->grow, b->gbase, b->gcount, sa);
+                if (r == NGMI_OK) r = mlp_forward_rows(b->net, b->gF, b->gy, maxrows, b->gcount, sa);
+                if (r == NGMI_OK) r = launch_guess_apply(G0.D, b->gy, b->grow, b->gbase, b->gcount, maxrows, sa);
+            }
+            if (r != NGMI_OK) break;
+            if (split) {
+                r = launch_tran_pass(tab0, &G0.D, 0, half, nullptr, nullptr, nullptr, 1, nullptr, half, b->lds, sa,
+                                     nullptr);
+                if (r == NGMI_OK)
+                    r = launch_tran_pass(tab0, &G0.D, half, C, nullptr, nullptr, nullptr, 1, nullptr, C - half,
+                                         b->lds, s2, nullptr);
+            } else {
+                r = launch_tran_pass(one ? tab0 : nullptr, one ? &G0.D : nullptr, 0, C, b->dTab, b->dD, b->mos_blk,
+                                     b->n_mos_blk, b->step_blk, b->n_step_blk, b->lds, sa, nullptr);
+            }
+        }
+        if (split && capturing && r == NGMI_OK) {
+            NGMI_HIP(hipEventRecord(join, s2));
+            NGMI_HIP(hipStreamWaitEvent(sa, join, 0));
+        }
+        return r;
+    };
     while (pass < max_passes && rc == NGMI_OK) {
         int k = 0;
-        for (; k < check_every && pass < max_passes; k++, pass++) {
+        if (use_graph && pass > 0 && max_passes - pass >= check_every) {
+            if (!exec) {
+                if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess ||
+                    (split && hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) ||
+                    hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    set_error("graph capture could not start: %s", hipGetErrorString(hipGetLastError()));
+                    rc = NGMI_E_DEVICE;
+                    break;
+                }
+                const int lrc = launch_chunk(cap, check_every, true);
+                const hipError_t e = hipStreamEndCapture(cap, &graph);
+                if (lrc != NGMI_OK || e != hipSuccess ||
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                    set_error("graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : hipGetLastError()));
+                    rc = lrc != NGMI_OK ? lrc : NGMI_E_DEVICE;
+                    break;
+                }
+            }
+            if (hipGraphLaunch(exec, s) != hipSuccess) {
+                set_error("graph launch failed: %s", hipGetErrorString(hipGetLastError()));
+                rc = NGMI_E_DEVICE;
+                break;
+            }
+            k = check_every;
+            pass += check_every;
+        } else if (!b->profile) {
+            k = (int)std::min<int64_t>(check_every, max_passes - pass);
+            rc = launch_chunk(s, k, false);
+            pass += k;
+            if (rc != NGMI_OK) break;
+            if (split && (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess)) {
+                set_error("stream join failed: %s", hipGetErrorString(hipGetLastError()));
+                rc = NGMI_E_DEVICE;
+                break;
+            }
+        }
+        for (; b->profile && k < check_every && pass < max_passes; k++, pass++) {
             if (b->net) {
                 if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k], s));
                 const int maxrows = G0.C * G0.N;
-                NGMI_HIP(hipMemsetAsync(b->gcount, 0, sizeof(int), s));
-                rc = launch_guess_features(G0.D, b->gF, b->grow, b->gbase, b->gcount, s);
+                /* a kernel, not a memset node, clears the row counter: the
+                 * chunk may be captured into a graph */
+                rc = launch_zero_int(b->gcount, s);
+                if (rc == NGMI_OK) rc = launch_guess_features(G0.D, b->gF, b->grow, b->gbase, b->gcount, s);
                 if (rc == NGMI_OK) rc = mlp_forward_rows(b->net, b->gF, b->gy, maxrows, b->gcount, s);
                 if (rc == NGMI_OK)
                     rc = launch_guess_apply(G0.D, b->gy, b->grow, b->gbase, b->gcount, maxrows, s);
@@ -1373,7 +1429,8 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
             }
             if (rc != NGMI_OK) break;
         }
-        if (split && (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess)) {
+        if (b->profile && split &&
+            (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess)) {
             set_error("stream join failed: %s", hipGetErrorString(hipGetLastError()));
             rc = NGMI_E_DEVICE;
             break;
@@ -1409,6 +1466,10 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     for (auto &e : ev) (void)hipEventDestroy(e);
     for (auto &e : ev2) (void)hipEventDestroy(e);
     for (auto &e : gev) (void)hipEventDestroy(e);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (fork) (void)hipEventDestroy(fork);
     if (split) {
         (void)hipStreamSynchronize(s2);
         (void)hipEventDestroy(join);

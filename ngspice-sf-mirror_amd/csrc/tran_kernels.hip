@@ -1691,6 +1691,15 @@ extern "C" int ngmi_guess_features(int M, const double *t, const double *h, cons
     return NGMI_OK;
 }
 
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+int launch_zero_int(int *p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, p);
+    NGMI_HIP(hipGetLastError());
+    return NGMI_OK;
+}
+
 int set_tran_lds_limit(size_t bytes)
 {
     NGMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tran_step),

@@ -531,6 +531,116 @@ class SMP:
             pass
 
 
+class Large:
+    """One circuit too large for the batched programs (ngmi_large_*): the
+    power-grid mesh's transient with DCtran / NIiter on the host and every
+    vector operation, the BSIM4 loads and the whole-GPU LU on the device.
+    probe: 1-based unknowns recorded at each accepted point (None: all)."""
+
+    def __init__(self, circuit, probe=None, max_points=100000):
+        L = lib()
+        vp, i64 = ctypes.c_void_p, ctypes.c_int64
+        L.ngmi_large_new.argtypes = [vp, vp, ctypes.c_int, i64]
+        L.ngmi_large_new.restype = vp
+        L.ngmi_large_free.argtypes = [vp]
+        L.ngmi_large_info.argtypes = [vp] + [ctypes.POINTER(i64)] * 3 + [ctypes.POINTER(ctypes.c_double)]
+        L.ngmi_large_tran.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
+        L.ngmi_large_stats.argtypes = [vp] + [ctypes.POINTER(i64)] * 4
+        L.ngmi_large_waveform.argtypes = [vp, vp, vp, i64, ctypes.POINTER(i64)]
+        L.ngmi_large_times.argtypes = [vp] + [ctypes.POINTER(ctypes.c_double)] * 4
+        L.ngmi_large_kcl.argtypes = [vp, ctypes.POINTER(ctypes.c_double), vp]
+        self.circuit = circuit
+        pr = None if probe is None else np.ascontiguousarray(probe, dtype=np.int32)
+        self.nprobe = None if pr is None else len(pr)
+        self.h = L.ngmi_large_new(circuit.h, pr.ctypes.data if pr is not None else None,
+                                  len(pr) if pr is not None else 0, max_points)
+        if not self.h:
+            raise EngineError(1, 'large_new')
+        if self.nprobe is None:
+            self.nprobe = self.info()['unknowns']
+
+    def info(self):
+        v = [ctypes.c_int64() for _ in range(3)]
+        fl = ctypes.c_double()
+        check(lib().ngmi_large_info(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(fl)), 'large_info')
+        return dict(unknowns=v[0].value, nonzeros=v[1].value, lu_nonzeros=v[2].value, flops=fl.value)
+
+    def tran(self, tstep, tstop, tmax=0.0, stream=0):
+        return lib().ngmi_large_tran(self.h, tstep, tstop, tmax, stream)
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        check(lib().ngmi_large_stats(self.h, *[ctypes.byref(x) for x in v]), 'large_stats')
+        return dict(points=v[0].value, accepted=v[1].value, rejected=v[2].value, iterations=v[3].value)
+
+    def waveform(self):
+        n = self.stats()['points']
+        t = np.zeros(max(n, 1))
+        x = np.zeros((max(n, 1), self.nprobe))
+        got = ctypes.c_int64()
+        check(lib().ngmi_large_waveform(self.h, t.ctypes.data, x.ctypes.data, n, ctypes.byref(got)),
+              'large_waveform')
+        return t[:got.value], x[:got.value]
+
+    def times(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        check(lib().ngmi_large_times(self.h, *[ctypes.byref(x) for x in v]), 'large_times')
+        return dict(load_ms=v[0].value, factor_ms=v[1].value, solve_ms=v[2].value, conv_ms=v[3].value)
+
+    def kcl(self):
+        """KCL residual of the final point: max |F(x)| over the unknowns, with
+        F the circuit equations at the last accepted solution, scaled by the
+        largest current term; returns (scaled residual, max |F| per row kind
+        [nodes, branches])"""
+        r = ctypes.c_double()
+        per = np.zeros(2)
+        check(lib().ngmi_large_kcl(self.h, ctypes.byref(r), per.ctypes.data), 'large_kcl')
+        return r.value, per
+
+    def __del__(self):
+        try:
+            lib().ngmi_large_free(self.h)
+        except Exception:
+            pass
+
+
+def autotune(circuits, tstep, tstop, trial_passes=200, cache_path=None):
+    """ngmi_autotune_batch over the Circuit objects; returns the timing
+    report (the chosen knobs stay set process-wide)"""
+    L = lib()
+    vp = ctypes.c_void_p
+    L.ngmi_autotune_batch.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int64,
+                                      ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    arr = (vp * len(circuits))(*[c.h for c in circuits])
+    buf = ctypes.create_string_buffer(4096)
+    check(L.ngmi_autotune_batch(arr, len(circuits), tstep, tstop, trial_passes,
+                                cache_path.encode() if cache_path else None, buf, 4096), 'autotune')
+    return buf.value.decode()
+
+
+def autotune_load(circuits, cache_path):
+    """apply a cached autotuner choice; False when the cache has none"""
+    L = lib()
+    vp = ctypes.c_void_p
+    L.ngmi_autotune_load.argtypes = [ctypes.c_char_p, vp, ctypes.c_int]
+    arr = (vp * len(circuits))(*[c.h for c in circuits])
+    rc = L.ngmi_autotune_load(cache_path.encode(), arr, len(circuits))
+    if rc == 14:
+        return False
+    check(rc, 'autotune_load')
+    return True
+
+
+def get_tune(knob):
+    """current value of a launch knob"""
+    L = lib()
+    L.ngmi_tune_get.argtypes = [ctypes.c_char_p]
+    v = L.ngmi_tune_get(knob.encode())
+    if v < 0:
+        raise EngineError(-v, 'tune get %s' % knob)
+    return v
+
+
 def guess_features(t4, h, v4, nbv, nbmask, stream=0):
     """The engine's guess feature rows (the device function the in-loop
     features kernel uses) for caller-supplied windows, through

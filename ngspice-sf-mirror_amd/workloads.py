@@ -320,3 +320,42 @@ def power_grid_jacobian(side=MESH_SIDE, loads=MESH_LOADS, pad_every=MESH_PAD_EVE
     A.sum_duplicates()
     A.sort_indices()
     return n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
+MESH_CLOCK_PERIOD = 1e-9
+MESH_CLOCK_EDGE = 20e-12
+
+
+def power_grid(side=MESH_SIDE, loads=None, pad_every=MESH_PAD_EVERY, seed=0, tstep=10e-12, tstop=10e-9):
+    """The power-grid mesh as a circuit description: side x side nodes on a
+    0.1 ohm segment grid (node (i, j) = 1 + i*side + j), 10 fF from every
+    mesh node to ground, 1 V pads (voltage sources to ground) every
+    pad_every nodes in both directions, and `loads` CMOS inverters at seeded
+    mesh nodes (Philox stream (4, seed), the taps power_grid_jacobian uses):
+    inverter k's PMOS draws from its mesh node, its input (node
+    side^2 + 2k + 1) is driven by its own clock PULSE(0 1 td 20p 20p 480p 1n)
+    with td uniform in [0, 1 ns], its output (node side^2 + 2k + 2) carries
+    1 fF.  loads defaults to 16384 scaled with the mesh area."""
+    if loads is None:
+        loads = MESH_LOADS * side * side // (MESH_SIDE * MESH_SIDE)
+    g = rng(4, seed)
+    nm = side * side
+    tap = g.integers(0, nm, size=loads)
+    td = rng(6, seed).uniform(0.0, MESH_CLOCK_PERIOD, loads)
+    ij = np.arange(nm).reshape(side, side) + 1
+    res = [(int(a), int(b), 1.0 / MESH_G) for a, b in zip(ij[:, :-1].ravel(), ij[:, 1:].ravel())]
+    res += [(int(a), int(b), 1.0 / MESH_G) for a, b in zip(ij[:-1, :].ravel(), ij[1:, :].ravel())]
+    caps = [(int(u), 0, MESH_C) for u in range(1, nm + 1)]
+    vsrc = [(int(ij[a * pad_every, b * pad_every]), 0, VDD, None)
+            for a in range(side // pad_every) for b in range(side // pad_every)]
+    mos = []
+    for k in range(loads):
+        vin, out, t = nm + 2 * k + 1, nm + 2 * k + 2, int(tap[k]) + 1
+        mos.append(('n', out, vin, 0, 0, {'w': RING_WN, 'l': RING_L}))
+        mos.append(('p', out, vin, t, t, {'w': RING_WP, 'l': RING_L}))
+        caps.append((out, 0, MESH_LOAD_C))
+        edge = MESH_CLOCK_EDGE
+        vsrc.append((vin, 0, 0.0, (0.0, VDD, float(td[k]), edge, edge, MESH_CLOCK_PERIOD / 2 - edge,
+                                   MESH_CLOCK_PERIOD)))
+    return {'nnodes': nm + 2 * loads, 'models': {'n': dict(BASE_NMOS), 'p': dict(BASE_PMOS)}, 'mos': mos,
+            'caps': caps, 'res': res, 'vsrc': vsrc, 'ic': {}, 'tstep': tstep, 'tstop': tstop}

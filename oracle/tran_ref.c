@@ -37,7 +37,9 @@
 
 int or_slot_rc(int slot, int *row_role, int *col_role);
 
-#define MAXBRK 64
+/* breakpoint table size (CKTsetBreak grows its table; one pending
+ * breakpoint per PULSE source plus the run's ends fits here) */
+#define MAXBRK 65536
 
 typedef struct {
     int p, n, br;
@@ -80,7 +82,7 @@ typedef struct {
     int *piv;
     ORCkt ck;
     double time, delta, saveDelta, finalTime, step, maxStep, delmin, minBreak;
-    double breaks[MAXBRK];
+    double *breaks;
     int nbreaks;
     /* output */
     int64_t npts, cap_pts;
@@ -100,6 +102,13 @@ typedef struct {
     /* optional: a drop-in BSIM4 load and linear solver behind the same
      * DCtran / NIiter (the engine's per-circuit cores, through function
      * pointers the caller supplies; the oracle links nothing of it) */
+    /* sparse mode (large circuits): the matrix lives only in Ax, addressed
+     * by a hash of (row, column); ground rows/columns go to a trash cell */
+    int sparse;
+    int64_t *hkey;
+    int *hval;
+    int64_t hsize;
+    double trash, miss;
     void *dl_ctx;
     int (*dl_run)(void *, const void *, const double *, double *const *, double *, double *, int *);
     void *smp_ctx;
@@ -116,6 +125,8 @@ typedef struct {
     double xfact;
     double gmin, reltol, abstol;
 } ORDropInScalars;
+
+static int build_csc(ORTran *T);
 
 /* equality.c:22-60 */
 static int almost_equal_ulps(double A, double B, int maxUlps)
@@ -234,7 +245,21 @@ static int vsrc_accept(ORTran *T, int mode)
     return 0;
 }
 
-#define AX(r, c) T->A[(size_t)(r) * (T->n + 1) + (c)]
+static double *ax_sparse(ORTran *T, int r, int c)
+{
+    int64_t key, h;
+    if (r == 0 || c == 0) return &T->trash;
+    key = (int64_t)r * (T->n + 1) + c;
+    h = (key * 0x9E3779B97F4A7C15ULL) >> 11;
+    for (h %= T->hsize;; h = (h + 1) % T->hsize) {
+        if (T->hkey[h] == key) return &T->Ax[T->hval[h]];
+        if (T->hkey[h] < 0) break;
+    }
+    T->miss = 0.0;   /* an element the pattern does not hold reads as 0 */
+    return &T->miss;
+}
+
+#define AX(r, c) (*(T->sparse ? ax_sparse(T, (r), (c)) : &T->A[(size_t)(r) * (T->n + 1) + (c)]))
 
 /* CKTload (cktload.c:31-180): device types in registry order BSIM4, CAP,
  * RES, VSRC (dev.c:142-206), instances newest first as the instance lists
@@ -244,7 +269,8 @@ static int ckt_load(ORTran *T)
     ORCkt *ck = &T->ck;
     const int mode = ck->mode, n = T->n;
     int k, s;
-    memset(T->A, 0, sizeof(double) * (size_t)(n + 1) * (n + 1));
+    if (T->sparse) memset(T->Ax, 0, sizeof(double) * (size_t)T->Ap[n]);
+    else memset(T->A, 0, sizeof(double) * (size_t)(n + 1) * (n + 1));
     memset(T->rhs, 0, sizeof(double) * (size_t)(n + 1));
     ck->rhsOld = T->rhsOld;
     if (T->dl_run) {
@@ -257,6 +283,7 @@ static int ckt_load(ORTran *T)
         sc.xfact = ck->deltaOld[1] != 0.0 ? ck->delta / ck->deltaOld[1] : 0.0;
         sc.gmin = ck->gmin; sc.reltol = ck->reltol; sc.abstol = ck->abstol;
         st[0] = ck->states[0]; st[1] = ck->states[1]; st[2] = ck->states[2];
+        if (T->sparse) return -1;   /* the drop-in adds at dense positions */
         if (T->dl_run(T->dl_ctx, &sc, T->rhsOld, st, T->A, T->rhs, &nc)) return -1;
         ck->noncon += nc;
     }
@@ -360,8 +387,9 @@ static int solve_klu(ORTran *T)
 {
     const int n = T->n;
     int c, p, full;
-    for (c = 0; c < n; c++)
-        for (p = T->Ap[c]; p < T->Ap[c + 1]; p++) T->Ax[p] = AX(T->Ai[p] + 1, c + 1);
+    if (!T->sparse)
+        for (c = 0; c < n; c++)
+            for (p = T->Ap[c]; p < T->Ap[c + 1]; p++) T->Ax[p] = AX(T->Ai[p] + 1, c + 1);
     full = (T->ck.mode & OR_MODEINITJCT) || ((T->ck.mode & OR_MODEINITTRAN) && T->iterno == 1);
     c = T->kg_factor(T->klu, T->Ax, full, 1e-3);
     if (c) return c;
@@ -380,8 +408,9 @@ static int solve(ORTran *T)
          * the diagonal itself (LoadGmin_CSC) */
         const int n = T->n;
         int c, p, rc;
-        for (c = 0; c < n; c++)
-            for (p = T->Ap[c]; p < T->Ap[c + 1]; p++) T->Ax[p] = AX(T->Ai[p] + 1, c + 1);
+        if (!T->sparse)
+            for (c = 0; c < n; c++)
+                for (p = T->Ap[c]; p < T->Ap[c + 1]; p++) T->Ax[p] = AX(T->Ai[p] + 1, c + 1);
         rc = T->smp_factor(T->smp_ctx, T->Ax, T->diagGmin);
         if (rc) return rc;
         if (T->smp_solve(T->smp_ctx, T->rhs)) return 1;
@@ -396,6 +425,7 @@ static int solve(ORTran *T)
             if (T->hasdiag[u]) AX(u, u) += T->diagGmin;
     }
     if (T->klu) return solve_klu(T);
+    if (T->sparse) return 1;   /* sparse mode solves with the reference KLU only */
     const int n = T->n, ld = n + 1;
     double *a = T->lu;
     int i, j, k;
@@ -976,6 +1006,7 @@ ORTran *or_tran_new(int nnodes, int nmos, int ncap, int nres, int nvs)
     T->noOpIter = 0; T->numGminSteps = 1; T->numSrcSteps = 1; T->dcTrcvMaxIter = 50;
     T->gminFactor = 10; T->gshunt = 0; T->srcFact = 1; T->diagGmin = 0;
     T->method = OR_TRAPEZOIDAL;
+    T->breaks = (double *)calloc(MAXBRK, sizeof(double));
     return T;
 }
 
@@ -1054,8 +1085,12 @@ int or_tran_setup(ORTran *T)
     T->stbuf = (double *)calloc((size_t)8 * (T->nstates + 1), sizeof(double));
     for (k = 0; k < 8; k++) T->states[k] = T->stbuf + (size_t)k * (T->nstates + 1);
     for (k = 0; k < 8; k++) T->ck.states[k] = T->states[k];
-    T->A = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
-    T->lu = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
+    /* large circuits keep the matrix sparse (KLU only); small ones dense */
+    T->sparse = T->n > 3000;
+    if (!T->sparse) {
+        T->A = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
+        T->lu = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
+    }
     T->piv = (int *)calloc((size_t)T->n + 1, sizeof(int));
     T->rhs = (double *)calloc((size_t)T->n + 1, sizeof(double));
     T->rhsOld = (double *)calloc((size_t)T->n + 1, sizeof(double));
@@ -1075,20 +1110,31 @@ int or_tran_setup(ORTran *T)
     T->ck.voltTol = T->vntol; T->ck.temp = T->temp;
     for (k = 0; k < T->nmos; k++)
         if (or_bsim4_inst_temp(T->mos[k], T->temp)) return -1;
-    return (T->A && T->lu && T->rhs && T->rhsOld && T->stbuf) ? 0 : -1;
+    if (T->sparse && build_csc(T)) return -1;
+    return ((T->sparse || (T->A && T->lu)) && T->rhs && T->rhsOld && T->stbuf) ? 0 : -1;
 }
 
 /* switch the linear solve to the reference's KLU in the library at path
  * (oracle/_ref/libklu_ref.so); call after or_tran_setup.  Returns 0. */
 /* the structural CSC pattern (0-based) of the circuit's matrix: every
  * element the devices' SMPmakeElt calls create */
+static int cmp_i64(const void *a, const void *b)
+{
+    const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
 static int build_csc(ORTran *T)
 {
-    int n = T->n, r, c, k, s, nz = 0;
-    char *S;
+    int n = T->n, k, s, c;
+    int64_t ne = 0, cap, i, nz;
+    int64_t *E;
     if (T->Ap) return 0;
-    S = (char *)calloc((size_t)(n + 1) * (n + 1), 1);
-#define MARK(a, b) do { if ((a) && (b)) S[(size_t)(a) * (n + 1) + (b)] = 1; } while (0)
+    cap = (int64_t)T->nmos * P__COUNT + 4 * (int64_t)(T->ncap + T->nres + T->nvs) + T->nnodes + 1;
+    E = (int64_t *)malloc(sizeof(int64_t) * (size_t)cap);
+    if (!E) return -1;
+    /* key = column-major position (c, r), both 1-based */
+#define MARK(a, b) do { if ((a) && (b)) E[ne++] = (int64_t)(b) * (n + 1) + (a); } while (0)
     for (k = 0; k < T->nmos; k++)
         for (s = 0; s < P__COUNT; s++) {
             int rr, cc;
@@ -1110,20 +1156,37 @@ static int build_csc(ORTran *T)
     for (k = 1; k <= T->nnodes; k++)
         if (T->ic_given[k]) MARK(k, k);
 #undef MARK
+    qsort(E, (size_t)ne, sizeof(int64_t), cmp_i64);
+    nz = 0;
+    for (i = 0; i < ne; i++)
+        if (i == 0 || E[i] != E[i - 1]) E[nz++] = E[i];
     T->Ap = (int *)calloc((size_t)n + 1, sizeof(int));
-    for (c = 1; c <= n; c++)
-        for (r = 1; r <= n; r++) nz += S[(size_t)r * (n + 1) + c];
     T->Ai = (int *)calloc((size_t)nz + 1, sizeof(int));
     T->Ax = (double *)calloc((size_t)nz + 1, sizeof(double));
     T->kb = (double *)calloc((size_t)n + 1, sizeof(double));
-    nz = 0;
-    for (c = 1; c <= n; c++) {
-        T->Ap[c - 1] = nz;
-        for (r = 1; r <= n; r++)
-            if (S[(size_t)r * (n + 1) + c]) T->Ai[nz++] = r - 1;
+    for (i = 0, c = 1; c <= n; c++) {
+        T->Ap[c - 1] = (int)i;
+        while (i < nz && E[i] / (n + 1) == c) {
+            T->Ai[i] = (int)(E[i] % (n + 1)) - 1;
+            i++;
+        }
     }
-    T->Ap[n] = nz;
-    free(S);
+    T->Ap[n] = (int)nz;
+    if (T->sparse) {
+        T->hsize = 2 * nz + 7;
+        T->hkey = (int64_t *)malloc(sizeof(int64_t) * (size_t)T->hsize);
+        T->hval = (int *)malloc(sizeof(int) * (size_t)T->hsize);
+        for (i = 0; i < T->hsize; i++) T->hkey[i] = -1;
+        for (c = 0; c < n; c++)
+            for (k = T->Ap[c]; k < T->Ap[c + 1]; k++) {
+                const int64_t key = (int64_t)(T->Ai[k] + 1) * (n + 1) + (c + 1);
+                int64_t h = ((key * 0x9E3779B97F4A7C15ULL) >> 11) % T->hsize;
+                while (T->hkey[h] >= 0) h = (h + 1) % T->hsize;
+                T->hkey[h] = key;
+                T->hval[h] = k;
+            }
+    }
+    free(E);
     return 0;
 }
 
@@ -1220,5 +1283,6 @@ void or_tran_free(ORTran *T)
     free(T->rhsOld); free(T->out_t); free(T->out_x);
     if (T->klu) T->kg_free(T->klu);
     free(T->Ap); free(T->Ai); free(T->Ax); free(T->kb);
+    free(T->hkey); free(T->hval); free(T->breaks);
     free(T);
 }

@@ -1,0 +1,113 @@
+"""One large circuit on the whole GPU: the power-grid mesh's transient
+(configuration 3's mesh: an RC mesh of 0.1 ohm segments and 10 fF per node,
+1 V pads every 64 nodes, CMOS inverter loads driven by phased clocks)
+through ngmi_large_*, against the oracle's DCtran restatement solving with
+the reference's KLU on the same circuit.
+
+Bar: the same accepted timepoints (1e-9 relative) and Newton iteration
+count; node voltages within 1e-6 of full scale (1 V) and within 1e-6 of
+their own value above 10 mV; source branch currents within 1e-6 of their
+full scale, floored at 1 uA (a source that never carries more is held to
+abstol = 1e-12 A, the current tolerance NIconvTest applies), and within
+1e-5 of their own value above 1e-2 of full scale.  The branch-current bar is
+looser because the two solvers pivot differently on a stiff mesh: the
+oracle run with the reference KLU and the same oracle run with a dense LU
+differ by up to 5e-7 relative in these currents on a 24 x 24 mesh, while
+their node voltages agree to 4e-14 V.  At the full 1024 x 1024 size, where
+the oracle would take hours, the final point must satisfy KCL: the circuit
+equations assembled at the accepted solution balance to within the Newton
+tolerance (scaled residual below 1e-4, the reltol-level change NIconvTest
+allows over the row's largest current term)."""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from engine_path import ngmi, workloads
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), '..', 'oracle'))
+import pyoracle as po  # noqa: E402
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+WAVE_TOL = 1e-6
+
+
+def _compare(tag, te, xe, ora, nn):
+    to, xo, so = ora
+    print('%s: engine %d pts, oracle %d pts (%d rejected, %d iters)'
+          % (tag, len(te), len(to), so['rejected'], so['iterations']), flush=True)
+    assert len(te) == len(to)
+    assert np.allclose(te, to, rtol=1e-9, atol=1e-21)
+    xo = xo[:, 1:]
+    scale = np.ones(xo.shape[1])
+    scale[nn:] = np.maximum(np.abs(xo[:, nn:]).max(axis=0), 1e-6)
+    d = np.abs(xe - xo)
+    sd = d / scale
+    big = np.abs(xo) >= 1e-2 * scale
+    rel = np.where(big, d / np.where(big, np.abs(xo), 1.0), 0.0)
+    err_n, rel_n = sd[:, :nn].max(), rel[:, :nn].max()
+    err_b = sd[:, nn:].max() if xo.shape[1] > nn else 0.0
+    rel_b = rel[:, nn:].max() if xo.shape[1] > nn else 0.0
+    print('%s: nodes: max |dv| %.3e V, max relative %.3e; branches: max scaled %.3e, max relative %.3e'
+          % (tag, err_n, rel_n, err_b, rel_b), flush=True)
+    assert err_n <= WAVE_TOL and rel_n <= WAVE_TOL
+    assert err_b <= WAVE_TOL and rel_b <= 10 * WAVE_TOL
+
+
+@pytest.mark.parametrize('side,pad_every,tstop', [(48, 16, 1e-9), (128, 64, 0.05e-9)])
+def test_mesh_against_oracle(side, pad_every, tstop):
+    d = workloads.power_grid(side=side, pad_every=pad_every, tstop=tstop)
+    loads = len(d['mos']) // 2
+    print('case: power-grid mesh %dx%d, %d inverter loads, .tran 10p %gn' % (side, side, loads, tstop * 1e9),
+          flush=True)
+    t0 = time.time()
+    lg = ngmi.Large(ngmi.Circuit(d), max_points=100000)
+    info = lg.info()
+    print('setup %.1fs: %s' % (time.time() - t0, info), flush=True)
+    rc = lg.tran(d['tstep'], tstop)
+    assert rc == 0, (rc, ngmi.last_error())
+    st = lg.stats()
+    te, xe = lg.waveform()
+    print('engine: %s, times %s' % (st, lg.times()), flush=True)
+    T = po.Transient(d)
+    assert T.use_reference_klu()
+    rc, to, xo = T.run(d['tstep'], tstop)
+    assert rc == 0, rc
+    _compare('mesh%d' % side, te, xe, (to, xo, T.stats()), d['nnodes'])
+    assert st['iterations'] == T.stats()['iterations']
+    r, per = lg.kcl()
+    print('KCL at the final point: scaled %.2e, max |F| nodes %.2e A, branches %.2e V' % (r, per[0], per[1]),
+          flush=True)
+    assert r < 1e-4
+
+
+def test_full_mesh_kcl():
+    """the full configuration-3 mesh: 1024 x 1024, 16384 inverter loads,
+    1.1M unknowns; the first 3 ps of its .tran 10p 10n (TMAX 10 ps; the
+    16384 clock phases put a PULSE corner every ~15 fs)"""
+    tstop = 3e-12
+    print('case: power-grid mesh 1024x1024, 16384 loads, first 3 ps of .tran 10p 10n, KCL at the end', flush=True)
+    t0 = time.time()
+    d = workloads.power_grid(side=1024, tstop=tstop)
+    probe = [1, 1 + 512 * 1024 + 512, d['nnodes']]
+    lg = ngmi.Large(ngmi.Circuit(d), probe=probe, max_points=10000)
+    print('setup %.1fs: %s' % (time.time() - t0, lg.info()), flush=True)
+    del d
+    t1 = time.time()
+    rc = lg.tran(10e-12, tstop, 10e-12)
+    assert rc == 0, (rc, ngmi.last_error())
+    st = lg.stats()
+    t, x = lg.waveform()
+    print('run %.1fs: %s, times %s' % (time.time() - t1, st, lg.times()), flush=True)
+    assert st['accepted'] >= 10 and abs(t[-1] - tstop) < 1e-18
+    # pads hold the mesh near 1 V; every recorded node within the rails
+    assert np.all(x[:, :2] > 0.9) and np.all(x[:, :2] <= 1.0 + 1e-6)
+    assert np.all(x[:, 2] > -0.1) and np.all(x[:, 2] < 1.1)
+    r, per = lg.kcl()
+    print('KCL at the final point: scaled %.2e, max |F| nodes %.2e A, branches %.2e V' % (r, per[0], per[1]),
+          flush=True)
+    assert r < 1e-4
