@@ -189,6 +189,7 @@ def _tran_protos():
     L.or_tran_csc.restype = ctypes.c_int64
     L.or_tran_mos_info.argtypes = [vp, i, ctypes.POINTER(i), vp]
     L.or_tran_use_dropin.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.or_set_sparse_above.argtypes = [i]
     L.or_tran_set_op_options.argtypes = [vp, i, i, i, d, d, i]
     ip = ctypes.POINTER(ctypes.c_int)
     L.or_tran_klu_order.argtypes = [vp, ip, ip, ip]
@@ -208,6 +209,12 @@ def comcof(method, order, delta, delta_old):
     if rc != 0:
         raise ValueError('order %d not available for method %d' % (order, method))
     return ag
+
+
+def set_sparse_above(n):
+    """circuits set up afterwards with more than n unknowns keep their matrix
+    sparse (solved by the reference KLU); returns the previous threshold"""
+    return _tran_protos().or_set_sparse_above(int(n))
 
 
 class Transient:

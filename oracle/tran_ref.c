@@ -128,6 +128,18 @@ typedef struct {
 
 static int build_csc(ORTran *T);
 
+/* circuits with more unknowns than this keep their matrix sparse */
+static int or_sparse_above = 3000;
+
+/* set the sparse-mode threshold for circuits set up afterwards; returns the
+ * previous value */
+int or_set_sparse_above(int n)
+{
+    const int old = or_sparse_above;
+    or_sparse_above = n;
+    return old;
+}
+
 /* equality.c:22-60 */
 static int almost_equal_ulps(double A, double B, int maxUlps)
 {
@@ -1086,7 +1098,7 @@ int or_tran_setup(ORTran *T)
     for (k = 0; k < 8; k++) T->states[k] = T->stbuf + (size_t)k * (T->nstates + 1);
     for (k = 0; k < 8; k++) T->ck.states[k] = T->states[k];
     /* large circuits keep the matrix sparse (KLU only); small ones dense */
-    T->sparse = T->n > 3000;
+    T->sparse = T->n > or_sparse_above;
     if (!T->sparse) {
         T->A = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));
         T->lu = (double *)calloc((size_t)(T->n + 1) * (T->n + 1), sizeof(double));

@@ -181,3 +181,28 @@ def test_operating_point_stepping_reaches_the_direct_solution(how):
     assert len(t) == len(t0) and np.allclose(t, t0, rtol=1e-9)
     assert np.abs(x[0] - x0[0]).max() < 1e-6 * 10
     assert np.abs(x - x0).max() < 1e-6
+
+
+def test_sparse_mode_matches_dense_mode():
+    """The oracle's sparse mode (the matrix only in CSC, for large meshes) adds
+    every stamp in the same order as its dense mode, so with the reference
+    KLU both give the same run bit for bit: the ring and the adder forced
+    into sparse mode against their dense runs."""
+    import pyoracle as po
+    from engine_path import workloads
+    for desc, tstop in ((workloads.ring_oscillator(stages=11, tstop=1e-9), 1e-9),
+                        (workloads.mc_adder(1), 1e-9)):
+        Td = po.Transient(desc)
+        if not Td.use_reference_klu():
+            pytest.skip('reference KLU not built')
+        old = po.set_sparse_above(0)
+        try:
+            Ts = po.Transient(desc)
+            assert Ts.use_reference_klu()
+        finally:
+            po.set_sparse_above(old)
+        rd, td, xd = Td.run(desc['tstep'], tstop)
+        rs, ts, xs = Ts.run(desc['tstep'], tstop)
+        assert rd == 0 and rs == 0
+        assert np.array_equal(td, ts) and np.array_equal(xd, xs)
+        assert Td.stats() == Ts.stats()
