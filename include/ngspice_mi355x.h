@@ -471,6 +471,16 @@ int ngmi_mlp_get_params(const ngmi_mlp *n, float *host, int grads);
 int ngmi_mlp_set_params(ngmi_mlp *n, const float *host);
 /* y[M] = net(X[M][16]), device pointers, asynchronous on stream */
 int ngmi_mlp_forward(ngmi_mlp *n, const float *X, float *y, int M, void *stream);
+/* ngmi_mlp_forward without allocating device memory, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The work buffers are a fixed per-device
+ * scratch of NGMI_MLP_NOALLOC_MAX_ROWS rows (17 MB, held by every GPU the
+ * library runs on until the process ends); larger M returns
+ * NGMI_E_PARMVAL.  Every call on one GPU shares that scratch, so calls that
+ * overlap on one GPU (other streams or threads) race.  Same results as
+ * ngmi_mlp_forward. */
+#define NGMI_MLP_NOALLOC_MAX_ROWS 16384
+int ngmi_mlp_forward_noalloc(const ngmi_mlp *n, const float *X, float *y, int M, void *stream);
 /* MSE gradients over M samples; lr > 0 also applies one Adam step; the loss
  * before the step goes to *loss_host when given (synchronises) */
 int ngmi_mlp_train_step(ngmi_mlp *n, const float *X, const float *target, int M, float lr,

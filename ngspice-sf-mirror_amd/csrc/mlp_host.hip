@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 #include "ngmi_internal.h"
@@ -140,6 +141,73 @@ int forward(ngmi_mlp *n, const float *X, int M, int Mpad, hipStream_t s)
 
 }  // namespace
 
+namespace {
+/* inference over at most maxRows rows (the first *dcount when dcount is
+ * set) with caller-provided work buffers: xb (Mpad x 32 BF16 input rows),
+ * ia and ib (Mpad x 256 BF16 activations, ping-pong) */
+int forward_infer(const ngmi_mlp *n, const float *X, float *y, int maxRows, int Mpad, unsigned short *xb,
+                  unsigned short *ia, unsigned short *ib, const int *dcount, hipStream_t s)
+{
+    int rc;
+    hipLaunchKernelGGL(k_pack_input, dim3(std::min(blocks((int64_t)Mpad * IN_PAD), 2048)), dim3(256), 0, s,
+                       X, xb, maxRows, Mpad, dcount);
+    NGMI_HIP(hipGetLastError());
+    const unsigned short *wb = n->wb;
+    if ((rc = launch_gemm_bf16(xb, IN_PAD, 1, wb + OFF_W1, IN_PAD, 1, n->w + OFF_B1, nullptr,
+                               nullptr, ia, H, Mpad, H, IN_PAD, 2, s, dcount)))
+        return rc;
+    if ((rc = launch_gemm_bf16(ia, H, 1, wb + OFF_W2, H, 1, n->w + OFF_B2, nullptr, nullptr, ib,
+                               H, Mpad, H, H, 2, s, dcount)))
+        return rc;
+    if ((rc = launch_gemm_bf16(ib, H, 1, wb + OFF_W3, H, 1, n->w + OFF_B3, nullptr, nullptr, ia,
+                               H, Mpad, H, H, 2, s, dcount)))
+        return rc;
+    hipLaunchKernelGGL(k_head, dim3(std::min(blocks(maxRows), 1024)), dim3(256), 0, s, ia, wb + OFF_W4,
+                       n->w + OFF_B4, nullptr, y, nullptr, maxRows, H, dcount);
+    NGMI_HIP(hipGetLastError());
+    return NGMI_OK;
+}
+
+/* Work memory of ngmi_mlp_forward_noalloc: NGMI_MLP_NOALLOC_MAX_ROWS rows of
+ * packed BF16 input (32 wide) and two BF16 activation buffers (256 wide),
+ * 17 MB, in device memory of every GPU the library runs on until the
+ * process ends.  Host code addresses it only through the pointer
+ * hipGetSymbolAddress returns for the current device. */
+constexpr int64_t NOALLOC_XB = (int64_t)NGMI_MLP_NOALLOC_MAX_ROWS * IN_PAD;
+constexpr int64_t NOALLOC_ACT = (int64_t)NGMI_MLP_NOALLOC_MAX_ROWS * H;
+}  // namespace
+
+static __device__ __align__(256) unsigned short scratch[NOALLOC_XB + 2 * NOALLOC_ACT];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned short *scratch_ptr[MAX_DEVICES] = {};
+
+/* the device address of this device's scratch, looked up once per device */
+int scratch_for_current_device(unsigned short **out)
+{
+    int dev = 0;
+    NGMI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error("mlp_forward_noalloc: device %d beyond the %d the library tracks", dev, MAX_DEVICES);
+        return NGMI_E_DEVICE;
+    }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("mlp_forward_noalloc: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+            return NGMI_E_DEVICE;
+        }
+        scratch_ptr[dev] = (unsigned short *)p;
+    }
+    *out = scratch_ptr[dev];
+    return NGMI_OK;
+}
+}  // namespace
+
 namespace ngmi {
 /* forward over the first *dcount of at most maxRows rows (device-side count) */
 int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const int *dcount,
@@ -148,23 +216,7 @@ int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const i
     const int Mpad = (maxRows + 31) / 32 * 32;
     int rc = n->ensure_infer(Mpad);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pack_input, dim3(std::min(blocks((int64_t)Mpad * IN_PAD), 2048)), dim3(256), 0, s,
-                       X, n->ixb, maxRows, Mpad, dcount);
-    NGMI_HIP(hipGetLastError());
-    const unsigned short *wb = n->wb;
-    if ((rc = launch_gemm_bf16(n->ixb, IN_PAD, 1, wb + OFF_W1, IN_PAD, 1, n->w + OFF_B1, nullptr,
-                               nullptr, n->ia, H, Mpad, H, IN_PAD, 2, s, dcount)))
-        return rc;
-    if ((rc = launch_gemm_bf16(n->ia, H, 1, wb + OFF_W2, H, 1, n->w + OFF_B2, nullptr, nullptr, n->ib,
-                               H, Mpad, H, H, 2, s, dcount)))
-        return rc;
-    if ((rc = launch_gemm_bf16(n->ib, H, 1, wb + OFF_W3, H, 1, n->w + OFF_B3, nullptr, nullptr, n->ia,
-                               H, Mpad, H, H, 2, s, dcount)))
-        return rc;
-    hipLaunchKernelGGL(k_head, dim3(std::min(blocks(maxRows), 1024)), dim3(256), 0, s, n->ia, wb + OFF_W4,
-                       n->w + OFF_B4, nullptr, y, nullptr, maxRows, H, dcount);
-    NGMI_HIP(hipGetLastError());
-    return NGMI_OK;
+    return forward_infer(n, X, y, maxRows, Mpad, n->ixb, n->ia, n->ib, dcount, s);
 }
 }  // namespace ngmi
 
@@ -244,6 +296,27 @@ int ngmi_mlp_forward(ngmi_mlp *n, const float *X, float *y, int M, void *stream)
                        n->w + OFF_B4, nullptr, y, nullptr, M, H, nullptr);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
+}
+
+/* ngmi_mlp_forward without allocating: the work buffers are the library's
+ * fixed per-device scratch, so calls that overlap on one GPU race */
+int ngmi_mlp_forward_noalloc(const ngmi_mlp *n, const float *X, float *y, int M, void *stream)
+{
+    if (!n || M < 0 || (M > 0 && (!X || !y))) {
+        set_error("mlp_forward_noalloc: bad arguments");
+        return NGMI_E_PANIC;
+    }
+    if (M > NGMI_MLP_NOALLOC_MAX_ROWS) {
+        set_error("mlp_forward_noalloc: %d rows exceed the scratch maximum of %d", M, NGMI_MLP_NOALLOC_MAX_ROWS);
+        return NGMI_E_PARMVAL;
+    }
+    if (M == 0) return NGMI_OK;
+    unsigned short *base = nullptr;
+    int rc = scratch_for_current_device(&base);
+    if (rc) return rc;
+    const int Mpad = (M + 31) / 32 * 32;
+    return forward_infer(n, X, y, M, Mpad, base, base + NOALLOC_XB, base + NOALLOC_XB + NOALLOC_ACT, nullptr,
+                         (hipStream_t)stream);
 }
 
 /* gradients of the mean squared error over M samples into the gradient
