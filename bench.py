@@ -240,7 +240,9 @@ def bench_mc(args):
     for _ in range(args.warmup):
         rc, passes = b.tran(workloads.ADDER_TSTEP, tstop, stream=stream)
         log('warmup rc %d passes %d' % (rc, passes))
-    b.set_profile(True)
+    # the timed runs replay captured graphs without per-launch events; the
+    # kernels' own durations come from a separate profiled run below
+    b.set_profile(False)
     times = []
     for _ in range(args.steps):
         torch.cuda.synchronize()
@@ -256,42 +258,37 @@ def bench_mc(args):
     acc = sum(s['accepted'] for s in st)
     rej = sum(s['rejected'] for s in st)
     its = sum(s['iterations'] for s in st)
-    mos_ms, step_ms, evals, npasses = b.kernel_times()
+    _, _, evals, npasses = b.kernel_times()
     # algorithmic bytes of one BSIM4 evaluation in the transient kernel: the
     # per-instance table columns, 4 node voltages, state reads (4 terminal
     # voltages, 3 charges and 3 charge currents of the previous level), 21
     # state writes, 38 cached values read and written, 16 + 4 outputs, the
     # convergence vote and the model index
     b_eval = 8 * (icols + 4 + 4 + 6 + 21 + 38 + 38 + 20) + 4 + 4
-    achieved = evals * b_eval / (mos_ms * 1e-3) / 1e9 if mos_ms > 0 else None
     mb = MEASURED_HBM_BYTES_PER_EVAL.get(('k_tran_mos', args.mos_waves or 2))
-    # with two streams every pass launches the BSIM4 kernel once per half
-    mos_launches = passes * (args.streams or 2)
-    traffic = mb * evals / mos_launches if (mb and mos_launches) else None
-    fp64_tflops = (MEASURED_FLOP_PER_EVAL['k_tran_mos'] * evals / (mos_ms * 1e-3) / 1e12
-                   if mos_ms > 0 else None)
-    log('passes %d, accepted %d, rejected %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
-        'step %.1f ms, wall %.1f ms' % (passes, acc, rej, its, failed, mos_ms, step_ms,
-                                        wall * 1e3))
+    log('passes %d, accepted %d, rejected %d, iterations %d, failed %d; wall %.1f ms (%d stream(s), graphs)'
+        % (passes, acc, rej, its, failed, wall * 1e3, args.streams or 2))
     single = None
-    if args.streams is None and not args.no_single_stream:
-        # the default overlaps two halves of the batch on two streams, so each
-        # kernel's launch duration includes time shared with the other
-        # half's kernel; one more timed run on one stream gives the BSIM4
-        # kernel's own roofline
+    if not args.no_single_stream:
+        # one more run, on one stream and with per-launch events, gives the
+        # BSIM4 kernel's own duration: on two streams a half's launch shares
+        # the chip with the other half's step kernel
+        keep = ngmi.get_tune('streams')
         ngmi.tune('streams', 1)
+        b.set_profile(True)
         torch.cuda.synchronize()
         w0 = time.perf_counter()
         rc1, passes1 = b.tran(workloads.ADDER_TSTEP, tstop, stream=stream)
         torch.cuda.synchronize()
         wall1 = time.perf_counter() - w0
-        ngmi.tune('streams', 2)
+        b.set_profile(False)
+        ngmi.tune('streams', keep)
         if rc1 != 0:
             raise RuntimeError('single-stream transient failed: %d %s' % (rc1, ngmi.last_error()))
         acc1 = sum(b.stats(c)['accepted'] for c in range(C))
         mos1, step1, evals1, _ = b.kernel_times()
         ach1 = evals1 * b_eval / (mos1 * 1e-3) / 1e9 if mos1 > 0 else None
-        single = {'value': acc1 / wall1, 'ms': wall1 * 1e3, 'bsim4_kernel_ms_total': mos1,
+        single = {'value': acc1 / wall1, 'ms': wall1 * 1e3, 'profiled': True, 'bsim4_kernel_ms_total': mos1,
                   'step_kernel_ms_total': step1,
                   'roofline': {'bound': 'hbm', 'achieved': ach1, 'peak': HBM_PEAK_GBS,
                                'unit': 'GB/s', 'frac': ach1 / HBM_PEAK_GBS if ach1 else None,
@@ -301,27 +298,21 @@ def bench_mc(args):
                                'fp64_tflops': (MEASURED_FLOP_PER_EVAL['k_tran_mos'] * evals1 /
                                                (mos1 * 1e-3) / 1e12) if mos1 > 0 else None,
                                'fp64_peak_tflops': FP64_PEAK_TFLOPS}}
-        log('one stream: %.0f tp*c/s, wall %.1f ms, bsim4 %.1f ms (%.0f GB/s), step %.1f ms'
+        log('one stream, profiled: %.0f tp*c/s, wall %.1f ms, bsim4 %.1f ms (%.0f GB/s), step %.1f ms'
             % (single['value'], wall1 * 1e3, mos1, ach1 or 0, step1))
     learned = None
     if not args.no_guess:
-        learned = learned_guess_run(args, b, probe, stream, acc, its)
-    overlapped = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                  'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': traffic,
-                  'kernel': 'k_tran_mos', 'kernel_ms_total': mos_ms,
-                  'launches': mos_launches, 'bytes_per_eval': b_eval, 'evals': evals,
-                  'fp64_tflops': fp64_tflops, 'fp64_peak_tflops': FP64_PEAK_TFLOPS}
-    # The headline roofline is priced on launches that do not share the chip:
-    # with two streams a half's BSIM4 launch runs beside the other half's step
-    # kernel, so its duration overstates the kernel's own time (the summed
-    # durations exceed the wall time).  The one-stream run's launches run alone.
+        learned = learned_guess_run(args, b, probe, stream, acc, its, acc / wall)
+    # The headline roofline is priced on launches that do not share the chip
+    # (the one-stream profiled run), so the kernel's summed durations stay
+    # below that run's wall time.
     if single is not None:
         roof = dict(single['roofline'])
-        roof['timed_on'] = 'the one-stream run (no overlap; kernel time per .tran %.0f ms <= its wall %.0f ms)' \
+        roof['timed_on'] = 'the one-stream profiled run (kernel time per .tran %.0f ms <= its wall %.0f ms)' \
             % (single['bsim4_kernel_ms_total'], single['ms'])
     else:
-        roof = dict(overlapped)
-        roof['timed_on'] = 'this run (%d stream(s))' % (args.streams or 2)
+        roof = {'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': None,
+                'traffic': None, 'kernel': 'k_tran_mos', 'timed_on': 'not measured (--no-single-stream)'}
     return {
         'metric': 'MC transient timepoints x circuits/s at reltol 1e-3 (plain vs learned NR guess)',
         'value': acc / wall,
@@ -333,10 +324,10 @@ def bench_mc(args):
                    'circuits': C, 'unknowns': info['unknowns'],
                    'lu_nonzeros': info['lu_nonzeros']},
         'roofline': roof,
-        'extra': {'accepted': acc, 'two_stream_roofline': overlapped, 'rejected': rej, 'newton_iterations': its,
+        'extra': {'accepted': acc, 'rejected': rej, 'newton_iterations': its,
+                  'max_circuit_iterations': max(s['iterations'] for s in st),
                   'iterations_per_point': its / max(acc, 1), 'passes': passes,
-                  'failed_circuits': failed, 'step_kernel_ms_total': step_ms,
-                  'bsim4_kernel_ms_total': mos_ms, 'streams': args.streams or 2,
+                  'failed_circuits': failed, 'streams': args.streams or 2, 'graphs': True,
                   'one_stream': single, 'learned_guess': learned},
     }
 
@@ -614,7 +605,7 @@ def cpu_baseline_mesh(side):
                       '%dx%d mesh matrix; klu_factor took %.1f s' % (side, side, tf)}
 
 
-def learned_guess_run(args, b, probe, stream, plain_acc, plain_its):
+def learned_guess_run(args, b, probe, stream, plain_acc, plain_its, plain_value):
     """Train the guess network on engine waveforms of held-out seeds, then
     run the same batch with the learned guess; returns its figures and the
     waveform error of circuit 0 (plain and learned) against the oracle."""
@@ -675,20 +666,43 @@ def learned_guess_run(args, b, probe, stream, plain_acc, plain_its):
     gs = [b.guess_stats(c) for c in range(C)]
     acc = sum(s['accepted'] for s in st)
     its = sum(s['iterations'] for s in st)
-    mos_ms, step_ms, evals, _ = b.kernel_times()
+    rows = b.guess_rows()
     out = {'value': acc / wall, 'unit': 'timepoints x circuits/s', 'ms': wall * 1e3,
+           'vs_plain': (acc / wall) / plain_value,
            'accepted': acc, 'newton_iterations': its, 'iterations_per_point': its / max(acc, 1),
            'plain_iterations_per_point': plain_its / max(plain_acc, 1),
            'steps_from_guess': sum(x['guessed'] for x in gs),
            'fallbacks_to_predictor': sum(x['fallbacks'] for x in gs),
-           'guess_kernels_ms_total': gs[0]['guess_ms'], 'bsim4_kernel_ms_total': mos_ms,
-           'step_kernel_ms_total': step_ms, 'failed_circuits': sum(1 for s in st if s['status']),
+           'steps_paused': sum(x['paused'] for x in gs),
+           'network_rows': rows, 'network_rows_per_pass': rows / max(passes, 1), 'passes': passes,
+           'rejected': sum(s['rejected'] for s in st),
+           'max_circuit_iterations': max(s['iterations'] for s in st),
+           'slowest_circuit': int(np.argmax([s['iterations'] for s in st])),
+           'failed_circuits': sum(1 for s in st if s['status']),
            'training': {'seeds': '%d..%d' % (seeds[0], seeds[-1]), 'rows': int(len(F)),
                         'steps': args.train_steps, 'target_mean_square': base_ms,
                         'final_loss': loss, 'seconds': t2 - t0}}
-    log('learned: %.0f tp*c/s, %.3f it/pt (plain %.3f), %d fallbacks, guess kernels %.1f ms'
-        % (out['value'], out['iterations_per_point'], out['plain_iterations_per_point'],
-           out['fallbacks_to_predictor'], out['guess_kernels_ms_total']))
+    if not args.no_single_stream:
+        # the guess kernels' own durations: the same learned run once more,
+        # one stream, with per-launch events
+        keep = ngmi.get_tune('streams')
+        ngmi.tune('streams', 1)
+        b.set_profile(True)
+        rc2, passes2 = b.tran(workloads.ADDER_TSTEP, args.tstop, stream=stream)
+        torch.cuda.synchronize()
+        b.set_profile(False)
+        ngmi.tune('streams', keep)
+        if rc2 != 0:
+            raise RuntimeError('profiled learned run failed: %s' % ngmi.last_error())
+        mos_ms, step_ms, _, _ = b.kernel_times()
+        g_ms = b.guess_stats(0)['guess_ms']
+        out.update({'profiled': {'guess_kernels_ms_total': g_ms, 'guess_us_per_pass': g_ms * 1e3 / max(passes2, 1),
+                                 'bsim4_kernel_ms_total': mos_ms, 'step_kernel_ms_total': step_ms,
+                                 'passes': passes2}})
+    log('learned: %.0f tp*c/s (%.3f x plain), %.3f it/pt (plain %.3f), %d fallbacks, %.0f network rows/pass%s'
+        % (out['value'], out['vs_plain'], out['iterations_per_point'], out['plain_iterations_per_point'],
+           out['fallbacks_to_predictor'], out['network_rows_per_pass'],
+           (', guess kernels %.1f us/pass' % out['profiled']['guess_us_per_pass']) if 'profiled' in out else ''))
     if ora:
         out['waveform_error_vs_oracle'] = {'plain': plain_err,
                                            'learned': waveform_error(b, probe, ora)}

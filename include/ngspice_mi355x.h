@@ -64,7 +64,11 @@ const char *ngmi_last_error(void);
  * elimination tree (fewer factor and solve levels, possibly more fill);
  * it applies to batches built afterwards;
  * "graph" (0 or 1, default 1): ngmi_batch_tran replays chunks of 32 Newton
- * passes as one captured HIP graph instead of launching each kernel.
+ * passes as one captured HIP graph instead of launching each kernel;
+ * "guess_fused" (0 or 1, default 1): the learned guess and ngmi_mlp_forward
+ * run the network in one fused kernel (weights in registers, activations in
+ * LDS) instead of the layer-by-layer kernels; both give the same values bit
+ * for bit.
  * Process-wide; defaults are the measured best on MI355X. */
 int ngmi_tune(const char *knob, int value);
 /* current value of a knob, or minus NGMI_E_BADPARM for an unknown name */
@@ -373,12 +377,19 @@ int ngmi_batch_kernel_times(const ngmi_batch *b, double *mos_ms, double *step_ms
  * criteria are unchanged; a run that fails redoes the step from the
  * predictor.  NULL detaches.  The network must outlive the batch's runs. */
 int ngmi_batch_set_guess(ngmi_batch *b, ngmi_mlp *net);
+/* network rows (nodes given a learned starting value) of the last
+ * ngmi_batch_tran, summed over its passes; 0 without a network */
+int ngmi_batch_guess_rows(const ngmi_batch *b, int64_t *rows);
 /* the guess features' neighbour table: [nnodes+1][8] node numbers, -1 = none */
 int ngmi_batch_neighbours(const ngmi_batch *b, int *out);
-/* per circuit: steps started from the guess, fallbacks to the predictor;
- * guess kernels' total milliseconds of the last profiled run */
+/* per circuit: steps started from the guess, fallbacks to the predictor
+ * (a Newton run from the guess that failed), steps started from the
+ * predictor while the circuit's guess was paused (after three guessed
+ * steps in a row that needed more than two iterations, for 32 steps, then
+ * twice as long each time); guess kernels' total milliseconds of the last
+ * profiled run */
 int ngmi_batch_guess_stats(const ngmi_batch *b, int c, int64_t *guessed, int64_t *fallbacks,
-                           double *guess_ms);
+                           int64_t *paused, double *guess_ms);
 /* Live data stream, the batched counterpart of SendData / OUTpData
  * (sharedspice.c:2280, outitf.c:633): while ngmi_batch_tran runs, after
  * every 32 Newton passes and once at its end, cb receives each watched
@@ -492,11 +503,14 @@ int ngmi_mlp_train_step(ngmi_mlp *n, const float *X, const float *target, int M,
  * t[4i..4i+3] and values v[4i..4i+3] of a node (newest first), the step
  * h[i], neighbour values nbv[8i..8i+7] present where bit k of nbmask[i] is
  * set.  F[16i..16i+15] receives the row and act[i] the node's activity;
- * rows too quiet for the network get zeros and act[i] <= 0.  For training
+ * rows the network cannot matter for get zeros and act[i] <= 0: activity
+ * below 10 uV, or 4 x activity (the largest correction the network may
+ * make) within the Newton tolerance reltol |v0| + vntol.  For training
  * pipelines that build rows on the host and check them against the
  * engine's (no counterpart in the reference).  Asynchronous on stream. */
 int ngmi_guess_features(int M, const double *t, const double *h, const double *v, const double *nbv,
-                        const unsigned *nbmask, float *F, double *act, void *stream);
+                        const unsigned *nbmask, double reltol, double vntol, float *F, double *act,
+                        void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device memory helpers for callers without a HIP binding             */

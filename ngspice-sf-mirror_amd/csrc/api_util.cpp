@@ -10,7 +10,7 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 1, 1};
+static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 1, 1, 1};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
 
@@ -39,6 +39,7 @@ static int knob_index(const char *knob)
     if (!std::strcmp(knob, "specialise")) return TUNE_SPECIALISE;
     if (!std::strcmp(knob, "order_slack")) return TUNE_ORDER_SLACK;
     if (!std::strcmp(knob, "graph")) return TUNE_GRAPH;
+    if (!std::strcmp(knob, "guess_fused")) return TUNE_GUESS_FUSED;
     return -1;
 }
 
@@ -62,12 +63,14 @@ int ngmi_tune(const char *knob, int value)
     else if (!std::strcmp(knob, "specialise")) k = TUNE_SPECIALISE;
     else if (!std::strcmp(knob, "order_slack")) k = TUNE_ORDER_SLACK;
     else if (!std::strcmp(knob, "graph")) k = TUNE_GRAPH;
+    else if (!std::strcmp(knob, "guess_fused")) k = TUNE_GUESS_FUSED;
     if (k < 0) {
         set_error("unknown tuning knob '%s'", knob);
         return NGMI_E_BADPARM;
     }
-    const int vmin = (k == TUNE_SPECIALISE || k == TUNE_ORDER_SLACK || k == TUNE_GRAPH) ? 0 : 1;
-    const int vmax = (k == TUNE_STREAMS) ? 2 : (k == TUNE_SPECIALISE || k == TUNE_GRAPH) ? 1 : 4;
+    const bool flag = k == TUNE_SPECIALISE || k == TUNE_GRAPH || k == TUNE_GUESS_FUSED;
+    const int vmin = (flag || k == TUNE_ORDER_SLACK) ? 0 : 1;
+    const int vmax = (k == TUNE_STREAMS) ? 2 : flag ? 1 : 4;
     if (value < vmin || value > vmax) {
         set_error("knob '%s' takes %d..%d", knob, vmin, vmax);
         return NGMI_E_PARMVAL;

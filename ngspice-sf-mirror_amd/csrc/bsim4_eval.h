@@ -42,7 +42,12 @@ namespace ngmi {
 enum : int {
     M_TRAN = 0x1, M_AC = 0x2, M_DCOP = 0x10, M_TRANOP = 0x20, M_DCTRANCURVE = 0x40,
     M_INITFLOAT = 0x100, M_INITJCT = 0x200, M_INITFIX = 0x400, M_INITSMSIG = 0x800,
-    M_INITTRAN = 0x1000, M_INITPRED = 0x2000, M_UIC = 0x10000
+    M_INITTRAN = 0x1000, M_INITPRED = 0x2000, M_UIC = 0x10000,
+    /* engine-internal, no reference counterpart: the first Newton iteration
+     * of a step that starts from the learned guess.  The terminal voltages
+     * come from the iterate as in INITFLOAT, and state0 first takes state1's
+     * voltages as in INITPRED, so limiting is against the last accepted point */
+    M_GUESSED = 0x40000000
 };
 
 struct B4Ckt {
@@ -345,29 +350,34 @@ NGMI_HD int b4_load(const P &prm, S &st, C &ca, const B4Ckt &ck, const double *v
         const double s0vds = st.s(0, B4X_vds), s0vgs = st.s(0, B4X_vgs), s0vbs = st.s(0, B4X_vbs);
         const double s0vbd = st.s(0, B4X_vbd);
         double delvbd, delvgd, delvds, delvgs, delvbs, vgdo;
-        if (mode & M_INITPRED) {
-            const double xf = ck.xfact;
-            /* b4ld.c:322-372: state0 takes state1, voltages extrapolate */
-            double s1vds = st.s(1, B4X_vds), s1vgs = st.s(1, B4X_vgs), s1vbs = st.s(1, B4X_vbs);
-            double s1vdbs = st.s(1, B4X_vdbs);
+        const int from1 = mode & (M_INITPRED | M_GUESSED);
+        if (from1) {
+            /* b4ld.c:322-372: state0 takes state1, and in INITPRED the
+             * voltages extrapolate */
+            const double s1vds = st.s(1, B4X_vds), s1vgs = st.s(1, B4X_vgs), s1vbs = st.s(1, B4X_vbs);
+            const double s1vdbs = st.s(1, B4X_vdbs);
             st.set(0, B4X_vds, s1vds);
-            vds = (1.0 + xf) * s1vds - (xf * st.s(2, B4X_vds));
             st.set(0, B4X_vgs, s1vgs);
-            vgs = (1.0 + xf) * s1vgs - (xf * st.s(2, B4X_vgs));
             st.set(0, B4X_vbs, s1vbs);
-            vbs = (1.0 + xf) * s1vbs - (xf * st.s(2, B4X_vbs));
             st.set(0, B4X_vbd, s1vbs - s1vds);
             st.set(0, B4X_vdbs, s1vdbs);
             st.set(0, B4X_vdbd, s1vdbs - s1vds);
-        } else {
+            if (mode & M_INITPRED) {
+                const double xf = ck.xfact;
+                vds = (1.0 + xf) * s1vds - (xf * st.s(2, B4X_vds));
+                vgs = (1.0 + xf) * s1vgs - (xf * st.s(2, B4X_vgs));
+                vbs = (1.0 + xf) * s1vbs - (xf * st.s(2, B4X_vbs));
+            }
+        }
+        if (!(mode & M_INITPRED)) {
             vds = type * (vnode[0] - vnode[2]);
             vgs = type * (vnode[1] - vnode[2]);
             vbs = type * (vnode[3] - vnode[2]);
         }
-        const double r0vds = (mode & M_INITPRED) ? st.s(0, B4X_vds) : s0vds;
-        const double r0vgs = (mode & M_INITPRED) ? st.s(0, B4X_vgs) : s0vgs;
-        const double r0vbs = (mode & M_INITPRED) ? st.s(0, B4X_vbs) : s0vbs;
-        const double r0vbd = (mode & M_INITPRED) ? st.s(0, B4X_vbd) : s0vbd;
+        const double r0vds = from1 ? st.s(0, B4X_vds) : s0vds;
+        const double r0vgs = from1 ? st.s(0, B4X_vgs) : s0vgs;
+        const double r0vbs = from1 ? st.s(0, B4X_vbs) : s0vbs;
+        const double r0vbd = from1 ? st.s(0, B4X_vbd) : s0vbd;
         vgdo = r0vgs - r0vds;
         vbd = vbs - vds;
         vgd = vgs - vds;

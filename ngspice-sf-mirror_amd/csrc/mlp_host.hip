@@ -29,19 +29,19 @@ __global__ void k_adam(float *w, unsigned short *wb, const float *g, float *m, f
 __global__ void k_head(const unsigned short *H, const unsigned short *w, const float *b,
                        const float *target, float *y, float *dy, int M, int K, const int *Mdev);
 __global__ void k_head_back(const unsigned short *w, const float *dy, float *dH, int M, int K);
+int launch_guess_net(const unsigned short *wb, const float *wf, const float *X, int *count, int maxRows,
+                     const int64_t *dest, const double2 *base, double *x, float *yout,
+                     unsigned long long *rows_total, hipStream_t s);
 }  // namespace ngmi
 
 using namespace ngmi;
 
 namespace {
-constexpr int IN_PAD = 32;
+constexpr int IN_PAD = MLP_IN_PAD;
 constexpr int H = MLP_HID;
-/* parameter blocks, in this order in one FP32 array */
-constexpr int64_t SZ_W1 = (int64_t)H * IN_PAD, SZ_B1 = H, SZ_W2 = (int64_t)H * H, SZ_B2 = H,
-                  SZ_W3 = (int64_t)H * H, SZ_B3 = H, SZ_W4 = H, SZ_B4 = 1;
-constexpr int64_t OFF_W1 = 0, OFF_B1 = OFF_W1 + SZ_W1, OFF_W2 = OFF_B1 + SZ_B1,
-                  OFF_B2 = OFF_W2 + SZ_W2, OFF_W3 = OFF_B2 + SZ_B2, OFF_B3 = OFF_W3 + SZ_W3,
-                  OFF_W4 = OFF_B3 + SZ_B3, OFF_B4 = OFF_W4 + SZ_W4, NPARAM = OFF_B4 + SZ_B4;
+constexpr int64_t OFF_W1 = MLP_OFF_W1, OFF_B1 = MLP_OFF_B1, OFF_W2 = MLP_OFF_W2, OFF_B2 = MLP_OFF_B2,
+                  OFF_W3 = MLP_OFF_W3, OFF_B3 = MLP_OFF_B3, OFF_W4 = MLP_OFF_W4, OFF_B4 = MLP_OFF_B4,
+                  NPARAM = MLP_NPARAM;
 }  // namespace
 
 struct ngmi_mlp {
@@ -112,10 +112,7 @@ __global__ void k_pack_input(const float *__restrict__ X, unsigned short *__rest
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(i / IN_PAD), c = (int)(i % IN_PAD);
-        float v = (r < M && c < MLP_IN) ? X[(int64_t)r * MLP_IN + c] : 0.0f;
-        unsigned int u = __float_as_uint(v);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        xb[i] = (unsigned short)(u >> 16);
+        xb[i] = f2bf((r < M && c < MLP_IN) ? X[(int64_t)r * MLP_IN + c] : 0.0f);
     }
 }
 
@@ -218,6 +215,14 @@ int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const i
     if (rc) return rc;
     return forward_infer(n, X, y, maxRows, Mpad, n->ixb, n->ia, n->ib, dcount, s);
 }
+
+/* the fused inference of the transient's guess pass: rows from count[0]
+ * (cleared, with the finish counter count[1], by the kernel) */
+int mlp_guess_rows(const ngmi_mlp *n, const float *X, int *count, int maxRows, const int64_t *dest,
+                   const double2 *base, double *x, float *yout, unsigned long long *rows_total, hipStream_t s)
+{
+    return launch_guess_net(n->wb, n->w, X, count, maxRows, dest, base, x, yout, rows_total, s);
+}
 }  // namespace ngmi
 
 extern "C" {
@@ -288,6 +293,8 @@ int ngmi_mlp_forward(ngmi_mlp *n, const float *X, float *y, int M, void *stream)
     if (!n || M < 0) return NGMI_E_PANIC;
     if (M == 0) return NGMI_OK;
     const hipStream_t s = (hipStream_t)stream;
+    if (tune_get(TUNE_GUESS_FUSED))
+        return launch_guess_net(n->wb, n->w, X, nullptr, M, nullptr, nullptr, nullptr, y, nullptr, s);
     const int Mpad = (M + 31) / 32 * 32;
     int rc = n->ensure(Mpad);
     if (rc) return rc;

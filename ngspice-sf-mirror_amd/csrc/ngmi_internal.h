@@ -23,6 +23,7 @@ enum {
     TUNE_SPECIALISE = 3,
     TUNE_ORDER_SLACK = 4,
     TUNE_GRAPH = 5,
+    TUNE_GUESS_FUSED = 6,
     TUNE__COUNT
 };
 int tune_get(int knob);

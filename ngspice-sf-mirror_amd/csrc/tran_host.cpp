@@ -37,12 +37,12 @@ int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, int c_begin, int c_en
                      int n_step_blk, size_t lds_bytes, hipStream_t s, hipEvent_t *ev);
 int set_tran_lds_limit(size_t bytes);
 int launch_zero_int(int *p, hipStream_t s);
-int launch_guess_features(const TranDev &D, float *F, int *rowmap, double2 *base, int *count,
-                          hipStream_t s);
-int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double2 *base,
-                       const int *count, int maxrows, hipStream_t s);
+int launch_guess_apply(const TranDev &D, const float *y, const int64_t *dest, const double2 *base,
+                       const int *count, int maxrows, unsigned long long *rows_total, hipStream_t s);
 int mlp_forward_rows(ngmi_mlp *n, const float *X, float *y, int maxRows, const int *dcount,
                      hipStream_t s);
+int mlp_guess_rows(const ngmi_mlp *n, const float *X, int *count, int maxRows, const int64_t *dest,
+                   const double2 *base, double *x, float *yout, unsigned long long *rows_total, hipStream_t s);
 const B4Tab *set_table(const ngmi_bsim4_set *s);
 }
 
@@ -104,8 +104,10 @@ struct ngmi_batch {
     /* learned initial guess (single-topology batches) */
     ngmi_mlp *net = nullptr;
     float *gF = nullptr, *gy = nullptr;
-    int *grow = nullptr, *gcount = nullptr;
+    int64_t *gdest = nullptr;   /* per row: the node's offset in the iterate array */
+    int *gcount = nullptr;      /* per half of the circuits: rows this pass, the fused kernel's finish counter */
     double2 *gbase = nullptr;   /* per row: linear extrapolation, activity */
+    unsigned long long *grows = nullptr;   /* network rows evaluated, summed over passes */
     unsigned long long counters[2] = {0, 0};
     /* live data stream: watched circuits and the points already handed out */
     ngmi_data_cb data_cb = nullptr;
@@ -1284,20 +1286,30 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     }
     NGMI_HIP(hipMemsetAsync(b->done_count, 0, sizeof(int), s));
     std::fill(b->sent.begin(), b->sent.end(), 0);
+    /* the guess rows' counters start from zero (eager, before any capture) */
+    if (b->net) {
+        NGMI_HIP(hipMemsetAsync(b->gcount, 0, 4 * sizeof(int), s));
+        NGMI_HIP(hipMemsetAsync(b->grows, 0, sizeof(unsigned long long), s));
+    }
     const TopoGroup &G0 = *b->groups[0];
     const bool one = b->groups.size() == 1;
     const B4Tab *tab0 = set_table(G0.set);
+    const bool fused = tune_get(TUNE_GUESS_FUSED) != 0;
     /* two streams: the circuits' halves alternate, so one half's latency-
      * bound step kernel runs beside the other half's BSIM4 kernel (single
-     * topology, no learned guess) */
-    const bool split = one && !b->net && C >= 2 && tune_get(TUNE_STREAMS) == 2;
+     * topology; with the learned guess each half has its own feature rows
+     * and network launch, which needs the fused kernel, and profiling then
+     * stays on one stream) */
+    const bool split = one && C >= 2 && tune_get(TUNE_STREAMS) == 2 &&
+                       (!b->net || (fused && !b->profile));
+    const int half = C / 2;
+    b->groups[0]->D.gsplit = split ? half : C;
     hipStream_t s2 = nullptr;
     hipEvent_t join = nullptr;
     if (split) {
         NGMI_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         NGMI_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     }
-    const int half = C / 2;
     if (split) {   /* the second stream starts after the resets above */
         NGMI_HIP(hipEventRecord(join, s));
         NGMI_HIP(hipStreamWaitEvent(s2, join, 0));
@@ -1332,6 +1344,25 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t fork = nullptr;
+    /* One pass's learned guess for the circuits of half h (all circuits
+     * when not split): the network over the feature rows the step kernels
+     * of the previous pass appended (circuits starting a step from the
+     * guess), whose outputs become those nodes' starting values.  The fused
+     * kernel clears the row counter itself; the layer-by-layer path clears
+     * it with a kernel, not a memset node, since the chunk may be captured
+     * into a graph. */
+    auto guess_pass = [&](hipStream_t sa, int h) -> int {
+        const int64_t r0 = h ? (int64_t)G0.D.gsplit * G0.N : 0;   /* the half's first row */
+        const int maxrows = (int)(h ? (int64_t)(C - G0.D.gsplit) * G0.N : (int64_t)G0.D.gsplit * G0.N);
+        int *count = b->gcount + 2 * h;
+        if (fused)
+            return mlp_guess_rows(b->net, b->gF + 16 * r0, count, maxrows, b->gdest + r0, b->gbase + r0, G0.D.x,
+                                  nullptr, b->grows, sa);
+        int r = mlp_forward_rows(b->net, b->gF, b->gy, maxrows, b->gcount, sa);
+        if (r == NGMI_OK) r = launch_guess_apply(G0.D, b->gy, b->gdest, b->gbase, b->gcount, maxrows, b->grows, sa);
+        if (r == NGMI_OK) r = launch_zero_int(b->gcount, sa);
+        return r;
+    };
     auto launch_chunk = [&](hipStream_t sa, int count, bool capturing) -> int {
         int r = NGMI_OK;
         if (split && capturing) {
@@ -1339,17 +1370,12 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
             NGMI_HIP(hipStreamWaitEvent(s2, fork, 0));
         }
         for (int q = 0; q < count && r == NGMI_OK; q++) {
-            if (b->net) {
-                const int maxrows = G0.C * G0.N;
-                r = launch_zero_int(b->gcount, sa);
-                if (r == NGMI_OK) r = launch_guess_features(G0.D, b->gF, b->grow, b->gbase, b->gcount, sa);
-                if (r == NGMI_OK) r = mlp_forward_rows(b->net, b->gF, b->gy, maxrows, b->gcount, sa);
-                if (r == NGMI_OK) r = launch_guess_apply(G0.D, b->gy, b->grow, b->gbase, b->gcount, maxrows, sa);
-            }
+            if (b->net) r = guess_pass(sa, 0);
             if (r != NGMI_OK) break;
             if (split) {
                 r = launch_tran_pass(tab0, &G0.D, 0, half, nullptr, nullptr, nullptr, 1, nullptr, half, b->lds, sa,
                                      nullptr);
+                if (r == NGMI_OK && b->net) r = guess_pass(s2, 1);
                 if (r == NGMI_OK)
                     r = launch_tran_pass(tab0, &G0.D, half, C, nullptr, nullptr, nullptr, 1, nullptr, C - half,
                                          b->lds, s2, nullptr);
@@ -1405,14 +1431,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         for (; b->profile && k < check_every && pass < max_passes; k++, pass++) {
             if (b->net) {
                 if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k], s));
-                const int maxrows = G0.C * G0.N;
-                /* a kernel, not a memset node, clears the row counter: the
-                 * chunk may be captured into a graph */
-                rc = launch_zero_int(b->gcount, s);
-                if (rc == NGMI_OK) rc = launch_guess_features(G0.D, b->gF, b->grow, b->gbase, b->gcount, s);
-                if (rc == NGMI_OK) rc = mlp_forward_rows(b->net, b->gF, b->gy, maxrows, b->gcount, s);
-                if (rc == NGMI_OK)
-                    rc = launch_guess_apply(G0.D, b->gy, b->grow, b->gbase, b->gcount, maxrows, s);
+                rc = guess_pass(s, 0);
                 if (rc != NGMI_OK) break;
                 if (b->profile) NGMI_HIP(hipEventRecord(gev[2 * k + 1], s));
             }
@@ -1529,14 +1548,27 @@ int ngmi_batch_set_guess(ngmi_batch *b, ngmi_mlp *net)
     TranDev &D = G.D;
     const size_t rows = (size_t)G.C * G.N;
     if (!b->gF) {
-        if (alloc(b, rows * 16, &b->gF) || alloc(b, rows, &b->gy) || alloc(b, rows, &b->grow) ||
-            alloc(b, rows, &b->gbase) || alloc(b, 1, &b->gcount))
+        if (alloc(b, rows * 16, &b->gF) || alloc(b, rows, &b->gy) || alloc(b, rows, &b->gdest) ||
+            alloc(b, rows, &b->gbase) || alloc(b, 4, &b->gcount) || alloc(b, 1, &b->grows))
             return NGMI_E_DEVICE;
         double *h;
         if (alloc(b, (size_t)4 * G.C * (G.N + 1), &h)) return NGMI_E_DEVICE;
         D.hist = h;
+        D.gF = b->gF;
+        D.gdest = b->gdest;
+        D.gbase = b->gbase;
+        D.gcount = b->gcount;
         NGMI_HIP(hipMemcpy(b->dD, &D, sizeof(TranDev), hipMemcpyHostToDevice));
     }
+    return NGMI_OK;
+}
+
+int ngmi_batch_guess_rows(const ngmi_batch *b, int64_t *rows)
+{
+    if (!b || !rows) return NGMI_E_PANIC;
+    unsigned long long r = 0;
+    if (b->grows) NGMI_HIP(hipMemcpy(&r, b->grows, sizeof(r), hipMemcpyDeviceToHost));
+    *rows = (int64_t)r;
     return NGMI_OK;
 }
 
@@ -1549,13 +1581,14 @@ int ngmi_batch_neighbours(const ngmi_batch *b, int *out)
 }
 
 int ngmi_batch_guess_stats(const ngmi_batch *b, int c, int64_t *guessed, int64_t *fallbacks,
-                           double *guess_ms)
+                           int64_t *paused, double *guess_ms)
 {
     if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
     CktCtl k;
     NGMI_HIP(hipMemcpy(&k, b->ctl_of(c), sizeof(k), hipMemcpyDeviceToHost));
     if (guessed) *guessed = k.guessed;
     if (fallbacks) *fallbacks = k.fallbacks;
+    if (paused) *paused = k.gpaused;
     if (guess_ms) *guess_ms = b->guess_ms;
     return NGMI_OK;
 }

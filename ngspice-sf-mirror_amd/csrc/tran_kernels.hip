@@ -22,6 +22,7 @@
 #include "ngmi_internal.h"
 #include "bsim4_eval.h"
 #include "tran_types.h"
+#include "guess_types.h"
 
 namespace ngmi {
 
@@ -74,7 +75,9 @@ __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D
     const CktCtl *K = D.ctl + c;
     if (K->phase >= PH_DONE) return;
     B4Ckt ck;
-    ck.mode = K->mode;
+    /* a circuit whose step starts from the learned guess (set by
+     * k_guess_features this pass) limits against its last accepted point */
+    ck.mode = K->mode | (K->guess == 1 ? M_GUESSED : 0);
     ck.order = K->order;
     ck.method = K->method;
 #pragma unroll
@@ -702,14 +705,45 @@ __device__ int loop_top(CktCtl *K)
     return com_cof(K);
 }
 
+/* The learned guess's back-off.  NIiter cannot stop before its second
+ * iteration, so a step that starts from the guess and needs more than two
+ * has cost more than the guess can ever save.  After GUESS_STRIKES such
+ * steps in a row (a failed run counts too) the circuit starts its next
+ * gbackoff steps from the predictor, and each further pause is twice as
+ * long (up to GUESS_PAUSE_MAX): a circuit on which the network is poor
+ * falls back to the plain method instead of holding the whole batch. */
+constexpr int GUESS_STRIKES = 3;
+constexpr int GUESS_PAUSE_MIN = 32;
+constexpr int GUESS_PAUSE_MAX = 4096;
+
+__device__ void guess_outcome(CktCtl *K, bool slow)
+{
+    if (!slow) {
+        K->gstrike = 0;
+        return;
+    }
+    if (++K->gstrike < GUESS_STRIKES) return;
+    K->gstrike = 0;
+    const int b = K->gbackoff < GUESS_PAUSE_MIN ? GUESS_PAUSE_MIN : K->gbackoff;
+    K->gpause = b;
+    K->gbackoff = b < GUESS_PAUSE_MAX ? 2 * b : GUESS_PAUSE_MAX;
+}
+
 /* start the next Newton run from the learned guess when the circuit has
- * four accepted points of history */
-__device__ void request_guess(CktCtl *K)
+ * four accepted points of history and is not pausing; true when it will */
+__device__ bool request_guess(CktCtl *K)
 {
     if (K->learned && K->nhist >= 4 && !K->firsttime && K->phase == PH_TRAN) {
+        if (K->gpause > 0) {
+            K->gpause--;
+            K->gpaused++;
+            return false;
+        }
         K->guess = 1;
         K->mode = M_TRAN | M_INITFLOAT;
+        return true;
     }
+    return false;
 }
 
 /* a read-only table load through the global address space (the tables'
@@ -992,12 +1026,144 @@ constexpr int PLAIN_UNROLL = 8;
 /* unknowns (and device votes) a lane loads together in the convergence test */
 constexpr int CONV_CHUNK = 8;
 
+/* ---------------- learned initial guess ---------------- */
+
+constexpr double GUESS_RATIO_MAX = 4.0;       /* step ratios are clamped to [0, 4] */
+constexpr double GUESS_MIN_ACTIVITY = 1e-5;   /* V: quieter nodes keep the linear extrapolation */
+
+__device__ __forceinline__ double clamp_ratio(double r)
+{
+    return r != r ? 0.0 : fmin(fmax(r, 0.0), GUESS_RATIO_MAX);
+}
+
+/* One feature row of the guess network for a node whose last four accepted
+ * values v0..v3 sit at t0 > t1 > t2 > t3, for the step t0 -> t0 + h;
+ * nbv[k] is neighbour k's value at t0 where bit k of nbmask is set.  Writes
+ * the linear extrapolation and the activity; returns false (f untouched)
+ * when the network cannot matter: the node is quieter than
+ * GUESS_MIN_ACTIVITY, or the largest correction the network may make,
+ * GUESS_CLAMP times the activity, stays inside the node's Newton tolerance
+ * reltol |v0| + vntol (niconv.c:50-60), so the step converges as from the
+ * extrapolation.  The training rows are built by the same formula on the
+ * host (products and sums rounded separately, never fused). */
+__device__ __forceinline__ bool guess_feature_row(double v0, double v1, double v2, double v3, double t0,
+                                                  double t1, double t2, double t3, double h,
+                                                  const double *nbv, unsigned nbmask, double reltol,
+                                                  double vntol, float *f, double *lin_out, double *act_out)
+{
+    const double r0 = clamp_ratio(h / (t0 - t1));
+    const double r1 = clamp_ratio((t0 - t1) / (t1 - t2));
+    const double r2 = clamp_ratio((t1 - t2) / (t2 - t3));
+    const double d0 = v0 - v1, d1 = v1 - v2, d2 = v2 - v3;
+    const double act = fabs(d0) + fabs(d1) + fabs(d2);
+    *lin_out = v0 + r0 * d0;
+    *act_out = act;
+    if (!(act >= GUESS_MIN_ACTIVITY)) return false;
+    if (!(__dmul_rn(GUESS_CLAMP, act) > __dadd_rn(__dmul_rn(reltol, fabs(v0)), vntol))) return false;
+    if (!f) return true;   /* the selection only */
+    f[0] = (float)v0;
+    f[1] = (float)(d0 / act);
+    f[2] = (float)(d1 / act);
+    f[3] = (float)(d2 / act);
+    f[4] = (float)r0;
+    f[5] = (float)r1;
+    f[6] = (float)r2;
+    for (int k = 0; k < 8; k++) f[7 + k] = (nbmask >> k) & 1u ? (float)(nbv[k] - v0) : 0.0f;
+    f[15] = (float)(log10(act) / 5.0);
+    return true;
+}
+
+
+/*
+ * Features of node u of circuit c for the step the circuit is starting
+ * (t_n -> t_n + h), from the last four accepted points v0..v3 at t0 > t1 >
+ * t2 > t3, with the node's activity a = |v0 - v1| + |v1 - v2| + |v2 - v3|:
+ *   v0, (v0 - v1) / a, (v1 - v2) / a, (v2 - v3) / a, h / (t0 - t1),
+ *   (t0 - t1) / (t1 - t2), (t1 - t2) / (t2 - t3), the voltages of up to 8
+ *   neighbour nodes minus v0 (0 where absent), log10(a) / 5.
+ * The guess is v0 + r (v0 - v1) + a * clamp(net, +-4): the network predicts
+ * the linear extrapolation's error in units of the node's own recent
+ * motion.  Nodes the network cannot matter for (guess_feature_row) take the
+ * extrapolation without a network row; branch currents start from their
+ * last accepted value.
+ *
+ * Run by the circuit's step workgroup (one wavefront) right after it asks
+ * for a guess, in two sweeps so that the dependent loads of a row (its
+ * neighbours) happen once per circuit rather than once per 64 nodes: the
+ * first sweep reads each node's four accepted values, gives the nodes the
+ * network cannot matter for their extrapolation and lists the others in
+ * LDS (list, at least N ints, free once the step is accepted); one atomic
+ * reserves the circuit's rows at the end of its half's row buffer; the
+ * second sweep builds the listed rows.  The network kernel at the start of
+ * the next pass turns them into the nodes' starting values.
+ */
+__device__ __forceinline__ void guess_rows(const TranDev &D, const CktCtl *K, int c, int *list)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = (int)__lane_id();
+    const int s0 = (K->hpos + 3) & 3, s1 = (K->hpos + 2) & 3, s2 = (K->hpos + 1) & 3, s3 = K->hpos;
+    const double t0 = K->ht[s0], t1 = K->ht[s1], t2 = K->ht[s2], t3 = K->ht[s3], h = K->delta;
+    const int64_t stride = (int64_t)D.C * (D.N + 1);
+    const double *h0p = D.hist + ((int64_t)s0 * D.C + c) * (D.N + 1);
+    const int64_t xoff = ((int64_t)K->xsel * D.C + c) * (D.N + 1);
+    double *xo = D.x + xoff;
+    const int hs = c >= D.gsplit;   /* the half whose row buffer and counter this circuit uses */
+    const int64_t r0 = hs ? (int64_t)D.gsplit * D.N : 0;
+    const double nbz[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int n_on = 0;   /* uniform: rows listed so far */
+    for (int b0 = 1; b0 <= D.N; b0 += nt) {
+        const int u = b0 + tid;
+        bool on = false;
+        if (u <= D.N) {
+            if (u > D.NN) {
+                xo[u] = h0p[u];   /* branch currents start from their last accepted value */
+            } else {
+                double lin, act;
+                on = guess_feature_row(h0p[u], h0p[u + ((s1 - s0) * stride)], h0p[u + ((s2 - s0) * stride)],
+                                       h0p[u + ((s3 - s0) * stride)], t0, t1, t2, t3, h, nbz, 0u, D.reltol,
+                                       D.vntol, nullptr, &lin, &act);
+                if (!on) xo[u] = lin;
+            }
+        }
+        const unsigned long long m = __ballot(on);
+        if (on) list[n_on + __popcll(m & ((1ull << lane) - 1ull))] = u;
+        n_on += __popcll(m);
+    }
+    if (n_on == 0) return;
+    int first = 0;
+    if (tid == 0) first = atomicAdd(D.gcount + 2 * hs, n_on);
+    first = __shfl(first, 0);
+    __syncthreads();   /* the list is complete */
+    for (int k = tid; k < n_on; k += nt) {
+        const int u = list[k];
+        double nbv[8];
+        unsigned nbmask = 0;
+        for (int q = 0; q < 8; q++) {
+            const int nb = D.nbr[u * 8 + q];
+            nbv[q] = nb >= 0 ? h0p[nb] : 0.0;
+            nbmask |= nb >= 0 ? 1u << q : 0u;
+        }
+        float f[16];
+        double lin, act;
+        guess_feature_row(h0p[u], h0p[u + ((s1 - s0) * stride)], h0p[u + ((s2 - s0) * stride)],
+                          h0p[u + ((s3 - s0) * stride)], t0, t1, t2, t3, h, nbv, nbmask, D.reltol, D.vntol, f,
+                          &lin, &act);
+        const int64_t row = r0 + first + k;
+        float4 *fr = reinterpret_cast<float4 *>(D.gF + row * 16);
+#pragma unroll
+        for (int q = 0; q < 4; q++) fr[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+        D.gdest[row] = xoff + u;
+        D.gbase[row] = make_double2(lin, act);
+    }
+}
+
 /* circuit c of the topology D, one workgroup */
 __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
 {
     extern __shared__ double sm[];
     __shared__ int s_flag[4];
     __shared__ int s_act;
+    __shared__ int s_guess;
 
     const int tid = threadIdx.x, nt = blockDim.x;
     CktCtl *K = D.ctl + c;
@@ -1352,6 +1518,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                  * step from the plain predictor */
                 K->guess = 0;
                 K->fallbacks++;
+                guess_outcome(K, true);
                 K->mode = M_TRAN | M_INITPRED;
                 K->iterno = 0;
                 s_act = ACT_STOP;
@@ -1359,6 +1526,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                 if (K->guess == 2) {
                     K->guess = 0;
                     K->guessed++;
+                    guess_outcome(K, rc != 0 || iterno > 2);
                 }
                 K->mode = M_TRAN | M_INITPRED;
                 s_act = K->firsttime ? 20 : 21;   /* 20: copy state1 -> state2, state3 first */
@@ -1464,28 +1632,34 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                 K->hpos = (K->hpos + 1) & 3;
                 K->nhist++;
             }
+            s_guess = 0;
             if (almost_equal_ulps(K->time, K->finalTime, 100)) {
                 K->phase = PH_DONE;
                 atomicAdd(D.done_count, 1);
             } else {
                 resume_block(K);
                 if (loop_top(K)) fail(D, K, 104);
-                else request_guess(K);
+                else s_guess = request_guess(K);
             }
         }
+    } else if (act == ACT_RETRY) {
+        if (tid == 0) {
+            s_guess = 0;
+            if (K->delta <= K->delmin && K->olddelta <= K->delmin) {
+                fail(D, K, 106);
+            } else {
+                if (K->delta <= K->delmin) K->delta = K->delmin;
+                if (loop_top(K)) fail(D, K, 104);
+                else s_guess = request_guess(K);
+            }
+        }
+    } else {
         return;
     }
-    if (act == ACT_RETRY && tid == 0) {
-        if (K->delta <= K->delmin) {
-            if (K->olddelta > K->delmin) K->delta = K->delmin;
-            else {
-                fail(D, K, 106);
-                return;
-            }
-        }
-        if (loop_top(K)) fail(D, K, 104);
-        else request_guess(K);
-    }
+    /* the next step starts from the learned guess: its feature rows (the
+     * LU workspace in LDS is free by now) */
+    __syncthreads();
+    if (s_guess) guess_rows(D, K, c, reinterpret_cast<int *>(sm));
 }
 
 /* one topology, one workgroup per circuit, circuits c0.. */
@@ -1503,54 +1677,12 @@ k_tran_step_groups(const TranDev *__restrict__ Ds, const int2 *__restrict__ blk)
     tran_step_body(Ds[bg.x], bg.y);
 }
 
-/* ---------------- learned initial guess ---------------- */
-
-constexpr double GUESS_RATIO_MAX = 4.0;       /* step ratios are clamped to [0, 4] */
-constexpr double GUESS_MIN_ACTIVITY = 1e-5;   /* V: quieter nodes keep the linear extrapolation */
-constexpr double GUESS_CLAMP = 4.0;           /* |network output| limit, in units of activity */
-
-__device__ __forceinline__ double clamp_ratio(double r)
-{
-    return r != r ? 0.0 : fmin(fmax(r, 0.0), GUESS_RATIO_MAX);
-}
-
-/* One feature row of the guess network for a node whose last four accepted
- * values v0..v3 sit at t0 > t1 > t2 > t3, for the step t0 -> t0 + h;
- * nbv[k] is neighbour k's value at t0 where bit k of nbmask is set.  Writes
- * the linear extrapolation and the activity; returns false (f untouched)
- * when the node is too quiet for the network.  The training rows are built
- * by the same formula on the host. */
-__device__ __forceinline__ bool guess_feature_row(double v0, double v1, double v2, double v3, double t0,
-                                                  double t1, double t2, double t3, double h,
-                                                  const double *nbv, unsigned nbmask, float *f,
-                                                  double *lin_out, double *act_out)
-{
-    const double r0 = clamp_ratio(h / (t0 - t1));
-    const double r1 = clamp_ratio((t0 - t1) / (t1 - t2));
-    const double r2 = clamp_ratio((t1 - t2) / (t2 - t3));
-    const double d0 = v0 - v1, d1 = v1 - v2, d2 = v2 - v3;
-    const double act = fabs(d0) + fabs(d1) + fabs(d2);
-    *lin_out = v0 + r0 * d0;
-    *act_out = act;
-    if (!(act >= GUESS_MIN_ACTIVITY)) return false;
-    f[0] = (float)v0;
-    f[1] = (float)(d0 / act);
-    f[2] = (float)(d1 / act);
-    f[3] = (float)(d2 / act);
-    f[4] = (float)r0;
-    f[5] = (float)r1;
-    f[6] = (float)r2;
-    for (int k = 0; k < 8; k++) f[7 + k] = (nbmask >> k) & 1u ? (float)(nbv[k] - v0) : 0.0f;
-    f[15] = (float)(log10(act) / 5.0);
-    return true;
-}
-
 /* the same rows for caller-supplied windows: row i from t[4i..4i+3],
  * v[4i..4i+3], h[i], nbv[8i..8i+7], nbmask[i]; quiet rows get zeros and a
  * negative activity in act[i] */
 __global__ void __launch_bounds__(256)
 k_guess_feature_rows(int M, const double *t, const double *h, const double *v, const double *nbv,
-                     const unsigned *nbmask, float *F, double *act)
+                     const unsigned *nbmask, double reltol, double vntol, float *F, double *act)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
@@ -1558,127 +1690,36 @@ k_guess_feature_rows(int M, const double *t, const double *h, const double *v, c
     double lin, a;
     const bool on = guess_feature_row(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3], t[4 * i],
                                       t[4 * i + 1], t[4 * i + 2], t[4 * i + 3], h[i], nbv + 8 * i, nbmask[i],
-                                      f, &lin, &a);
+                                      reltol, vntol, f, &lin, &a);
     for (int k = 0; k < 16; k++) F[(int64_t)i * 16 + k] = on ? f[k] : 0.0f;
     act[i] = on ? a : -a;
 }
 
-/*
- * Features of node u of circuit c for the step the circuit is starting
- * (t_n -> t_n + h), from the last four accepted points v0..v3 at t0 > t1 >
- * t2 > t3, with the node's activity a = |v0 - v1| + |v1 - v2| + |v2 - v3|:
- *   v0, (v0 - v1) / a, (v1 - v2) / a, (v2 - v3) / a, h / (t0 - t1),
- *   (t0 - t1) / (t1 - t2), (t1 - t2) / (t2 - t3), the voltages of up to 8
- *   neighbour nodes minus v0 (0 where absent), log10(a) / 5.
- * The guess is v0 + r (v0 - v1) + a * clamp(net, +-4): the network predicts
- * the linear extrapolation's error in units of the node's own recent
- * motion, so a node at rest keeps the extrapolation.  Nodes with a below
- * 10 uV take the extrapolation without a network row; branch currents
- * start from their last accepted value.
- */
+/* the network's output y[row] becomes the starting value of the row's node
+ * (rows produced by k_guess_features; grid-stride) */
 __global__ void __launch_bounds__(256)
-k_guess_features(TranDev D, float *F, int *rowmap, double2 *base, int *count)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)D.C * D.N) return;
-    const int c = (int)(i / D.N), u = (int)(i - (int64_t)c * D.N) + 1;
-    const CktCtl *K = D.ctl + c;
-    if (K->phase != PH_TRAN || K->guess != 1) return;
-    const int s0 = (K->hpos + 3) & 3, s1 = (K->hpos + 2) & 3, s2 = (K->hpos + 1) & 3, s3 = K->hpos;
-    const int64_t stride = (int64_t)D.C * (D.N + 1);
-    const double *h0p = D.hist + ((int64_t)s0 * D.C + c) * (D.N + 1);
-    double *xo = D.x + ((int64_t)K->xsel * D.C + c) * (D.N + 1);
-    if (u > D.NN) {
-        xo[u] = h0p[u];
-        return;
-    }
-    const double v0 = h0p[u];
-    const double v1 = h0p[u + ((s1 - s0) * stride)];
-    const double v2 = h0p[u + ((s2 - s0) * stride)];
-    const double v3 = h0p[u + ((s3 - s0) * stride)];
-    double nbv[8];
-    unsigned nbmask = 0;
-    for (int k = 0; k < 8; k++) {
-        const int nb = D.nbr[u * 8 + k];
-        nbv[k] = nb >= 0 ? h0p[nb] : 0.0;
-        nbmask |= nb >= 0 ? 1u << k : 0u;
-    }
-    float f[16];
-    double lin, act;
-    if (!guess_feature_row(v0, v1, v2, v3, K->ht[s0], K->ht[s1], K->ht[s2], K->ht[s3], K->delta, nbv,
-                           nbmask, f, &lin, &act)) {
-        xo[u] = lin;
-        return;
-    }
-    const int row = atomicAdd(count, 1);
-    float *fr = F + (int64_t)row * 16;
-    for (int k = 0; k < 16; k++) fr[k] = f[k];
-    rowmap[row] = c * (D.N + 1) + u;
-    base[row] = make_double2(lin, act);
-}
-
-/* grid-stride over the rows the features kernel produced */
-__global__ void __launch_bounds__(256)
-k_guess_apply(TranDev D, const float *y, const int *rowmap, const double2 *base, const int *count)
+k_guess_apply(double *x, const float *y, const int64_t *dest, const double2 *base, const int *count,
+              unsigned long long *rows_total)
 {
     const int n = *count;
-    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
-        const int m = rowmap[row];
-        const int c = m / (D.N + 1), u = m - c * (D.N + 1);
-        const CktCtl *K = D.ctl + c;
-        const double2 b = base[row];
-        const double g = fmin(fmax((double)y[row], -GUESS_CLAMP), GUESS_CLAMP);
-        D.x[((int64_t)K->xsel * D.C + c) * (D.N + 1) + u] = b.x + b.y * g;
-    }
+    if (rows_total && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(rows_total, (unsigned long long)n);
+    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x)
+        x[dest[row]] = guess_value(base[row], y[row]);
 }
 
-/* the first iteration from a guess limits against the last accepted terminal
- * voltages, as the predictor step does (b4ld.c:322-372 copies state1 into
- * state0 before extrapolating) */
-__global__ void __launch_bounds__(256)
-k_guess_states(TranDev D)
-{
-    const int64_t cnm = (int64_t)D.C * D.NM;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnm) return;
-    const int c = (int)(i / D.NM);
-    const CktCtl *K = D.ctl + c;
-    if (K->phase != PH_TRAN || K->guess != 1) return;
-    const int p0 = lvl(K->rb, 0), p1 = lvl(K->rb, 1);
-    const int ks[6] = {B4X_vds, B4X_vgs, B4X_vbs, B4X_vbd, B4X_vdbs, B4X_vdbd};
-#pragma unroll
-    for (int q = 0; q < 6; q++)
-        D.mstate[((int64_t)p0 * B4X__COUNT + ks[q]) * cnm + i] =
-            D.mstate[((int64_t)p1 * B4X__COUNT + ks[q]) * cnm + i];
-}
-
-int launch_guess_features(const TranDev &D, float *F, int *rowmap, double2 *base, int *count,
-                          hipStream_t s)
-{
-    const int64_t n = (int64_t)D.C * D.N;
-    hipLaunchKernelGGL(k_guess_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, F,
-                       rowmap, base, count);
-    NGMI_HIP(hipGetLastError());
-    const int64_t cnm = (int64_t)D.C * D.NM;
-    if (cnm > 0) {
-        hipLaunchKernelGGL(k_guess_states, dim3((unsigned)((cnm + 255) / 256)), dim3(256), 0, s, D);
-        NGMI_HIP(hipGetLastError());
-    }
-    return NGMI_OK;
-}
-
-int launch_guess_apply(const TranDev &D, const float *y, const int *rowmap, const double2 *base,
-                       const int *count, int maxrows, hipStream_t s)
+int launch_guess_apply(const TranDev &D, const float *y, const int64_t *dest, const double2 *base,
+                       const int *count, int maxrows, unsigned long long *rows_total, hipStream_t s)
 {
     const int blocks = std::max(1, std::min((maxrows + 255) / 256, 1024));
-    hipLaunchKernelGGL(k_guess_apply, dim3((unsigned)blocks), dim3(256), 0, s, D, y,
-                       rowmap, base, count);
+    hipLaunchKernelGGL(k_guess_apply, dim3((unsigned)blocks), dim3(256), 0, s, D.x, y, dest, base, count,
+                       rows_total);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }
 
 extern "C" int ngmi_guess_features(int M, const double *t, const double *h, const double *v, const double *nbv,
-                                   const unsigned *nbmask, float *F, double *act, void *stream)
+                                   const unsigned *nbmask, double reltol, double vntol, float *F, double *act,
+                                   void *stream)
 {
     if (M < 0 || (M > 0 && (!t || !h || !v || !nbv || !nbmask || !F || !act))) {
         set_error("guess_features: bad arguments");
@@ -1686,7 +1727,7 @@ extern "C" int ngmi_guess_features(int M, const double *t, const double *h, cons
     }
     if (M == 0) return NGMI_OK;
     hipLaunchKernelGGL(k_guess_feature_rows, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, M, t, h, v, nbv, nbmask, F, act);
+                       (hipStream_t)stream, M, t, h, v, nbv, nbmask, reltol, vntol, F, act);
     NGMI_HIP(hipGetLastError());
     return NGMI_OK;
 }

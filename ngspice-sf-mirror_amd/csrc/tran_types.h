@@ -44,6 +44,11 @@ struct CktCtl {
     int learned, guess, nhist, hpos;
     double ht[4];
     int64_t guessed, fallbacks;
+    /* back-off: gstrike = guessed steps in a row that needed more Newton
+     * iterations than the floor of 2 (or failed); gpause = steps still to
+     * start from the predictor; gbackoff = the next pause's length */
+    int gstrike, gpause, gbackoff, gpad;
+    int64_t gpaused;   /* steps started from the predictor during pauses */
     /* CKTop's stepping (cktop.c:161-670): CKTdiagGmin, the devices' CKTgmin,
      * CKTsrcFact, and the scalars of the stage running */
     double diagGmin, gminCur, srcFact;
@@ -158,6 +163,18 @@ struct TranDev {
     int *done_count;          /* circuits finished or failed */
     double *hist;             /* [4][C][N+1] accepted node vectors (learned guess) */
     const int *nbr;           /* [NN+1][8] neighbour nodes (0 = none) */
+    /* the guess network's rows, appended by the step kernel of a circuit
+     * that requests a guess and consumed at the start of the next pass:
+     * features [rows][16], the node's offset in x, (extrapolation,
+     * activity).  Circuits below gsplit use rows from 0 and the counters
+     * gcount[0] (rows) and gcount[1] (the network kernel's finish counter);
+     * the others (the second stream's half) rows from gsplit * N and
+     * gcount[2], gcount[3]. */
+    float *gF;
+    int64_t *gdest;
+    double2 *gbase;
+    int *gcount;
+    int gsplit;
 };
 
 }  // namespace ngmi

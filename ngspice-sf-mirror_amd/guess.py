@@ -12,8 +12,10 @@ node's activity a = |v0 - v1| + |v1 - v2| + |v2 - v3|:
 with r0 = h / (t_n - t_{n-1}), r1 = (t_n - t_{n-1}) / (t_{n-1} - t_{n-2}),
 r2 = (t_{n-1} - t_{n-2}) / (t_{n-2} - t_{n-3}), each clamped to [0, 4], and
 nb_k the accepted voltage of the k-th neighbour node (0 where absent).
-Only nodes with a >= 10 uV get a row (the engine gives quieter nodes the
-linear extrapolation).  The target is the linear extrapolation's error in
+Only nodes the network can matter for get a row: a >= 10 uV, and the
+largest correction the network may make, 4 a, above the node's Newton
+tolerance reltol |v0| + vntol (niconv.c:50-60); the engine gives the
+other nodes the linear extrapolation.  The target is the linear extrapolation's error in
 units of a, (v_{n+1} - (v0 + r0 (v0 - v1))) / a, clipped to [-4, 4] as the
 engine clamps the network output.  In training h is the step actually taken
 to t_{n+1}.
@@ -32,7 +34,7 @@ def _ratio(a, b):
     return np.clip(r, 0.0, RATIO_MAX)
 
 
-def features(t, x, nbr, nnodes, every=1, with_scale=False):
+def features(t, x, nbr, nnodes, every=1, with_scale=False, reltol=1e-3, vntol=1e-6):
     """t[P], x[P, N] (unknowns 1..N in columns 0..N-1), nbr[(nnodes+1)*8].
     Returns (F [rows, 16] float32, target [rows] float32) over steps
     n = 3 .. P-2 (every `every`-th) and the active nodes 1..nnodes; with
@@ -52,7 +54,7 @@ def features(t, x, nbr, nnodes, every=1, with_scale=False):
     r2 = _ratio(t1 - t2, t2 - t3)[:, None]
     d0, d1, d2 = v0 - v1, v1 - v2, v2 - v3
     act = np.abs(d0) + np.abs(d1) + np.abs(d2)
-    keep = act >= MIN_ACTIVITY
+    keep = (act >= MIN_ACTIVITY) & (CLAMP * act > reltol * np.abs(v0) + vntol)
     a = np.where(keep, act, 1.0)
     S, NN = v0.shape
     F = np.zeros((S, NN, 16))

@@ -641,7 +641,7 @@ def get_tune(knob):
     return v
 
 
-def guess_features(t4, h, v4, nbv, nbmask, stream=0):
+def guess_features(t4, h, v4, nbv, nbmask, reltol=1e-3, vntol=1e-6, stream=0):
     """The engine's guess feature rows (the device function the in-loop
     features kernel uses) for caller-supplied windows, through
     ngmi_guess_features: t4[M,4] and v4[M,4] the last four accepted times
@@ -654,9 +654,9 @@ def guess_features(t4, h, v4, nbv, nbmask, stream=0):
     act = torch.empty(M, dtype=torch.float64, device=h.device)
     L = lib()
     vp = ctypes.c_void_p
-    L.ngmi_guess_features.argtypes = [ctypes.c_int] + [vp] * 8
+    L.ngmi_guess_features.argtypes = [ctypes.c_int] + [vp] * 5 + [ctypes.c_double] * 2 + [vp] * 3
     check(L.ngmi_guess_features(M, t4.data_ptr(), h.data_ptr(), v4.data_ptr(), nbv.data_ptr(),
-                                nbmask.data_ptr(), F.data_ptr(), act.data_ptr(), stream),
+                                nbmask.data_ptr(), reltol, vntol, F.data_ptr(), act.data_ptr(), stream),
           'guess features')
     return F, act
 
@@ -819,11 +819,20 @@ class Batch:
         L = lib()
         i64 = ctypes.c_int64
         L.ngmi_batch_guess_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(i64),
-                                             ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
-        g, f, ms = i64(), i64(), ctypes.c_double()
-        check(L.ngmi_batch_guess_stats(self.h, c, ctypes.byref(g), ctypes.byref(f),
+                                             ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                             ctypes.POINTER(ctypes.c_double)]
+        g, f, pz, ms = i64(), i64(), i64(), ctypes.c_double()
+        check(L.ngmi_batch_guess_stats(self.h, c, ctypes.byref(g), ctypes.byref(f), ctypes.byref(pz),
                                        ctypes.byref(ms)), 'guess stats')
-        return dict(guessed=g.value, fallbacks=f.value, guess_ms=ms.value)
+        return dict(guessed=g.value, fallbacks=f.value, paused=pz.value, guess_ms=ms.value)
+
+    def guess_rows(self):
+        """network rows of the last run, summed over its passes"""
+        L = lib()
+        L.ngmi_batch_guess_rows.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+        r = ctypes.c_int64()
+        check(L.ngmi_batch_guess_rows(self.h, ctypes.byref(r)), 'guess rows')
+        return r.value
 
     def set_profile(self, on=True):
         check(lib().ngmi_batch_set_profile(self.h, 1 if on else 0), 'profile')

@@ -138,8 +138,9 @@ def test_features_match_the_engine_bit_for_bit():
     """guess.features (the host side that builds training rows) and the
     engine's features kernel (through ngmi_guess_features, the device
     function the in-loop kernel calls) give the same float32 rows, bit for
-    bit, on the windows of real adder waveforms; the quiet-node rule
-    (activity below 10 uV) selects the same rows."""
+    bit, on the windows of real adder waveforms; the rule that drops rows
+    the network cannot matter for (activity below 10 uV, or 4 x activity
+    within the Newton tolerance) selects the same rows."""
     print('case: guess feature rows, host vs engine, adder seeds 0..1 .tran 5p 2.5n', flush=True)
     b, descs = run([0, 1])
     nn = descs[0]['nnodes']
@@ -222,3 +223,36 @@ def test_learned_guess_full_length_against_oracle(trained, seed):
           'max |v - v_oracle| per output: %s; worst error / Newton tolerance %.2e'
           % (len(te), len(to), same, g['guessed'], g['fallbacks'], ', '.join(per), worst), flush=True)
     assert worst <= 1.0
+
+
+def test_fused_network_kernel_matches_layered_path(trained):
+    """The transient's fused network kernel (features -> three layers ->
+    head -> starting values in one launch, weights in registers) against the
+    layer-by-layer kernels it replaces: the same learned runs, bit for bit
+    (waveforms, Newton statistics and the number of network rows), since
+    both issue the same matrix-core steps and the same element-wise code."""
+    net = trained[0]
+    seeds = [0, 1, 2, 3]
+    print('case: fused vs layered guess network, adder seeds 0..3 .tran 5p %gn' % (TSTOP * 1e9), flush=True)
+    out = {}
+    try:
+        for fused in (1, 0):
+            ngmi.tune('guess_fused', fused)
+            b, descs = run(seeds, net=net)
+            n = b.info()['unknowns']
+            out[fused] = (b.guess_rows(), [b.waveform(c, n) for c in range(len(seeds))],
+                          [b.stats(c) for c in range(len(seeds))],
+                          [b.guess_stats(c)['guessed'] for c in range(len(seeds))])
+    finally:
+        ngmi.tune('guess_fused', 1)
+    rows_f, wf_f, st_f, g_f = out[1]
+    rows_l, wf_l, st_l, g_l = out[0]
+    print('network rows: fused %d, layered %d; steps from the guess %s / %s'
+          % (rows_f, rows_l, g_f, g_l), flush=True)
+    assert rows_f == rows_l and rows_f > 0
+    assert g_f == g_l
+    for c in range(len(seeds)):
+        assert st_f[c]['accepted'] == st_l[c]['accepted']
+        assert st_f[c]['iterations'] == st_l[c]['iterations']
+        assert np.array_equal(wf_f[c][0], wf_l[c][0])
+        assert np.array_equal(wf_f[c][1], wf_l[c][1])
