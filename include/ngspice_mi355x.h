@@ -329,6 +329,26 @@ int ngmi_circuit_set_op_options(ngmi_circuit *c, int noopiter, int gminsteps, in
  * with their node rows, then minimum degree).  A batch uses the sequence of
  * its first circuit. */
 int ngmi_circuit_set_pivot_order(ngmi_circuit *c, int n, const int *rows, const int *cols);
+/* on = 1: without a pivot sequence from the caller, the static LU takes
+ * KLU's own ordering, computed by the engine (ngmi_klu_analyze on the
+ * circuit's MNA pattern: P as the pivot rows, Q as the pivot columns, what
+ * SMPpreOrder's klu_analyze fixes, klusmp.c:1255-1285); on = 0 (the
+ * default): the engine's own ordering.  A batch uses its first circuit's
+ * choice. */
+int ngmi_circuit_use_klu_order(ngmi_circuit *c, int on);
+/* KLU's symbolic analysis on the host, restated (no device work): the
+ * permutations klu_analyze computes with KLU's defaults -- a block upper
+ * triangular form (maximum transversal, then strongly connected components;
+ * btf_order.c:36-132) and AMD on each block larger than 3 x 3
+ * (klu_analyze.c:14-250, amd_order.c, amd_2.c) -- for the n x n CSC
+ * pattern (Ap, Ai), 0-based.  P[k], Q[k]: row and column of pivot k;
+ * R[0..nblocks]: block boundaries (R needs n + 1 entries); nzoff: entries
+ * outside the diagonal blocks; structural_rank: the matching size.  Equal
+ * to Symbolic->P, Q, R of the reference on the same pattern; replaces
+ * SMPpreOrder's klu_analyze call (klusmp.c:1255-1285).  Any of nblocks,
+ * nzoff, structural_rank may be NULL. */
+int ngmi_klu_analyze(int n, const int *Ap, const int *Ai, int *P, int *Q, int *R, int *nblocks, int *nzoff,
+                     int *structural_rank);
 /* Host-only symbolic analysis of the circuit's topology (no device work):
  * unknowns, L+U nonzeros, factor terms, strictly-lower L nonzeros, levels
  * [factor, forward, backward] and the pivot sequence used (rows/cols,

@@ -38,8 +38,11 @@ struct ngmi_circuit {
     /* CKTop options (cktntask.c:94-132 defaults) */
     int noopiter = 0, gminsteps = 1, srcsteps = 1, itl2 = 50;
     double gminfactor = 10.0, gshunt = 0.0;
-    /* optional pivot sequence (1-based unknowns); empty = the engine's own */
+    /* optional pivot sequence (1-based unknowns); empty = the engine's own,
+     * or KLU's (restated klu_analyze on the circuit's pattern) when
+     * klu_order is set */
     std::vector<int> order_rows, order_cols;
+    int klu_order = 0;
 };
 
 #endif

@@ -316,6 +316,27 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         if ((int)c0.order_rows.size() != N) return fail("pivot order length differs from the unknown count");
         rowp = c0.order_rows;
         colq = c0.order_cols;
+    } else if (c0.klu_order) {
+        /* KLU's symbolic analysis of the pattern SMPpreOrder hands it:
+         * distinct entries, 0-based, rows ascending per column */
+        std::vector<std::vector<int>> colrows(N);
+        for (const Elt &e : elts) colrows[e.c - 1].push_back(e.r - 1);
+        std::vector<int> Ap(N + 1, 0), Ai;
+        for (int j = 0; j < N; j++) {
+            std::vector<int> &v = colrows[j];
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            Ai.insert(Ai.end(), v.begin(), v.end());
+            Ap[j + 1] = (int)Ai.size();
+        }
+        std::vector<int> kp(N), kq(N), kr(N + 1);
+        const int rc = ngmi_klu_analyze(N, Ap.data(), Ai.data(), kp.data(), kq.data(), kr.data(), nullptr, nullptr,
+                                        nullptr);
+        if (rc != NGMI_OK) return rc;
+        for (int k = 0; k < N; k++) {
+            rowp[k] = kp[k] + 1;
+            colq[k] = kq[k] + 1;
+        }
     } else {
         std::vector<int> prow(N + 1), prow_inv(N + 1);
         for (int u = 0; u <= N; u++) prow[u] = u;
@@ -1052,6 +1073,13 @@ int ngmi_circuit_set_pivot_order(ngmi_circuit *c, int n, const int *rows, const 
     }
     c->order_rows.assign(rows, rows + n);
     c->order_cols.assign(cols, cols + n);
+    return NGMI_OK;
+}
+
+int ngmi_circuit_use_klu_order(ngmi_circuit *c, int on)
+{
+    if (!c) return NGMI_E_PANIC;
+    c->klu_order = on ? 1 : 0;
     return NGMI_OK;
 }
 

@@ -641,6 +641,24 @@ def get_tune(knob):
     return v
 
 
+def klu_analyze(n, Ap, Ai):
+    """KLU's symbolic analysis restated in the engine (ngmi_klu_analyze, host
+    code): (P, Q, R, nblocks, nzoff, structural_rank), 0-based, for the n x n
+    CSC pattern."""
+    L = lib()
+    vp = ctypes.c_void_p
+    L.ngmi_klu_analyze.argtypes = [ctypes.c_int] + [vp] * 5 + [ctypes.POINTER(ctypes.c_int)] * 3
+    Ap = np.ascontiguousarray(Ap, dtype=np.int32)
+    Ai = np.ascontiguousarray(Ai, dtype=np.int32)
+    P = np.zeros(max(n, 1), np.int32)
+    Q = np.zeros(max(n, 1), np.int32)
+    R = np.zeros(n + 1, np.int32)
+    nb, nz, rk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(L.ngmi_klu_analyze(int(n), Ap.ctypes.data, Ai.ctypes.data, P.ctypes.data, Q.ctypes.data, R.ctypes.data,
+                             ctypes.byref(nb), ctypes.byref(nz), ctypes.byref(rk)), 'klu_analyze')
+    return P[:n], Q[:n], R[:nb.value + 1], nb.value, nz.value, rk.value
+
+
 def guess_features(t4, h, v4, nbv, nbmask, reltol=1e-3, vntol=1e-6, stream=0):
     """The engine's guess feature rows (the device function the in-loop
     features kernel uses) for caller-supplied windows, through
@@ -714,6 +732,13 @@ class Circuit:
                 check(L.ngmi_circuit_add_vsrc(self.h, p, n, dc, arr, len(pulse)), 'add vsrc')
         for node, v in desc.get('ic', {}).items():
             check(L.ngmi_circuit_set_ic(self.h, node, v), 'set ic')
+
+    def use_klu_order(self, on=True):
+        """Static LU on KLU's own ordering, computed by the engine (the
+        restated klu_analyze), instead of the engine's own."""
+        L = lib()
+        L.ngmi_circuit_use_klu_order.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        check(L.ngmi_circuit_use_klu_order(self.h, 1 if on else 0), 'klu order')
 
     def set_pivot_order(self, rows, cols):
         """Pivot sequence of the static LU (1-based unknowns), e.g. KLU's

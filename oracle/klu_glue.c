@@ -89,3 +89,25 @@ int kg_order(void *hp, int *rows, int *cols, int *counts)
     counts[3] = (int)h->sym->nblocks;
     return 0;
 }
+
+/* The symbolic analysis of klu_analyze alone (SMPpreOrder,
+ * klusmp.c:1255-1285): P, Q (0-based, size n) and the block boundaries
+ * R[0..nblocks]; info = {nblocks, nzoff, structural rank}.  Returns 0, or 1
+ * when klu_analyze fails. */
+int kg_symbolic(int n, const int *Ap, const int *Ai, int *P, int *Q, int *R, int *info)
+{
+    void *hp = kg_new(n, Ap, Ai);
+    kg_t *h = (kg_t *)hp;
+    int k;
+    if (!h) return 1;
+    for (k = 0; k < n; k++) {
+        P[k] = h->sym->P[k];
+        Q[k] = h->sym->Q[k];
+    }
+    for (k = 0; k <= h->sym->nblocks; k++) R[k] = h->sym->R[k];
+    info[0] = h->sym->nblocks;
+    info[1] = h->sym->nzoff;
+    info[2] = h->sym->structural_rank;
+    kg_free(hp);
+    return 0;
+}

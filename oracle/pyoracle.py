@@ -393,3 +393,23 @@ def klu_csc(n, Ap, Ai, Ax, b, pivrel=1e-3, refactors=0):
         return x, tf, tr
     finally:
         K.kg_free(h)
+
+
+def klu_symbolic(n, Ap, Ai):
+    """The reference klu_analyze (BTF + AMD, KLU's defaults) on a CSC
+    pattern: returns (P, Q, R, nblocks, nzoff, structural_rank), 0-based."""
+    import numpy as np
+    K = klu_ref()
+    vp = ctypes.c_void_p
+    K.kg_symbolic.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, vp]
+    Ap32 = np.ascontiguousarray(Ap, dtype=np.int32)
+    Ai32 = np.ascontiguousarray(Ai, dtype=np.int32)
+    P = np.zeros(max(n, 1), np.int32)
+    Q = np.zeros(max(n, 1), np.int32)
+    R = np.zeros(n + 1, np.int32)
+    info = np.zeros(3, np.int32)
+    if K.kg_symbolic(int(n), Ap32.ctypes.data, Ai32.ctypes.data, P.ctypes.data, Q.ctypes.data,
+                     R.ctypes.data, info.ctypes.data) != 0:
+        raise RuntimeError('klu_analyze failed')
+    nb = int(info[0])
+    return P[:n], Q[:n], R[:nb + 1], nb, int(info[1]), int(info[2])
