@@ -312,6 +312,12 @@ int analyze(const ngmi_circuit &c0, Plan &P)
      * exact +-1 entries of the source stamp, then a minimum-degree symmetric
      * ordering of the row-paired pattern limits fill. */
     std::vector<int> rowp(N), colq(N);
+    /* step -> diagonal block of KLU's block triangular form (one block
+     * unless the circuit uses KLU's ordering): the LU then factors the
+     * diagonal blocks only and the solve runs block by block, as klu_factor
+     * and klu_solve do (klu_factor.c:15-260, klu_solve.c:14-400) */
+    std::vector<int> blk(N, 0);
+    bool blocks = false;
     if (!c0.order_rows.empty()) {
         if ((int)c0.order_rows.size() != N) return fail("pivot order length differs from the unknown count");
         rowp = c0.order_rows;
@@ -330,13 +336,17 @@ int analyze(const ngmi_circuit &c0, Plan &P)
             Ap[j + 1] = (int)Ai.size();
         }
         std::vector<int> kp(N), kq(N), kr(N + 1);
-        const int rc = ngmi_klu_analyze(N, Ap.data(), Ai.data(), kp.data(), kq.data(), kr.data(), nullptr, nullptr,
+        int nb = 0;
+        const int rc = ngmi_klu_analyze(N, Ap.data(), Ai.data(), kp.data(), kq.data(), kr.data(), &nb, nullptr,
                                         nullptr);
         if (rc != NGMI_OK) return rc;
         for (int k = 0; k < N; k++) {
             rowp[k] = kp[k] + 1;
             colq[k] = kq[k] + 1;
         }
+        for (int b = 0; b < nb; b++)
+            for (int k = kr[b]; k < kr[b + 1]; k++) blk[k] = b;
+        blocks = nb > 1;
     } else {
         std::vector<int> prow(N + 1), prow_inv(N + 1);
         for (int u = 0; u <= N; u++) prow[u] = u;
@@ -381,11 +391,12 @@ int analyze(const ngmi_circuit &c0, Plan &P)
     for (int k = 0; k < N; k++) S[k][k] = 1;
     for (int u = 1; u <= NN; u++)
         if (P.ic_nodes[u]) S[rinv[u]][cinv[u]] = 1;
+    /* fill stays inside the diagonal blocks (no entries lie below them) */
     for (int k = 0; k < N; k++)
         for (int i = k + 1; i < N; i++)
             if (S[i][k])
                 for (int j = k + 1; j < N; j++)
-                    if (S[k][j]) S[i][j] = 1;
+                    if (S[k][j] && blk[j] == blk[k]) S[i][j] = 1;
     std::vector<int> slot((size_t)N * N, -1);
     int nslots = 0;
     for (int i = 0; i < N; i++)
@@ -414,7 +425,8 @@ int analyze(const ngmi_circuit &c0, Plan &P)
             e.t = slot[(size_t)i * N + j];
             e.piv = (i > j) ? slot[(size_t)j * N + j] : -1;
             int L = 0;
-            for (int k = 0; k < m; k++) {
+            /* an entry off the diagonal blocks keeps its assembled value */
+            for (int k = 0; k < m && blk[i] == blk[j]; k++) {
                 if (S[i][k] && S[k][j]) {
                     const int a = slot[(size_t)i * N + k], c2 = slot[(size_t)k * N + j];
                     e.terms.push_back(make_int2(a, c2));
@@ -487,7 +499,53 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         nterms += (int)e.terms.size();
     }
     P.nterms = nterms;
-    {
+    if (blocks) {
+        /* klu_solve's order (klu_solve.c:14-400): blocks from the last; in
+         * block b, y = L_b \ (b_b - A_b,later x_later), then x_b = U_b \ y.
+         * One program: the forward entries (no pivot) of a block wait for
+         * the back-substituted unknowns of later blocks their rows touch. */
+        std::vector<int> lvf(N, -1), lvb(N, -1);
+        int nb = 0;
+        for (int k = 0; k < N; k++) nb = std::max(nb, blk[k] + 1);
+        std::vector<int> bstart(nb + 1, N);
+        for (int k = N - 1; k >= 0; k--) bstart[blk[k]] = k;
+        for (int b = nb - 1; b >= 0; b--) {
+            const int k1 = bstart[b], k2 = bstart[b + 1];
+            for (int a = k1; a < k2; a++) {
+                PEnt r{a, -1, 0, 0, {}};
+                int L = 0;
+                for (int k = k1; k < a; k++)
+                    if (S[a][k]) {
+                        r.terms.push_back(make_int2(slot[(size_t)a * N + k], k));
+                        L = std::max(L, lvf[k] + 1);
+                    }
+                for (int j = k2; j < N; j++)
+                    if (S[a][j]) {
+                        r.terms.push_back(make_int2(slot[(size_t)a * N + j], j));
+                        L = std::max(L, lvb[j] + 1);
+                    }
+                if (r.terms.empty()) continue;
+                r.plev = L;
+                if ((int)r.terms.size() > SPLIT_TERMS) L++;
+                lvf[a] = r.lev = L;
+                be.push_back(std::move(r));
+            }
+            for (int a = k2 - 1; a >= k1; a--) {
+                PEnt r{a, slot[(size_t)a * N + a], 0, 0, {}};
+                int L = lvf[a] + 1;
+                for (int j = a + 1; j < k2; j++)
+                    if (S[a][j]) {
+                        r.terms.push_back(make_int2(slot[(size_t)a * N + j], j));
+                        L = std::max(L, lvb[j] + 1);
+                    }
+                r.plev = L;
+                if ((int)r.terms.size() > SPLIT_TERMS) L++;
+                lvb[a] = r.lev = L;
+                be.push_back(std::move(r));
+            }
+        }
+    }
+    if (!blocks) {
         std::vector<int> lv(N, 0);
         for (int a = 0; a < N; a++) {
             PEnt r{a, -1, 0, 0, {}};
@@ -503,7 +561,7 @@ int analyze(const ngmi_circuit &c0, Plan &P)
             if (!r.terms.empty()) we.push_back(std::move(r));   /* rows without terms change nothing */
         }
     }
-    {
+    if (!blocks) {
         std::vector<int> lv(N, 0);
         for (int a = N - 1; a >= 0; a--) {
             PEnt r{a, slot[(size_t)a * N + a], 0, 0, {}};

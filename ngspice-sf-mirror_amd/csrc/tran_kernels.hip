@@ -1423,6 +1423,10 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                    }
                    const double acc = (kind == PROG_COMBINE) ? add_parts(z[a], r, part)
                                                              : lu_row_terms(z[a], r, BT, lu, z);
+                   if (r.h.y < 0) {   /* a forward row of the block-by-block solve */
+                       z[a] = acc;
+                       return;
+                   }
                    const double piv = lu[r.h.y];
                    if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
                    z[a] = acc / piv;

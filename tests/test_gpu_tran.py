@@ -143,6 +143,36 @@ def test_klu_pivot_order():
         _compare(tag, (te, xe, st), (t, x, T.stats()), desc['nnodes'])
 
 
+def test_klu_order_block_solve():
+    """The engine on KLU's ordering computed by itself (use_klu_order: the
+    restated klu_analyze), factoring only the diagonal blocks of KLU's block
+    triangular form and solving block by block from the last, as klu_factor
+    and klu_solve do, against the oracle solving with the reference KLU:
+    the 101-stage ring (3 blocks) and the adder (69 blocks, 1042 entries
+    off the blocks)."""
+    print('case: KLU order + block solve, 101-stage ring .tran 10p 2n and adder seed 0 .tran 5p 2.5n',
+          flush=True)
+    for tag, desc, tstop in [('ring101-kluorder', workloads.ring_oscillator(tstop=2e-9), 2e-9),
+                             ('adder0-kluorder', workloads.mc_adder(0), 2.5e-9)]:
+        T = po.Transient(desc)
+        if not T.use_reference_klu():
+            pytest.skip('reference KLU (oracle/_ref) not built')
+        rc, t, x = T.run(desc['tstep'], tstop)
+        assert rc == 0, rc
+        circ = ngmi.Circuit(desc)
+        circ.use_klu_order()
+        b = ngmi.Batch([circ, ngmi.Circuit(desc)], max_points=200000)
+        rc, _ = b.tran(desc['tstep'], tstop)
+        assert rc == 0, (rc, ngmi.last_error())
+        info = b.info()
+        print(tag, 'lu', info, flush=True)
+        for c in range(2):
+            st = b.stats(c)
+            assert st['status'] == 0, st
+            te, xe = b.waveform(c, info['unknowns'])
+            _compare(tag, (te, xe, st), (t, x, T.stats()), desc['nnodes'])
+
+
 def test_gear_method():
     """method=gear (NIcomCof's Gear branch, NIintegrate's Gear sum and
     CKTterr's Gear factor) on the engine against the oracle."""

@@ -15,7 +15,9 @@ count, the off-block entry count and the structural rank -- on
   mesh as the oracle builds them for KLU (the pattern SMPpreOrder passes).
 Then the engine's own pivot sequence with use_klu_order equals the pivot
 sequence the reference KLU used in a transient (Numeric->Pnum, Symbolic->Q):
-on these circuits klu_factor keeps the analysis' row order."""
+on these circuits klu_factor keeps the analysis' row order; and the static
+LU on it has KLU's structure: nnz(L) and nnz(U) of the diagonal blocks plus
+the off-block entries, with no fill outside the blocks."""
 import os
 import sys
 
@@ -121,6 +123,13 @@ def test_circuit_patterns_and_pivot_sequence(name, make):
           % (name, n, nb, nzoff, bool((a['rows'] == rows).all() and (a['cols'] == cols).all())))
     assert (a['cols'] == Q + 1).all() and (a['rows'] == P + 1).all()
     assert (a['rows'] == rows).all() and (a['cols'] == cols).all()
+    # the factor keeps to KLU's structure: fill only inside the diagonal
+    # blocks, the off-block entries as they are (klu_factor.c:15-260)
+    total = cnt['lnz'] - n + cnt['unz'] + cnt['nzoff']
+    print('  engine L %d (KLU lnz - n %d), L+U+off %d (KLU %d), levels %s'
+          % (a['l_nonzeros'], cnt['lnz'] - n, a['lu_nonzeros'], total, a['levels']))
+    assert a['l_nonzeros'] == cnt['lnz'] - n
+    assert a['lu_nonzeros'] == total
     c.use_klu_order(False)
     assert not (c.analyze()['rows'] == rows).all() or name == 'ring11'
 
