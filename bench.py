@@ -352,7 +352,9 @@ def bench_rings(args):
     for _ in range(args.warmup):
         rc, passes = b.tran(10e-12, args.tstop, stream=stream)
         log('warmup rc %d passes %d' % (rc, passes))
-    b.set_profile(True)
+    # timed on graphs without per-launch events; kernel durations from one
+    # more, profiled run
+    b.set_profile(False)
     times = []
     for _ in range(args.steps):
         torch.cuda.synchronize()
@@ -366,6 +368,12 @@ def bench_rings(args):
     st = [b.stats(c) for c in range(C)]
     acc = sum(s['accepted'] for s in st)
     its = sum(s['iterations'] for s in st)
+    b.set_profile(True)
+    rc2, _ = b.tran(10e-12, args.tstop, stream=stream)
+    torch.cuda.synchronize()
+    b.set_profile(False)
+    if rc2 != 0:
+        raise RuntimeError('profiled transient failed: %d %s' % (rc2, ngmi.last_error()))
     mos_ms, step_ms, evals, npasses = b.kernel_times()
     log('rings: passes %d, accepted %d, iterations %d, failed %d; kernels: bsim4 %.1f ms, '
         'step %.1f ms, wall %.1f ms' % (passes, acc, its, sum(1 for s in st if s['status']),
@@ -458,13 +466,12 @@ def bench_mesh(args):
 
 
 def bench_c3(args):
-    """Configuration 3 together: the uneven batch of 2048 rings runs its
-    transient on one stream while a host thread runs the 1024 x 1024
-    power-grid mesh's transient (windows of --mesh-window simulated
-    seconds, back to back, each from its operating point) on a second
-    stream for as long as the rings run.  Reports the rings' rate with the
-    mesh alongside and the mesh's accepted timepoints per second in the
-    same window."""
+    """Configuration 3 together: one window (--mesh-window simulated seconds
+    from the operating point) of the 1024 x 1024 power-grid mesh's transient
+    runs on a second stream from a host thread, while the uneven batch of
+    2048 rings repeats its whole transient on the first stream until the
+    mesh window ends.  Reports the rings' rate over that time (beside their
+    rate alone) and the mesh's accepted timepoints per second alongside."""
     import threading
     import torch
     import ngmi
@@ -482,45 +489,50 @@ def bench_c3(args):
     n = lg.info()['unknowns']
     del d
     ms = torch.cuda.Stream()
-    win = args.mesh_window
-    if lg.tran(10e-12, win, 10e-12, stream=ms.cuda_stream) != 0:   # warm-up
-        raise RuntimeError('mesh transient failed: %s' % ngmi.last_error())
-    torch.cuda.synchronize()
-    stop = threading.Event()
-    mesh_pts = [0, 0]   # accepted points, windows finished
-
-    def mesh_loop():
-        while not stop.is_set():
-            if lg.tran(10e-12, win, 10e-12, stream=ms.cuda_stream) != 0:
-                break
-            mesh_pts[0] += lg.stats()['accepted']
-            mesh_pts[1] += 1
-
     rs = torch.cuda.Stream()
-    th = threading.Thread(target=mesh_loop)
+    win = args.mesh_window
+    rc, _ = b.tran(10e-12, args.tstop, stream=rs.cuda_stream)   # warm-up of this batch
+    torch.cuda.synchronize()
+    mesh = {}
+
+    def mesh_window():
+        t0 = time.perf_counter()
+        mesh['rc'] = lg.tran(10e-12, win, 10e-12, stream=ms.cuda_stream)
+        ms.synchronize()
+        mesh['s'] = time.perf_counter() - t0
+        mesh['accepted'] = lg.stats()['accepted']
+
+    th = threading.Thread(target=mesh_window)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     th.start()
-    rc, passes = b.tran(10e-12, args.tstop, stream=rs.cuda_stream)
-    rs.synchronize()   # the rings' stream only: the mesh window in flight is not waited for
+    acc = runs = 0
+    while True:
+        rc, passes = b.tran(10e-12, args.tstop, stream=rs.cuda_stream)
+        rs.synchronize()
+        if rc != 0:
+            break
+        acc += sum(b.stats(c)['accepted'] for c in range(C))
+        runs += 1
+        if not th.is_alive():
+            break
     wall = time.perf_counter() - w0
-    mesh_done = list(mesh_pts)   # mesh windows finished inside the rings' window
-    stop.set()
     th.join()
     if rc != 0:
         raise RuntimeError('transient failed: %d %s' % (rc, ngmi.last_error()))
-    st = [b.stats(c) for c in range(C)]
-    acc = sum(s_['accepted'] for s_ in st)
-    log('c3: rings %.0f tp*c/s with the mesh transient alongside (%.0f alone); mesh %d accepted points in %d '
-        'windows in %.2f s = %.1f timepoints/s alongside' % (acc / wall, alone, mesh_done[0], mesh_done[1], wall,
-                                                            mesh_done[0] / wall))
+    if mesh.get('rc', 1) != 0:
+        raise RuntimeError('mesh transient failed: %s' % ngmi.last_error())
+    log('c3: rings %.0f tp*c/s over %d runs with the mesh transient alongside (%.0f alone); mesh %d accepted '
+        'points in %.2f s = %.1f timepoints/s alongside' % (acc / wall, runs, alone, mesh['accepted'], mesh['s'],
+                                                           mesh['accepted'] / mesh['s']))
     res['metric'] = 'uneven-batch transient timepoints x circuits/s with the power-grid mesh transient alongside'
     res['value'] = acc / wall
-    res['ms_per_step'] = wall * 1e3
-    res['config']['workload'] += ('; alongside: the %dx%d power-grid mesh transient (%d unknowns, 16384 clocked '
-                                  'inverters, windows of %g ps) on a second stream' % (side, side, n, win * 1e12))
-    res['extra'].update({'rings_alone': alone, 'mesh_timepoints_per_s_alongside': mesh_done[0] / wall,
-                         'mesh_accepted_points': mesh_done[0], 'mesh_windows': mesh_done[1],
+    res['ms_per_step'] = wall * 1e3 / max(runs, 1)
+    res['config']['workload'] += ('; alongside: a %g ps window of the %dx%d power-grid mesh transient (%d unknowns, '
+                                  '16384 clocked inverters) on a second stream' % (win * 1e12, side, side, n))
+    res['extra'].update({'rings_alone': alone, 'rings_runs_alongside': runs,
+                         'mesh_timepoints_per_s_alongside': mesh['accepted'] / mesh['s'],
+                         'mesh_accepted_points': mesh['accepted'], 'mesh_window_s': mesh['s'],
                          'accepted_alongside': acc})
     return res
 

@@ -8,10 +8,11 @@
  *   src/spicelib/analysis/traninit.c:17-40   TMAX / delmin
  *   src/spicelib/analysis/dctran.c:66-971    DCtran (the XSPICE build, no
  *                                             event-driven instances)
- *   src/spicelib/analysis/cktop.c:26-116     CKTop (direct Newton only; the
- *                                             gmin/source-stepping fallbacks
- *                                             are not restated and report
- *                                             failure instead)
+ *   src/spicelib/analysis/cktop.c:26-720     CKTop: direct Newton, then
+ *                                             dynamic_gmin / new_gmin /
+ *                                             spice3_gmin and gillespie_src /
+ *                                             spice3_src (optran is not
+ *                                             restated)
  *   src/maths/ni/niiter.c:28-367             NIiter (INITF state machine,
  *                                             >= 2 iterations, maxIter >= 100)
  *   src/maths/ni/nicomcof.c:17-130           NIcomCof (trapezoidal, Gear)
@@ -24,9 +25,11 @@
  *   src/spicelib/devices/vsrc/vsrcload.c:40-170 (PULSE), vsrcacct.c:20-160
  *   src/maths/misc/equality.c:22-60          AlmostEqualUlps
  *
- * The linear solve is a dense LU with partial pivoting on the MNA matrix
- * (the reference uses KLU; results agree to rounding).  Device loads sum into
- * the matrix in device-type order CAP, BSIM4, RES, VSRC as CKTload does.
+ * The linear solve is a dense LU with partial pivoting on the MNA matrix,
+ * or the reference's own KLU (oracle/_ref, or_tran_use_klu) driven as
+ * klusmp.c drives it; above or_sparse_above unknowns the matrix is held in
+ * CSC only.  Device loads sum into the matrix in device-type order CAP,
+ * BSIM4, RES, VSRC as CKTload does.
  */
 #include <math.h>
 #include <stdint.h>
