@@ -326,15 +326,38 @@ int ngmi_circuit_set_op_options(ngmi_circuit *c, int noopiter, int gminsteps, in
  * caller passes Numeric->Pnum[k] + 1 and Symbolic->Q[k] + 1, and every later
  * refactor runs on that sequence as klu_refactor does (klu_refactor.c:
  * 18-482).  n = 0 restores the engine's own ordering (source rows paired
- * with their node rows, then minimum degree).  A batch uses the sequence of
- * its first circuit. */
+ * with their node rows, then minimum degree).  Circuits of one batch with
+ * different sequences (or options) run as separate topology groups. */
 int ngmi_circuit_set_pivot_order(ngmi_circuit *c, int n, const int *rows, const int *cols);
+/* CKTpivotRelTol (.options pivrel, default 1e-3), the tolerance of KLU's
+ * threshold partial pivoting (klu_kernel.c:370-530 via SMPreorder,
+ * klusmp.c:752-806): in (0, 1].  Where NIiter factors afresh -- INITJCT,
+ * the first INITTRAN iteration, the iteration after either, and after a
+ * refactor met a zero pivot (niiter.c:107-195, 335-347) -- the engine tests
+ * whether klu_factor would keep the static pivot sequence (each diagonal
+ * candidate at least pivrel times the column's largest candidate, values
+ * scaled by their row maxima, klu_scale.c:19-159); where it would not, or
+ * a static pivot is zero, the circuit factors with KLU's pivot search on a
+ * dense workspace and refactors on those rows until the next fresh
+ * factorization.  A zero pivot in a refactor is factored afresh in the
+ * same iteration; a zero pivot there is E_SINGULAR. */
+int ngmi_circuit_set_pivrel(ngmi_circuit *c, double pivrel);
+/* optran (optran.c:285-851, set by the 'optran' command, com_optran): when
+ * CKTop's Newton run and its gmin and source stepping all fail, the
+ * operating point is sought as a transient from the state they left, the
+ * sources held at their time-0 values, in steps of at most opstep up to
+ * opfinal (no output), and the transient starts from where it ends.
+ * Defaults: ngspice's own 'optran 1 1 1 100n 10u 0' (init.c:77-93), i.e.
+ * opstep 100 ns, opfinal 10 us; a step above opfinal / 50 becomes
+ * opfinal / 50; opstep = 0 deselects it (the operating point then fails
+ * with the stepping's error).  Supply ramping (opramptime) is not
+ * offered. */
+int ngmi_circuit_set_optran(ngmi_circuit *c, double opstep, double opfinal);
 /* on = 1: without a pivot sequence from the caller, the static LU takes
  * KLU's own ordering, computed by the engine (ngmi_klu_analyze on the
  * circuit's MNA pattern: P as the pivot rows, Q as the pivot columns, what
  * SMPpreOrder's klu_analyze fixes, klusmp.c:1255-1285); on = 0 (the
- * default): the engine's own ordering.  A batch uses its first circuit's
- * choice. */
+ * default): the engine's own ordering. */
 int ngmi_circuit_use_klu_order(ngmi_circuit *c, int on);
 /* KLU's symbolic analysis on the host, restated (no device work): the
  * permutations klu_analyze computes with KLU's defaults -- a block upper
@@ -426,6 +449,14 @@ int ngmi_batch_set_data_callback(ngmi_batch *b, ngmi_data_cb cb, const int *circ
  * recorded points, accepted / rejected timepoints, Newton iterations */
 int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int64_t *accepted,
                      int64_t *rejected, int64_t *iterations);
+/* circuit c's pivoting (ngmi_circuit_set_pivrel): pv = 1 while it
+ * refactors on pivot rows of its own, the counts of pivoted factorizations
+ * (klu_factor) and of refactors on their rows (klu_refactor), and the pivot
+ * sequence in use (rows[k], cols[k]: the unknowns, 1-based, whose row and
+ * column are the k-th pivot; Numeric->Pnum and Symbolic->Q on KLU's
+ * ordering).  rows, cols may be NULL; otherwise n_unknowns long. */
+int ngmi_batch_pivot_stats(const ngmi_batch *b, int c, int *pv, int64_t *factors, int64_t *refactors, int *rows,
+                           int *cols);
 /* waveform of circuit c (CKTdump rows, cktdump.c:33-44): t[n], x[n][nprobe] */
 int ngmi_batch_waveform(const ngmi_batch *b, int c, double *t, double *x, int64_t maxpts,
                         int64_t *npts);

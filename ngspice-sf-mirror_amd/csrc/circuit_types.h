@@ -35,6 +35,10 @@ struct ngmi_circuit {
     std::vector<char> ic_given;
     double reltol = 1e-3, abstol = 1e-12, vntol = 1e-6, chgtol = 1e-14, trtol = 7.0, gmin = 1e-12;
     int method = 1, maxord = 2;
+    double pivrel = 1e-3;   /* CKTpivotRelTol, KLU's pivot tolerance */
+    /* optran's step and final time (ngspice's default 'optran 1 1 1 100n
+     * 10u 0', init.c:77-93); opstep 0 deselects it */
+    double opstep = 1e-7, opfinal = 1e-5;
     /* CKTop options (cktntask.c:94-132 defaults) */
     int noopiter = 0, gminsteps = 1, srcsteps = 1, itl2 = 50;
     double gminfactor = 10.0, gshunt = 0.0;

@@ -403,7 +403,7 @@ struct ngmi_netlist {
     double reltol = 1e-3, abstol = 1e-12, vntol = 1e-6, chgtol = 1e-14, trtol = 7.0, gmin = 1e-12;
     int method = 1, maxord = 2;
     int noopiter = 0, gminsteps = 1, srcsteps = 1, itl2 = 50;
-    double gminfactor = 10.0, gshunt = 0.0;
+    double gminfactor = 10.0, gshunt = 0.0, pivrel = 1e-3;
     double temp = 300.15, tnom = 300.15;
     std::vector<std::pair<int, double>> ic;
     ~ngmi_netlist()
@@ -801,6 +801,7 @@ struct Parser {
                 else if (o == "itl2") nl->itl2 = (int)v;
                 else if (o == "gminfactor") nl->gminfactor = v;
                 else if (o == "gshunt") nl->gshunt = v;
+                else if (o == "pivrel") nl->pivrel = v;
                 else if (o == "temp") nl->temp = v + 273.15;
                 else if (o == "tnom") nl->tnom = v + 273.15;
                 else if (o == "seed" || o == "itl1" || o == "itl4") {}
@@ -980,6 +981,7 @@ ngmi_circuit *ngmi_netlist_circuit(const ngmi_netlist *nl)
     if (ngmi_circuit_set_op_options(c, nl->noopiter, nl->gminsteps, nl->srcsteps, nl->gminfactor,
                                     nl->gshunt, nl->itl2) != NGMI_OK)
         return bail("operating-point options");
+    if (ngmi_circuit_set_pivrel(c, nl->pivrel) != NGMI_OK) return bail("pivrel");
     auto num = [&](const std::string &n) { return nl->index.at(n); };
     for (const Elt &e : nl->elts) {
         int rc = NGMI_OK;

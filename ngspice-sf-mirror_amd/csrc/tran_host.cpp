@@ -62,6 +62,12 @@ constexpr int PROG_INTS = 4 + 4 * PROG_INLINE;
 constexpr int PROG_PLAIN = 0, PROG_PARTIAL = 1, PROG_COMBINE = 2;
 /* entries with more terms run as partial sums over several lanes */
 constexpr int SPLIT_TERMS = 24;
+/* dense workspaces for KLU's pivoting: at most PV_POOL_SLOTS per topology
+ * within PV_POOL_BYTES (one at least), none when one would exceed
+ * PV_SLOT_MAX_BYTES */
+constexpr int PV_POOL_SLOTS = 256;
+constexpr int64_t PV_POOL_BYTES = 512ll << 20;
+constexpr int64_t PV_SLOT_MAX_BYTES = 1ll << 30;
 
 #include "circuit_types.h"
 
@@ -164,7 +170,13 @@ bool same_topology(const ngmi_circuit &a, const ngmi_circuit &b)
         if (a.vs[k].p != b.vs[k].p || a.vs[k].n != b.vs[k].n) return false;
     for (int u = 0; u <= a.nnodes; u++)
         if (a.ic_given[u] != b.ic_given[u]) return false;
-    return true;
+    /* a group runs on its first circuit's options and pivot sequence */
+    if (a.reltol != b.reltol || a.abstol != b.abstol || a.vntol != b.vntol || a.chgtol != b.chgtol ||
+        a.trtol != b.trtol || a.gmin != b.gmin || a.method != b.method || a.maxord != b.maxord ||
+        a.pivrel != b.pivrel || a.opstep != b.opstep || a.opfinal != b.opfinal || a.noopiter != b.noopiter || a.gminsteps != b.gminsteps ||
+        a.srcsteps != b.srcsteps || a.itl2 != b.itl2 || a.gminfactor != b.gminfactor || a.gshunt != b.gshunt)
+        return false;
+    return a.klu_order == b.klu_order && a.order_rows == b.order_rows && a.order_cols == b.order_cols;
 }
 
 /* one matrix contribution: original (row, col) 1-based, code */
@@ -243,6 +255,7 @@ struct Plan {
     std::vector<int> a_dest, r_dest;
     std::vector<int> perm, unk_pos, ic_row, nbr;
     std::vector<int> mos_pos, rmos_pos, diag_slot;
+    std::vector<int> row_start, slot_col;   /* the slots row by row (row-major) */
     int NA = 0, NR = 0;
 };
 
@@ -399,12 +412,17 @@ int analyze(const ngmi_circuit &c0, Plan &P)
                     if (S[k][j] && blk[j] == blk[k]) S[i][j] = 1;
     std::vector<int> slot((size_t)N * N, -1);
     int nslots = 0;
-    for (int i = 0; i < N; i++)
+    P.row_start.assign(N + 1, 0);
+    for (int i = 0; i < N; i++) {
+        P.row_start[i] = nslots;
         for (int j = 0; j < N; j++)
             if (S[i][j]) {
                 slot[(size_t)i * N + j] = nslots++;
+                P.slot_col.push_back(j);
                 if (i > j) P.nnz_l++;
             }
+    }
+    P.row_start[N] = nslots;
     P.nslots = nslots;
 
     /* ---- factor program: entries ordered by m = min(i, j), diagonal first */
@@ -886,6 +904,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.trtol = c0.trtol; D.gmin = c0.gmin;
     D.numGminSteps = c0.gminsteps; D.numSrcSteps = c0.srcsteps; D.dcTrcvMaxIter = c0.itl2;
     D.gminFactor = c0.gminfactor; D.gshunt = c0.gshunt;
+    D.opstep = c0.opstep; D.opfinal = c0.opfinal;
     G->noopiter = c0.noopiter;
     int *ip;
     double *dp;
@@ -927,6 +946,8 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     UP(P.rmos_pos, rmos_pos, int, ip);
     UP(G->probe, probe, int, ip);
     UP(G->nbr_host, nbr, int, ip);
+    UP(P.row_start, row_start, int, ip);
+    UP(P.slot_col, slot_col, int, ip);
 #undef UP
     const int64_t cnm = (int64_t)C * NM, cnc = (int64_t)C * NCAP;
     if (alloc(b, (size_t)C * NVS, &dp) != NGMI_OK) return fail(nullptr);
@@ -982,11 +1003,34 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     if (alloc(b, (size_t)C * b->cap_pts * D.nprobe, &dp) != NGMI_OK) return fail(nullptr);
     D.out_x = dp;
     D.done_count = b->done_count;
+    /* KLU's pivoting: row scale factors and pivot rows per circuit, and a
+     * pool of dense workspaces (the assembled matrix with its right-hand
+     * side, and its elimination), one per circuit factoring with pivots at
+     * the same moment, within a budget of PV_POOL_BYTES */
+    D.pivrel = c0.pivrel;
+    D.pv_ld = N + 1;
+    D.pv_stride = 2 * (int64_t)N * D.pv_ld;
+    {
+        const int64_t per = D.pv_stride * (int64_t)sizeof(double);
+        const int64_t fit = PV_POOL_BYTES / per;
+        D.pv_slots = (int)std::min<int64_t>(std::min<int64_t>(C, PV_POOL_SLOTS), std::max<int64_t>(fit, 1));
+        if (per > PV_SLOT_MAX_BYTES) D.pv_slots = 0;
+    }
+    if (alloc(b, (size_t)C * N, &dp) != NGMI_OK) return fail(nullptr);
+    D.pv_rs = dp;
+    if (alloc(b, (size_t)C * N, &ip) != NGMI_OK) return fail(nullptr);
+    D.pv_row = ip;
+    if (alloc(b, (size_t)std::max(D.pv_slots, 1) * D.pv_stride, &dp) != NGMI_OK) return fail(nullptr);
+    D.pv_pool = dp;
+    if (alloc(b, (size_t)std::max(D.pv_slots, 1), &ip) != NGMI_OK) return fail(nullptr);
+    D.pv_owner = ip;
     /* LDS: lu, z, capacitor and source values, partial sums (doubles), then
-     * the three level tables (ints) */
+     * the three level tables (ints); at least the dense factorization's
+     * scratch (36 N + 12 bytes from the start) */
     G->lds = sizeof(double) * ((size_t)P.nslots + 1 + N + 1 + 2 * (size_t)NCAP + NVS +
                               std::max(P.n_apart + P.n_rpart, P.n_ppart)) +
              sizeof(int) * ((size_t)G->nlev_f + G->nlev_fw + G->nlev_bw + 3);
+    G->lds = std::max<size_t>(G->lds, 36 * (size_t)N + 16);
     if (G->lds > 150 * 1024) return fail("circuit too large for the per-circuit LDS factorisation");
     G->method = c0.method;
     G->maxord = c0.maxord;
@@ -1105,6 +1149,31 @@ int ngmi_circuit_set_op_options(ngmi_circuit *c, int noopiter, int gminsteps, in
     c->gminfactor = gminfactor;
     c->gshunt = gshunt;
     c->itl2 = itl2;
+    return NGMI_OK;
+}
+
+int ngmi_circuit_set_optran(ngmi_circuit *c, double opstep, double opfinal)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (!(opstep >= 0 && opfinal > 0 && opstep <= opfinal)) {
+        set_error("optran step must lie in [0, final time]");
+        return NGMI_E_PARMVAL;
+    }
+    /* com_optran (optran.c:168-172): at most a fiftieth of the final time */
+    if (opstep > opfinal / 50.) opstep = opfinal / 50.;
+    c->opstep = opstep;
+    c->opfinal = opfinal;
+    return NGMI_OK;
+}
+
+int ngmi_circuit_set_pivrel(ngmi_circuit *c, double pivrel)
+{
+    if (!c) return NGMI_E_PANIC;
+    if (!(pivrel > 0.0 && pivrel <= 1.0)) {
+        set_error("pivrel must lie in (0, 1]");
+        return NGMI_E_PARMVAL;
+    }
+    c->pivrel = pivrel;
     return NGMI_OK;
 }
 
@@ -1326,6 +1395,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
             k.method = G->method;
             k.maxOrder = G->maxord;
             k.phase = PH_OP;
+            k.pvre = 1;   /* NIreinit sets NISHOULDREORDER (nireinit.c:49) */
             for (int i = 0; i < 7; i++) k.deltaOld[i] = maxStep;
             k.learned = b->net ? 1 : 0;
             /* CKTop: direct Newton first, or straight to the stepping when
@@ -1365,6 +1435,7 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         NGMI_HIP(hipMemsetAsync(D.mstate, 0, sizeof(double) * NLEV * B4X__COUNT * (size_t)cnm, s));
         NGMI_HIP(hipMemsetAsync(D.cstate, 0, sizeof(double) * NLEV * 2 * (size_t)cnc, s));
         NGMI_HIP(hipMemsetAsync(D.cache, 0, sizeof(double) * B4C__COUNT * (size_t)cnm, s));
+        NGMI_HIP(hipMemsetAsync(D.pv_owner, 0, sizeof(int) * (size_t)std::max(D.pv_slots, 1), s));
         std::vector<double> brk((size_t)G->C * G->NVS, -1.0);
         NGMI_HIP(hipMemcpyAsync(D.vs_brk, brk.data(), brk.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
@@ -1731,6 +1802,34 @@ int ngmi_batch_stats(const ngmi_batch *b, int c, int *status, int64_t *npts, int
     if (accepted) *accepted = k.accepted;
     if (rejected) *rejected = k.rejected;
     if (iterations) *iterations = k.iters;
+    return NGMI_OK;
+}
+
+/* KLU's pivoting in circuit c: pv = 1 while it refactors on pivot rows of
+ * its own (then rows[k] = the unknown whose row is the pivot of the column
+ * solved k-th, cols[k] = that column's unknown, 1-based; rows and cols may
+ * be NULL), and the counts of pivoted factorizations and refactors */
+int ngmi_batch_pivot_stats(const ngmi_batch *b, int c, int *pv, int64_t *factors, int64_t *refactors, int *rows,
+                           int *cols)
+{
+    if (!b || c < 0 || c >= b->C) return NGMI_E_PANIC;
+    CktCtl k;
+    NGMI_HIP(hipMemcpy(&k, b->ctl_of(c), sizeof(k), hipMemcpyDeviceToHost));
+    const TopoGroup &G = b->group_of(c);
+    const int N = G.N;
+    if (pv) *pv = k.pv;
+    if (factors) *factors = k.pvfull;
+    if (refactors) *refactors = k.pvrefac;
+    if (rows || cols) {
+        std::vector<int> pr(N), perm(2 * (size_t)N);
+        NGMI_HIP(hipMemcpy(pr.data(), G.D.pv_row + (size_t)b->where[c].y * N, N * sizeof(int),
+                           hipMemcpyDeviceToHost));
+        NGMI_HIP(hipMemcpy(perm.data(), G.D.perm_unknown, 2 * (size_t)N * sizeof(int), hipMemcpyDeviceToHost));
+        for (int q = 0; q < N; q++) {
+            if (rows) rows[q] = perm[N + (k.pv ? pr[q] : q)];
+            if (cols) cols[q] = perm[q];
+        }
+    }
     return NGMI_OK;
 }
 

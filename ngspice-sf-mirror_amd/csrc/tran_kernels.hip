@@ -19,6 +19,7 @@
  * cktload.c:146-173 (.ic rows in the transient operating point).
  */
 #include <algorithm>
+#include <climits>
 #include "ngmi_internal.h"
 #include "bsim4_eval.h"
 #include "tran_types.h"
@@ -73,7 +74,8 @@ __device__ __forceinline__ void tran_mos_body(const B4Tab &tab, const TranDev &D
     const int c = (int)(i / D.NM);
     const int j = (int)(i - (int64_t)c * D.NM);
     const CktCtl *K = D.ctl + c;
-    if (K->phase >= PH_DONE) return;
+    /* pvhold: the circuit's iteration continues with the loads it has */
+    if (K->phase >= PH_DONE || K->pvhold) return;
     B4Ckt ck;
     /* a circuit whose step starts from the learned guess (set by
      * k_guess_features this pass) limits against its last accepted point */
@@ -671,7 +673,59 @@ __device__ __forceinline__ void fail(const TranDev &D, CktCtl *K, int code)
     atomicAdd(D.done_count, 1);
 }
 
-/* DCtran "resume:" block (dctran.c:459-600, XSPICE branch) and state rotation */
+/* OPtran's set-up (optran.c:297-420): the transient's time scalars are
+ * kept aside; CKTstep = CKTmaxStep = opstep, final time opfinal, its own
+ * breakpoints {0, opfinal}, the first step's delta, INITTRAN */
+__device__ void optran_start(const TranDev &D, CktCtl *K)
+{
+    K->optran = 1;
+    K->phase = PH_TRAN;
+    K->trFinal = K->finalTime;
+    K->trStep = K->step;
+    K->trMax = K->maxStep;
+    K->finalTime = D.opfinal;
+    K->step = K->maxStep = D.opstep;
+    K->delta = fmin(D.opfinal / 100, K->step) / 10;
+    K->time = 0;
+    K->breaks[0] = 0;
+    K->breaks[1] = D.opfinal;
+    K->nbreaks = 2;
+    K->firsttime = 1;
+    K->order = 1;
+    for (int i = 0; i < 7; i++) K->deltaOld[i] = K->maxStep;
+    K->saveDelta = D.opfinal / 50;
+    K->mode = (K->mode & M_UIC) | M_TRAN | M_INITTRAN;
+    K->ag[0] = K->ag[1] = 0;
+}
+
+/* OPtran reached its final time: CKTmaxStep and CKTstep restored, and
+ * DCtran goes on as after a converged CKTop (dctran.c:268-330) with the
+ * scalars it set before it (time 0, the first delta, breakpoints
+ * {0, final time}) */
+__device__ void optran_end(CktCtl *K)
+{
+    K->optran = 0;
+    K->finalTime = K->trFinal;
+    K->step = K->trStep;
+    K->maxStep = K->trMax;
+    K->time = 0;
+    K->delta = fmin(K->finalTime / 100, K->step) / 10;
+    K->breaks[0] = 0;
+    K->breaks[1] = K->finalTime;
+    K->nbreaks = 2;
+    K->firsttime = 1;
+    K->order = 1;
+    for (int i = 0; i < 7; i++) K->deltaOld[i] = K->maxStep;
+    K->saveDelta = K->finalTime / 50;
+    K->mode = (K->mode & M_UIC) | M_TRAN | M_INITTRAN;
+    K->ag[0] = K->ag[1] = 0;
+}
+
+/* DCtran "resume:" block (dctran.c:459-600, XSPICE branch) and state
+ * rotation.  OPtran's (optran.c:490-600) has no 1.9-step rule, and its loop
+ * that throws out passed breakpoints has no final-time guard: once the table
+ * holds only the final time and the time is within CKTminBreak of it,
+ * OPclrBreak changes nothing and the reference would loop; here it ends. */
 __device__ void resume_block(CktCtl *K)
 {
     K->delta = fmin(K->delta, K->maxStep);
@@ -680,15 +734,25 @@ __device__ void resume_block(CktCtl *K)
         K->delta = fmin(K->delta, .1 * fmin(K->saveDelta, K->breaks[1] - K->breaks[0]));
         if (K->firsttime) K->delta /= 10;
     }
-    while ((K->breaks[0] <= K->time + K->minBreak || almost_equal_ulps(K->breaks[0], K->time, 100)) &&
-           K->breaks[0] < K->finalTime)
-        clr_break(K);
-    if (K->time + K->delta > K->breaks[0]) {
-        K->saveDelta = K->delta;
-        K->delta = K->breaks[0] - K->time;
-    } else if (K->time + 1.9 * K->delta > K->breaks[0]) {
-        K->saveDelta = K->delta;
-        K->delta = (K->breaks[0] - K->time) / 2.;
+    if (K->optran) {
+        while ((almost_equal_ulps(K->breaks[0], K->time, 100) || K->breaks[0] <= K->time + K->minBreak) &&
+               !(K->nbreaks == 2 && K->breaks[0] == K->finalTime && K->breaks[1] == K->finalTime))
+            clr_break(K);
+        if (K->time + K->delta > K->breaks[0]) {
+            K->saveDelta = K->delta;
+            K->delta = K->breaks[0] - K->time;
+        }
+    } else {
+        while ((K->breaks[0] <= K->time + K->minBreak || almost_equal_ulps(K->breaks[0], K->time, 100)) &&
+               K->breaks[0] < K->finalTime)
+            clr_break(K);
+        if (K->time + K->delta > K->breaks[0]) {
+            K->saveDelta = K->delta;
+            K->delta = K->breaks[0] - K->time;
+        } else if (K->time + 1.9 * K->delta > K->breaks[0]) {
+            K->saveDelta = K->delta;
+            K->delta = (K->breaks[0] - K->time) / 2.;
+        }
     }
     for (int i = 5; i >= 0; i--) K->deltaOld[i + 1] = K->deltaOld[i];
     K->deltaOld[0] = K->delta;
@@ -733,7 +797,7 @@ __device__ void guess_outcome(CktCtl *K, bool slow)
  * four accepted points of history and is not pausing; true when it will */
 __device__ bool request_guess(CktCtl *K)
 {
-    if (K->learned && K->nhist >= 4 && !K->firsttime && K->phase == PH_TRAN) {
+    if (K->learned && K->nhist >= 4 && !K->firsttime && K->phase == PH_TRAN && !K->optran) {
         if (K->gpause > 0) {
             K->gpause--;
             K->gpaused++;
@@ -1157,6 +1221,229 @@ __device__ __forceinline__ void guess_rows(const TranDev &D, const CktCtl *K, in
     }
 }
 
+/* ---------------- KLU's numeric pivoting ---------------- */
+
+constexpr int PV_OK = 0, PV_SINGULAR = 1, PV_BUSY = 2;
+/* a circuit waits for a free dense workspace at most this many polls; each
+ * holder finishes its factorization without waiting on anything, so the
+ * bound is only reached if the pool were never released */
+constexpr int PV_POLLS = 1 << 22;
+/* the failure status when it is */
+constexpr int PV_NO_WORKSPACE = 108;
+
+/* the largest of one value per lane (ties to the lowest key) with its
+ * index, and a sum, over the one-wavefront workgroup; fixed exchange order */
+__device__ __forceinline__ void wave_argmax(double &v, int &key, int &i)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const double ov = __shfl_xor(v, s);
+        const int ok = __shfl_xor(key, s), oi = __shfl_xor(i, s);
+        if (ov > v || (ov == v && ok < key)) {
+            v = ov;
+            key = ok;
+            i = oi;
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s);
+    return v;
+}
+
+/* klu_factor / klu_refactor for one circuit whose static pivots do not do,
+ * by one wavefront on a dense workspace (the permuted matrix with its
+ * right-hand side, columns in the static order).
+ * full: KLU's threshold partial pivoting with diagonal preference (lpivot,
+ * klu_kernel.c:370-530): in column k the diagonal candidate stays the pivot
+ * when its scaled magnitude is at least pivrel times the largest candidate's,
+ * else the largest is taken (ties to the lowest original row; KLU takes the
+ * first in its column pattern, which lists the column's own entries by
+ * ascending row, so the two can differ on some exact ties) and the
+ * displaced candidate becomes the diagonal of the
+ * column the chosen row was the candidate of (klu_kernel.c:930-945).  The
+ * candidates start as the static rows (Symbolic->P); magnitudes are scaled
+ * by the row maxima (klu_scale.c:19-159 with scale = 2, a zero row counting
+ * 1).  The rows chosen go to pv_row.
+ * !full: the rows in pv_row are refactored as they are (klu_refactor); a
+ * zero pivot there restarts as a full factorization in the same call (the
+ * same-iteration klu_factor of niiter.c:160-195), and *refailed is set.
+ * lu and z hold the assembled slots and right-hand side; the LDS from sm on
+ * (at least 36 N + 12 bytes, lu and z included) is scratch.  The solution of
+ * permuted column k goes to z[k], as the static solve leaves it.  Returns
+ * PV_OK, PV_SINGULAR (a zero or non-finite pivot, E_SINGULAR) or PV_BUSY. */
+__device__ __forceinline__ int pivot_solve(const TranDev &D, CktCtl *K, int c, const double *lu, double *z,
+                                           double *sm, bool full, bool *refailed)
+{
+    __shared__ int s_ws;
+    const int tid = threadIdx.x, N = D.N, ld = D.pv_ld;
+    const uint64_t below = (1ull << tid) - 1ull;
+    if (tid == 0) {
+        int got = -1;
+        for (int poll = 0; got < 0 && poll < PV_POLLS; poll++) {
+            for (int k = 0; k < D.pv_slots && got < 0; k++) {
+                const int w = (c + k) % D.pv_slots;
+                if (atomicCAS(D.pv_owner + w, 0, c + 1) == 0) got = w;
+            }
+            if (got < 0) __builtin_amdgcn_s_sleep(16);
+        }
+        s_ws = got;
+    }
+    __syncthreads();
+    const int ws = s_ws;
+    if (ws < 0) return PV_BUSY;
+    const int64_t nld = (int64_t)N * ld;
+    double *A0 = D.pv_pool + (int64_t)ws * D.pv_stride;   /* the assembled matrix */
+    double *W = A0 + nld;                                   /* its elimination */
+    double *Rs = D.pv_rs + (int64_t)c * N;
+    int *prow_g = D.pv_row + (int64_t)c * N;
+
+    for (int64_t e = tid; e < nld; e += STEP_THREADS) A0[e] = 0.0;
+    __syncthreads();
+    for (int a = tid; a < N; a += STEP_THREADS) {
+        const int s0 = gload(D.row_start + a), s1 = gload(D.row_start + a + 1);
+        double *row = A0 + (int64_t)a * ld;
+        double m = 0.0;
+        for (int s = s0; s < s1; s++) {
+            const double v = lu[s];
+            row[gload(D.slot_col + s)] = v;
+            m = fmax(m, fabs(v));
+        }
+        row[N] = z[a];
+        Rs[a] = m == 0.0 ? 1.0 : m;
+    }
+    __syncthreads();
+    /* scratch over lu, z and what follows: multipliers (later the solution),
+     * the pivot row's values, then the bookkeeping */
+    double *lval = sm, *uval = sm + N, *x = sm;
+    int *Pinv = reinterpret_cast<int *>(sm + 2 * N + 1);   /* row -> its column once pivotal, else
+                                                              -1 - (column it is the candidate of) */
+    int *Pd = Pinv + N;                                      /* column -> diagonal candidate row */
+    int *rl = Pd + N, *cl = rl + N, *prw = cl + N + 1;
+
+    int status;
+    for (;;) {
+        for (int64_t e = tid; e < nld; e += STEP_THREADS) W[e] = A0[e];
+        for (int a = tid; a < N; a += STEP_THREADS) {
+            Pinv[a] = full ? -1 - a : -1;
+            Pd[a] = a;
+            prw[a] = full ? a : prow_g[a];
+        }
+        if (tid == 0) {
+            if (full) K->pvfull++;
+            else K->pvrefac++;
+        }
+        __syncthreads();
+        status = PV_OK;
+        for (int k = 0; k < N; k++) {
+            int p;
+            if (full) {
+                /* the candidates' original rows break ties */
+                double best = -1.0;
+                int bkey = INT_MAX, bi = N;
+                for (int a = tid; a < N; a += STEP_THREADS)
+                    if (Pinv[a] < 0) {
+                        const double v = fabs(W[(int64_t)a * ld + k]) / Rs[a];
+                        const int key = gload(D.perm_unknown + N + a);
+                        if (v > best || (v == best && key < bkey)) {
+                            best = v;
+                            bkey = key;
+                            bi = a;
+                        }
+                    }
+                wave_argmax(best, bkey, bi);
+                const int d = Pd[k];
+                p = bi < N ? bi : d;
+                if (Pinv[d] < 0 && fabs(W[(int64_t)d * ld + k]) / Rs[d] >= D.pivrel * best) p = d;
+            } else {
+                p = prw[k];
+            }
+            const double piv = W[(int64_t)p * ld + k];
+            if (piv == 0.0 || !isfinite(piv)) {
+                status = PV_SINGULAR;
+                break;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                if (full) {
+                    const int d = Pd[k];
+                    if (p != d && Pinv[d] < 0) {
+                        const int kbar = -1 - Pinv[p];
+                        Pd[kbar] = d;
+                        Pinv[d] = -1 - kbar;
+                    }
+                    prw[k] = p;
+                }
+                Pinv[p] = k;
+            }
+            __syncthreads();
+            /* rows still to eliminate with a nonzero in column k, and their
+             * multipliers; the pivot row's nonzeros right of column k */
+            int nr = 0, nc = 0;
+            for (int a0 = 0; a0 < N; a0 += STEP_THREADS) {
+                const int a = a0 + tid;
+                const double xv = (a < N && Pinv[a] < 0) ? W[(int64_t)a * ld + k] : 0.0;
+                const bool on = xv != 0.0;
+                const uint64_t m = __ballot(on);
+                if (on) {
+                    const int q = nr + __popcll(m & below);
+                    rl[q] = a;
+                    lval[q] = xv / piv;
+                }
+                nr += __popcll(m);
+            }
+            const double *pr = W + (int64_t)p * ld;
+            for (int j0 = k + 1; j0 <= N; j0 += STEP_THREADS) {
+                const int j = j0 + tid;
+                const double u = j <= N ? pr[j] : 0.0;
+                const bool on = u != 0.0;
+                const uint64_t m = __ballot(on);
+                if (on) {
+                    const int q = nc + __popcll(m & below);
+                    cl[q] = j;
+                    uval[q] = u;
+                }
+                nc += __popcll(m);
+            }
+            __syncthreads();
+            const int tot = nr * nc;
+            for (int e = tid; e < tot; e += STEP_THREADS) {
+                const int r = e / nc, q = e - r * nc;
+                double *w = W + (int64_t)rl[r] * ld + cl[q];
+                *w = fma(-lval[r], uval[q], *w);
+            }
+            __syncthreads();
+        }
+        if (status == PV_OK || full) break;
+        full = true;
+        *refailed = true;
+        __syncthreads();
+    }
+    if (status == PV_OK) {
+        __syncthreads();
+        for (int k = N - 1; k >= 0; k--) {
+            const double *row = W + (int64_t)prw[k] * ld;
+            double acc = 0.0;
+            for (int j = k + 1 + tid; j < N; j += STEP_THREADS) acc = fma(row[j], x[j], acc);
+            acc = wave_sum(acc);
+            if (tid == 0) x[k] = (row[N] - acc) / row[k];
+            __syncthreads();
+        }
+        if (full)
+            for (int k = tid; k < N; k += STEP_THREADS) prow_g[k] = prw[k];
+        __syncthreads();
+        /* x lies in lu's part of the scratch (nslots >= N), clear of z */
+        for (int k = tid; k < N; k += STEP_THREADS) z[k] = x[k];
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) atomicExch(D.pv_owner + ws, 0);
+    return status;
+}
+
 /* circuit c of the topology D, one workgroup */
 __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
 {
@@ -1164,6 +1451,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     __shared__ int s_flag[4];
     __shared__ int s_act;
     __shared__ int s_guess;
+    __shared__ int s_pvbad;
 
     const int tid = threadIdx.x, nt = blockDim.x;
     CktCtl *K = D.ctl + c;
@@ -1176,15 +1464,18 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
     double *vsv = capq + D.NCAP;
 
     const int mode = K->mode, rb = K->rb, xs = K->xsel;
-    const double tnow = K->time;
+    /* OPtran steps its own time; the sources stay at CKTtime, 0 */
+    const double tnow = K->optran ? 0.0 : K->time;
     const int64_t cnm = (int64_t)D.C * D.NM, cnc = (int64_t)D.C * D.NCAP;
     double *xo = D.x + ((int64_t)xs * D.C + c) * (N + 1);
     double *xn = D.x + ((int64_t)(xs ^ 1) * D.C + c) * (N + 1);
     if (tid < 4) s_flag[tid] = 0;
-    if (tid == 0) {
-        lu[D.nslots] = 0.0;
-        z[N] = 0.0;
-    }
+    if (tid == 0) s_pvbad = 0;
+    /* NIiter factors afresh (SMPreorder, klu_factor) when NISHOULDREORDER
+     * is set, on INITJCT and on the first INITTRAN iteration
+     * (niiter.c:107-146), else refactors on the current pivot rows */
+    const bool reorder = K->pvre || (mode & M_INITJCT) || ((mode & M_INITTRAN) && K->iterno == 0);
+    const bool can_pivot = D.pv_slots > 0;
     if (tid == 0) {
         K->passes++;   /* this circuit's Newton passes (its BSIM4 evaluations: passes x NM) */
         if (K->guess == 1) K->guess = 2;   /* this iteration starts from the learned guess */
@@ -1241,6 +1532,18 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         vsv[k] = (mode & M_TRANOP) ? v * K->srcFact : v;
     }
     __syncthreads();
+
+    /* dense: this iteration factors on pivot rows other than the static
+     * ones (pivot_solve); full: afresh, with KLU's pivot search.  pvhold:
+     * in the last pass the static pivots failed KLU's test or met a zero;
+     * that iteration goes on here, its loads kept (the BSIM4 kernel skipped
+     * the circuit) and the matrix assembled again from them */
+    const bool hold = K->pvhold != 0;
+    const bool dense = can_pivot && (hold || (K->pv && !reorder)), full = hold;
+    if (tid == 0) {
+        lu[D.nslots] = 0.0;
+        z[N] = 0.0;
+    }
 
     /* assembly into the permuted L+U slots and RHS rows.  Each leaf sums at
      * most a few groups of four contributions in element order; entries fed
@@ -1388,49 +1691,110 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         }
     }
     __syncthreads();
+    bool refailed = false;
+    if (dense) {
+        const int r = pivot_solve(D, K, c, lu, z, sm, full, &refailed);
+        if (r == PV_BUSY) {
+            if (tid == 0) fail(D, K, PV_NO_WORKSPACE);
+            return;
+        }
+        if (r == PV_SINGULAR && tid == 0) s_flag[0] = 1;
+        __syncthreads();
+    } else {
+        if (reorder && can_pivot) {
+            /* KLU's row scale factors of the matrix being factored (klu_scale.c:
+             * 19-159, scale = 2): the largest magnitude in each row, 1 for a
+             * zero row */
+            double *Rs = D.pv_rs + (int64_t)c * N;
+            for (int a = tid; a < N; a += nt) {
+                const int s0 = gload(D.row_start + a), s1 = gload(D.row_start + a + 1);
+                double m = 0.0;
+                for (int s = s0; s < s1; s++) m = fmax(m, fabs(lu[s]));
+                Rs[a] = m == 0.0 ? 1.0 : m;
+            }
+        }
 
-    /* LU factorisation, entry-wise Doolittle by dependency levels, then the
-     * two substitutions; see run_levels for the load schedule */
-    const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
-    run_levels(levf, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
-               [&](const Prog &r) {
-                   const int t = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
-                   if (kind == PROG_PARTIAL) {
-                       part[t] = lu_row_terms(0.0, r, FT, lu, lu);
-                       return;
-                   }
-                   const double acc = (kind == PROG_COMBINE) ? add_parts(lu[t], r, part)
-                                                             : lu_row_terms(lu[t], r, FT, lu, lu);
-                   lu[t] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
-               });
-    /* forward substitution (unit L) */
-    const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
-    run_levels(levw, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
-               [&](const Prog &r) {
-                   const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
-                   if (kind == PROG_PARTIAL) part[a] = lu_row_terms(0.0, r, WT, lu, z);
-                   else if (kind == PROG_COMBINE) z[a] = add_parts(z[a], r, part);
-                   else z[a] = lu_row_terms(z[a], r, WT, lu, z);
-               });
-    /* back substitution */
-    const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
-    run_levels(levb, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
-               [&](const Prog &r) {
-                   const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
-                   if (kind == PROG_PARTIAL) {
-                       part[a] = lu_row_terms(0.0, r, BT, lu, z);
-                       return;
-                   }
-                   const double acc = (kind == PROG_COMBINE) ? add_parts(z[a], r, part)
-                                                             : lu_row_terms(z[a], r, BT, lu, z);
-                   if (r.h.y < 0) {   /* a forward row of the block-by-block solve */
-                       z[a] = acc;
-                       return;
-                   }
-                   const double piv = lu[r.h.y];
-                   if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
-                   z[a] = acc / piv;
-               });
+        /* LU factorisation, entry-wise Doolittle by dependency levels, then the
+         * two substitutions; see run_levels for the load schedule */
+        const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
+        run_levels(levf, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
+                   [&](const Prog &r) {
+                       const int t = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                       if (kind == PROG_PARTIAL) {
+                           part[t] = lu_row_terms(0.0, r, FT, lu, lu);
+                           return;
+                       }
+                       const double acc = (kind == PROG_COMBINE) ? add_parts(lu[t], r, part)
+                                                                 : lu_row_terms(lu[t], r, FT, lu, lu);
+                       lu[t] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
+                   });
+        /* forward substitution (unit L) */
+        const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
+        run_levels(levw, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
+                   [&](const Prog &r) {
+                       const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                       if (kind == PROG_PARTIAL) part[a] = lu_row_terms(0.0, r, WT, lu, z);
+                       else if (kind == PROG_COMBINE) z[a] = add_parts(z[a], r, part);
+                       else z[a] = lu_row_terms(z[a], r, WT, lu, z);
+                   });
+        /* back substitution */
+        const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
+        run_levels(levb, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
+                   [&](const Prog &r) {
+                       const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                       if (kind == PROG_PARTIAL) {
+                           part[a] = lu_row_terms(0.0, r, BT, lu, z);
+                           return;
+                       }
+                       const double acc = (kind == PROG_COMBINE) ? add_parts(z[a], r, part)
+                                                                 : lu_row_terms(z[a], r, BT, lu, z);
+                       if (r.h.y < 0) {   /* a forward row of the block-by-block solve */
+                           z[a] = acc;
+                           return;
+                       }
+                       const double piv = lu[r.h.y];
+                       if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
+                       z[a] = acc / piv;
+                   });
+        if (can_pivot && reorder) {
+            /* would klu_factor keep the static rows?  Column k's diagonal stays
+             * the pivot when its scaled magnitude is at least pivrel times every
+             * other candidate's: with x(a, k) = L(a, k) U(k, k), when
+             * pivrel |L(a, k)| Rs[k] <= Rs[a] for each row a below it */
+            __syncthreads();
+            const double *Rs = D.pv_rs + (int64_t)c * N;
+            int bad = 0;
+            for (int a = tid; a < N; a += nt) {
+                const int s0 = gload(D.row_start + a), s1 = gload(D.row_start + a + 1);
+                const double ra = Rs[a];
+                for (int s = s0; s < s1; s++) {
+                    const int k = gload(D.slot_col + s);
+                    if (k >= a) break;
+                    if (D.pivrel * fabs(lu[s]) * Rs[k] > ra) bad = 1;
+                }
+            }
+            if (bad) s_pvbad = 1;
+        }
+        if (can_pivot) {
+            __syncthreads();
+            if (s_flag[0] || s_pvbad) {
+                /* klu_factor would pivot elsewhere, or the refactor met a zero
+                 * pivot: the iteration continues next pass with a dense
+                 * factorization */
+                if (tid == 0) K->pvhold = 1;
+                return;
+            }
+        }
+    }
+    if (tid == 0 && can_pivot && !s_flag[0]) {
+        /* a successful SMPreorder clears NISHOULDREORDER; a refactor that
+         * met a zero pivot and was factored afresh in the same iteration
+         * leaves it set (niiter.c:113-195) */
+        if (reorder) K->pvre = 0;
+        else if (full || refailed) K->pvre = 1;
+        K->pv = dense ? 1 : 0;
+    }
+    if (tid == 0) K->pvhold = 0;
     /* new iterate; NIconvTest (niconv.c:21-90) and the device votes */
     /* unknown u is z[unk_pos[u]]; the loads of a lane's unknowns (and of
      * its device votes) are independent, so each chunk issues them all
@@ -1479,17 +1843,22 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         const int iterno = K->iterno + 1;
         K->iterno = iterno;
         K->iters++;
-        if (s_flag[0]) rc = 102;
-        else if (iterno > 100) rc = 103;
+        if (s_flag[0]) {
+            rc = 102;
+            K->pvre = 1;   /* E_SINGULAR returns with NISHOULDREORDER set */
+            K->pv = 0;
+        } else if (iterno > 100) rc = 103;
         else {
             const int noncon = (s_flag[1] == 0 && iterno != 1) ? s_flag[2] : 1;
             if (m & M_INITFLOAT) {
                 if (noncon == 0) rc = 0;
             } else if (m & M_INITJCT) {
                 m = (m & ~INITF_MASK) | M_INITFIX;
+                K->pvre = 1;   /* niiter.c:335-338 */
             } else if (m & M_INITFIX) {
                 if (noncon == 0) m = (m & ~INITF_MASK) | M_INITFLOAT;
             } else {
+                if ((m & M_INITTRAN) && iterno <= 1) K->pvre = 1;   /* niiter.c:343-347 */
                 m = (m & ~INITF_MASK) | M_INITFLOAT;   /* INITSMSIG, INITTRAN, INITPRED */
             }
         }
@@ -1500,7 +1869,11 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
             int opa = OPA_OK;
             if (K->phase == PH_OP) opa = op_next(D, K, rc);
             if (K->phase == PH_OP && opa != OPA_OK) {
-                if (opa == OPA_FAIL) {
+                if (opa == OPA_FAIL && D.opstep > 0.0) {
+                    /* cktop.c:93-105: OPtran, last (optran.c:297-420) */
+                    optran_start(D, K);
+                    s_act = 10;   /* copy state0 -> state1, then accept */
+                } else if (opa == OPA_FAIL) {
                     fail(D, K, K->status);
                     s_act = ACT_STOP;
                 } else {
@@ -1559,7 +1932,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
             const int rc = s_flag[3];
             if (rc != 0) {
                 K->time = K->time - K->delta;
-                K->rejected++;
+                if (!K->optran) K->rejected++;
                 K->delta = K->delta / 8;
                 if (K->firsttime) K->mode = M_TRAN | M_INITTRAN;
                 K->order = 1;
@@ -1584,7 +1957,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                     else K->delta = nd;
                 } else {
                     K->time = K->time - K->delta;
-                    K->rejected++;
+                    if (!K->optran) K->rejected++;
                     K->delta = nd;
                     s_act = ACT_RETRY;
                 }
@@ -1601,6 +1974,28 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                 act = ACT_ACCEPT;
             }
         }
+    }
+    if (act == ACT_ACCEPT && K->optran) {
+        /* OPtran's nextTime (optran.c:430-490): no output.  Its CKTaccept
+         * runs VSRCaccept at CKTtime 0 into the transient's breakpoints,
+         * which leaves them as the transient's own first accept at time 0
+         * does; that accept runs below, so it is not repeated here. */
+        if (tid == 0) {
+            s_act = ACT_NONE;
+            if (K->time > K->breaks[0]) clr_break(K);
+            if (almost_equal_ulps(K->time, K->finalTime, 100)) {
+                optran_end(K);
+                s_act = 10;
+            } else {
+                resume_block(K);
+                if (loop_top(K)) fail(D, K, 104);
+            }
+        }
+        __syncthreads();
+        if (s_act != 10) return;
+        /* the operating point is found: into the transient, as after CKTop */
+        copy_level(D, c, K->rb, 0, 1);
+        __syncthreads();
     }
     if (act == ACT_ACCEPT) {
         /* nextTime: (dctran.c:352-455) */

@@ -55,6 +55,21 @@ struct CktCtl {
     double opFactor, opOldGmin, opGtarget, opRaise, opConvFact, opStartGmin;
     int opStage, opSub;
     int64_t opIter0;
+    /* KLU's numeric pivoting (niiter.c:107-195 with klu_factor /
+     * klu_refactor): pv = 1 while the circuit's pivot rows
+     * (TranDev::pv_row) differ from the static sequence, which the step
+     * kernel then refactors in dense form; pvre is NIiter's NISHOULDREORDER
+     * (the next iteration factors afresh); pvhold = 1 while an iteration
+     * whose static pivots failed waits a pass for its dense factorization
+     * (its loads kept); pvfull counts pivoted
+     * factorizations, pvrefac refactors on their rows */
+    int pv, pvre, pvhold, pvpad;
+    int64_t pvfull, pvrefac;
+    /* OPtran (optran.c:285-851): optran = 1 while the operating point runs
+     * as a transient; the transient's final time, step and largest step
+     * are kept in trFinal, trStep, trMax meanwhile */
+    int optran, oppad;
+    double trFinal, trStep, trMax;
 };
 
 /* CKTop stages: direct Newton, dynamic_gmin and its final solve, new_gmin
@@ -87,6 +102,7 @@ struct TranDev {
     /* CKTop options: gminsteps, srcsteps (0 or 1), itl2, gminfactor, gshunt */
     int numGminSteps, numSrcSteps, dcTrcvMaxIter;
     double gminFactor, gshunt;
+    double opstep, opfinal;   /* optran's step and final time; opstep 0: deselected */
     const int *diag_slot;     /* [N+1] L+U slot of A(u, u), -1 where A has no such element */
     double *bak_x;            /* [C][N+1] CKTop's saved CKTrhsOld */
     double *bak_s;            /* [C][29 NM + 2 NCAP] CKTop's saved CKTstate0 */
@@ -175,6 +191,23 @@ struct TranDev {
     double2 *gbase;
     int *gcount;
     int gsplit;
+    /* pivoting.  The L+U slots are row-major over the permuted matrix: row
+     * a holds slots [row_start[a], row_start[a+1]), slot s in permuted
+     * column slot_col[s] (ascending).  pivrel is KLU's pivot tolerance
+     * (CKTpivotRelTol).  A circuit that needs pivoting takes one of
+     * pv_slots dense workspaces (pv_owner[k]: 0 free, else circuit + 1) of
+     * pv_stride doubles: the assembled matrix with its right-hand side as
+     * column N, twice (a pristine copy and the working one), rows pv_ld
+     * doubles apart; pv_slots = 0 when the matrices are too large for
+     * dense workspaces (a singular static pivot is then E_SINGULAR). */
+    const int *row_start, *slot_col;
+    double pivrel;
+    double *pv_rs;            /* [C][N] KLU's row scale factors (max |a_ij|, 0 -> 1) */
+    int *pv_row;              /* [C][N] permuted pivot row of each permuted column */
+    double *pv_pool;
+    int *pv_owner;
+    int pv_slots, pv_ld;
+    int64_t pv_stride;
 };
 
 }  // namespace ngmi

@@ -351,6 +351,8 @@ def _tran_protos(L):
     L.ngmi_circuit_set_ic.argtypes = [vp, i, d]
     L.ngmi_circuit_set_options.argtypes = [vp, d, d, d, d, d, d, i, i]
     L.ngmi_circuit_set_op_options.argtypes = [vp, i, i, i, d, d, i]
+    L.ngmi_circuit_set_pivrel.argtypes = [vp, d]
+    L.ngmi_circuit_set_optran.argtypes = [vp, d, d]
     ip = ctypes.POINTER(i)
     L.ngmi_circuit_set_pivot_order.argtypes = [vp, i, ip, ip]
     L.ngmi_circuit_analyze.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
@@ -700,6 +702,10 @@ class Circuit:
         check(L.ngmi_circuit_set_op_options(self.h, op['noopiter'], op['gminsteps'],
                                             op['srcsteps'], op['gminfactor'], op['gshunt'],
                                             op['itl2']), 'operating-point options')
+        check(L.ngmi_circuit_set_pivrel(self.h, o.get('pivrel', 1e-3)), 'pivrel')
+        opt = o.get('optran', (1e-7, 1e-5))
+        check(L.ngmi_circuit_set_optran(self.h, float(opt[0]) if opt else 0.0, float(opt[1]) if opt else 1.0),
+              'optran')
         mos = desc['mos']
         if mos:
             keys = list(self.models)
@@ -850,6 +856,24 @@ class Batch:
         check(L.ngmi_batch_guess_stats(self.h, c, ctypes.byref(g), ctypes.byref(f), ctypes.byref(pz),
                                        ctypes.byref(ms)), 'guess stats')
         return dict(guessed=g.value, fallbacks=f.value, paused=pz.value, guess_ms=ms.value)
+
+    def pivot_stats(self, c):
+        """KLU's pivoting in circuit c: pv (1 while it refactors on pivot
+        rows of its own), the counts of pivoted factorizations and of
+        refactors on their rows, and the pivot sequence in use (1-based
+        unknowns of the k-th pivot's row and column)."""
+        L = lib()
+        i64 = ctypes.c_int64
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.ngmi_batch_pivot_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ctypes.POINTER(i64),
+                                             ctypes.POINTER(i64), ip, ip]
+        n = self.circuit_info(c)['unknowns']
+        rows = np.zeros(n, dtype=np.int32)
+        cols = np.zeros(n, dtype=np.int32)
+        pv, f, r = ctypes.c_int(), i64(), i64()
+        check(L.ngmi_batch_pivot_stats(self.h, c, ctypes.byref(pv), ctypes.byref(f), ctypes.byref(r),
+                                       rows.ctypes.data_as(ip), cols.ctypes.data_as(ip)), 'pivot stats')
+        return dict(pv=pv.value, factors=f.value, refactors=r.value, rows=rows, cols=cols)
 
     def guess_rows(self):
         """network rows of the last run, summed over its passes"""

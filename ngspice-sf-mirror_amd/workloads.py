@@ -124,6 +124,51 @@ def uneven_rings(count=UNEVEN_RINGS, seed=3, tstop=UNEVEN_TSTOP):
     return [ring_oscillator(stages=int(k), tstop=tstop) for k in stages]
 
 
+def ring_with_divider(stages=11, tstop=0.3e-9):
+    """A ring oscillator with a floating 0.3 V source between two extra
+    nodes: node a to ground through 10 uOhm, node b to ground through
+    100 kOhm and to ring node 1 through 10 kOhm.  The rows of the source's
+    two nodes differ by ten orders in scale, so KLU's threshold pivoting
+    (klu_kernel.c:370-530) takes an off-diagonal pivot at every fresh
+    factorization on KLU's own ordering."""
+    d = ring_oscillator(stages=stages, tstop=tstop)
+    a, b = d['nnodes'] + 1, d['nnodes'] + 2
+    d['nnodes'] = b
+    d['vsrc'] = d['vsrc'] + [(a, b, 0.3, None)]
+    d['res'] = [(a, 0, 1e-5), (b, 0, 1e5), (b, 1, 1e4)]
+    return d
+
+
+def inverter_chain(stages=5, cload=1e-15, tstep=10e-12, tstop=1e-9):
+    """CMOS inverters in a chain: node 1 is driven by a source pulsing from
+    0 to 1 V at 0.1 ns (rise and fall 20 ps, width 0.3 ns), stage k drives
+    node k+1, 1 fF from every node to ground, VDD on node stages+2."""
+    vdd = stages + 2
+    mos, caps = [], []
+    for k in range(1, stages + 1):
+        mos.append(('n', k + 1, k, 0, 0, {'w': RING_WN, 'l': RING_L}))
+        mos.append(('p', k + 1, k, vdd, vdd, {'w': RING_WP, 'l': RING_L}))
+        caps.append((k + 1, 0, cload))
+    pulse = (0.0, VDD, 0.1e-9, 20e-12, 20e-12, 0.3e-9, 1.0)
+    return {'nnodes': vdd, 'models': {'n': dict(BASE_NMOS), 'p': dict(BASE_PMOS)}, 'mos': mos,
+            'caps': caps, 'res': [], 'vsrc': [(vdd, 0, VDD, None), (1, 0, 0.0, pulse)], 'ic': {},
+            'tstep': tstep, 'tstop': tstop}
+
+
+def with_floating_node(desc, node=2, gshunt=0.0):
+    """The circuit with one more node tied to `node` by a 2 fF capacitor
+    alone: at DC its row and column are empty and the operating point's
+    matrix is singular (E_SINGULAR) unless .options gshunt puts a
+    conductance on every diagonal; gmin and source stepping then fail and
+    CKTop ends with optran, the transient operating point."""
+    d = dict(desc)
+    f = d['nnodes'] + 1
+    d['nnodes'] = f
+    d['caps'] = list(d['caps']) + [(f, node, 2e-15)]
+    d['options'] = dict(d.get('options', {}), gshunt=gshunt)
+    return d
+
+
 def mirror_adder_topology(bits=16):
     """Node map and transistor list of a ripple-carry adder of 28-transistor
     mirror full adders (carry: 10 T, sum: 14 T, two output inverters).

@@ -18,6 +18,9 @@ typedef struct {
     klu_common common;
     klu_symbolic *sym;
     klu_numeric *num;
+    /* klu_factor calls, those that chose an off-diagonal pivot, klu_refactor
+     * calls and those that met a zero pivot */
+    int nfactor, noffdiag, nrefactor, nrefail;
 } kg_t;
 
 void *kg_new(int n, const int *Ap, const int *Ai)
@@ -40,19 +43,26 @@ void *kg_new(int n, const int *Ap, const int *Ai)
 }
 
 /* full = 1: klu_factor (pivoting); full = 0: klu_refactor, then klu_factor
- * if that fails.  Returns 0, or 102 (E_SINGULAR) / 1 on failure. */
+ * if that fails.  Returns 0, 2 (the refactor failed and the fresh factor
+ * succeeded), or 102 (E_SINGULAR) / 1 on failure. */
 int kg_factor(void *hp, double *Ax, int full, double pivrel)
 {
     kg_t *h = (kg_t *)hp;
+    int refailed = 0;
     h->common.tol = pivrel;
     if (!full && h->num) {
+        h->nrefactor++;
         if (klu_refactor(h->Ap, h->Ai, Ax, h->sym, h->num, &h->common)) return 0;
+        h->nrefail++;
+        refailed = 1;
     }
     if (h->num) klu_free_numeric(&h->num, &h->common);
+    h->nfactor++;
     h->num = klu_factor(h->Ap, h->Ai, Ax, h->sym, &h->common);
     if (!h->num) return h->common.status == KLU_SINGULAR ? 102 : 1;
+    if (h->common.noffdiag > 0) h->noffdiag++;
     if (h->common.status == KLU_SINGULAR) return 102;
-    return 0;
+    return refailed ? 2 : 0;
 }
 
 int kg_solve(void *hp, double *b)
@@ -73,7 +83,10 @@ void kg_free(void *hp)
 /* The pivot sequence of the last klu_factor: rows[k] = Numeric->Pnum[k],
  * cols[k] = Symbolic->Q[k] (0-based), and counts = {lnz, unz, nzoff,
  * nblocks} (klu.h: nnz of L and U including their diagonals, entries of
- * the off-diagonal blocks, BTF blocks).  Returns 0, or 1 before a factor. */
+ * the off-diagonal blocks, BTF blocks), then the klu_factor calls, those
+ * that chose an off-diagonal pivot (Common->noffdiag), the klu_refactor
+ * calls and those that met a zero pivot (8 ints).  Returns 0, or 1 before
+ * a factor. */
 int kg_order(void *hp, int *rows, int *cols, int *counts)
 {
     kg_t *h = (kg_t *)hp;
@@ -87,6 +100,10 @@ int kg_order(void *hp, int *rows, int *cols, int *counts)
     counts[1] = (int)h->num->unz;
     counts[2] = (int)h->num->nzoff;
     counts[3] = (int)h->sym->nblocks;
+    counts[4] = h->nfactor;
+    counts[5] = h->noffdiag;
+    counts[6] = h->nrefactor;
+    counts[7] = h->nrefail;
     return 0;
 }
 

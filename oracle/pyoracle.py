@@ -245,6 +245,14 @@ class Transient:
         if L.or_tran_set_op_options(self.h, op['noopiter'], op['gminsteps'], op['srcsteps'],
                                     op['gminfactor'], op['gshunt'], op['itl2']) != 0:
             raise ValueError('operating-point stepping options not restated')
+        L.or_tran_set_pivrel.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        if L.or_tran_set_pivrel(self.h, float(o.get('pivrel', 1e-3))) != 0:
+            raise ValueError('pivrel out of range')
+        # optran (step, final time); None or step 0 deselects it
+        opt = o.get('optran', (1e-7, 1e-5))
+        L.or_tran_set_optran.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        if L.or_tran_set_optran(self.h, float(opt[0]) if opt else 0.0, float(opt[1]) if opt else 1.0) != 0:
+            raise ValueError('optran parameters out of range')
         self.inst = []
         for key, dn, gn, sn, bn, ip in c['mos']:
             inst = Instance(self.models[key], ip)
@@ -331,13 +339,14 @@ class Transient:
         n = L.or_tran_n(self.h)
         rows = np.zeros(n, dtype=np.int32)
         cols = np.zeros(n, dtype=np.int32)
-        cnt = np.zeros(4, dtype=np.int32)
+        cnt = np.zeros(8, dtype=np.int32)
         ip = ctypes.POINTER(ctypes.c_int)
         if L.or_tran_klu_order(self.h, rows.ctypes.data_as(ip), cols.ctypes.data_as(ip),
                                cnt.ctypes.data_as(ip)) != 0:
             return None
         return rows, cols, dict(lnz=int(cnt[0]), unz=int(cnt[1]), nzoff=int(cnt[2]),
-                                nblocks=int(cnt[3]))
+                                nblocks=int(cnt[3]), factors=int(cnt[4]), offdiag_factors=int(cnt[5]),
+                                refactors=int(cnt[6]), refactor_fails=int(cnt[7]))
 
     def stats(self):
         L = _tran_protos()
@@ -391,6 +400,60 @@ def klu_csc(n, Ap, Ai, Ax, b, pivrel=1e-3, refactors=0):
         if K.kg_solve(h, x.ctypes.data) != 0:
             raise RuntimeError('klu_solve failed')
         return x, tf, tr
+    finally:
+        K.kg_free(h)
+
+
+def klu_pivot(A, P, Q, tol=1e-3, b=None):
+    """The oracle's restatement of klu_factor's pivot choice on a dense
+    matrix A (numpy, n x n) with the symbolic P, Q (0-based): returns
+    (rc, Pnum, x) -- rc 0 or 102 (singular), x the solution of A x = b."""
+    import numpy as np
+    L = lib()
+    vp = ctypes.c_void_p
+    L.or_klu_pivot.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_double, vp, vp, vp]
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    P32 = np.ascontiguousarray(P, dtype=np.int32)
+    Q32 = np.ascontiguousarray(Q, dtype=np.int32)
+    bb = np.ascontiguousarray(np.zeros(n) if b is None else b, dtype=np.float64)
+    Pnum = np.zeros(max(n, 1), np.int32)
+    x = np.zeros(max(n, 1))
+    rc = L.or_klu_pivot(n, A.ctypes.data, P32.ctypes.data, Q32.ctypes.data, tol, bb.ctypes.data,
+                        Pnum.ctypes.data, x.ctypes.data)
+    return rc, Pnum[:n], x[:n]
+
+
+def klu_factor_ref(n, Ap, Ai, Ax, tol=1e-3, b=None):
+    """The reference's klu_analyze + klu_factor (+ klu_solve of b) on a CSC
+    matrix: returns (rc, Pnum, Q, x) with rc 0 or 102, 0-based."""
+    import numpy as np
+    K = klu_ref()
+    vp = ctypes.c_void_p
+    K.kg_new.argtypes = [ctypes.c_int, vp, vp]
+    K.kg_new.restype = vp
+    K.kg_factor.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double]
+    K.kg_solve.argtypes = [vp, vp]
+    K.kg_order.argtypes = [vp, vp, vp, vp]
+    K.kg_free.argtypes = [vp]
+    Ap32 = np.ascontiguousarray(Ap, dtype=np.int32)
+    Ai32 = np.ascontiguousarray(Ai, dtype=np.int32)
+    Ax64 = np.ascontiguousarray(Ax, dtype=np.float64).copy()
+    h = K.kg_new(int(n), Ap32.ctypes.data, Ai32.ctypes.data)
+    if not h:
+        raise RuntimeError('klu_analyze failed')
+    try:
+        rc = K.kg_factor(h, Ax64.ctypes.data, 1, float(tol))
+        rows = np.zeros(max(n, 1), np.int32)
+        cols = np.zeros(max(n, 1), np.int32)
+        cnt = np.zeros(8, np.int32)
+        x = None
+        if rc == 0:
+            K.kg_order(h, rows.ctypes.data, cols.ctypes.data, cnt.ctypes.data)
+            if b is not None:
+                x = np.ascontiguousarray(b, dtype=np.float64).copy()
+                K.kg_solve(h, x.ctypes.data)
+        return rc, rows[:n], cols[:n], x
     finally:
         K.kg_free(h)
 

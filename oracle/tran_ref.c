@@ -71,6 +71,8 @@ typedef struct {
     /* operating point: noopiter, gminsteps, srcsteps, gminfactor, gshunt,
      * itl2 (cktntask.c:94-132 defaults 0, 1, 1, 10, 0, 50) */
     int noOpIter, numGminSteps, numSrcSteps, dcTrcvMaxIter;
+    /* optran's step size and final time (0: optran deselected) */
+    double opstep, opfinal;
     double gminFactor, gshunt;
     /* CKTdiagGmin, CKTsrcFact; hasdiag[u]: A(u, u) exists */
     double diagGmin, srcFact;
@@ -93,6 +95,8 @@ typedef struct {
     int64_t niter, accepted, rejected;
     int noncon_dc_fail;
     int iterno;
+    int reorder;     /* NIiter's NISHOULDREORDER (KLU mode) */
+    double pivrel;   /* CKTpivotRelTol */
     /* optional: the reference's KLU (oracle/_ref) instead of the dense LU */
     void *klu;
     int *Ap, *Ai;
@@ -395,9 +399,13 @@ static int ckt_load(ORTran *T)
     return 0;
 }
 
-/* the reference's KLU on the structural pattern; NIiter reorders (a fresh
- * klu_factor) on INITJCT and on the first INITTRAN iteration (niiter.c:
- * 110-125), refactors otherwise */
+/* the reference's KLU on the structural pattern.  NIiter factors afresh
+ * (klu_factor) while NISHOULDREORDER is set -- it is set by NIreinit, on
+ * INITJCT and the first INITTRAN iteration, after either (niiter.c:335-347)
+ * and by a failed factorization -- and clears it after a successful one;
+ * otherwise it refactors, and a refactor that meets a zero pivot is
+ * factored afresh in the same iteration with the flag left set
+ * (niiter.c:107-195) */
 static int solve_klu(ORTran *T)
 {
     const int n = T->n;
@@ -405,9 +413,15 @@ static int solve_klu(ORTran *T)
     if (!T->sparse)
         for (c = 0; c < n; c++)
             for (p = T->Ap[c]; p < T->Ap[c + 1]; p++) T->Ax[p] = AX(T->Ai[p] + 1, c + 1);
-    full = (T->ck.mode & OR_MODEINITJCT) || ((T->ck.mode & OR_MODEINITTRAN) && T->iterno == 1);
-    c = T->kg_factor(T->klu, T->Ax, full, 1e-3);
-    if (c) return c;
+    if ((T->ck.mode & OR_MODEINITJCT) || ((T->ck.mode & OR_MODEINITTRAN) && T->iterno == 1)) T->reorder = 1;
+    full = T->reorder;
+    c = T->kg_factor(T->klu, T->Ax, full, T->pivrel);
+    if (c == 0 || c == 2) T->reorder = (c == 2);
+    if (c == 2) c = 0;
+    if (c) {
+        T->reorder = 1;
+        return c;
+    }
     for (c = 0; c < n; c++) T->kb[c] = T->rhs[c + 1];
     if (T->kg_solve(T->klu, T->kb)) return 1;
     for (c = 0; c < n; c++) T->rhs[c + 1] = T->kb[c];
@@ -519,11 +533,13 @@ static int ni_iter(ORTran *T, int maxIter)
             if (ck->noncon == 0) return 0;
         } else if (ck->mode & OR_MODEINITJCT) {
             ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFIX;
+            T->reorder = 1;
         } else if (ck->mode & OR_MODEINITFIX) {
             if (ck->noncon == 0) ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
         } else if (ck->mode & OR_MODEINITSMSIG) {
             ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
         } else if (ck->mode & OR_MODEINITTRAN) {
+            if (iterno <= 1) T->reorder = 1;
             ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
         } else if (ck->mode & OR_MODEINITPRED) {
             ck->mode = (ck->mode & ~0x3f00) | OR_MODEINITFLOAT;
@@ -741,6 +757,8 @@ static int spice3_src(ORTran *T, int firstmode, int continuemode)
     return 0;
 }
 
+static int op_tran(ORTran *T, int oldconverged);
+
 static int ckt_op(ORTran *T, int firstmode, int continuemode, int iterlim)
 {
     int converged = 1;
@@ -763,6 +781,8 @@ static int ckt_op(ORTran *T, int firstmode, int continuemode, int iterlim)
     }
     free(ox);
     free(os);
+    /* cktop.c:93-105: the transient operating point, last */
+    if (converged != 0) converged = op_tran(T, converged);
     return converged;
 }
 
@@ -879,6 +899,128 @@ static void rotate_states(ORTran *T)
     int i;
     for (i = T->maxOrder; i >= 0; i--) T->ck.states[i + 1] = T->ck.states[i];
     T->ck.states[0] = temp;
+}
+
+/* OPtran (optran.c:285-851, the XSPICE build with no event instances): the
+ * operating point as a transient from the state the failed stepping left,
+ * sources held at CKTtime (0: the transient has not begun), steps of
+ * opstep (CKTstep = CKTmaxStep during it) up to opfinal on breakpoints of
+ * its own, without output.  Its accepts are CKTaccept's (VSRCaccept at
+ * CKTtime 0 into the transient's breakpoints).  Unlike DCtran's, its
+ * resume block has no 1.9-step rule, and its loop that throws out passed
+ * breakpoints has no final-time guard: with the table down to the final
+ * time and the time within CKTminBreak of it, OPclrBreak changes nothing
+ * and the reference would loop; here the loop ends.  Returns 0, 106
+ * (timestep too small) or oldconverged when deselected. */
+static int op_tran(ORTran *T, int oldconverged)
+{
+    ORCkt *ck = &T->ck;
+    const size_t ns = (size_t)T->nstates;
+    const int uic = ck->mode & OR_MODEUIC;
+    const double prevmax = T->maxStep, prevstep = T->step;
+    double optime = 0, ob[2], olddelta, newdelta;
+    int firsttime = 1, converged, i;
+    if (T->opstep == 0) return oldconverged;
+    T->maxStep = T->step = T->opstep;
+    T->delta = fmin(T->opfinal / 100, T->step) / 10;
+    ob[0] = 0;
+    ob[1] = T->opfinal;
+    ck->order = 1;
+    for (i = 0; i < 7; i++) ck->deltaOld[i] = T->maxStep;
+    ck->delta = T->delta;
+    T->saveDelta = T->opfinal / 50;
+    ck->mode = uic | OR_MODETRAN | OR_MODEINITTRAN;
+    ck->ag[0] = ck->ag[1] = 0;
+    memcpy(ck->states[1], ck->states[0], ns * sizeof(double));
+    for (;;) {
+        /* nextTime: */
+        if (vsrc_accept(T, ck->mode)) return 1;
+        if (optime > ob[0]) {
+            ob[0] = ob[1];
+            ob[1] = T->opfinal;
+        }
+        if (almost_equal_ulps(optime, T->opfinal, 100)) {
+            T->maxStep = prevmax;
+            T->step = prevstep;
+            return 0;
+        }
+        /* resume: */
+        T->delta = fmin(T->delta, T->maxStep);
+        if (almost_equal_ulps(optime, ob[0], 100) || ob[0] - optime <= T->delmin) {
+            ck->order = 1;
+            T->delta = fmin(T->delta, .1 * fmin(T->saveDelta, ob[1] - ob[0]));
+            if (firsttime) T->delta /= 10;
+        }
+        while ((almost_equal_ulps(ob[0], optime, 100) || ob[0] <= optime + T->minBreak) &&
+               !(ob[0] == T->opfinal && ob[1] == T->opfinal)) {
+            ob[0] = ob[1];
+            ob[1] = T->opfinal;
+        }
+        if (optime + T->delta > ob[0]) {
+            T->saveDelta = T->delta;
+            T->delta = ob[0] - optime;
+        }
+        for (i = 5; i >= 0; i--) ck->deltaOld[i + 1] = ck->deltaOld[i];
+        ck->deltaOld[0] = T->delta;
+        rotate_states(T);
+        for (;;) {
+            int accept = 0;
+            olddelta = T->delta;
+            optime += T->delta;
+            ck->delta = T->delta;
+            ck->deltaOld[0] = T->delta;
+            if (com_cof(T)) return 1;
+            converged = ni_iter(T, 10);
+            ck->mode = uic | OR_MODETRAN | OR_MODEINITPRED;
+            if (firsttime) {
+                memcpy(ck->states[2], ck->states[1], ns * sizeof(double));
+                memcpy(ck->states[3], ck->states[1], ns * sizeof(double));
+            }
+            if (converged != 0) {
+                optime -= T->delta;
+                T->delta = T->delta / 8;
+                if (firsttime) ck->mode = uic | OR_MODETRAN | OR_MODEINITTRAN;
+                ck->order = 1;
+            } else if (firsttime) {
+                firsttime = 0;
+                accept = 1;
+            } else {
+                newdelta = T->delta;
+                ckt_trunc(T, &newdelta);
+                if (newdelta > .9 * T->delta) {
+                    if (ck->order == 1 && T->maxOrder > 1) {
+                        newdelta = T->delta;
+                        ck->order = 2;
+                        ckt_trunc(T, &newdelta);
+                        if (newdelta <= 1.05 * T->delta) ck->order = 1;
+                    }
+                    T->delta = newdelta;
+                    accept = 1;
+                } else {
+                    optime -= T->delta;
+                    T->delta = newdelta;
+                }
+            }
+            if (accept) break;
+            if (T->delta <= T->delmin) {
+                if (olddelta > T->delmin) T->delta = T->delmin;
+                else return 106;
+            }
+        }
+    }
+}
+
+/* optran's step size and final time (com_optran, optran.c:75-190: a step
+ * above final / 50 becomes final / 50; step 0 deselects it).  Defaults
+ * 100n and 10u, ngspice's own (init.c:77-93).  Returns 0, or -1 when the
+ * step exceeds the final time. */
+int or_tran_set_optran(ORTran *T, double opstep, double opfinal)
+{
+    if (opstep < 0 || opfinal <= 0 || opstep > opfinal) return -1;
+    if (opstep > opfinal / 50.) opstep = opfinal / 50.;
+    T->opstep = opstep;
+    T->opfinal = opfinal;
+    return 0;
 }
 
 /* traninit.c + dctran.c (XSPICE build, no event instances) */
@@ -1020,6 +1162,7 @@ ORTran *or_tran_new(int nnodes, int nmos, int ncap, int nres, int nvs)
     T->trtol = 7; T->gmin = 1e-12; T->temp = 300.15; T->maxOrder = 2;
     T->noOpIter = 0; T->numGminSteps = 1; T->numSrcSteps = 1; T->dcTrcvMaxIter = 50;
     T->gminFactor = 10; T->gshunt = 0; T->srcFact = 1; T->diagGmin = 0;
+    T->opstep = 1e-7; T->opfinal = 1e-5;
     T->method = OR_TRAPEZOIDAL;
     T->breaks = (double *)calloc(MAXBRK, sizeof(double));
     return T;
@@ -1034,6 +1177,110 @@ int or_tran_set_options(ORTran *T, double reltol, double abstol, double vntol, d
     if (maxord > 6) maxord = 6;
     T->trtol = trtol; T->gmin = gmin; T->method = method; T->maxOrder = maxord; T->uic = uic;
     return 0;
+}
+
+/* CKTpivotRelTol, KLU's pivot tolerance (default 1e-3) */
+int or_tran_set_pivrel(ORTran *T, double pivrel)
+{
+    if (!(pivrel > 0 && pivrel <= 1)) return -1;
+    T->pivrel = pivrel;
+    return 0;
+}
+
+/* KLU's numeric pivot choice, restated: klu_factor's partial pivoting with
+ * diagonal preference (lpivot, klu_kernel.c:370-530; the displaced diagonal
+ * re-logged, klu_kernel.c:930-945) on the row-scaled matrix (klu_scale.c:
+ * 19-159, scale = 2: each row divided by its largest magnitude, a zero row
+ * by 1), columns taken in the order Q with P[k] the initial diagonal
+ * candidate of column Q[k] -- Symbolic->P and Q.  A is dense, row-major
+ * n x n in original (0-based) coordinates.  The elimination is dense and
+ * right-looking: the candidates of column k are its Schur-complement
+ * entries in the rows not yet pivotal, which is what KLU's left-looking
+ * kernel sees for that column.  Among equal largest candidates the lowest
+ * row wins: KLU takes the first in its column pattern, which lists the
+ * column's own entries by ascending row among the rows its depth-first
+ * search adds, so the two differ only on some exact ties.  Pnum[k] gets the
+ * pivot row of column Q[k], x (may be NULL) the solution of A x = b.
+ * Returns 0, or 102 on a zero or non-finite pivot (KLU_SINGULAR with
+ * halt_if_singular). */
+int or_klu_pivot(int n, const double *A, const int *P, const int *Q, double tol, const double *b, int *Pnum,
+                 double *x)
+{
+    double *W = (double *)malloc(sizeof(double) * (size_t)n * (n + 1) + 1);
+    double *Rs = (double *)malloc(sizeof(double) * (size_t)n + 1);
+    int *Pinv = (int *)malloc(sizeof(int) * (size_t)n + 1);   /* row -> its step, else -1 - candidate step */
+    int *Pd = (int *)malloc(sizeof(int) * (size_t)n + 1);     /* step -> diagonal candidate row */
+    const int ld = n + 1;
+    int i, j, k, rc = 0;
+    for (i = 0; i < n; i++) {
+        double m = 0;
+        for (j = 0; j < n; j++) {
+            const double v = A[(size_t)i * n + j];
+            W[(size_t)i * ld + j] = v;
+            if (fabs(v) > m) m = fabs(v);
+        }
+        W[(size_t)i * ld + n] = b ? b[i] : 0.0;
+        Rs[i] = m == 0.0 ? 1.0 : m;
+    }
+    for (k = 0; k < n; k++) {
+        Pd[k] = P[k];
+        Pinv[P[k]] = -1 - k;
+    }
+    for (k = 0; k < n; k++) {
+        const int col = Q[k];
+        const int d = Pd[k];
+        double best = -1.0, piv;
+        int p = -1;
+        for (i = 0; i < n; i++) {
+            double v;
+            if (Pinv[i] >= 0) continue;
+            v = fabs(W[(size_t)i * ld + col]) / Rs[i];
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        if (p < 0) p = d;
+        if (Pinv[d] < 0 && fabs(W[(size_t)d * ld + col]) / Rs[d] >= tol * best) p = d;
+        piv = W[(size_t)p * ld + col];
+        if (piv == 0.0 || !isfinite(piv)) {
+            rc = 102;
+            break;
+        }
+        if (p != d && Pinv[d] < 0) {
+            const int kbar = -1 - Pinv[p];
+            Pd[kbar] = d;
+            Pinv[d] = -1 - kbar;
+        }
+        Pnum[k] = p;
+        Pinv[p] = k;
+        for (i = 0; i < n; i++) {
+            double l;
+            if (Pinv[i] >= 0) continue;
+            l = W[(size_t)i * ld + col];
+            if (l == 0.0) continue;
+            l /= piv;
+            /* the columns still to come, and the right-hand side */
+            for (j = k + 1; j < n; j++) {
+                const int cj = Q[j];
+                W[(size_t)i * ld + cj] -= l * W[(size_t)p * ld + cj];
+            }
+            W[(size_t)i * ld + n] -= l * W[(size_t)p * ld + n];
+        }
+    }
+    if (rc == 0 && x) {
+        for (k = n - 1; k >= 0; k--) {
+            const int p = Pnum[k];
+            double acc = W[(size_t)p * ld + n];
+            for (j = k + 1; j < n; j++) acc -= W[(size_t)p * ld + Q[j]] * x[Q[j]];
+            x[Q[k]] = acc / W[(size_t)p * ld + Q[k]];
+        }
+    }
+    free(W);
+    free(Rs);
+    free(Pinv);
+    free(Pd);
+    return rc;
 }
 
 /* the instance must be set up (defaults) but not yet temperature-resolved */
@@ -1093,6 +1340,8 @@ int or_tran_set_ic(ORTran *T, int node, double v)
 int or_tran_setup(ORTran *T)
 {
     int k;
+    T->reorder = 1;   /* NIreinit (nireinit.c:49) */
+    if (!(T->pivrel > 0)) T->pivrel = 1e-3;
     T->nbr = T->nvs;
     T->n = T->nnodes + T->nbr;
     for (k = 0; k < T->nvs; k++) T->vs[k].br = T->nnodes + 1 + k;
@@ -1263,7 +1512,9 @@ int or_tran_use_dropin(ORTran *T, void *run, void *dl_ctx, void *factor, void *s
 
 /* The reference KLU's pivot sequence from its last klu_factor, as 1-based
  * unknowns (rows[k] = Pnum[k] + 1, cols[k] = Q[k] + 1), and counts = {lnz,
- * unz, nzoff, nblocks}.  Returns 0, or nonzero without a KLU factor. */
+ * unz, nzoff, nblocks, klu_factor calls, those with an off-diagonal pivot,
+ * klu_refactor calls, those that met a zero pivot} (8 ints).  Returns 0, or
+ * nonzero without a KLU factor. */
 int or_tran_klu_order(const ORTran *T, int *rows, int *cols, int *counts)
 {
     int k, rc;
