@@ -11,8 +11,9 @@
  *   src/spicelib/analysis/cktop.c:26-720     CKTop: direct Newton, then
  *                                             dynamic_gmin / new_gmin /
  *                                             spice3_gmin and gillespie_src /
- *                                             spice3_src (optran is not
- *                                             restated)
+ *                                             spice3_src, then OPtran
+ *   src/spicelib/analysis/optran.c:285-851   OPtran (defaults of init.c:
+ *                                             77-93; no supply ramping)
  *   src/maths/ni/niiter.c:28-367             NIiter (INITF state machine,
  *                                             >= 2 iterations, maxIter >= 100)
  *   src/maths/ni/nicomcof.c:17-130           NIcomCof (trapezoidal, Gear)
@@ -27,8 +28,11 @@
  *
  * The linear solve is a dense LU with partial pivoting on the MNA matrix,
  * or the reference's own KLU (oracle/_ref, or_tran_use_klu) driven as
- * klusmp.c drives it; above or_sparse_above unknowns the matrix is held in
- * CSC only.  Device loads sum into the matrix in device-type order CAP,
+ * klusmp.c and niiter.c:107-195 drive it (fresh factorizations while
+ * NISHOULDREORDER is set, refactors otherwise); above or_sparse_above
+ * unknowns the matrix is held in CSC only.  or_klu_pivot restates
+ * klu_factor's pivot choice (klu_kernel.c:370-530, klu_scale.c:19-159) on
+ * its own, for checking against the reference's.  Device loads sum into the matrix in device-type order CAP,
  * BSIM4, RES, VSRC as CKTload does.
  */
 #include <math.h>
