@@ -666,6 +666,12 @@ def learned_guess_run(args, b, probe, stream, plain_acc, plain_its, plain_value)
     ora = None if args.no_cpu_baseline else oracle_waveform(args)
     plain_err = waveform_error(b, probe, ora) if ora else None
     b.set_guess(net)
+    # untimed warm-up runs, as for the plain run: the learned passes are
+    # captured as graphs of their own and the guess buffers are first used
+    for _ in range(max(args.warmup, 1)):
+        rc, _ = b.tran(workloads.ADDER_TSTEP, args.tstop, stream=stream)
+        if rc != 0:
+            raise RuntimeError('learned warm-up run failed: %s' % ngmi.last_error())
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     rc, passes = b.tran(workloads.ADDER_TSTEP, args.tstop, stream=stream)

@@ -54,9 +54,10 @@ const char *ngmi_last_error(void);
 /* Launch-shape knobs (no counterpart in the reference): "eval_waves" and
  * "mos_waves" (1..4) set the occupancy the BSIM4 kernels' register
  * allocation targets for ngmi_bsim4_eval and the batched transient;
- * "streams" (1 or 2, default 2): with 2, a single-topology transient
- * without the learned guess alternates the two halves of its circuits on two
- * streams so one half's step kernel overlaps the other half's BSIM4 kernel;
+ * "streams" (1 or 2, default 2): with 2, a single-topology transient (with
+ * the learned guess only when its network runs fused) alternates the two
+ * halves of its circuits on two streams so one half's step kernel overlaps
+ * the other half's BSIM4 kernel;
  * "specialise" (0 or 1, default 1): compile the default card's selectors as
  * constants when every instance of a set shares them;
  * "order_slack" (0..4, default 1): the engine's own pivot order may pick a

@@ -56,14 +56,7 @@ int ngmi_tune_get(const char *knob)
 int ngmi_tune(const char *knob, int value)
 {
     if (!knob) return NGMI_E_PANIC;
-    int k = -1;
-    if (!std::strcmp(knob, "eval_waves")) k = TUNE_EVAL_WAVES;
-    else if (!std::strcmp(knob, "mos_waves")) k = TUNE_MOS_WAVES;
-    else if (!std::strcmp(knob, "streams")) k = TUNE_STREAMS;
-    else if (!std::strcmp(knob, "specialise")) k = TUNE_SPECIALISE;
-    else if (!std::strcmp(knob, "order_slack")) k = TUNE_ORDER_SLACK;
-    else if (!std::strcmp(knob, "graph")) k = TUNE_GRAPH;
-    else if (!std::strcmp(knob, "guess_fused")) k = TUNE_GUESS_FUSED;
+    const int k = knob_index(knob);
     if (k < 0) {
         set_error("unknown tuning knob '%s'", knob);
         return NGMI_E_BADPARM;
