@@ -69,7 +69,11 @@ const char *ngmi_last_error(void);
  * "guess_fused" (0 or 1, default 1): the learned guess and ngmi_mlp_forward
  * run the network in one fused kernel (weights in registers, activations in
  * LDS) instead of the layer-by-layer kernels; both give the same values bit
- * for bit.
+ * for bit;
+ * "guess_grid" (0..1024, default 64): workgroups of the fused network
+ * kernel, each holding a whole compute unit's registers; 0 = one per
+ * compute unit.  Fewer leave compute units to the other half's BSIM4
+ * kernel on the second stream.
  * Process-wide; defaults are the measured best on MI355X. */
 int ngmi_tune(const char *knob, int value);
 /* current value of a knob, or minus NGMI_E_BADPARM for an unknown name */

@@ -10,7 +10,7 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 1, 1, 1};
+static int g_tune[TUNE__COUNT] = {2, 2, 2, 1, 1, 1, 1, 64};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
 
@@ -40,6 +40,7 @@ static int knob_index(const char *knob)
     if (!std::strcmp(knob, "order_slack")) return TUNE_ORDER_SLACK;
     if (!std::strcmp(knob, "graph")) return TUNE_GRAPH;
     if (!std::strcmp(knob, "guess_fused")) return TUNE_GUESS_FUSED;
+    if (!std::strcmp(knob, "guess_grid")) return TUNE_GUESS_GRID;
     return -1;
 }
 
@@ -62,8 +63,8 @@ int ngmi_tune(const char *knob, int value)
         return NGMI_E_BADPARM;
     }
     const bool flag = k == TUNE_SPECIALISE || k == TUNE_GRAPH || k == TUNE_GUESS_FUSED;
-    const int vmin = (flag || k == TUNE_ORDER_SLACK) ? 0 : 1;
-    const int vmax = (k == TUNE_STREAMS) ? 2 : flag ? 1 : 4;
+    const int vmin = (flag || k == TUNE_ORDER_SLACK || k == TUNE_GUESS_GRID) ? 0 : 1;
+    const int vmax = (k == TUNE_STREAMS) ? 2 : flag ? 1 : (k == TUNE_GUESS_GRID) ? 1024 : 4;
     if (value < vmin || value > vmax) {
         set_error("knob '%s' takes %d..%d", knob, vmin, vmax);
         return NGMI_E_PARMVAL;

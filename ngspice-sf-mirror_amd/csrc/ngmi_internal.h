@@ -24,6 +24,7 @@ enum {
     TUNE_ORDER_SLACK = 4,
     TUNE_GRAPH = 5,
     TUNE_GUESS_FUSED = 6,
+    TUNE_GUESS_GRID = 7,
     TUNE__COUNT
 };
 int tune_get(int knob);
