@@ -358,6 +358,19 @@ int ngmi_circuit_set_pivrel(ngmi_circuit *c, double pivrel);
  * with the stepping's error).  Supply ramping (opramptime) is not
  * offered. */
 int ngmi_circuit_set_optran(ngmi_circuit *c, double opstep, double opfinal);
+/* uic = 1: '.tran ... uic' (TRANuic, traninit.c; MODEUIC).  The operating
+ * point is skipped: CKTic puts the .ic values into the solution vector (0
+ * at the other unknowns, cktic.c:19-80), each BSIM4 instance without its
+ * own icvds/icvgs/icvbs takes them from the .ic values of its terminals
+ * (BSIM4getic, b4getic.c:20-55) and each capacitor its initial voltage
+ * (CAPgetic, capgetic.c:19-35), and CKTop's NIiter is a single load on
+ * that vector (niiter.c:41-46).  The transient starts from the state that
+ * load leaves, with a breakpoint at tstep (dctran.c:509-512); the time-0
+ * point is not output (dctran.c:416-419).  noopiter is not taken under
+ * uic.  The .ic values are no longer forced during the operating point
+ * (cktload.c:118-150 is skipped under MODEUIC).  Default 0; the large-
+ * circuit transient rejects it. */
+int ngmi_circuit_set_uic(ngmi_circuit *c, int uic);
 /* on = 1: without a pivot sequence from the caller, the static LU takes
  * KLU's own ordering, computed by the engine (ngmi_klu_analyze on the
  * circuit's MNA pattern: P as the pivot rows, Q as the pivot columns, what

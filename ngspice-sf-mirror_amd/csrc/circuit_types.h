@@ -39,6 +39,7 @@ struct ngmi_circuit {
     /* optran's step and final time (ngspice's default 'optran 1 1 1 100n
      * 10u 0', init.c:77-93); opstep 0 deselects it */
     double opstep = 1e-7, opfinal = 1e-5;
+    int uic = 0;   /* .tran ... uic: MODEUIC, the .ic values as the initial state */
     /* CKTop options (cktntask.c:94-132 defaults) */
     int noopiter = 0, gminsteps = 1, srcsteps = 1, itl2 = 50;
     double gminfactor = 10.0, gshunt = 0.0;

@@ -536,6 +536,10 @@ ngmi_large *ngmi_large_new(const ngmi_circuit *c, const int *probe, int nprobe, 
             set_error("large_new: .ic values are not supported for a large circuit");
             return nullptr;
         }
+    if (c->uic) {
+        set_error("large_new: .tran uic is not supported for a large circuit");
+        return nullptr;
+    }
     if (c->method == 2 && c->maxord > 2) {
         set_error("large_new: Gear order %d above 2 is not supported for a large circuit", c->maxord);
         return nullptr;

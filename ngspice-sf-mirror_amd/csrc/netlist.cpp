@@ -964,10 +964,6 @@ const char *ngmi_netlist_element_name(const ngmi_netlist *nl, int k)
 ngmi_circuit *ngmi_netlist_circuit(const ngmi_netlist *nl)
 {
     if (!nl) return nullptr;
-    if (nl->uic) {
-        set_error(".tran uic is not supported");
-        return nullptr;
-    }
     ngmi_circuit *c = ngmi_circuit_new(nl->nnodes);
     if (!c) return nullptr;
     auto bail = [&](const std::string &what) -> ngmi_circuit * {
@@ -982,6 +978,7 @@ ngmi_circuit *ngmi_netlist_circuit(const ngmi_netlist *nl)
                                     nl->gshunt, nl->itl2) != NGMI_OK)
         return bail("operating-point options");
     if (ngmi_circuit_set_pivrel(c, nl->pivrel) != NGMI_OK) return bail("pivrel");
+    if (ngmi_circuit_set_uic(c, nl->uic) != NGMI_OK) return bail("uic");
     auto num = [&](const std::string &n) { return nl->index.at(n); };
     for (const Elt &e : nl->elts) {
         int rc = NGMI_OK;

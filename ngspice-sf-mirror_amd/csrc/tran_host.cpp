@@ -84,7 +84,7 @@ struct TopoGroup {
     int nslots = 0, nterms = 0, nlev_f = 0, nlev_fw = 0, nlev_bw = 0;
     std::vector<char> ic_nodes_host;
     std::vector<int> nbr_host;
-    int method = 1, maxord = 2, noopiter = 0;
+    int method = 1, maxord = 2, noopiter = 0, uic = 0;
     ~TopoGroup()
     {
         if (set) ngmi_bsim4_set_free(set);
@@ -173,7 +173,8 @@ bool same_topology(const ngmi_circuit &a, const ngmi_circuit &b)
     /* a group runs on its first circuit's options and pivot sequence */
     if (a.reltol != b.reltol || a.abstol != b.abstol || a.vntol != b.vntol || a.chgtol != b.chgtol ||
         a.trtol != b.trtol || a.gmin != b.gmin || a.method != b.method || a.maxord != b.maxord ||
-        a.pivrel != b.pivrel || a.opstep != b.opstep || a.opfinal != b.opfinal || a.noopiter != b.noopiter || a.gminsteps != b.gminsteps ||
+        a.pivrel != b.pivrel || a.opstep != b.opstep || a.opfinal != b.opfinal || a.uic != b.uic ||
+        a.noopiter != b.noopiter || a.gminsteps != b.gminsteps ||
         a.srcsteps != b.srcsteps || a.itl2 != b.itl2 || a.gminfactor != b.gminfactor || a.gshunt != b.gshunt)
         return false;
     return a.klu_order == b.klu_order && a.order_rows == b.order_rows && a.order_cols == b.order_cols;
@@ -308,9 +309,11 @@ int analyze(const ngmi_circuit &c0, Plan &P)
         rhs_codes[br].push_back(mk(CT_VSRC, k, false));
     }
     /* .ic rows: voltage columns are zeroed in the transient operating point
-     * (cktload.c:146-173); the row must hold no branch-current column */
+     * (cktload.c:146-173); the row must hold no branch-current column.
+     * Under UIC the rows are never forced (the .ic values only seed the
+     * initial vector), so there are none. */
     P.ic_nodes.assign(NN + 1, 0);
-    for (int u = 1; u <= NN; u++) P.ic_nodes[u] = c0.ic_given[u];
+    for (int u = 1; u <= NN; u++) P.ic_nodes[u] = c0.uic ? 0 : c0.ic_given[u];
     for (Elt &e : elts) {
         if (e.r <= NN && P.ic_nodes[e.r]) {
             if (e.c > NN) return fail(".ic on a node with a voltage source is not supported");
@@ -838,14 +841,32 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     /* ---- BSIM4 instances, circuit-major */
     G->set = ngmi_bsim4_set_new();
     for (int c = 0; c < C; c++) {
+        const ngmi_circuit &ck = *ckts[c];
         for (int j = 0; j < NM; j++) {
-            const ngmi_circuit::Mos &mo = ckts[c]->mos[j];
+            const ngmi_circuit::Mos &mo = ck.mos[j];
             std::vector<const char *> nm;
             for (const auto &s : mo.names) nm.push_back(s.c_str());
+            std::vector<double> vals = mo.values;
+            if (c0.uic) {
+                /* BSIM4getic (b4getic.c:32-55): under UIC an instance IC not
+                 * given is the .ic difference of its terminals (CKTrhs after
+                 * CKTic: the .ic values, 0 elsewhere) */
+                auto icv = [&](int u) { return (u > 0 && u <= NN && ck.ic_given[u]) ? ck.ic[u] : 0.0; };
+                const double vs = icv(mo.node[2]);
+                static const char *const icn[3] = {"icvds", "icvgs", "icvbs"};
+                const double icval[3] = {icv(mo.node[0]) - vs, icv(mo.node[1]) - vs, icv(mo.node[3]) - vs};
+                for (int q = 0; q < 3; q++) {
+                    bool given = false;
+                    for (const auto &s : mo.names) given |= (s == icn[q]);
+                    if (!given) {
+                        nm.push_back(icn[q]);
+                        vals.push_back(icval[q]);
+                    }
+                }
+            }
             ngmi_bsim4_model *mp = mo.model;
             const int zero = 0;
-            const int rc = ngmi_bsim4_set_add(G->set, &mp, 1, &zero, 1, nm.data(), (int)nm.size(),
-                                              mo.values.data());
+            const int rc = ngmi_bsim4_set_add(G->set, &mp, 1, &zero, 1, nm.data(), (int)nm.size(), vals.data());
             if (rc < 0) return fail(nullptr);
         }
     }
@@ -853,10 +874,17 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
 
     /* ---- per-circuit element values */
     std::vector<double> cap_c((size_t)C * NCAP), res_g((size_t)C * NRES), vs_par((size_t)C * NVS * VSP);
+    std::vector<double> cap_ic((size_t)C * NCAP, 0.0);
     std::vector<double> ic_val((size_t)C * (NN + 1), 0.0);
     for (int c = 0; c < C; c++) {
         const ngmi_circuit &ck = *ckts[c];
         for (int k = 0; k < NCAP; k++) cap_c[(size_t)c * NCAP + k] = ck.caps[k][2];
+        if (c0.uic) {
+            /* CAPgetic (capgetic.c:19-35) under UIC */
+            auto icv = [&](int u) { return (u > 0 && u <= NN && ck.ic_given[u]) ? ck.ic[u] : 0.0; };
+            for (int k = 0; k < NCAP; k++)
+                cap_ic[(size_t)c * NCAP + k] = icv((int)ck.caps[k][0]) - icv((int)ck.caps[k][1]);
+        }
         for (int k = 0; k < NRES; k++) res_g[(size_t)c * NRES + k] = 1.0 / ck.res[k][2];
         for (int k = 0; k < NVS; k++) {
             double *v = &vs_par[((size_t)c * NVS + k) * VSP];
@@ -906,6 +934,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     D.gminFactor = c0.gminfactor; D.gshunt = c0.gshunt;
     D.opstep = c0.opstep; D.opfinal = c0.opfinal;
     G->noopiter = c0.noopiter;
+    G->uic = c0.uic;
     int *ip;
     double *dp;
     uint32_t *up;
@@ -918,6 +947,7 @@ int build_group(ngmi_batch *b, TopoGroup *G, const std::vector<const ngmi_circui
     UP(vs_nodes, vs_nodes, int, ip);
     UP(ic_nodes, ic_nodes, int, ip);
     UP(cap_c, cap_c, double, dp);
+    UP(cap_ic, cap_ic, double, dp);
     UP(res_g, res_g, double, dp);
     UP(vs_par, vs_par, double, dp);
     UP(ic_val, ic_val, double, dp);
@@ -1166,6 +1196,13 @@ int ngmi_circuit_set_optran(ngmi_circuit *c, double opstep, double opfinal)
     return NGMI_OK;
 }
 
+int ngmi_circuit_set_uic(ngmi_circuit *c, int uic)
+{
+    if (!c) return NGMI_E_PANIC;
+    c->uic = uic ? 1 : 0;
+    return NGMI_OK;
+}
+
 int ngmi_circuit_set_pivrel(ngmi_circuit *c, double pivrel)
 {
     if (!c) return NGMI_E_PANIC;
@@ -1404,7 +1441,12 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
             k.gminCur = D.gmin;
             k.srcFact = 1.0;
             k.opStage = OP_DIRECT;
-            if (G->noopiter) {
+            if (G->uic) {
+                /* .tran uic: CKTop's NIiter is one CKTload on the initial
+                 * conditions (niiter.c:41-46); noopiter's stepping would
+                 * reload on the load's right-hand side, so it is not taken */
+                k.mode |= NGMI_MODEUIC;
+            } else if (G->noopiter) {
                 if (D.numGminSteps == 1) {
                     k.opStage = OP_DYN;
                     k.opFactor = D.gminFactor;
@@ -1432,6 +1474,14 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         NGMI_HIP(hipMemcpyAsync(D.ctl, ctl.data(), sizeof(CktCtl) * G->C, hipMemcpyHostToDevice, s));
         const int64_t cnm = (int64_t)G->C * G->NM, cnc = (int64_t)G->C * G->NCAP;
         NGMI_HIP(hipMemsetAsync(D.x, 0, sizeof(double) * 2 * G->C * (size_t)(G->N + 1), s));
+        if (G->uic) {
+            /* CKTic (cktic.c:19-80): CKTrhs and CKTrhsOld hold the .ic
+             * values at their nodes and 0 elsewhere; the UIC NIiter swaps
+             * them and loads on that vector */
+            NGMI_HIP(hipMemcpy2DAsync(D.x, sizeof(double) * (size_t)(G->N + 1), D.ic_val,
+                                      sizeof(double) * (size_t)(G->NN + 1), sizeof(double) * (size_t)(G->NN + 1),
+                                      (size_t)G->C, hipMemcpyDeviceToDevice, s));
+        }
         NGMI_HIP(hipMemsetAsync(D.mstate, 0, sizeof(double) * NLEV * B4X__COUNT * (size_t)cnm, s));
         NGMI_HIP(hipMemsetAsync(D.cstate, 0, sizeof(double) * NLEV * 2 * (size_t)cnc, s));
         NGMI_HIP(hipMemsetAsync(D.cache, 0, sizeof(double) * B4C__COUNT * (size_t)cnm, s));

@@ -732,7 +732,12 @@ __device__ void resume_block(CktCtl *K)
     if (almost_equal_ulps(K->time, K->breaks[0], 100) || K->breaks[0] - K->time <= K->delmin) {
         K->order = 1;
         K->delta = fmin(K->delta, .1 * fmin(K->saveDelta, K->breaks[1] - K->breaks[0]));
-        if (K->firsttime) K->delta /= 10;
+        if (K->firsttime) {
+            /* dctran.c:509-512: a breakpoint at the first step under UIC;
+             * OPtran never runs under UIC (its operating point succeeds) */
+            if ((K->mode & M_UIC) && !K->optran) set_break(K, K->step);
+            K->delta /= 10;
+        }
     }
     if (K->optran) {
         while ((almost_equal_ulps(K->breaks[0], K->time, 100) || K->breaks[0] <= K->time + K->minBreak) &&
@@ -804,7 +809,7 @@ __device__ bool request_guess(CktCtl *K)
             return false;
         }
         K->guess = 1;
-        K->mode = M_TRAN | M_INITFLOAT;
+        K->mode = (K->mode & M_UIC) | M_TRAN | M_INITFLOAT;
         return true;
     }
     return false;
@@ -1475,6 +1480,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
      * is set, on INITJCT and on the first INITTRAN iteration
      * (niiter.c:107-146), else refactors on the current pivot rows */
     const bool reorder = K->pvre || (mode & M_INITJCT) || ((mode & M_INITTRAN) && K->iterno == 0);
+    const bool uic_op = (mode & M_TRANOP) && (mode & M_UIC);
     const bool can_pivot = D.pv_slots > 0;
     if (tid == 0) {
         K->passes++;   /* this circuit's Newton passes (its BSIM4 evaluations: passes x NM) */
@@ -1490,7 +1496,8 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         if (mode & (M_TRAN | M_AC | M_TRANOP)) {
             const int cond1 = ((mode & MODEDC_MASK) && (mode & M_INITJCT)) ||
                               ((mode & M_UIC) && (mode & M_INITTRAN));
-            const double vcap = cond1 ? 0.0 : xo[p] - xo[q];
+            /* CAPinitCond: 0, or under UIC the .ic difference (capgetic.c) */
+            const double vcap = cond1 ? D.cap_ic[ci] : xo[p] - xo[q];
             double *q0 = D.cstate + ((int64_t)lvl(rb, 0) * 2) * cnc + ci;
             double *q1 = D.cstate + ((int64_t)lvl(rb, 1) * 2) * cnc + ci;
             double *cc0 = D.cstate + ((int64_t)lvl(rb, 0) * 2 + 1) * cnc + ci;
@@ -1677,7 +1684,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         for (int e = tid; e < D.n_icrow; e += nt) {
             const int a = gload(D.ic_row + e);            /* permuted row of an .ic node */
             const int u = gload(D.perm_unknown + N + a);
-            z[a] = gload(D.ic_val + (int64_t)c * (D.NN + 1) + u);
+            z[a] = gload(D.ic_val + (int64_t)c * (D.NN + 1) + u) * K->srcFact;   /* cktload.c:160 */
         }
     }
     const double dgmin = K->diagGmin;
@@ -1691,159 +1698,165 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         }
     }
     __syncthreads();
-    bool refailed = false;
-    if (dense) {
-        const int r = pivot_solve(D, K, c, lu, z, sm, full, &refailed);
-        if (r == PV_BUSY) {
-            if (tid == 0) fail(D, K, PV_NO_WORKSPACE);
-            return;
-        }
-        if (r == PV_SINGULAR && tid == 0) s_flag[0] = 1;
-        __syncthreads();
-    } else {
-        if (reorder && can_pivot) {
-            /* KLU's row scale factors of the matrix being factored (klu_scale.c:
-             * 19-159, scale = 2): the largest magnitude in each row, 1 for a
-             * zero row */
-            double *Rs = D.pv_rs + (int64_t)c * N;
-            for (int a = tid; a < N; a += nt) {
-                const int s0 = gload(D.row_start + a), s1 = gload(D.row_start + a + 1);
-                double m = 0.0;
-                for (int s = s0; s < s1; s++) m = fmax(m, fabs(lu[s]));
-                Rs[a] = m == 0.0 ? 1.0 : m;
-            }
-        }
-
-        /* LU factorisation, entry-wise Doolittle by dependency levels, then the
-         * two substitutions; see run_levels for the load schedule */
-        const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
-        run_levels(levf, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
-                   [&](const Prog &r) {
-                       const int t = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
-                       if (kind == PROG_PARTIAL) {
-                           part[t] = lu_row_terms(0.0, r, FT, lu, lu);
-                           return;
-                       }
-                       const double acc = (kind == PROG_COMBINE) ? add_parts(lu[t], r, part)
-                                                                 : lu_row_terms(lu[t], r, FT, lu, lu);
-                       lu[t] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
-                   });
-        /* forward substitution (unit L) */
-        const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
-        run_levels(levw, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
-                   [&](const Prog &r) {
-                       const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
-                       if (kind == PROG_PARTIAL) part[a] = lu_row_terms(0.0, r, WT, lu, z);
-                       else if (kind == PROG_COMBINE) z[a] = add_parts(z[a], r, part);
-                       else z[a] = lu_row_terms(z[a], r, WT, lu, z);
-                   });
-        /* back substitution */
-        const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
-        run_levels(levb, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
-                   [&](const Prog &r) {
-                       const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
-                       if (kind == PROG_PARTIAL) {
-                           part[a] = lu_row_terms(0.0, r, BT, lu, z);
-                           return;
-                       }
-                       const double acc = (kind == PROG_COMBINE) ? add_parts(z[a], r, part)
-                                                                 : lu_row_terms(z[a], r, BT, lu, z);
-                       if (r.h.y < 0) {   /* a forward row of the block-by-block solve */
-                           z[a] = acc;
-                           return;
-                       }
-                       const double piv = lu[r.h.y];
-                       if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
-                       z[a] = acc / piv;
-                   });
-        if (can_pivot && reorder) {
-            /* would klu_factor keep the static rows?  Column k's diagonal stays
-             * the pivot when its scaled magnitude is at least pivrel times every
-             * other candidate's: with x(a, k) = L(a, k) U(k, k), when
-             * pivrel |L(a, k)| Rs[k] <= Rs[a] for each row a below it */
-            __syncthreads();
-            const double *Rs = D.pv_rs + (int64_t)c * N;
-            int bad = 0;
-            for (int a = tid; a < N; a += nt) {
-                const int s0 = gload(D.row_start + a), s1 = gload(D.row_start + a + 1);
-                const double ra = Rs[a];
-                for (int s = s0; s < s1; s++) {
-                    const int k = gload(D.slot_col + s);
-                    if (k >= a) break;
-                    if (D.pivrel * fabs(lu[s]) * Rs[k] > ra) bad = 1;
-                }
-            }
-            if (bad) s_pvbad = 1;
-        }
-        if (can_pivot) {
-            __syncthreads();
-            if (s_flag[0] || s_pvbad) {
-                /* klu_factor would pivot elsewhere, or the refactor met a zero
-                 * pivot: the iteration continues next pass with a dense
-                 * factorization */
-                if (tid == 0) K->pvhold = 1;
+    /* NIiter under UIC in the operating point (niiter.c:41-46): the load
+     * above and nothing else, no solve, no iteration counted */
+    if (!uic_op) {
+        bool refailed = false;
+        if (dense) {
+            const int r = pivot_solve(D, K, c, lu, z, sm, full, &refailed);
+            if (r == PV_BUSY) {
+                if (tid == 0) fail(D, K, PV_NO_WORKSPACE);
                 return;
             }
-        }
-    }
-    if (tid == 0 && can_pivot && !s_flag[0]) {
-        /* a successful SMPreorder clears NISHOULDREORDER; a refactor that
-         * met a zero pivot and was factored afresh in the same iteration
-         * leaves it set (niiter.c:113-195) */
-        if (reorder) K->pvre = 0;
-        else if (full || refailed) K->pvre = 1;
-        K->pv = dense ? 1 : 0;
-    }
-    if (tid == 0) K->pvhold = 0;
-    /* new iterate; NIconvTest (niconv.c:21-90) and the device votes */
-    /* unknown u is z[unk_pos[u]]; the loads of a lane's unknowns (and of
-     * its device votes) are independent, so each chunk issues them all
-     * before it waits */
-    if (tid == 0) xn[0] = 0.0;
-    int conv = 0;
-    for (int u0 = tid + 1; u0 <= N; u0 += CONV_CHUNK * nt) {
-        int pos[CONV_CHUNK];
-        double old[CONV_CHUNK];
-#pragma unroll
-        for (int k = 0; k < CONV_CHUNK; k++) {
-            const int u = u0 + k * nt;
-            if (u <= N) {
-                pos[k] = gload(D.unk_pos + u);
-                old[k] = xo[u];
+            if (r == PV_SINGULAR && tid == 0) s_flag[0] = 1;
+            __syncthreads();
+        } else {
+            if (reorder && can_pivot) {
+                /* KLU's row scale factors of the matrix being factored (klu_scale.c:
+                 * 19-159, scale = 2): the largest magnitude in each row, 1 for a
+                 * zero row */
+                double *Rs = D.pv_rs + (int64_t)c * N;
+                for (int a = tid; a < N; a += nt) {
+                    const int s0 = gload(D.row_start + a), s1 = gload(D.row_start + a + 1);
+                    double m = 0.0;
+                    for (int s = s0; s < s1; s++) m = fmax(m, fabs(lu[s]));
+                    Rs[a] = m == 0.0 ? 1.0 : m;
+                }
+            }
+
+            /* LU factorisation, entry-wise Doolittle by dependency levels, then the
+             * two substitutions; see run_levels for the load schedule */
+            const uint4 *FT = reinterpret_cast<const uint4 *>(D.f_term);
+            run_levels(levf, reinterpret_cast<const int4 *>(D.f_ent), D.nlev_f, tid, nt,
+                       [&](const Prog &r) {
+                           const int t = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                           if (kind == PROG_PARTIAL) {
+                               part[t] = lu_row_terms(0.0, r, FT, lu, lu);
+                               return;
+                           }
+                           const double acc = (kind == PROG_COMBINE) ? add_parts(lu[t], r, part)
+                                                                     : lu_row_terms(lu[t], r, FT, lu, lu);
+                           lu[t] = (r.h.y >= 0) ? acc / lu[r.h.y] : acc;
+                       });
+            /* forward substitution (unit L) */
+            const uint4 *WT = reinterpret_cast<const uint4 *>(D.fw_term);
+            run_levels(levw, reinterpret_cast<const int4 *>(D.fw_ent), D.nlev_fw, tid, nt,
+                       [&](const Prog &r) {
+                           const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                           if (kind == PROG_PARTIAL) part[a] = lu_row_terms(0.0, r, WT, lu, z);
+                           else if (kind == PROG_COMBINE) z[a] = add_parts(z[a], r, part);
+                           else z[a] = lu_row_terms(z[a], r, WT, lu, z);
+                       });
+            /* back substitution */
+            const uint4 *BT = reinterpret_cast<const uint4 *>(D.bw_term);
+            run_levels(levb, reinterpret_cast<const int4 *>(D.bw_ent), D.nlev_bw, tid, nt,
+                       [&](const Prog &r) {
+                           const int a = r.h.x & 0xffff, kind = (r.h.x >> 20) & 3;
+                           if (kind == PROG_PARTIAL) {
+                               part[a] = lu_row_terms(0.0, r, BT, lu, z);
+                               return;
+                           }
+                           const double acc = (kind == PROG_COMBINE) ? add_parts(z[a], r, part)
+                                                                     : lu_row_terms(z[a], r, BT, lu, z);
+                           if (r.h.y < 0) {   /* a forward row of the block-by-block solve */
+                               z[a] = acc;
+                               return;
+                           }
+                           const double piv = lu[r.h.y];
+                           if (piv == 0.0 || !isfinite(piv)) s_flag[0] = 1;   /* E_SINGULAR */
+                           z[a] = acc / piv;
+                       });
+            if (can_pivot && reorder) {
+                /* would klu_factor keep the static rows?  Column k's diagonal stays
+                 * the pivot when its scaled magnitude is at least pivrel times every
+                 * other candidate's: with x(a, k) = L(a, k) U(k, k), when
+                 * pivrel |L(a, k)| Rs[k] <= Rs[a] for each row a below it */
+                __syncthreads();
+                const double *Rs = D.pv_rs + (int64_t)c * N;
+                int bad = 0;
+                for (int a = tid; a < N; a += nt) {
+                    const int s0 = gload(D.row_start + a), s1 = gload(D.row_start + a + 1);
+                    const double ra = Rs[a];
+                    for (int s = s0; s < s1; s++) {
+                        const int k = gload(D.slot_col + s);
+                        if (k >= a) break;
+                        if (D.pivrel * fabs(lu[s]) * Rs[k] > ra) bad = 1;
+                    }
+                }
+                if (bad) s_pvbad = 1;
+            }
+            if (can_pivot) {
+                __syncthreads();
+                if (s_flag[0] || s_pvbad) {
+                    /* klu_factor would pivot elsewhere, or the refactor met a zero
+                     * pivot: the iteration continues next pass with a dense
+                     * factorization */
+                    if (tid == 0) K->pvhold = 1;
+                    return;
+                }
             }
         }
-#pragma unroll
-        for (int k = 0; k < CONV_CHUNK; k++) {
-            const int u = u0 + k * nt;
-            if (u > N) continue;
-            const double nw = z[pos[k]];
-            xn[u] = nw;
-            if (isnan(nw)) { conv = 1; continue; }
-            const double tol = D.reltol * fmax(fabs(old[k]), fabs(nw)) + (u <= D.NN ? D.vntol : D.abstol);
-            if (fabs(nw - old[k]) > tol) conv = 1;
+        if (tid == 0 && can_pivot && !s_flag[0]) {
+            /* a successful SMPreorder clears NISHOULDREORDER; a refactor that
+             * met a zero pivot and was factored afresh in the same iteration
+             * leaves it set (niiter.c:113-195) */
+            if (reorder) K->pvre = 0;
+            else if (full || refailed) K->pvre = 1;
+            K->pv = dense ? 1 : 0;
         }
-    }
-    int vote = 0;
-    const int *nc = D.noncon + (int64_t)c * D.NM;
-    for (int j0 = tid; j0 < D.NM; j0 += CONV_CHUNK * nt) {
-        int v[CONV_CHUNK];
-#pragma unroll
-        for (int k = 0; k < CONV_CHUNK; k++) v[k] = (j0 + k * nt < D.NM) ? gload(nc + j0 + k * nt) : 0;
-#pragma unroll
-        for (int k = 0; k < CONV_CHUNK; k++) vote |= v[k];
-    }
-    if (vote) s_flag[1] = 1;
-    if (conv) s_flag[2] = 1;
-    __syncthreads();
+        if (tid == 0) K->pvhold = 0;
+        /* new iterate; NIconvTest (niconv.c:21-90) and the device votes */
+        /* unknown u is z[unk_pos[u]]; the loads of a lane's unknowns (and of
+         * its device votes) are independent, so each chunk issues them all
+         * before it waits */
+        if (tid == 0) xn[0] = 0.0;
+        int conv = 0;
+        for (int u0 = tid + 1; u0 <= N; u0 += CONV_CHUNK * nt) {
+            int pos[CONV_CHUNK];
+            double old[CONV_CHUNK];
+    #pragma unroll
+            for (int k = 0; k < CONV_CHUNK; k++) {
+                const int u = u0 + k * nt;
+                if (u <= N) {
+                    pos[k] = gload(D.unk_pos + u);
+                    old[k] = xo[u];
+                }
+            }
+    #pragma unroll
+            for (int k = 0; k < CONV_CHUNK; k++) {
+                const int u = u0 + k * nt;
+                if (u > N) continue;
+                const double nw = z[pos[k]];
+                xn[u] = nw;
+                if (isnan(nw)) { conv = 1; continue; }
+                const double tol = D.reltol * fmax(fabs(old[k]), fabs(nw)) + (u <= D.NN ? D.vntol : D.abstol);
+                if (fabs(nw - old[k]) > tol) conv = 1;
+            }
+        }
+        int vote = 0;
+        const int *nc = D.noncon + (int64_t)c * D.NM;
+        for (int j0 = tid; j0 < D.NM; j0 += CONV_CHUNK * nt) {
+            int v[CONV_CHUNK];
+    #pragma unroll
+            for (int k = 0; k < CONV_CHUNK; k++) v[k] = (j0 + k * nt < D.NM) ? gload(nc + j0 + k * nt) : 0;
+    #pragma unroll
+            for (int k = 0; k < CONV_CHUNK; k++) vote |= v[k];
+        }
+        if (vote) s_flag[1] = 1;
+        if (conv) s_flag[2] = 1;
+        __syncthreads();
+    }   /* !uic_op */
 
     /* NIiter bookkeeping (niiter.c:95-365); thread 0 owns the scalars */
     if (tid == 0) {
         int m = K->mode;
         int rc = -1;   /* -1: keep iterating */
-        const int iterno = K->iterno + 1;
+        const int iterno = K->iterno + (uic_op ? 0 : 1);
         K->iterno = iterno;
-        K->iters++;
-        if (s_flag[0]) {
+        if (!uic_op) K->iters++;
+        if (uic_op) {
+            rc = 0;
+        } else if (s_flag[0]) {
             rc = 102;
             K->pvre = 1;   /* E_SINGULAR returns with NISHOULDREORDER set */
             K->pv = 0;
@@ -1886,7 +1899,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                     K->order = 1;
                     for (int i = 0; i < 7; i++) K->deltaOld[i] = K->maxStep;
                     K->saveDelta = K->finalTime / 50;
-                    K->mode = M_TRAN | M_INITTRAN;
+                    K->mode = (K->mode & M_UIC) | M_TRAN | M_INITTRAN;
                     K->ag[0] = K->ag[1] = 0;
                     s_act = 10;   /* copy state0 -> state1, then accept */
                 }
@@ -1896,7 +1909,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                 K->guess = 0;
                 K->fallbacks++;
                 guess_outcome(K, true);
-                K->mode = M_TRAN | M_INITPRED;
+                K->mode = (K->mode & M_UIC) | M_TRAN | M_INITPRED;
                 K->iterno = 0;
                 s_act = ACT_STOP;
             } else {
@@ -1905,7 +1918,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                     K->guessed++;
                     guess_outcome(K, rc != 0 || iterno > 2);
                 }
-                K->mode = M_TRAN | M_INITPRED;
+                K->mode = (K->mode & M_UIC) | M_TRAN | M_INITPRED;
                 s_act = K->firsttime ? 20 : 21;   /* 20: copy state1 -> state2, state3 first */
                 s_flag[3] = rc;
             }
@@ -1934,7 +1947,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
                 K->time = K->time - K->delta;
                 if (!K->optran) K->rejected++;
                 K->delta = K->delta / 8;
-                if (K->firsttime) K->mode = M_TRAN | M_INITTRAN;
+                if (K->firsttime) K->mode = (K->mode & M_UIC) | M_TRAN | M_INITTRAN;
                 K->order = 1;
                 s_act = ACT_RETRY;
             } else if (K->firsttime) {
@@ -2011,9 +2024,11 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         }
         __syncthreads();
         if (s_act == ACT_STOP) return;
-        /* CKTdump: rhsOld of the accepted point */
+        /* CKTdump: rhsOld of the accepted point; under UIC not at time 0
+         * (dctran.c:416-419) */
         const double *xold = D.x + ((int64_t)K->xsel * D.C + c) * (N + 1);
-        const int64_t np = K->npts;
+        const bool dump = !((K->mode & M_UIC) && K->time <= 0.0);
+        const int64_t np = dump ? K->npts : D.cap_pts;
         if (np < D.cap_pts) {
             double *row = D.out_x + ((int64_t)c * D.cap_pts + np) * D.nprobe;
             for (int k = tid; k < D.nprobe; k += nt) row[k] = xold[D.probe[k]];
@@ -2025,7 +2040,7 @@ __device__ __forceinline__ void tran_step_body(const TranDev &D, int c)
         }
         __syncthreads();
         if (tid == 0) {
-            K->npts = np + 1;
+            if (dump) K->npts = np + 1;
             if (K->learned) {
                 K->ht[K->hpos] = K->time;
                 K->hpos = (K->hpos + 1) & 3;

@@ -114,6 +114,7 @@ struct TranDev {
     const int *ic_nodes;      /* [NN+1] 1 where the node has an .ic */
     /* per-circuit element values */
     const double *cap_c;      /* [C][NCAP] */
+    const double *cap_ic;     /* [C][NCAP] CAPinitCond: under UIC the .ic difference, else 0 */
     const double *res_g;      /* [C][NRES] */
     const double *vs_par;     /* [C][NVS][VSP]: dc, pulse flag, coefficient count, 7 coefficients */
     double *vs_brk;           /* [C][NVS] break_time */

@@ -353,6 +353,7 @@ def _tran_protos(L):
     L.ngmi_circuit_set_op_options.argtypes = [vp, i, i, i, d, d, i]
     L.ngmi_circuit_set_pivrel.argtypes = [vp, d]
     L.ngmi_circuit_set_optran.argtypes = [vp, d, d]
+    L.ngmi_circuit_set_uic.argtypes = [vp, ctypes.c_int]
     ip = ctypes.POINTER(i)
     L.ngmi_circuit_set_pivot_order.argtypes = [vp, i, ip, ip]
     L.ngmi_circuit_analyze.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
@@ -706,6 +707,7 @@ class Circuit:
         opt = o.get('optran', (1e-7, 1e-5))
         check(L.ngmi_circuit_set_optran(self.h, float(opt[0]) if opt else 0.0, float(opt[1]) if opt else 1.0),
               'optran')
+        check(L.ngmi_circuit_set_uic(self.h, int(o.get('uic', 0))), 'uic')
         mos = desc['mos']
         if mos:
             keys = list(self.models)

@@ -169,6 +169,24 @@ def with_floating_node(desc, node=2, gshunt=0.0):
     return d
 
 
+def rc_charge(v=1.0, r=1e3, c=1e-12, ic=None, uic=False, tstep=20e-12, tstop=5e-9):
+    """A source of v volts on node 1 charging c through r into node 2;
+    ic: {node: volts} for .ic; uic: .tran ... uic."""
+    return {'nnodes': 2, 'models': {}, 'mos': [], 'caps': [(2, 0, c)], 'res': [(1, 2, r)],
+            'vsrc': [(1, 0, v, None)], 'ic': dict(ic or {}), 'tstep': tstep, 'tstop': tstop,
+            'options': {'uic': int(uic)}}
+
+
+def with_uic(desc, ic=None):
+    """desc under .tran ... uic, with its .ic values replaced by ic when
+    given"""
+    d = dict(desc)
+    if ic is not None:
+        d['ic'] = dict(ic)
+    d['options'] = dict(desc.get('options', {}), uic=1)
+    return d
+
+
 def mirror_adder_topology(bits=16):
     """Node map and transistor list of a ripple-carry adder of 28-transistor
     mirror full adders (carry: 10 T, sum: 14 T, two output inverters).
@@ -275,12 +293,13 @@ def deck(desc, title='engine workload'):
     ics = ' '.join('v(%d)=%r' % (k, float(v)) for k, v in desc.get('ic', {}).items())
     if ics:
         out.append('.ic ' + ics)
-    o = desc.get('options', {})
+    o = {k: v for k, v in desc.get('options', {}).items() if k not in ('uic', 'optran')}
     if o:
         out.append('.options ' + ' '.join('%s=%s' % (k, ('gear' if v == 2 else 'trap')
                                                       if k == 'method' else repr(v))
                                           for k, v in o.items()))
-    out.append('.tran %r %r' % (float(desc['tstep']), float(desc['tstop'])))
+    out.append('.tran %r %r%s' % (float(desc['tstep']), float(desc['tstop']),
+                                  ' uic' if desc.get('options', {}).get('uic') else ''))
     out.append('.end')
     return '\n'.join(out) + '\n'
 

@@ -25,6 +25,9 @@
  *   src/spicelib/devices/res/resload.c:16-38
  *   src/spicelib/devices/vsrc/vsrcload.c:40-170 (PULSE), vsrcacct.c:20-160
  *   src/maths/misc/equality.c:22-60          AlmostEqualUlps
+ *   src/spicelib/analysis/cktic.c:19-80, devices/bsim4/b4getic.c:32-55,
+ *   devices/cap/capgetic.c:19-35, niiter.c:41-46   .tran ... uic (CKTic,
+ *                                             DEVsetic, the single load)
  *
  * The linear solve is a dense LU with partial pivoting on the MNA matrix,
  * or the reference's own KLU (oracle/_ref, or_tran_use_klu) driven as
@@ -64,6 +67,7 @@ typedef struct {
     ORInst **mos;
     int *cap_p, *cap_n, *cap_q;
     double *cap_c;
+    double *cap_ic;           /* CAPinitCond: the .ic difference under UIC (CAPgetic), else 0 */
     int *res_p, *res_n;
     double *res_g;
     ORVsrc *vs;
@@ -334,7 +338,7 @@ static int ckt_load(ORTran *T)
         for (k = T->ncap - 1; k >= 0; k--) {
             const int p = T->cap_p[k], q = T->cap_n[k], qc = T->cap_q[k];
             double vcap, geq, ceq;
-            vcap = cond1 ? 0.0 : T->rhsOld[p] - T->rhsOld[q];
+            vcap = cond1 ? T->cap_ic[k] : T->rhsOld[p] - T->rhsOld[q];
             if (mode & (OR_MODETRAN | OR_MODEAC)) {
                 if (mode & OR_MODEINITPRED) {
                     ck->states[0][qc] = ck->states[1][qc];
@@ -392,10 +396,10 @@ static int ckt_load(ORTran *T)
                 }
             }
             if (currents) {
-                T->rhs[node] = 1.0e10 * T->ic[node];
+                T->rhs[node] = 1.0e10 * T->ic[node] * T->srcFact;
                 AX(node, node) += 1.0e10;
             } else {
-                T->rhs[node] = T->ic[node];
+                T->rhs[node] = T->ic[node] * T->srcFact;   /* cktload.c:160 */
                 AX(node, node) = 1.0;
             }
         }
@@ -518,6 +522,12 @@ static int ni_iter(ORTran *T, int maxIter)
     ORCkt *ck = &T->ck;
     int iterno = 0;
     if (maxIter < 100) maxIter = 100;
+    if ((ck->mode & OR_MODETRANOP) && (ck->mode & OR_MODEUIC)) {
+        /* niiter.c:41-46: under UIC the operating point is one load on the
+         * initial conditions, no solve */
+        double *t = T->rhs; T->rhs = T->rhsOld; T->rhsOld = t;
+        return ckt_load(T) ? 1 : 0;
+    }
     for (;;) {
         int rc;
         ck->noncon = 0;
@@ -769,7 +779,9 @@ static int ckt_op(ORTran *T, int firstmode, int continuemode, int iterlim)
     double *ox = (double *)malloc(sizeof(double) * (T->n + 1));
     double *os = (double *)malloc(sizeof(double) * (T->nstates + 1));
     T->ck.mode = firstmode;
-    if (!T->noOpIter) converged = ni_iter(T, iterlim);
+    /* under UIC noopiter is not taken: the stepping's NIiter calls would
+     * each swap the previous load's right-hand side in as the iterate */
+    if (!T->noOpIter || (firstmode & OR_MODEUIC)) converged = ni_iter(T, iterlim);
     /* cktop.c:54-91 (the frontend's dyngmin variable unset) */
     if (converged != 0 && T->numGminSteps >= 1) {
         if (T->numGminSteps == 1) {
@@ -1028,6 +1040,27 @@ int or_tran_set_optran(ORTran *T, double opstep, double opfinal)
 }
 
 /* traninit.c + dctran.c (XSPICE build, no event instances) */
+/* CKTic (cktic.c:19-80) with DEVsetic under UIC: CKTrhs = 0 but for the
+ * .ic nodes, where CKTrhs = CKTrhsOld = the value; BSIM4getic
+ * (b4getic.c:32-55) takes an instance's icvds/icvgs/icvbs not given from
+ * its terminals' CKTrhs, CAPgetic (capgetic.c:19-35) each capacitor's
+ * initial voltage */
+static void or_uic_setic(ORTran *T)
+{
+    int u, k;
+    memset(T->rhs, 0, sizeof(double) * (T->n + 1));
+    for (u = 1; u <= T->nnodes; u++)
+        if (T->ic_given[u]) T->rhs[u] = T->rhsOld[u] = T->ic[u];
+    for (k = 0; k < T->nmos; k++) {
+        ORInst *h = T->mos[k];
+        const double vs = T->rhs[h->node[N_s]];
+        if (!h->igiven[BI_icVDS]) h->ip[BI_icVDS] = T->rhs[h->node[N_d]] - vs;
+        if (!h->igiven[BI_icVGS]) h->ip[BI_icVGS] = T->rhs[h->node[N_g]] - vs;
+        if (!h->igiven[BI_icVBS]) h->ip[BI_icVBS] = T->rhs[h->node[N_b]] - vs;
+    }
+    for (k = 0; k < T->ncap; k++) T->cap_ic[k] = T->rhs[T->cap_p[k]] - T->rhs[T->cap_n[k]];
+}
+
 int or_tran_run(ORTran *T, double tstep, double tstop, double tmax)
 {
     ORCkt *ck = &T->ck;
@@ -1054,6 +1087,7 @@ int or_tran_run(ORTran *T, double tstep, double tstop, double tmax)
     ck->order = 1;
     for (i = 0; i < T->nvs; i++) T->vs[i].break_time = -1.0;
     memset(T->rhsOld, 0, sizeof(double) * (T->n + 1));
+    if (T->uic) or_uic_setic(T);
     converged = ckt_op(T, (T->uic ? OR_MODEUIC : 0) | OR_MODETRANOP | OR_MODEINITJCT,
                        (T->uic ? OR_MODEUIC : 0) | OR_MODETRANOP | OR_MODEINITFLOAT, 100);
     if (converged) { T->noncon_dc_fail = 1; return converged; }
@@ -1070,14 +1104,19 @@ int or_tran_run(ORTran *T, double tstep, double tstop, double tmax)
         if (vsrc_accept(T, ck->mode)) return 1;
         if (T->time > T->breaks[0]) clr_break(T);
         T->accepted++;
-        if (dump(T)) return 8;
+        /* dctran.c:416-419: no time-0 point under UIC */
+        if (!(T->uic && T->time <= 0) && dump(T)) return 8;
         if (almost_equal_ulps(T->time, T->finalTime, 100)) return 0;
         /* resume: */
         T->delta = fmin(T->delta, T->maxStep);
         if (almost_equal_ulps(T->time, T->breaks[0], 100) || T->breaks[0] - T->time <= T->delmin) {
             ck->order = 1;
             T->delta = fmin(T->delta, .1 * fmin(T->saveDelta, T->breaks[1] - T->breaks[0]));
-            if (firsttime) T->delta /= 10;
+            if (firsttime) {
+                /* dctran.c:509-512 */
+                if (T->uic && set_break(T, T->step)) return 1;
+                T->delta /= 10;
+            }
         }
         while ((T->breaks[0] <= T->time + T->minBreak ||
                 almost_equal_ulps(T->breaks[0], T->time, 100)) &&
@@ -1156,6 +1195,7 @@ ORTran *or_tran_new(int nnodes, int nmos, int ncap, int nres, int nvs)
     T->cap_n = (int *)calloc((size_t)ncap + 1, sizeof(int));
     T->cap_q = (int *)calloc((size_t)ncap + 1, sizeof(int));
     T->cap_c = (double *)calloc((size_t)ncap + 1, sizeof(double));
+    T->cap_ic = (double *)calloc((size_t)ncap + 1, sizeof(double));
     T->res_p = (int *)calloc((size_t)nres + 1, sizeof(int));
     T->res_n = (int *)calloc((size_t)nres + 1, sizeof(int));
     T->res_g = (double *)calloc((size_t)nres + 1, sizeof(double));
@@ -1547,7 +1587,7 @@ void or_tran_output(const ORTran *T, double *t, double *x)
 void or_tran_free(ORTran *T)
 {
     if (!T) return;
-    free(T->mos); free(T->cap_p); free(T->cap_n); free(T->cap_q); free(T->cap_c);
+    free(T->mos); free(T->cap_p); free(T->cap_n); free(T->cap_q); free(T->cap_c); free(T->cap_ic);
     free(T->res_p); free(T->res_n); free(T->res_g); free(T->vs); free(T->ic_given);
     free(T->ic); free(T->stbuf); free(T->hasdiag); free(T->A); free(T->lu); free(T->piv); free(T->rhs);
     free(T->rhsOld); free(T->out_t); free(T->out_x);

@@ -147,3 +147,13 @@ vdd a 0 1
     vals = [e[3] for e in ngmi.Netlist(deck).elements()][:6]
     expect = [1e6, 1e-3, 25.4e-6, 500.0, 1.0, 25.0]
     assert all(math.isclose(v, e, rel_tol=1e-14) for v, e in zip(vals, expect)), vals
+
+
+def test_tran_uic_reaches_the_circuit():
+    """'.tran ... uic' sets MODEUIC on the circuit (TRANuic, transetp.c:51-53)"""
+    d = workloads.with_uic(workloads.rc_charge(), {2: 0.5})
+    text = workloads.deck(d)
+    assert '.tran 2e-11 5e-09 uic' in text
+    nl = ngmi.Netlist(text)
+    assert nl.tran()['uic'] == 1
+    assert nl.circuit().analyze()['unknowns'] == 3
