@@ -828,7 +828,8 @@ def main():
                     help='timed steps (mc: whole .tran runs of the batch; default 1)')
     ap.add_argument('--warmup', type=int, default=None,
                     help='untimed steps before (mc default 1)')
-    ap.add_argument('--workload', default='mc', choices=['mc', 'eval', 'copy', 'rings', 'mesh', 'mesh_tran', 'c3'])
+    ap.add_argument('--workload', default='mc',
+                    choices=['mc', 'eval', 'copy', 'rings', 'mesh', 'mesh_tran', 'c3', 'c0'])
     ap.add_argument('--mesh-window', type=float, default=MESH_TRAN_WINDOW,
                     help='mesh_tran / c3: simulated seconds of the mesh transient timed')
     ap.add_argument('--mesh-side', type=int, default=1024, help='mesh: nodes per side')
@@ -864,9 +865,9 @@ def main():
     if args.tstop is None:
         args.tstop = 10e-9 if args.workload in ('rings', 'c3') else 20e-9
     if args.steps is None:
-        args.steps = 1 if args.workload in ('mc', 'rings', 'c3') else 20
+        args.steps = 1 if args.workload in ('mc', 'rings', 'c3', 'c0') else 20
     if args.warmup is None:
-        args.warmup = 1 if args.workload in ('mc', 'rings', 'c3') else 3
+        args.warmup = 1 if args.workload in ('mc', 'rings', 'c3', 'c0') else 3
     host = host_cores()
     if args.cpu_procs is None:
         args.cpu_procs = host['used']
@@ -886,6 +887,9 @@ def main():
     if args.workload == 'mesh' and not args.no_cpu_baseline:
         cpu = cpu_baseline_mesh(args.mesh_side)
         log('cpu baseline: %s' % cpu)
+    if args.workload == 'c0' and not args.no_cpu_baseline:
+        c0_cpu = cpu_baseline_c0()
+        log('cpu baseline, 101-stage ring: %s' % c0_cpu)
     import torch
     if not torch.cuda.is_available():
         print('bench.py needs a GPU', file=sys.stderr)
@@ -919,6 +923,13 @@ def main():
     elif args.workload == 'c3':
         res = bench_c3(args)
         cpu = None
+    elif args.workload == 'c0':
+        # the reference's CPU-sized case alone: one 101-stage ring
+        c0 = bench_c0(c0_cpu)
+        res = {'metric': 'single-circuit transient timepoints/s', 'value': c0['value'], 'unit': 'timepoints/s',
+               'ms_per_step': c0['ms'], 'dtype': 'f64', 'config': {'workload': c0['workload']},
+               'roofline': None, 'extra': c0}
+        cpu = c0_cpu
     else:
         res = bench_eval(args)
         cpu = None

@@ -54,6 +54,9 @@ const char *ngmi_last_error(void);
 /* Launch-shape knobs (no counterpart in the reference): "eval_waves" and
  * "mos_waves" (1..4) set the occupancy the BSIM4 kernels' register
  * allocation targets for ngmi_bsim4_eval and the batched transient;
+ * "mos_waves" 0 (its default) takes 2, or 1 for a launch whose instances
+ * fit one wave per SIMD (a single small circuit: no spills, shorter
+ * latency);
  * "streams" (1 or 2, default 2): with 2, a single-topology transient (with
  * the learned guess only when its network runs fused) alternates the two
  * halves of its circuits on two streams so one half's step kernel overlaps
@@ -197,6 +200,15 @@ int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream);
  * NGMI_E_SINGULAR when the last ngmi_splu_factor did not succeed (or none
  * ran), where klu_solve would run on the failed factors. */
 int ngmi_splu_solve(ngmi_splu *s, double *b, void *stream);
+/* the largest |l| among the multipliers of the last successful
+ * ngmi_splu_factor (no counterpart in the reference).  The pivot sequence
+ * is fixed by ngmi_splu_analyze, with no threshold test: where KLU's
+ * partial pivoting (klu_kernel.c:370-530, tol = pivrel = 1e-3) would have
+ * exchanged a pivot, its column holds a multiplier above 1/tol (up to KLU's
+ * row scaling), so a value above 1e3 flags a factorization that lost
+ * accuracy KLU would have kept.  NGMI_E_SINGULAR when the last factor
+ * failed or none ran. */
+int ngmi_splu_multiplier(const ngmi_splu *s, double *lmax);
 
 /* ------------------------------------------------------------------ */
 /* One large circuit on the whole GPU: transient analysis              */
@@ -224,6 +236,10 @@ int ngmi_large_info(const ngmi_large *L, int64_t *n_unknowns, int64_t *nnz, int6
 int ngmi_large_tran(ngmi_large *L, double tstep, double tstop, double tmax, void *stream);
 /* recorded points, accepted / rejected timepoints, Newton iterations */
 int ngmi_large_stats(const ngmi_large *L, int64_t *npts, int64_t *accepted, int64_t *rejected, int64_t *iterations);
+/* the largest |l| multiplier over the last ngmi_large_tran's
+ * factorizations (see ngmi_splu_multiplier; no counterpart in the
+ * reference) */
+int ngmi_large_multiplier(const ngmi_large *L, double *lmax);
 /* the record: t[npts], x[npts][nprobe] */
 int ngmi_large_waveform(const ngmi_large *L, double *t, double *x, int64_t maxpts, int64_t *npts);
 /* KCL at the last accepted solution x: the circuit equations F(x) =

@@ -110,6 +110,7 @@ struct ngmi_large {
     hipStream_t s = nullptr;
     LgStep S{};
     int64_t accepted = 0, rejected = 0, iters = 0;
+    double lmax = 0.0;   /* the largest |l| multiplier of the run's factorizations */
     double tstep_run = 0.0, finalTime = 0.0, minBreak = 0.0, time = 0.0;
     std::vector<double> breaks;
     double t_load = 0, t_factor = 0, t_solve = 0, t_other = 0;
@@ -301,6 +302,8 @@ struct ngmi_large {
             auto t2 = clk::now();
             iters++;
             if (rc) return rc;
+            double lm = 0.0;
+            if (ngmi_splu_multiplier(lu, &lm) == NGMI_OK) lmax = std::max(lmax, lm);
             if ((rc = ngmi_splu_solve(lu, x[xnew] + 1, s))) return rc;
             NGMI_HIP(hipStreamSynchronize(s));
             auto t3 = clk::now();
@@ -400,6 +403,7 @@ struct ngmi_large {
         minBreak = 10.0 * delmin;
         time = 0;
         accepted = rejected = iters = 0;
+        lmax = 0.0;
         t_load = t_factor = t_solve = t_other = 0;
         wt.clear();
         wx.clear();
@@ -770,6 +774,13 @@ int ngmi_large_stats(const ngmi_large *L, int64_t *npts, int64_t *accepted, int6
     if (accepted) *accepted = L->accepted;
     if (rejected) *rejected = L->rejected;
     if (iterations) *iterations = L->iters;
+    return NGMI_OK;
+}
+
+int ngmi_large_multiplier(const ngmi_large *L, double *lmax)
+{
+    if (!L || !lmax) return NGMI_E_PANIC;
+    *lmax = L->lmax;
     return NGMI_OK;
 }
 

@@ -7,7 +7,8 @@
  * kernels and the C ABI.
  *
  * Refactor, per call:
- *   k_sp_zero     clears every front (and the zero-pivot flag);
+ *   k_sp_zero     clears every front (and the zero-pivot flag and the
+ *                 largest-multiplier word);
  *   k_sp_scatter  stores each CSC value at its front position;
  *   then by tree height, leaves first:
  *   k_sp_extadd   adds each child's Schur complement into its parent, one
@@ -25,6 +26,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "ngmi_internal.h"
@@ -51,7 +53,7 @@ __global__ void __launch_bounds__(WG) k_sp_zero(double *__restrict__ F, int64_t 
 {
     const int64_t stride = (int64_t)gridDim.x * WG;
     for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += stride) F[i] = 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 4) flag[threadIdx.x] = 0;
 }
 
 __global__ void __launch_bounds__(WG) k_sp_scatter(const int64_t *__restrict__ pos, const double *__restrict__ Ax,
@@ -86,7 +88,11 @@ __global__ void __launch_bounds__(WG) k_sp_extadd(SpDev d, const int *__restrict
 /* Panel p of every front in lev[0..gridDim.x): factor the w x w diagonal
  * block, write it to dfac (the update launch copies it back, after every
  * block of this launch has read the unfactored one), then L21 rows
- * (x U11 = row) and U12 columns (L11 y = column). */
+ * (x U11 = row) and U12 columns (L11 y = column).  The largest |l| of the
+ * panel's multipliers goes into flag[2..3] (a double's bits, atomic max):
+ * a pivot sequence fixed in advance has no threshold test, and KLU's
+ * partial pivoting (tol 1e-3, klu_kernel.c:370-530) would have exchanged
+ * any pivot whose column holds a multiplier above 1/tol. */
 __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict__ lev, int p, double *__restrict__ F,
                                                  double *__restrict__ dfac, int *__restrict__ flag)
 {
@@ -100,6 +106,7 @@ __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict_
     __shared__ double D[NB][NB + 1];
     __shared__ double dinv[NB];
     const int tid = threadIdx.x;
+    double lmax = 0.0;
     if (tid < 64) {
         /* one wave: lane j holds column j; step k scales lane k's column by
          * the pivot's reciprocal and every later column subtracts the
@@ -133,6 +140,11 @@ __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict_
 #pragma unroll
             for (int i = 0; i < NB; i++) D[i][j] = c[i];
             dinv[j] = 1.0 / diag;
+        }
+        if (blockIdx.y == 0 && j < w) {
+#pragma unroll
+            for (int i = 0; i < NB; i++)
+                if (i > j && i < w) lmax = fmax(lmax, fabs(c[i]));
         }
         if (blockIdx.y == 0 && j == 0 && bad) flag[0] = 1;
         if (blockIdx.y == 0 && j < w) {
@@ -168,7 +180,10 @@ __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict_
         }
 #pragma unroll
         for (int k = 0; k < NB; k++)
-            if (k < w) Fr[(int64_t)(kb + k) * nf + i] = x[k];
+            if (k < w) {
+                Fr[(int64_t)(kb + k) * nf + i] = x[k];
+                lmax = fmax(lmax, fabs(x[k]));
+            }
     } else if (t < 2 * m) {
         const int j = t0 + (t - m);
         double *col = Fr + (int64_t)j * nf + kb;
@@ -191,6 +206,11 @@ __global__ void __launch_bounds__(WG) k_sp_panel(SpDev d, const int *__restrict_
         for (int k = 0; k < NB; k++)
             if (k < w) col[k] = y[k];
     }
+    /* every lane of the block reaches here: the wave's largest multiplier */
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o));
+    if ((tid & 63) == 0 && lmax > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long *>(flag + 2), (unsigned long long)__double_as_longlong(lmax));
 }
 
 /* F[t0:, t0:] -= L21 * U12 for panel p, TL x TL tile blockIdx.y; block 0
@@ -457,6 +477,7 @@ struct ngmi_splu {
     SpluSym S;
     bool ready = false;      /* device buffers allocated and plan built */
     bool factored = false;   /* the last ngmi_splu_factor succeeded */
+    double lmax = 0.0;       /* its largest |l| multiplier */
     SpDev d{};
     int *d_ns = nullptr, *d_nb = nullptr, *d_parent = nullptr, *d_idx = nullptr, *d_rel = nullptr;
     int *d_kid_ptr = nullptr, *d_kids = nullptr, *d_lev = nullptr, *d_ea = nullptr, *d_rowsrc = nullptr;
@@ -516,7 +537,7 @@ static int splu_device_alloc(ngmi_splu *s)
     UP(s->d_upd_off, S.upd_off);
     UP(s->d_ent, S.ent_pos);
 #undef UP
-    NGMI_HIP(hipMalloc((void **)&s->d_flag, sizeof(int)));
+    NGMI_HIP(hipMalloc((void **)&s->d_flag, 4 * sizeof(int)));   /* zero-pivot flag, pad, largest |l| */
     NGMI_HIP(hipMalloc((void **)&s->F, std::max<int64_t>(1, S.store) * sizeof(double)));
     NGMI_HIP(hipMalloc((void **)&s->dfac, (size_t)std::max(1, S.max_lev) * NB * NB * sizeof(double)));
     NGMI_HIP(hipMalloc((void **)&s->upd, std::max<int64_t>(1, S.upd_off[S.nf]) * sizeof(double)));
@@ -655,14 +676,26 @@ int ngmi_splu_factor(ngmi_splu *s, const double *Ax, void *stream)
             hipLaunchKernelGGL(k_sp_update, dim3(p.gx, p.gy), dim3(WG), 0, st, s->d, lev, p.p, s->F, s->dfac);
     }
     NGMI_HIP(hipGetLastError());
-    int flag = 0;
-    NGMI_HIP(hipMemcpyAsync(&flag, s->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    int fl[4] = {0, 0, 0, 0};
+    NGMI_HIP(hipMemcpyAsync(fl, s->d_flag, sizeof(fl), hipMemcpyDeviceToHost, st));
     NGMI_HIP(hipStreamSynchronize(st));
-    if (flag) {
+    std::memcpy(&s->lmax, fl + 2, sizeof(double));
+    if (fl[0]) {
         set_error("splu_factor: zero or non-finite pivot");
         return NGMI_E_SINGULAR;
     }
     s->factored = true;
+    return NGMI_OK;
+}
+
+int ngmi_splu_multiplier(const ngmi_splu *s, double *lmax)
+{
+    if (!s || !lmax) return NGMI_E_PANIC;
+    if (!s->factored) {
+        set_error("splu_multiplier: no successful factorization");
+        return NGMI_E_SINGULAR;
+    }
+    *lmax = s->lmax;
     return NGMI_OK;
 }
 

@@ -132,7 +132,9 @@ def sync():
 
 
 def tune(knob, value):
-    """launch-shape knob ('eval_waves' / 'mos_waves', 1 or 2)"""
+    """launch-shape knob (ngmi_tune: 'eval_waves' 1..4, 'mos_waves' 0..4 with
+    0 = automatic, 'streams', 'specialise', 'order_slack', 'graph',
+    'guess_fused', 'guess_grid')"""
     L = lib()
     L.ngmi_tune.argtypes = [ctypes.c_char_p, ctypes.c_int]
     check(L.ngmi_tune(knob.encode(), int(value)), 'tune %s' % knob)
@@ -267,6 +269,7 @@ class SparseLU:
         L.ngmi_splu_rowsrc.argtypes = [vp, vp]
         L.ngmi_splu_factor.argtypes = [vp, vp, vp]
         L.ngmi_splu_solve.argtypes = [vp, vp, vp]
+        L.ngmi_splu_multiplier.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         self.n = int(n)
         Ap = np.ascontiguousarray(Ap, dtype=np.int64)
         Ai = np.ascontiguousarray(Ai, dtype=np.int32)
@@ -304,6 +307,12 @@ class SparseLU:
 
     def solve(self, b_ptr, stream=0):
         check(lib().ngmi_splu_solve(self.h, b_ptr, stream), 'splu_solve')
+
+    def multiplier(self):
+        """the largest |l| multiplier of the last successful factor"""
+        v = ctypes.c_double()
+        check(lib().ngmi_splu_multiplier(self.h, ctypes.byref(v)), 'splu_multiplier')
+        return v.value
 
     def __del__(self):
         try:
@@ -549,6 +558,7 @@ class Large:
         L.ngmi_large_info.argtypes = [vp] + [ctypes.POINTER(i64)] * 3 + [ctypes.POINTER(ctypes.c_double)]
         L.ngmi_large_tran.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
         L.ngmi_large_stats.argtypes = [vp] + [ctypes.POINTER(i64)] * 4
+        L.ngmi_large_multiplier.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.ngmi_large_waveform.argtypes = [vp, vp, vp, i64, ctypes.POINTER(i64)]
         L.ngmi_large_times.argtypes = [vp] + [ctypes.POINTER(ctypes.c_double)] * 4
         L.ngmi_large_kcl.argtypes = [vp, ctypes.POINTER(ctypes.c_double), vp]
@@ -574,7 +584,10 @@ class Large:
     def stats(self):
         v = [ctypes.c_int64() for _ in range(4)]
         check(lib().ngmi_large_stats(self.h, *[ctypes.byref(x) for x in v]), 'large_stats')
-        return dict(points=v[0].value, accepted=v[1].value, rejected=v[2].value, iterations=v[3].value)
+        lm = ctypes.c_double()
+        check(lib().ngmi_large_multiplier(self.h, ctypes.byref(lm)), 'large_multiplier')
+        return dict(points=v[0].value, accepted=v[1].value, rejected=v[2].value, iterations=v[3].value,
+                    max_multiplier=lm.value)
 
     def waveform(self):
         n = self.stats()['points']

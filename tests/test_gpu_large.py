@@ -103,6 +103,9 @@ def test_mesh_against_oracle(side, pad_every, tstop):
     assert rc == 0, rc
     _compare('mesh%d' % side, te, xe, (to, xo, T.stats()), d['nnodes'], _pad_conductance(d))
     assert st['iterations'] == T.stats()['iterations']
+    # the static pivot order's largest multiplier: KLU (tol 1e-3) would
+    # have kept every one of these pivots
+    assert 0 < st['max_multiplier'] < 1e3, st
     r, per = lg.kcl()
     print('KCL at the final point: scaled %.2e, max |F| nodes %.2e A, branches %.2e V' % (r, per[0], per[1]),
           flush=True)

@@ -97,6 +97,30 @@ def test_zero_pivot_is_reported():
     assert np.allclose(db.cpu().numpy(), np.linalg.solve(np.array([[2.0, 1.0], [1.0, 1.0]]), np.ones(2)))
 
 
+def test_largest_multiplier():
+    """ngmi_splu_multiplier: the largest |l| of the factorization equals the
+    one numpy's dense-front factorization over the library's own fronts
+    gives (test_splu.frontal_solve); a 2x2 whose pivots are both 1e-6 of
+    their columns reports 1e6, the multiplier KLU's threshold test
+    (klu_kernel.c:370-530) would have refused"""
+    from test_splu import frontal_solve
+    print('case: largest multiplier, power grid 48x48 and a 2x2 with small pivots', flush=True)
+    n, Ap, Ai, Ax = workloads.power_grid_jacobian(48, 64, 16, seed=48)
+    b = workloads.rng(5, 48).uniform(-1, 1, n)
+    s = ngmi.SparseLU(n, Ap, Ai)
+    _gpu(s, Ax, b)
+    st = {}
+    frontal_solve(s, n, Ap, Ai, Ax, b, st)
+    got = s.multiplier()
+    print('power grid: library %.15g, numpy fronts %.15g' % (got, st['lmax']), flush=True)
+    assert abs(got - st['lmax']) <= 1e-12 * st['lmax']
+    eps = 1e-6
+    s2 = ngmi.SparseLU(2, np.array([0, 2, 4]), np.array([0, 1, 0, 1]))
+    _gpu(s2, np.array([eps, 1.0, 1.0, eps]), np.ones(2))
+    print('2x2: %.17g' % s2.multiplier(), flush=True)
+    assert s2.multiplier() == 1.0 / eps
+
+
 def test_concurrent_with_a_transient_batch():
     """The large-system LU refactoring and solving on its own stream and host
     thread while a ring batch runs its transient on another (how the

@@ -21,8 +21,9 @@ def _structure(s, n):
     return info, fronts, s.rowsrc()
 
 
-def frontal_solve(s, n, Ap, Ai, Ax, b):
-    """factor + solve with the library's fronts, dense per front"""
+def frontal_solve(s, n, Ap, Ai, Ax, b, stats=None):
+    """factor + solve with the library's fronts, dense per front; stats
+    (a dict), when given, receives the largest |l| multiplier"""
     info, fronts, rowsrc = _structure(s, n)
     pos = np.full(n, -1)
     owner = np.full(n, -1)
@@ -56,6 +57,8 @@ def frontal_solve(s, n, Ap, Ai, Ax, b):
         for k in range(ns):
             assert G[k, k] != 0.0, 'zero pivot in front %d' % f
             G[k + 1:, k] /= G[k, k]
+            if stats is not None and k + 1 < ns + nb:
+                stats['lmax'] = max(stats.get('lmax', 0.0), float(np.abs(G[k + 1:, k]).max()))
             G[k + 1:, k + 1:] -= np.outer(G[k + 1:, k], G[k, k + 1:])
     bp = np.asarray(b)[rowsrc]
     y = np.zeros(n)
