@@ -851,6 +851,9 @@ def main():
                     help='skip the extra one-stream run of the default two-stream mode')
     ap.add_argument('--eval-waves', type=int, default=None, help='BSIM4 eval kernel occupancy')
     ap.add_argument('--mos-waves', type=int, default=None, help='transient BSIM4 kernel occupancy')
+    ap.add_argument('--guess-fused', type=int, default=None,
+                    help='learned guess network: 0 layer by layer, 1 fused (weights in registers), '
+                         '2 fused with weights streamed per tile')
     ap.add_argument('--specialise', type=int, default=None,
                     help='0: BSIM4 kernels read the selector word per instance')
     ap.add_argument('--streams', type=int, default=None,
@@ -899,6 +902,8 @@ def main():
         ngmi.tune('eval_waves', args.eval_waves)
     if args.mos_waves:
         ngmi.tune('mos_waves', args.mos_waves)
+    if args.guess_fused is not None:
+        ngmi.tune('guess_fused', args.guess_fused)
     if args.streams:
         ngmi.tune('streams', args.streams)
     if args.specialise is not None:

@@ -69,11 +69,13 @@ const char *ngmi_last_error(void);
  * it applies to batches built afterwards;
  * "graph" (0 or 1, default 1): ngmi_batch_tran replays chunks of 32 Newton
  * passes as one captured HIP graph instead of launching each kernel;
- * "guess_fused" (0 or 1, default 1): the learned guess and ngmi_mlp_forward
- * run the network in one fused kernel (weights in registers, activations in
- * LDS) instead of the layer-by-layer kernels; both give the same values bit
- * for bit;
- * "guess_grid" (0..1024, default 64): workgroups of the fused network
+ * "guess_fused" (0..2, default 2): the learned guess and ngmi_mlp_forward
+ * run the network layer by layer (0), in one fused kernel with the weights
+ * in registers and the activations in LDS (1), or in one fused kernel of
+ * small workgroups that stream each layer's weights from L2 per 32-row
+ * tile (2), which fit beside the other half's BSIM4 kernel on the second
+ * stream; all three give the same values bit for bit;
+ * "guess_grid" (0..1024, default 64): workgroups of the guess_fused 1
  * kernel, each holding a whole compute unit's registers; 0 = one per
  * compute unit.  Fewer leave compute units to the other half's BSIM4
  * kernel on the second stream.

@@ -345,6 +345,117 @@ k_guess_net(const float *__restrict__ X, const unsigned short *__restrict__ wb, 
     }
 }
 
+/* The same network with the weights streamed from L2 per tile instead of
+ * held in registers: 4 waves, wave w owning hidden units 64w .. 64w + 63 as
+ * two 32-column groups, each group's weights for one layer loaded just
+ * before its matrix-core steps.  About 100 VGPRs a lane, so a workgroup
+ * fits where one BSIM4 workgroup of the other stream's half finished
+ * instead of needing a whole compute unit.  Every output column takes the
+ * same matrix-core steps in the same order as in k_guess_net, so both give
+ * the same values bit for bit. */
+constexpr int GL_THREADS = 256;
+static_assert(GL_THREADS / 64 * 64 == MLP_HID, "one wave per 64 hidden units");
+static_assert(GL_THREADS == 32 * GN_HEAD_BLOCKS, "one head partial per thread");
+
+template <int STEPS>
+__device__ __forceinline__ void gl_layer(const unsigned short *A, int lda, const unsigned short *__restrict__ W,
+                                         int ldw, const float *__restrict__ bias, unsigned short *out, int col0,
+                                         int l)
+{
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+        const int j = col0 + 32 * g + (l & 31);
+        const unsigned short *pw = W + (int64_t)j * ldw + (l >> 5) * 8;
+        bf16x8 w[STEPS];
+#pragma unroll
+        for (int q = 0; q < STEPS; q++) w[q] = *reinterpret_cast<const bf16x8 *>(pw + 16 * q);
+        gn_layer(A, lda, w, bias[j], out, col0 + 32 * g, l);
+    }
+}
+
+__global__ void __launch_bounds__(GL_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_guess_net_lite(const float *__restrict__ X, const unsigned short *__restrict__ wb, const float *__restrict__ wf,
+                 int *count, int maxRows, const int64_t *__restrict__ dest, const double2 *__restrict__ base,
+                 double *__restrict__ x, float *__restrict__ yout, unsigned long long *rows_total)
+{
+    __shared__ __align__(16) unsigned short xs[32 * GN_XLD];
+    __shared__ __align__(16) unsigned short ha[32 * GN_HLD];
+    __shared__ __align__(16) unsigned short hb[32 * GN_HLD];
+    __shared__ float w4s[MLP_HID];
+    __shared__ float ps[32][GN_HEAD_BLOCKS + 1];
+    const int tid = threadIdx.x, l = tid & 63, col0 = (tid >> 6) * 64;
+    const int M = count ? min(count[0], maxRows) : maxRows;
+    if ((int64_t)blockIdx.x * 32 < M) {
+        const float b4 = wf[MLP_OFF_B4];
+        for (int e = tid; e < MLP_HID; e += GL_THREADS) w4s[e] = bf2f(wb[MLP_OFF_W4 + e]);
+        for (int e = tid; e < 32 * GN_XLD; e += GL_THREADS) xs[e] = 0;   /* the zero columns stay */
+        __syncthreads();
+        /* this thread's two input elements (rows rr and rr + 16, feature
+         * cc), loaded one tile ahead */
+        const int rr = tid >> 4, cc = tid & 15;
+        const int64_t step = (int64_t)gridDim.x * 32;
+        int64_t i0 = (int64_t)blockIdx.x * 32;
+        float xa = i0 + rr < M ? X[(i0 + rr) * MLP_IN + cc] : 0.0f;
+        float xb = i0 + rr + 16 < M ? X[(i0 + rr + 16) * MLP_IN + cc] : 0.0f;
+        for (; i0 < M; i0 += step) {
+            xs[rr * GN_XLD + cc] = f2bf(xa);
+            xs[(rr + 16) * GN_XLD + cc] = f2bf(xb);
+            xa = i0 + step + rr < M ? X[(i0 + step + rr) * MLP_IN + cc] : 0.0f;
+            xb = i0 + step + rr + 16 < M ? X[(i0 + step + rr + 16) * MLP_IN + cc] : 0.0f;
+            const bool mine = tid < 32 && i0 + tid < M;
+            const int64_t d_row = (mine && x) ? dest[i0 + tid] : 0;
+            const double2 b_row = (mine && x) ? base[i0 + tid] : make_double2(0.0, 0.0);
+            /* the weight pointers made opaque per tile, so the compiler does
+             * not hoist every layer's weights out of the loop into registers */
+            const unsigned short *wt = wb;
+            const float *ft = wf;
+            asm volatile("" : "+s"(wt), "+s"(ft));
+            __syncthreads();
+            gl_layer<MLP_IN_PAD / 16>(xs, GN_XLD, wt + MLP_OFF_W1, MLP_IN_PAD, ft + MLP_OFF_B1, ha, col0, l);
+            __syncthreads();
+            gl_layer<MLP_HID / 16>(ha, GN_HLD, wt + MLP_OFF_W2, MLP_HID, ft + MLP_OFF_B2, hb, col0, l);
+            __syncthreads();
+            gl_layer<MLP_HID / 16>(hb, GN_HLD, wt + MLP_OFF_W3, MLP_HID, ft + MLP_OFF_B3, ha, col0, l);
+            __syncthreads();
+            {
+                const int hr = tid / GN_HEAD_BLOCKS, cb = tid % GN_HEAD_BLOCKS;
+                const uint4 *h = reinterpret_cast<const uint4 *>(ha + hr * GN_HLD + cb * MLP_HEAD_BLOCK);
+                const float *w4 = w4s + cb * MLP_HEAD_BLOCK;
+                float p = 0.0f;
+#pragma unroll
+                for (int q = 0; q < MLP_HEAD_BLOCK / 8; q++) {
+                    const uint4 u = h[q];
+                    const unsigned int wd[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        p = fmaf(bf2f((unsigned short)(wd[e] & 0xffffu)), w4[8 * q + 2 * e], p);
+                        p = fmaf(bf2f((unsigned short)(wd[e] >> 16)), w4[8 * q + 2 * e + 1], p);
+                    }
+                }
+                ps[hr][cb] = p;
+            }
+            __syncthreads();
+            if (mine) {
+                float y = b4;
+#pragma unroll
+                for (int cb = 0; cb < GN_HEAD_BLOCKS; cb++) y += ps[tid][cb];
+                if (yout) yout[i0 + tid] = y;
+                if (x) x[d_row] = guess_value(b_row, y);
+            }
+        }
+    }
+    if (!count) return;
+    __syncthreads();   /* every wave has read count[0] */
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(reinterpret_cast<unsigned int *>(count + 1), 1u) == gridDim.x - 1) {
+            if (rows_total) atomicAdd(rows_total, (unsigned long long)M);
+            count[0] = 0;
+            count[1] = 0;
+        }
+    }
+}
+
 int launch_guess_net(const unsigned short *wb, const float *wf, const float *X, int *count, int maxRows,
                      const int64_t *dest, const double2 *base, double *x, float *yout,
                      unsigned long long *rows_total, hipStream_t s)
@@ -361,6 +472,14 @@ int launch_guess_net(const unsigned short *wb, const float *wf, const float *X, 
      * BSIM4 kernel of the other stream's half (measured 0.933x plain
      * against 0.923x with all 256) */
     const int tiles = std::max(1, (maxRows + 31) / 32);
+    if (tune_get(TUNE_GUESS_FUSED) == 2) {
+        /* streamed weights: small workgroups, two per compute unit */
+        const int grid = std::min(tiles, 2 * cus);
+        hipLaunchKernelGGL(k_guess_net_lite, dim3((unsigned)grid), dim3(GL_THREADS), 0, s, X, wb, wf, count,
+                           maxRows, dest, base, x, yout, rows_total);
+        NGMI_HIP(hipGetLastError());
+        return NGMI_OK;
+    }
     const int want = tune_get(TUNE_GUESS_GRID);
     const int grid = std::min(tiles, want > 0 ? want : cus);
     hipLaunchKernelGGL(k_guess_net, dim3((unsigned)grid), dim3(GN_THREADS), 0, s, X, wb, wf, count, maxRows,

@@ -230,13 +230,17 @@ def test_fused_network_kernel_matches_layered_path(trained):
     head -> starting values in one launch, weights in registers) against the
     layer-by-layer kernels it replaces: the same learned runs, bit for bit
     (waveforms, Newton statistics and the number of network rows), since
-    both issue the same matrix-core steps and the same element-wise code."""
+    both issue the same matrix-core steps and the same element-wise code;
+    and the fused kernel with weights streamed per tile (guess_fused 2)
+    against both."""
     net = trained[0]
     seeds = [0, 1, 2, 3]
-    print('case: fused vs layered guess network, adder seeds 0..3 .tran 5p %gn' % (TSTOP * 1e9), flush=True)
+    print('case: fused vs layered vs streamed guess network, adder seeds 0..3 .tran 5p %gn' % (TSTOP * 1e9),
+          flush=True)
     out = {}
+    keep = ngmi.get_tune('guess_fused')
     try:
-        for fused in (1, 0):
+        for fused in (1, 0, 2):
             ngmi.tune('guess_fused', fused)
             b, descs = run(seeds, net=net)
             n = b.info()['unknowns']
@@ -244,7 +248,7 @@ def test_fused_network_kernel_matches_layered_path(trained):
                           [b.stats(c) for c in range(len(seeds))],
                           [b.guess_stats(c)['guessed'] for c in range(len(seeds))])
     finally:
-        ngmi.tune('guess_fused', 1)
+        ngmi.tune('guess_fused', keep)
     rows_f, wf_f, st_f, g_f = out[1]
     rows_l, wf_l, st_l, g_l = out[0]
     print('network rows: fused %d, layered %d; steps from the guess %s / %s'
@@ -256,3 +260,10 @@ def test_fused_network_kernel_matches_layered_path(trained):
         assert st_f[c]['iterations'] == st_l[c]['iterations']
         assert np.array_equal(wf_f[c][0], wf_l[c][0])
         assert np.array_equal(wf_f[c][1], wf_l[c][1])
+    rows_s, wf_s, st_s, g_s = out[2]
+    print('streamed weights: network rows %d, steps from the guess %s' % (rows_s, g_s), flush=True)
+    assert rows_s == rows_f and g_s == g_f
+    for c in range(len(seeds)):
+        assert st_s[c]['iterations'] == st_f[c]['iterations']
+        assert np.array_equal(wf_s[c][0], wf_f[c][0])
+        assert np.array_equal(wf_s[c][1], wf_f[c][1])
