@@ -10,7 +10,8 @@ the first step sets a breakpoint at tstep (dctran.c:509-512).
 
 Bars:
 * RC charging through 1 kOhm into 1 pF from .ic v(2) = 0.5 (and from no
-  .ic, i.e. 0): v(2) within 5e-5 V of v - (v - v0) exp(-t/RC), the closed
+  .ic, i.e. 0; and from 0.5 under Gear integration): v(2) within 5e-5 V
+  of v - (v - v0) exp(-t/RC), the closed
   form, at every output point; the first output point after t = 0; tstep
   among the output times; without uic the same .ic gives the t = 0 point
   at v0 (the .ic row of the operating point, cktload.c:146-173);
@@ -41,13 +42,14 @@ def _run(desc):
 
 
 def test_rc_charge_closed_form():
-    for v0, ic in ((0.5, {2: 0.5}), (0.0, {})):
+    for v0, ic, method in ((0.5, {2: 0.5}, 1), (0.0, {}, 1), (0.5, {2: 0.5}, 2)):
         d = workloads.rc_charge(ic=ic, uic=True)
+        d['options'] = dict(d['options'], method=method)
         T, t, x = _run(d)
         exact = 1.0 - (1.0 - v0) * np.exp(-t / (R * C))
         err = np.abs(x[:, 2] - exact).max()
-        print('uic, v(2) from %.2f: %d points from t = %.3g, max error %.2e V, %s'
-              % (v0, len(t), t[0], err, T.stats()))
+        print('uic, %s, v(2) from %.2f: %d points from t = %.3g, max error %.2e V, %s'
+              % ('gear' if method == 2 else 'trap', v0, len(t), t[0], err, T.stats()))
         assert t[0] > 0
         assert np.isclose(t, d['tstep'], rtol=1e-9, atol=0).any()
         assert np.abs(x[:, 1] - 1.0).max() < 1e-12
