@@ -12,8 +12,8 @@ values within 1e-6 of full scale and of themselves above 1e-2 of full
 scale) and the oracle's Newton iteration count, on RC charging from
 .ic v(2) = 0.5 and from no .ic (two topology groups in one batch), ring11
 with every node's .ic given, ring11 with the supply node's .ic left out
-(its PMOS start from 0 V on VDD), and ring11 with the instance ICs given
-explicitly.
+(its PMOS start from 0 V on VDD), ring11 with the instance ICs given
+explicitly, and ring11 under Gear integration.
 """
 import os
 import sys
@@ -96,6 +96,9 @@ def test_ring_uic():
         v = lambda u: ic.get(u, 0.0)  # noqa: E731
         mos.append((key, dn, g, s, bb, dict(p, icvds=v(dn) - v(s), icvgs=v(g) - v(s), icvbs=v(bb) - v(s))))
     cases.append(('instance ICs given', dict(explicit, mos=mos), 1))
+    gear = workloads.with_uic(d, ic)
+    gear['options'] = dict(gear['options'], method=2)
+    cases.append(('Gear', gear, 2))
     for name, desc, copies in cases:
         print('case: ring11 uic, %s, %d circuit(s)' % (name, copies), flush=True)
         T, t, x = _oracle(desc)
