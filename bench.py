@@ -854,6 +854,8 @@ def main():
     ap.add_argument('--guess-fused', type=int, default=None,
                     help='learned guess network: 0 layer by layer, 1 fused (weights in registers), '
                          '2 fused with weights streamed per tile')
+    ap.add_argument('--guess-grid', type=int, default=None,
+                    help='workgroups of the fused guess network kernel (0 = one per compute unit)')
     ap.add_argument('--specialise', type=int, default=None,
                     help='0: BSIM4 kernels read the selector word per instance')
     ap.add_argument('--streams', type=int, default=None,
@@ -904,6 +906,8 @@ def main():
         ngmi.tune('mos_waves', args.mos_waves)
     if args.guess_fused is not None:
         ngmi.tune('guess_fused', args.guess_fused)
+    if args.guess_grid is not None:
+        ngmi.tune('guess_grid', args.guess_grid)
     if args.streams:
         ngmi.tune('streams', args.streams)
     if args.specialise is not None:

@@ -75,10 +75,11 @@ const char *ngmi_last_error(void);
  * small workgroups that stream each layer's weights from L2 per 32-row
  * tile (2), which fit beside the other half's BSIM4 kernel on the second
  * stream; all three give the same values bit for bit;
- * "guess_grid" (0..1024, default 64): workgroups of the guess_fused 1
- * kernel, each holding a whole compute unit's registers; 0 = one per
- * compute unit.  Fewer leave compute units to the other half's BSIM4
- * kernel on the second stream.
+ * "guess_grid" (0..1024, default 64): workgroups of the fused network
+ * kernels (guess_fused 1 and 2); 0 = one per compute unit.  Fewer leave
+ * compute units to the other half's BSIM4 kernel on the second stream
+ * (guess_fused 2 on the adder batch: 0.977x plain at 64, 0.976x at 32,
+ * 0.972x at 128, 0.965x at 256, 0.950x at 512).
  * Process-wide; defaults are the measured best on MI355X. */
 int ngmi_tune(const char *knob, int value);
 /* current value of a knob, or minus NGMI_E_BADPARM for an unknown name */

@@ -472,15 +472,16 @@ int launch_guess_net(const unsigned short *wb, const float *wf, const float *X, 
      * BSIM4 kernel of the other stream's half (measured 0.933x plain
      * against 0.923x with all 256) */
     const int tiles = std::max(1, (maxRows + 31) / 32);
+    const int want = tune_get(TUNE_GUESS_GRID);
     if (tune_get(TUNE_GUESS_FUSED) == 2) {
-        /* streamed weights: small workgroups, two per compute unit */
-        const int grid = std::min(tiles, 2 * cus);
+        /* streamed weights: guess_grid small workgroups (0 = one per
+         * compute unit) */
+        const int grid = std::min(tiles, want > 0 ? want : cus);
         hipLaunchKernelGGL(k_guess_net_lite, dim3((unsigned)grid), dim3(GL_THREADS), 0, s, X, wb, wf, count,
                            maxRows, dest, base, x, yout, rows_total);
         NGMI_HIP(hipGetLastError());
         return NGMI_OK;
     }
-    const int want = tune_get(TUNE_GUESS_GRID);
     const int grid = std::min(tiles, want > 0 ? want : cus);
     hipLaunchKernelGGL(k_guess_net, dim3((unsigned)grid), dim3(GN_THREADS), 0, s, X, wb, wf, count, maxRows,
                        dest, base, x, yout, rows_total);
