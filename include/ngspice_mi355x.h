@@ -57,10 +57,11 @@ const char *ngmi_last_error(void);
  * "mos_waves" 0 (its default) takes 2, or 1 for a launch whose instances
  * fit one wave per SIMD (a single small circuit: no spills, shorter
  * latency);
- * "streams" (1 or 2, default 2): with 2, a single-topology transient (with
+ * "streams" (1..4, default 2): with 2, a single-topology transient (with
  * the learned guess only when its network runs fused) alternates the two
  * halves of its circuits on two streams so one half's step kernel overlaps
- * the other half's BSIM4 kernel;
+ * the other half's BSIM4 kernel; 3 and 4 split a plain run (no guess, no
+ * profiling) into that many parts on as many streams;
  * "specialise" (0 or 1, default 1): compile the default card's selectors as
  * constants when every instance of a set shares them;
  * "order_slack" (0..4, default 1): the engine's own pivot order may pick a

@@ -64,7 +64,7 @@ int ngmi_tune(const char *knob, int value)
     }
     const bool flag = k == TUNE_SPECIALISE || k == TUNE_GRAPH || k == TUNE_GUESS_FUSED;
     const int vmin = (flag || k == TUNE_ORDER_SLACK || k == TUNE_GUESS_GRID || k == TUNE_MOS_WAVES) ? 0 : 1;
-    const int vmax = (k == TUNE_STREAMS || k == TUNE_GUESS_FUSED) ? 2 : flag ? 1 : (k == TUNE_GUESS_GRID) ? 1024 : 4;
+    const int vmax = (k == TUNE_GUESS_FUSED) ? 2 : flag ? 1 : (k == TUNE_GUESS_GRID) ? 1024 : 4;
     if (value < vmin || value > vmax) {
         set_error("knob '%s' takes %d..%d", knob, vmin, vmax);
         return NGMI_E_PARMVAL;

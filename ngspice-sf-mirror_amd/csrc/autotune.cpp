@@ -63,7 +63,7 @@ int ngmi_autotune_batch(ngmi_circuit *const *ckts, int C, double tstep, double t
     using clk = std::chrono::steady_clock;
     std::vector<Knob> knobs = {{"order_slack", ngmi::TUNE_ORDER_SLACK, {0, 1, 2}, true},
                                {"mos_waves", ngmi::TUNE_MOS_WAVES, {1, 2, 3, 4}, false},
-                               {"streams", ngmi::TUNE_STREAMS, {1, 2}, false},
+                               {"streams", ngmi::TUNE_STREAMS, {1, 2, 3, 4}, false},
                                {"graph", ngmi::TUNE_GRAPH, {0, 1}, false}};
     std::ostringstream log;
     ngmi_batch *b = nullptr;

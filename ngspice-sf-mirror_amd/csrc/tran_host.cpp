@@ -1507,19 +1507,33 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
      * topology; with the learned guess each half has its own feature rows
      * and network launch, which needs the fused kernel, and profiling then
      * stays on one stream) */
-    const bool split = one && C >= 2 && tune_get(TUNE_STREAMS) == 2 &&
+    const bool split = one && C >= 2 && tune_get(TUNE_STREAMS) >= 2 &&
                        (!b->net || (fused && !b->profile));
     const int half = C / 2;
     b->groups[0]->D.gsplit = split ? half : C;
+    /* streams 3 or 4 (plain runs without profiling): the circuits in that
+     * many parts, parts 2.. each on a stream of its own; part 1 ends at
+     * cut2 */
+    const int nparts = !split ? 1 : (b->net || b->profile) ? 2 : std::min(tune_get(TUNE_STREAMS), C);
+    std::vector<int> cut(nparts + 1);
+    for (int p = 0; p <= nparts; p++) cut[p] = (int)((int64_t)C * p / nparts);
+    const int cut2 = nparts > 2 ? cut[2] : C;   /* with 2 parts cut[1] = half, as the guess rows assume */
     hipStream_t s2 = nullptr;
     hipEvent_t join = nullptr;
+    std::vector<hipStream_t> sx(nparts > 2 ? nparts - 2 : 0, nullptr);
+    std::vector<hipEvent_t> jx(sx.size(), nullptr);
     if (split) {
         NGMI_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         NGMI_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        for (size_t k = 0; k < sx.size(); k++) {
+            NGMI_HIP(hipStreamCreateWithFlags(&sx[k], hipStreamNonBlocking));
+            NGMI_HIP(hipEventCreateWithFlags(&jx[k], hipEventDisableTiming));
+        }
     }
-    if (split) {   /* the second stream starts after the resets above */
+    if (split) {   /* the other streams start after the resets above */
         NGMI_HIP(hipEventRecord(join, s));
         NGMI_HIP(hipStreamWaitEvent(s2, join, 0));
+        for (hipStream_t t : sx) NGMI_HIP(hipStreamWaitEvent(t, join, 0));
     }
     int64_t pass = 0;
     int done = 0;
@@ -1575,17 +1589,21 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         if (split && capturing) {
             NGMI_HIP(hipEventRecord(fork, sa));
             NGMI_HIP(hipStreamWaitEvent(s2, fork, 0));
+            for (hipStream_t t : sx) NGMI_HIP(hipStreamWaitEvent(t, fork, 0));
         }
         for (int q = 0; q < count && r == NGMI_OK; q++) {
             if (b->net) r = guess_pass(sa, 0);
             if (r != NGMI_OK) break;
             if (split) {
-                r = launch_tran_pass(tab0, &G0.D, 0, half, nullptr, nullptr, nullptr, 1, nullptr, half, b->lds, sa,
+                r = launch_tran_pass(tab0, &G0.D, 0, cut[1], nullptr, nullptr, nullptr, 1, nullptr, cut[1], b->lds, sa,
                                      nullptr);
                 if (r == NGMI_OK && b->net) r = guess_pass(s2, 1);
                 if (r == NGMI_OK)
-                    r = launch_tran_pass(tab0, &G0.D, half, C, nullptr, nullptr, nullptr, 1, nullptr, C - half,
-                                         b->lds, s2, nullptr);
+                    r = launch_tran_pass(tab0, &G0.D, cut[1], cut2, nullptr, nullptr, nullptr, 1, nullptr,
+                                         cut2 - cut[1], b->lds, s2, nullptr);
+                for (size_t t = 0; t < sx.size() && r == NGMI_OK; t++)
+                    r = launch_tran_pass(tab0, &G0.D, cut[t + 2], cut[t + 3], nullptr, nullptr, nullptr, 1, nullptr,
+                                         cut[t + 3] - cut[t + 2], b->lds, sx[t], nullptr);
             } else {
                 r = launch_tran_pass(one ? tab0 : nullptr, one ? &G0.D : nullptr, 0, C, b->dTab, b->dD, b->mos_blk,
                                      b->n_mos_blk, b->step_blk, b->n_step_blk, b->lds, sa, nullptr);
@@ -1594,6 +1612,10 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         if (split && capturing && r == NGMI_OK) {
             NGMI_HIP(hipEventRecord(join, s2));
             NGMI_HIP(hipStreamWaitEvent(sa, join, 0));
+            for (size_t t = 0; t < sx.size(); t++) {
+                NGMI_HIP(hipEventRecord(jx[t], sx[t]));
+                NGMI_HIP(hipStreamWaitEvent(sa, jx[t], 0));
+            }
         }
         return r;
     };
@@ -1634,6 +1656,12 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
                 rc = NGMI_E_DEVICE;
                 break;
             }
+            for (size_t t = 0; t < sx.size() && rc == NGMI_OK; t++)
+                if (hipEventRecord(jx[t], sx[t]) != hipSuccess || hipStreamWaitEvent(s, jx[t], 0) != hipSuccess) {
+                    set_error("stream join failed: %s", hipGetErrorString(hipGetLastError()));
+                    rc = NGMI_E_DEVICE;
+                }
+            if (rc != NGMI_OK) break;
         }
         for (; b->profile && k < check_every && pass < max_passes; k++, pass++) {
             if (b->net) {
@@ -1700,6 +1728,11 @@ int ngmi_batch_tran(ngmi_batch *b, double tstep, double tstop, double tmax, int6
         (void)hipStreamSynchronize(s2);
         (void)hipEventDestroy(join);
         (void)hipStreamDestroy(s2);
+        for (size_t t = 0; t < sx.size(); t++) {
+            (void)hipStreamSynchronize(sx[t]);
+            (void)hipEventDestroy(jx[t]);
+            (void)hipStreamDestroy(sx[t]);
+        }
     }
     /* evaluations and passes from the circuits' own counters */
     b->counters[0] = b->counters[1] = 0;

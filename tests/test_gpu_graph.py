@@ -48,6 +48,24 @@ def test_graph_matches_eager_single_topology():
     _same(_run(descs, 2e-9, 0), _run(descs, 2e-9, 1), 'adders')
 
 
+def test_stream_parts_are_bit_identical():
+    """knob "streams" 1..4: the single-topology batch in that many parts on
+    as many streams, under graphs and eagerly; the circuits are independent,
+    so every split gives the one-stream results bit for bit"""
+    print('case: streams 1..4, adders seeds 0..6 .tran 5p 1n, graph and eager', flush=True)
+    descs = [workloads.mc_adder(s) for s in range(7)]
+    keep = ngmi.get_tune('streams')
+    try:
+        ngmi.tune('streams', 1)
+        ref = _run(descs, 1e-9, 1)
+        for n in (2, 3, 4):
+            ngmi.tune('streams', n)
+            _same(ref, _run(descs, 1e-9, 1), 'streams %d, graph' % n)
+            _same(ref, _run(descs, 1e-9, 0), 'streams %d, eager' % n)
+    finally:
+        ngmi.tune('streams', keep)
+
+
 def test_graph_matches_eager_uneven_batch():
     print('case: graph vs eager, rings of 11/21/13/31 stages .tran 10p 2n (topology groups)', flush=True)
     descs = [workloads.ring_oscillator(stages=k, tstop=2e-9) for k in (11, 21, 13, 31, 11)]
