@@ -54,9 +54,8 @@ const char *ngmi_last_error(void);
 /* Launch-shape knobs (no counterpart in the reference): "eval_waves" and
  * "mos_waves" (1..4) set the occupancy the BSIM4 kernels' register
  * allocation targets for ngmi_bsim4_eval and the batched transient;
- * "mos_waves" 0 (its default) takes 2, or 1 for a launch whose instances
- * fit one wave per SIMD (a single small circuit: no spills, shorter
- * latency);
+ * 0 (the default of both) takes 2, or 1 for a launch whose instances fit
+ * one wave per SIMD (a single small circuit: no spills, shorter latency);
  * "streams" (1..4, default 2): with 2, a single-topology transient (with
  * the learned guess only when its network runs fused) alternates the two
  * halves of its circuits on two streams so one half's step kernel overlaps

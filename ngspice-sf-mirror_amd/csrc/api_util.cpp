@@ -10,9 +10,25 @@ namespace ngmi {
 
 static thread_local char g_err[512] = "";
 
-static int g_tune[TUNE__COUNT] = {2, 0, 2, 1, 1, 1, 2, 64};
+static int g_tune[TUNE__COUNT] = {0, 0, 2, 1, 1, 1, 2, 64};
 
 int tune_get(int knob) { return (knob >= 0 && knob < TUNE__COUNT) ? g_tune[knob] : 0; }
+
+int waves_for(int knob, int64_t instances)
+{
+    const int w = tune_get(knob);
+    if (w > 0) return w;
+    static int lanes[64] = {0};   /* per device: compute units x 4 SIMDs x 64 lanes */
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2;
+    if (!lanes[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            return 2;
+        lanes[dev] = cus * 4 * 64;
+    }
+    return instances <= lanes[dev] ? 1 : 2;
+}
 
 void set_error(const char *fmt, ...)
 {
@@ -63,7 +79,7 @@ int ngmi_tune(const char *knob, int value)
         return NGMI_E_BADPARM;
     }
     const bool flag = k == TUNE_SPECIALISE || k == TUNE_GRAPH || k == TUNE_GUESS_FUSED;
-    const int vmin = (flag || k == TUNE_ORDER_SLACK || k == TUNE_GUESS_GRID || k == TUNE_MOS_WAVES) ? 0 : 1;
+    const int vmin = (flag || k == TUNE_ORDER_SLACK || k == TUNE_GUESS_GRID || k == TUNE_MOS_WAVES || k == TUNE_EVAL_WAVES) ? 0 : 1;
     const int vmax = (k == TUNE_GUESS_FUSED) ? 2 : flag ? 1 : (k == TUNE_GUESS_GRID) ? 1024 : 4;
     if (value < vmin || value > vmax) {
         set_error("knob '%s' takes %d..%d", knob, vmin, vmax);

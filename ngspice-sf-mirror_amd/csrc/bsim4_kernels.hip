@@ -83,7 +83,7 @@ int launch_bsim4_eval(const B4Tab &tab, const B4Ckt &ck, const double *vterm,
     const bool dflt = tab.flags == B4FL_DEFAULT && tune_get(TUNE_SPECIALISE);
 #define NGMI_EVAL(W, F) \
     hipLaunchKernelGGL((k_bsim4_eval<W, F>), dim3((unsigned)nb), dim3(bs), 0, stream, tab, ck, io)
-    switch (tune_get(TUNE_EVAL_WAVES)) {
+    switch (waves_for(TUNE_EVAL_WAVES, tab.n)) {
     case 1: if (dflt) NGMI_EVAL(1, B4FL_DEFAULT); else NGMI_EVAL(1, -1); break;
     case 3: if (dflt) NGMI_EVAL(3, B4FL_DEFAULT); else NGMI_EVAL(3, -1); break;
     case 4: if (dflt) NGMI_EVAL(4, B4FL_DEFAULT); else NGMI_EVAL(4, -1); break;

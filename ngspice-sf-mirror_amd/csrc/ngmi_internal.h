@@ -28,6 +28,11 @@ enum {
     TUNE__COUNT
 };
 int tune_get(int knob);
+/* a BSIM4 kernel's occupancy variant for a launch of this many instances:
+ * the knob's value, or for 0 (automatic) 2, or 1 when one wave per SIMD
+ * holds every instance (its occupancy then buys nothing and the spill-free
+ * 1-wave variant has the shorter latency) */
+int waves_for(int knob, int64_t instances);
 
 #define NGMI_HIP(call)                                                          \
     do {                                                                        \

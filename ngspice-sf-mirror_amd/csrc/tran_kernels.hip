@@ -2169,31 +2169,13 @@ int set_tran_lds_limit(size_t bytes)
  * circuits [c_begin, c_end).  ev (optional) = 3
  * events recorded before the MOSFET kernel, between the two kernels and
  * after the step kernel. */
-/* mos_waves 0 (the default): the 2-wave variant, unless one wave per SIMD
- * already holds every instance of the launch, where its occupancy buys
- * nothing and the spill-free 1-wave variant has the shorter latency (a
- * single 101-stage ring: 49.7 against 53.2 us per launch) */
-static int mos_waves_for(int64_t instances)
-{
-    const int w = tune_get(TUNE_MOS_WAVES);
-    if (w > 0) return w;
-    static int lanes[64] = {0};   /* per device: compute units x 4 SIMDs x 64 lanes */
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2;
-    if (!lanes[dev]) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            return 2;
-        lanes[dev] = cus * 4 * 64;
-    }
-    return instances <= lanes[dev] ? 1 : 2;
-}
-
 int launch_tran_pass(const B4Tab *tab0, const TranDev *D0, int c_begin, int c_end, const B4Tab *tabs,
                      const TranDev *Ds, const int2 *mos_blk, int n_mos_blk, const int2 *step_blk,
                      int n_step_blk, size_t lds_bytes, hipStream_t s, hipEvent_t *ev)
 {
-    const int waves = mos_waves_for(D0 ? (int64_t)(c_end - c_begin) * D0->NM : (int64_t)n_mos_blk * 256);
+    /* mos_waves 0 (the default): one wave on a single 101-stage ring, 49.7
+     * against 53.2 us per launch with two */
+    const int waves = waves_for(TUNE_MOS_WAVES, D0 ? (int64_t)(c_end - c_begin) * D0->NM : (int64_t)n_mos_blk * 256);
     if (ev) NGMI_HIP(hipEventRecord(ev[0], s));
     if (n_mos_blk > 0) {
         const dim3 g((unsigned)n_mos_blk);

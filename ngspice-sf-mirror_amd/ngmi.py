@@ -132,7 +132,7 @@ def sync():
 
 
 def tune(knob, value):
-    """launch-shape knob (ngmi_tune: 'eval_waves' 1..4, 'mos_waves' 0..4 with
+    """launch-shape knob (ngmi_tune: 'eval_waves' and 'mos_waves' 0..4 with
     0 = automatic, 'streams', 'specialise', 'order_slack', 'graph',
     'guess_fused', 'guess_grid')"""
     L = lib()
