@@ -52,7 +52,7 @@ extern "C" {
 const char *ngmi_last_error(void);
 
 /* Launch-shape knobs (no counterpart in the reference): "eval_waves" and
- * "mos_waves" (1..4) set the occupancy the BSIM4 kernels' register
+ * "mos_waves" (0..4) set the occupancy the BSIM4 kernels' register
  * allocation targets for ngmi_bsim4_eval and the batched transient;
  * 0 (the default of both) takes 2, or 1 for a launch whose instances fit
  * one wave per SIMD (a single small circuit: no spills, shorter latency);
